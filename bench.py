@@ -1,0 +1,186 @@
+"""Benchmark: converged subsets per second of the full-field DIC solve
+(semper_run's pipeline) on one MI355X.
+
+Workload (default): 4096 x 4096 synthetic speckle with the heterogeneous field
+u = 3 sin(2 pi x / 1024) cos(2 pi y / 1536), v = 2 exp(-r^2 / (2 600^2)),
+step 8, subset 31, strain window 31, Keys 4x4, FP32 IC-GN (253k attempted
+subsets). One step = one full solve of the deformed frame against the cached
+reference (seeding, Path A/B, strain, packing), with both frames already in
+HBM. `value` = solved subsets (status 0, ZNSSD <= 0.15) per second, scaled by
+the fraction of them within 1e-3 px of the FP64 oracle on a sampled subset of
+nodes.
+
+Also reported: the roofline of the dominant kernel (the Path A subset solver
+launch, FP32 VALU bound: 109 flop per pixel-iteration + 8 per subset pixel for
+the SD build), timed with HIP events on the engine's stream, and the CPU
+baseline: the oracle (a C++/OpenMP restatement of the reference's
+run_full_field) on a bounded sample of the same workload on the host cores.
+"""
+import argparse
+import ctypes
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "semper-dic-engine_amd")
+
+
+def _load(name, path):
+    spec = importlib.util.spec_from_file_location(name, path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+FLOP_PER_PIXEL_ITER = {False: 109.0, True: 227.0}
+SD_FLOP_PER_PIXEL = 8.0
+PEAK_FP32_TFLOPS = 157.3
+
+
+def cpu_baseline(ref, dfm, prm, gpu_pts, sample_rect, use6, threads):
+    """The oracle on a bounded crop of the same workload, and the FP64 check of
+    a node sample of the GPU field (the CPU-side checker)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import OracleEngine, fp64_polish
+
+    oe = OracleEngine()
+    oe.set_reference(ref)
+    p = list(prm)
+    p[0:4] = sample_rect
+    t0 = time.perf_counter()
+    n, _, met = oe.run(dfm, p, threads=threads)
+    dt = time.perf_counter() - t0
+    solved = float(met[3] + met[4] + met[19])
+    # FP64 yardstick on a sample of the GPU's output points
+    rng = np.random.default_rng(0)
+    k = min(2000, len(gpu_pts))
+    idx = rng.choice(len(gpu_pts), size=k, replace=False)
+    pts = gpu_pts[idx]
+    nodes = [(int(x), int(y)) for x, y in pts[:, :2]]
+    h, w = ref.shape
+    # start from the GPU's displacement with zero gradients; FP64 IC-GN to 1e-10
+    p0 = np.zeros((k, 6), np.float64)
+    p0[:, 0], p0[:, 1] = pts[:, 2], pts[:, 3]
+    si, po, _ = fp64_polish(ref.astype(np.float32), dfm.astype(np.float32), prm[5], nodes, p0, use6=use6)
+    ok = si[:, 0] == 0
+    du = np.abs(po[ok, 0] - pts[ok, 2])
+    dv = np.abs(po[ok, 1] - pts[ok, 3])
+    frac = float(np.mean((du <= 1e-3) & (dv <= 1e-3))) if ok.any() else 0.0
+    return {
+        "value": solved / dt,
+        "unit": "converged subsets/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"oracle run_full_field on the {sample_rect[2]}x{sample_rect[3]} px ROI at ({sample_rect[0]}, "
+                  f"{sample_rect[1]}) of the same frame pair ({int(met[0])} attempted subsets), {dt:.2f} s",
+    }, {"sampled": int(k), "fp64_converged": int(ok.sum()), "within_1e-3px": frac,
+        "max_du": float(du.max()) if ok.any() else None, "max_dv": float(dv.max()) if ok.any() else None}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c4_4096_heterogeneous")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and FP64 check")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
+        sys.exit(2)
+
+    sa = _load("semper_amd", os.path.join(PKG, "semper_amd.py"))
+    wl = _load("workloads", os.path.join(PKG, "workloads.py"))
+    dev = sa.load_dev()
+    if dev.sdk_device_count() < 1:
+        raise RuntimeError("no GPU visible")
+    cfg = wl.config(args.workload)
+    w, h = cfg["w"], cfg["h"]
+    prm = np.array(wl.solve_params(args.workload), np.int32)
+    use6 = bool(prm[7])
+    t0 = time.perf_counter()
+    dref, ddef = wl.pair(dev, args.workload, on_device=True)
+    t_gen = time.perf_counter() - t0
+    eng = dev.sdk_engine_create()
+    assert dev.sdk_engine_set_reference(eng, dref, 1, w, h, None) == 0
+    gw, gh = w // cfg["step"], h // cfg["step"]
+    cap = gw * gh * 8
+    out = np.zeros(cap, np.float32)
+    met = np.zeros(24, np.float32)
+    tm = np.zeros(11, np.float64)
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    tol = ctypes.c_float(cfg["tol"])
+    for _ in range(args.warmup):
+        n = dev.sdk_engine_run(eng, ddef, 1, P(prm), P(out), cap, P(met), 24, tol)
+        assert n > 0, n
+    kern_ms, kern_iters, kern_nodes = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        n = dev.sdk_engine_run(eng, ddef, 1, P(prm), P(out), cap, P(met), 24, tol)
+        dev.sdk_engine_timings(eng, P(tm))
+        kern_ms.append(tm[8])
+        kern_iters.append(tm[10])
+        kern_nodes.append(tm[9])
+    dt = time.perf_counter() - t0
+    ms_per_step = dt * 1000.0 / args.steps
+    solved = float(met[3] + met[4] + met[19])
+    n_sub = cfg["subset"] ** 2
+    # dominant kernel: the Path A solver launch over the grid
+    avg_kms = float(np.mean(kern_ms))
+    flops = float(np.mean(kern_iters)) * n_sub * FLOP_PER_PIXEL_ITER[use6] + met[3] * n_sub * SD_FLOP_PER_PIXEL
+    achieved = flops / (avg_kms * 1e-3) / 1e12
+    pts = out[: n * 8].reshape(-1, 8).copy()
+
+    cpu = None
+    acc = None
+    if not args.no_cpu:
+        ref = np.zeros((h, w), np.uint8)
+        dfm = np.zeros((h, w), np.uint8)
+        dev.sdk_memcpy(P(ref), dref, w * h, 1)
+        dev.sdk_memcpy(P(dfm), ddef, w * h, 1)
+        threads = args.cpu_threads or (os.cpu_count() or 1)
+        side = 1024 if w >= 2048 else w
+        rect = [(w - side) // 2 // cfg["step"] * cfg["step"], (h - side) // 2 // cfg["step"] * cfg["step"], side, side]
+        cpu, acc = cpu_baseline(ref, dfm, prm, pts, rect, use6, threads)
+    frac = acc["within_1e-3px"] if acc else 1.0
+    value = solved * frac / (ms_per_step / 1000.0)
+    line = {
+        "metric": "converged subsets/sec at <=1e-3 px (plain GPU method)",
+        "value": value,
+        "unit": "converged subsets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {w}x{h} speckle, step {cfg['step']}, subset {cfg['subset']}, "
+                               f"strain window {cfg['window']}, {'Keys 6x6' if use6 else 'Keys 4x4'} IC-GN",
+                   "attempted_subsets": int(met[0]), "solved_subsets": int(solved), "output_points": int(n),
+                   "pair_generation_s": round(t_gen, 3)},
+        "roofline": {"bound": "fp32-valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "Path A subset solver launch (k_fast + k_slow)", "avg_launch_ms": avg_kms,
+                     "iterations_per_launch": float(np.mean(kern_iters))},
+        "cpu_baseline": cpu,
+        "accuracy_vs_fp64": acc,
+        "phases_ms": {k: float(v) for k, v in zip(
+            ["total", "prep", "seeding", "prepass", "mesh", "path_a", "path_b", "strain"], tm[:8])},
+    }
+    dev.sdk_engine_destroy(eng)
+    dev.sdk_free(dref)
+    dev.sdk_free(ddef)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

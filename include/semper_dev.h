@@ -1,0 +1,60 @@
+/*
+ * Kernel-level C interface of the MI355X DIC engine (libsemper_dev.so).
+ *
+ * These entry points have no counterpart in the reference's C ABI: they expose
+ * the pieces behind semper_run -- the subset solver, the Hessian pool, the
+ * reference planes, an engine whose frames already sit in HBM, and the
+ * synthetic speckle generator -- to the test suite and the benchmark. The
+ * subset solver mirrors the reference's OptimizationEngine::calculate_deformation
+ * (src/math/optimization_engine.cpp:55-118) over precompute_subset
+ * (src/math/subset_precomputer.cpp:31-35), node by node. libsemper_dev.so also
+ * exports every semper_* function of semper_c.h.
+ */
+#ifndef SEMPER_DEV_H
+#define SEMPER_DEV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#define SDK_API __attribute__((visibility("default")))
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+SDK_API int sdk_device_count(void);
+SDK_API void* sdk_malloc(size_t bytes);
+SDK_API void sdk_free(void* p);
+/* kind 0: host to device, 1: device to host, 2: device to device */
+SDK_API int sdk_memcpy(void* dst, const void* src, size_t bytes, int kind);
+
+/* calculate_deformation per node on host images; see devapi.cpp. */
+SDK_API int sdk_solve_nodes(const float* ref, const void* def, int def_is_f32, int w, int h, int dim, int use6, int lm,
+                            float tol, int n, const int32_t* cxy, const float* guess, const int32_t* modes,
+                            int32_t* iout, float* fout);
+
+/* compute_hessian_only per node centre (subset_precomputer.cpp:42-149). */
+SDK_API int sdk_pool(const float* ref, int w, int h, int dim, int lm, int n, const int32_t* cxy, int32_t* valid,
+                     float* stats2, float* hstep);
+
+/* Image + prepare_data(false) + Ghost Wall (image_processor.cpp:9-64,
+   reference_cache.cpp:18-35). */
+SDK_API int sdk_ref_planes(const uint8_t* img, const uint8_t* mask, int w, int h, float* I, float* gx, float* gy);
+
+/* An engine over frames in host or device memory (on_device != 0). */
+SDK_API void* sdk_engine_create(void);
+SDK_API void sdk_engine_destroy(void* h);
+SDK_API int sdk_engine_set_reference(void* h, const uint8_t* img, int on_device, int w, int hh, const uint8_t* mask);
+SDK_API int sdk_engine_run(void* h, const uint8_t* def, int on_device, const int32_t* params, float* out, int cap,
+                           float* metrics, int metrics_len, float tol);
+SDK_API int sdk_engine_timings(void* h, double* out11);
+
+/* Synthetic speckle pair (synth.hpp FieldSpec kinds). */
+SDK_API int sdk_synth_pair(uint64_t seed, int w, int h, int kind, const double* p, int np, double cx, double cy,
+                           float noise_sigma, uint8_t* ref, uint8_t* def, int on_device);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SEMPER_DEV_H */

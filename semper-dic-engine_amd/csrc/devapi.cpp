@@ -1,0 +1,249 @@
+// Kernel-level C entry points of libsemper_dev.so (declared in
+// include/semper_dev.h): the subset solver on float or u8 images, the Hessian
+// pool, the image planes, an engine whose frames are already in HBM, device
+// buffers and the synthetic speckle generator. The test suite and the
+// benchmark use them; applications use the semper_* ABI.
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "engine.hpp"
+#include "semper_dev.h"
+#include "synth.hpp"
+
+using namespace sdic;
+
+namespace {
+
+template <typename T>
+struct Dev {
+    T* p = nullptr;
+    explicit Dev(size_t n) {
+        if (hipMalloc(&p, sizeof(T) * (n ? n : 1)) != hipSuccess) p = nullptr;
+    }
+    ~Dev() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+struct DevEngine {
+    Engine eng;
+    Cancel cancel;
+};
+
+}  // namespace
+
+extern "C" {
+
+SDK_API int sdk_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+SDK_API void* sdk_malloc(size_t bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) return nullptr;
+    return p;
+}
+
+SDK_API void sdk_free(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+SDK_API int sdk_memcpy(void* dst, const void* src, size_t bytes, int kind) {
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    return (int)hipMemcpy(dst, src, bytes, k);
+}
+
+// Subset solver on host images (ref float; def float or u8): the Hessian pool
+// of each node centre, then calculate_deformation per node. lm selects the
+// full-field engine configuration. Outputs per node: ints {init, status,
+// iters, trigger_iters, simplex} and floats {u, v, ux, uy, vx, vy, corr}.
+SDK_API int sdk_solve_nodes(const float* ref, const void* def, int def_is_f32, int w, int h, int dim, int use6, int lm,
+                            float tol, int n, const int32_t* cxy, const float* guess, const int32_t* modes,
+                            int32_t* iout, float* fout) {
+    if (!ref || !def || w <= 0 || h <= 0 || n < 0 || ppl_for(dim) < 0) return -3;
+    if (n == 0) return 0;
+    const size_t npx = (size_t)w * h;
+    hipStream_t st;
+    if (hipStreamCreate(&st) != hipSuccess) return -3;
+    int rc = 0;
+    {
+        Dev<float> dref(npx), I(npx), gx(npx), gy(npx);
+        Dev<uint8_t> ddef(npx * (def_is_f32 ? 4 : 1));
+        Dev<int32_t> dcxy(2 * (size_t)n), bail(n), nbail(1);
+        Dev<PoolEntry> pool(n);
+        Dev<Task> tasks(n);
+        Dev<SolveOut> outd(n);
+        std::vector<Task> ht(n);
+        for (int i = 0; i < n; ++i) {
+            Task t = {};
+            t.cx = cxy[2 * i];
+            t.cy = cxy[2 * i + 1];
+            t.pool = i;
+            t.mode = modes[i];
+            std::memcpy(t.g, guess + 6 * i, sizeof(t.g));
+            t.node = i;
+            ht[i] = t;
+        }
+        if (!dref.p || !I.p || !gx.p || !gy.p || !ddef.p || !dcxy.p || !bail.p || !nbail.p || !pool.p || !tasks.p ||
+            !outd.p) {
+            rc = -3;
+        } else {
+            hipMemcpyAsync(dref.p, ref, sizeof(float) * npx, hipMemcpyHostToDevice, st);
+            hipMemcpyAsync(ddef.p, def, npx * (def_is_f32 ? 4 : 1), hipMemcpyHostToDevice, st);
+            hipMemcpyAsync(dcxy.p, cxy, sizeof(int32_t) * 2 * n, hipMemcpyHostToDevice, st);
+            hipMemcpyAsync(tasks.p, ht.data(), sizeof(Task) * n, hipMemcpyHostToDevice, st);
+            launch_prep_ref_f32(dref.p, w, h, I.p, gx.p, gy.p, st);
+            launch_pool_list(I.p, gx.p, gy.p, w, h, dcxy.p, n, dim, lm != 0, pool.p, st);
+            SolveArgs a = {};
+            a.tasks = tasks.p;
+            a.out = outd.p;
+            a.nitems = n;
+            a.bail = bail.p;
+            a.nbail = nbail.p;
+            a.pool = pool.p;
+            a.refI = I.p;
+            a.refGx = gx.p;
+            a.refGy = gy.p;
+            a.def = ddef.p;
+            a.w = w;
+            a.h = h;
+            a.dim = dim;
+            a.lm = lm;
+            a.tol = tol;
+            if (launch_solve(a, use6 != 0, def_is_f32 != 0, st) != hipSuccess) rc = -3;
+            std::vector<SolveOut> ho(n);
+            hipMemcpyAsync(ho.data(), outd.p, sizeof(SolveOut) * n, hipMemcpyDeviceToHost, st);
+            if (hipStreamSynchronize(st) != hipSuccess) rc = -3;
+            for (int i = 0; i < n && rc == 0; ++i) {
+                const SolveOut& o = ho[i];
+                iout[5 * i] = o.init;
+                iout[5 * i + 1] = o.init ? o.status : 1;
+                iout[5 * i + 2] = o.iters;
+                iout[5 * i + 3] = o.trigger_iters;
+                iout[5 * i + 4] = o.simplex;
+                const float v[7] = {o.p[0], o.p[1], o.p[2], o.p[3], o.p[4], o.p[5], o.corr};
+                for (int k = 0; k < 7; ++k) fout[7 * i + k] = o.init ? v[k] : 0.0f;
+            }
+        }
+    }
+    hipStreamDestroy(st);
+    return rc;
+}
+
+// Hessian pool entries of a host float image at node centres:
+// valid, mean, inv_std and the 36-float step matrix per node.
+SDK_API int sdk_pool(const float* ref, int w, int h, int dim, int lm, int n, const int32_t* cxy, int32_t* valid,
+                     float* stats2, float* hstep) {
+    if (!ref || w <= 0 || h <= 0 || n <= 0) return -3;
+    const size_t npx = (size_t)w * h;
+    Dev<float> dref(npx), I(npx), gx(npx), gy(npx);
+    Dev<int32_t> dcxy(2 * (size_t)n);
+    Dev<PoolEntry> pool(n);
+    if (!dref.p || !I.p || !gx.p || !gy.p || !dcxy.p || !pool.p) return -3;
+    hipMemcpy(dref.p, ref, sizeof(float) * npx, hipMemcpyHostToDevice);
+    hipMemcpy(dcxy.p, cxy, sizeof(int32_t) * 2 * n, hipMemcpyHostToDevice);
+    launch_prep_ref_f32(dref.p, w, h, I.p, gx.p, gy.p, nullptr);
+    launch_pool_list(I.p, gx.p, gy.p, w, h, dcxy.p, n, dim, lm != 0, pool.p, nullptr);
+    std::vector<PoolEntry> hp(n);
+    if (hipMemcpy(hp.data(), pool.p, sizeof(PoolEntry) * n, hipMemcpyDeviceToHost) != hipSuccess) return -3;
+    for (int i = 0; i < n; ++i) {
+        valid[i] = hp[i].valid;
+        stats2[2 * i] = hp[i].mean;
+        stats2[2 * i + 1] = hp[i].inv_std;
+        std::memcpy(hstep + 36 * i, hp[i].hstep, sizeof(float) * 36);
+    }
+    return 0;
+}
+
+// The reference planes of a u8 image (with an optional ghost-wall mask).
+SDK_API int sdk_ref_planes(const uint8_t* img, const uint8_t* mask, int w, int h, float* I, float* gx, float* gy) {
+    if (!img || w <= 0 || h <= 0) return -3;
+    const size_t npx = (size_t)w * h;
+    Dev<uint8_t> di(npx), dm(npx);
+    Dev<float> dI(npx), dgx(npx), dgy(npx);
+    if (!di.p || !dm.p || !dI.p || !dgx.p || !dgy.p) return -3;
+    hipMemcpy(di.p, img, npx, hipMemcpyHostToDevice);
+    if (mask) hipMemcpy(dm.p, mask, npx, hipMemcpyHostToDevice);
+    launch_prep_ref_u8(di.p, mask ? dm.p : nullptr, w, h, dI.p, dgx.p, dgy.p, nullptr);
+    hipMemcpy(I, dI.p, sizeof(float) * npx, hipMemcpyDeviceToHost);
+    hipMemcpy(gx, dgx.p, sizeof(float) * npx, hipMemcpyDeviceToHost);
+    return (int)hipMemcpy(gy, dgy.p, sizeof(float) * npx, hipMemcpyDeviceToHost);
+}
+
+// ---- an engine over frames already in HBM ----------------------------------
+
+SDK_API void* sdk_engine_create(void) {
+    try {
+        return new DevEngine();
+    } catch (...) {
+        return nullptr;
+    }
+}
+
+SDK_API void sdk_engine_destroy(void* h) { delete (DevEngine*)h; }
+
+SDK_API int sdk_engine_set_reference(void* h, const uint8_t* img, int on_device, int w, int hh, const uint8_t* mask) {
+    if (!h) return -3;
+    return ((DevEngine*)h)->eng.set_reference(img, on_device != 0, w, hh, mask);
+}
+
+// params: rect_x, rect_y, rect_w, rect_h, step, subset, strain_window, use6
+SDK_API int sdk_engine_run(void* h, const uint8_t* def, int on_device, const int32_t* params, float* out, int cap,
+                           float* metrics, int metrics_len, float tol) {
+    if (!h || !params) return -3;
+    DevEngine* e = (DevEngine*)h;
+    Params p;
+    p.rect_x = params[0];
+    p.rect_y = params[1];
+    p.rect_w = params[2];
+    p.rect_h = params[3];
+    p.step = params[4];
+    p.subset = params[5];
+    p.strain_window = params[6];
+    p.use6 = params[7] != 0;
+    e->cancel.enter();
+    try {
+        return e->eng.run(def, on_device != 0, nullptr, p, out, cap, metrics, metrics_len, e->cancel, nullptr, tol);
+    } catch (...) {
+        return -3;
+    }
+}
+
+// Last solve's timings: total, prep, seeding, prepass, mesh, path_a, path_b,
+// strain (host ms), the Path A solver launch (device ms), its grid nodes and
+// its IC-GN iterations.
+SDK_API int sdk_engine_timings(void* h, double* out11) {
+    if (!h) return -3;
+    const Timings& t = ((DevEngine*)h)->eng.timings();
+    const double v[11] = {t.total, t.prep,   t.seeding,       t.prepass,        t.mesh,          t.path_a,
+                          t.path_b, t.strain, t.kernel_path_a, (double)t.path_a_items, (double)t.path_a_iters};
+    std::memcpy(out11, v, sizeof(v));
+    return 0;
+}
+
+// ---- synthetic speckle pairs ------------------------------------------------
+
+// Renders a reference / deformed u8 pair of w x h into device buffers
+// (on_device) or host buffers. kind and p[] as FieldSpec (synth.hpp).
+SDK_API int sdk_synth_pair(uint64_t seed, int w, int h, int kind, const double* p, int np, double cx, double cy,
+                           float noise_sigma, uint8_t* ref, uint8_t* def, int on_device) {
+    if (w <= 0 || h <= 0 || np < 0 || np > 48) return -3;
+    FieldSpec f = {};
+    f.kind = kind;
+    f.cx = cx;
+    f.cy = cy;
+    for (int i = 0; i < np; ++i) f.p[i] = p[i];
+    const size_t npx = (size_t)w * h;
+    if (on_device) return synth_pair(seed, w, h, f, noise_sigma, ref, def, nullptr);
+    Dev<uint8_t> dr(npx), dd(npx);
+    if (!dr.p || !dd.p) return -3;
+    const int rc = synth_pair(seed, w, h, f, noise_sigma, dr.p, dd.p, nullptr);
+    if (rc != 0) return rc;
+    hipMemcpy(ref, dr.p, npx, hipMemcpyDeviceToHost);
+    return (int)hipMemcpy(def, dd.p, npx, hipMemcpyDeviceToHost);
+}
+
+}  // extern "C"

@@ -1,0 +1,1078 @@
+// Orchestration of the MI355X full-field solve; see engine.hpp. Phase order,
+// decisions and counters follow src/pipeline/full_field_solver.cpp:141-586,
+// full_field_anchors.cpp:283-607, full_field_path_{a,b,c}.cpp and
+// full_field_solver_stats.cpp of the reference; every per-pixel and per-node
+// computation runs in a kernel.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <queue>
+#include <thread>
+
+#include "seeding.hpp"
+
+namespace sdic {
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+double ms_since(clk::time_point t0) { return std::chrono::duration<double, std::milli>(clk::now() - t0).count(); }
+
+constexpr int kErrRoi = -2, kErrInit = -3, kErrStrainWindow = -4, kErrSeed = -5, kErrCancelled = -99;
+
+struct HipFail {
+    hipError_t e;
+};
+inline void ck(hipError_t e) {
+    if (e != hipSuccess) throw HipFail{e};
+}
+
+bool clip_axis(int& lo, int& len, int step, int extent) {
+    int64_t a = lo;
+    const int64_t b = std::min<int64_t>((int64_t)lo + (int64_t)len, extent);
+    if (a < 0) a += ((-a + step - 1) / step) * (int64_t)step;
+    if (b <= a) return false;
+    lo = (int)a;
+    len = (int)(b - a);
+    return true;
+}
+
+// validate_params (full_field_solver.cpp:97-111)
+int validate(Params& p, int width, int height) {
+    if (width <= 0 || height <= 0) return kErrInit;
+    if ((int64_t)width * height > (int64_t)INT32_MAX) return kErrInit;
+    if (p.step <= 0) return kErrRoi;
+    if (p.subset < 3 || p.subset % 2 == 0) return kErrInit;
+    if (p.rect_w <= 0 || p.rect_h <= 0) return kErrRoi;
+    if (!clip_axis(p.rect_x, p.rect_w, p.step, width)) return kErrRoi;
+    if (!clip_axis(p.rect_y, p.rect_h, p.step, height)) return kErrRoi;
+    if (p.rect_w / p.step <= 0 || p.rect_h / p.step <= 0) return kErrRoi;
+    const int64_t gmin = std::min(p.rect_w / p.step, p.rect_h / p.step);
+    if ((int64_t)p.strain_window > 2 * (gmin + 1) * (int64_t)p.step) return kErrInit;
+    return 0;
+}
+
+// StrainCalculator::vsg_window_node_count (strain_calculator.cpp:12-25)
+int vsg_count(int step, int window) {
+    if (step <= 0) return 0;
+    const float radius = window / 2.0f;
+    const int gr = (int)std::ceil(radius / (float)step);
+    const double r2 = (double)(radius * radius) + kVsgRadiusTiny;
+    int c = 0;
+    for (int dy = -gr; dy <= gr; ++dy)
+        for (int dx = -gr; dx <= gr; ++dx) {
+            const double px = (double)(dx * step), py = (double)(dy * step);
+            if (px * px + py * py <= r2) c++;
+        }
+    return c;
+}
+
+struct Counters {   // one ThreadStats bucket
+    int points = 0, iters = 0, calls = 0, saved = 0, dead = 0;
+    void record(const SolveOut& o) {
+        iters += o.iters;
+        if (o.simplex) {
+            calls++;
+            if (o.status == 0 && o.corr <= kCorrAccept) saved++;
+            else dead++;
+        }
+    }
+};
+
+// Progress ticker: 10..90 % from a thread of the solve, about every 100 ms
+// (full_field_solver.cpp:334-373).
+class Ticker {
+public:
+    Ticker(const std::function<void(int)>& cb, int total) : cb_(cb), total_(total) {
+        if (cb_) th_ = std::thread([this] { loop(); });
+    }
+    ~Ticker() { stop(); }
+    void set(int solved) { solved_.store(solved, std::memory_order_relaxed); }
+    void stop() {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        if (th_.joinable()) th_.join();
+    }
+
+private:
+    void loop() {
+        std::unique_lock<std::mutex> lk(m_);
+        while (!stop_) {
+            const int pct = std::max(0, std::min(100, (int)(((float)solved_.load() / (float)std::max(1, total_)) * 80.0f + 10.0f)));
+            lk.unlock();
+            try {
+                cb_(pct);
+            } catch (...) {
+            }
+            lk.lock();
+            cv_.wait_for(lk, std::chrono::milliseconds(100), [this] { return stop_; });
+        }
+    }
+    std::function<void(int)> cb_;
+    int total_;
+    std::atomic<int> solved_{0};
+    std::thread th_;
+    std::mutex m_;
+    std::condition_variable cv_;
+    bool stop_ = false;
+};
+
+}  // namespace
+
+Engine::Engine() {
+    ck(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    ck(hipEventCreate(&ev0_));
+    ck(hipEventCreate(&ev1_));
+}
+
+Engine::~Engine() {
+    if (st_) (void)hipStreamSynchronize(st_);
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
+    if (st_) (void)hipStreamDestroy(st_);
+}
+
+void Engine::reset() {
+    have_ref_ = false;
+    grid_valid_ = false;
+    w_ = h_ = 0;
+}
+
+int Engine::set_reference(const uint8_t* gray, bool on_device, int w, int h, const uint8_t* mask) {
+    reset();
+    if (!gray || w <= 0 || h <= 0 || (int64_t)w * h > (int64_t)INT32_MAX) return kErrInit;
+    try {
+        const size_t n = (size_t)w * h;
+        if (!ref_u8_.reserve(n) || !I_.reserve(n) || !gx_.reserve(n) || !gy_.reserve(n) || !mask_d_.reserve(n))
+            return kErrInit;
+        ck(hipMemcpyAsync(ref_u8_.p, gray, n, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st_));
+        if (mask) ck(hipMemcpyAsync(mask_d_.p, mask, n, hipMemcpyHostToDevice, st_));
+        ck(launch_prep_ref_u8(ref_u8_.p, mask ? mask_d_.p : nullptr, w, h, I_.p, gx_.p, gy_.p, st_));
+        // reference pyramid of the global-motion search (levels 1..3)
+        int pw = w, ph = h;
+        pyr_ref_levels_ = 0;
+        for (int l = 1; l <= 3; ++l) {
+            const int nw = pw / 2, nh = ph / 2;
+            if (nw < 1 || nh < 1) break;
+            if (!pyr_r_[l].reserve((size_t)nw * nh)) return kErrInit;
+            if (l == 1) ck(launch_downsample_u8(ref_u8_.p, pw, ph, pyr_r_[1].p, st_));
+            else ck(launch_downsample_f32(pyr_r_[l - 1].p, pw, ph, pyr_r_[l].p, st_));
+            pw = nw;
+            ph = nh;
+            pyr_ref_levels_ = l;
+        }
+        ck(hipStreamSynchronize(st_));
+    } catch (const HipFail&) {
+        return kErrInit;
+    }
+    w_ = w;
+    h_ = h;
+    have_ref_ = true;
+    return 0;
+}
+
+// Grid scan and Hessian pool, cached while rect / step / subset / mask hold.
+int Engine::ensure_grid(const Params& p, const uint8_t* mask, bool& cached) {
+    const int gw = p.rect_w / p.step, gh = p.rect_h / p.step;
+    const size_t n = (size_t)w_ * h_;
+    const bool mask_same = (!mask && grid_mask_.empty()) ||
+                           (mask && grid_mask_.size() == n && std::memcmp(grid_mask_.data(), mask, n) == 0);
+    cached = grid_valid_ && grid_p_.rect_x == p.rect_x && grid_p_.rect_y == p.rect_y &&
+             grid_p_.rect_w == p.rect_w && grid_p_.rect_h == p.rect_h && grid_p_.step == p.step &&
+             grid_p_.subset == p.subset && mask_same && gw_ == gw && gh_ == gh;
+    if (cached) return 0;
+    grid_valid_ = false;
+    const int N = gw * gh;
+    if (!skip_d_.reserve(N) || !pool_.reserve(N) || !counters_.reserve(16)) return kErrInit;
+    const uint8_t* md = nullptr;
+    if (mask) {
+        if (!mask_d_.reserve(n)) return kErrInit;
+        ck(hipMemcpyAsync(mask_d_.p, mask, n, hipMemcpyHostToDevice, st_));
+        md = mask_d_.p;
+    }
+    ck(hipMemsetAsync(counters_.p, 0, sizeof(int32_t), st_));
+    ck(launch_scan(md, w_, h_, p.rect_x, p.rect_y, p.step, gw, gh, p.subset, skip_d_.p, counters_.p, st_));
+    int total = 0;
+    skip_h_.resize(N);
+    ck(hipMemcpyAsync(&total, counters_.p, sizeof(int), hipMemcpyDeviceToHost, st_));
+    ck(hipMemcpyAsync(skip_h_.data(), skip_d_.p, N, hipMemcpyDeviceToHost, st_));
+    ck(hipStreamSynchronize(st_));
+    total_valid_ = total;
+    if (total == 0) return kErrRoi;
+    ck(launch_pool(I_.p, gx_.p, gy_.p, w_, h_, skip_d_.p, gw, gh, p.rect_x, p.rect_y, p.step, p.subset, true,
+                   pool_.p, st_));
+    ck(hipStreamSynchronize(st_));
+    grid_p_ = p;
+    grid_mask_ = mask ? std::vector<uint8_t>(mask, mask + n) : std::vector<uint8_t>();
+    gw_ = gw;
+    gh_ = gh;
+    grid_valid_ = true;
+    return 0;
+}
+
+int Engine::solve_tasks(const std::vector<Task>& tasks, std::vector<SolveOut>& out, bool use6, int dim, float tol) {
+    const int n = (int)tasks.size();
+    out.assign(n, SolveOut{});
+    if (n == 0) return 0;
+    if (!tasks_d_.reserve(n) || !out_d_.reserve(n) || !bail_.reserve(n) || !counters_.reserve(16)) return kErrInit;
+    ck(hipMemcpyAsync(tasks_d_.p, tasks.data(), sizeof(Task) * n, hipMemcpyHostToDevice, st_));
+    SolveArgs a = {};
+    a.tasks = tasks_d_.p;
+    a.out = out_d_.p;
+    a.nitems = n;
+    a.bail = bail_.p;
+    a.nbail = counters_.p + 8;
+    a.pool = pool_.p;
+    a.refI = I_.p;
+    a.refGx = gx_.p;
+    a.refGy = gy_.p;
+    a.def = def_u8_.p;
+    a.w = w_;
+    a.h = h_;
+    a.dim = dim;
+    a.lm = 1;
+    a.tol = tol;
+    ck(launch_solve(a, use6, false, st_));
+    ck(hipMemcpyAsync(out.data(), out_d_.p, sizeof(SolveOut) * n, hipMemcpyDeviceToHost, st_));
+    ck(hipStreamSynchronize(st_));
+    return 0;
+}
+
+int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const Params& pin, float* out,
+                int out_capacity, float* metrics, int metrics_len, Cancel& cancel,
+                const std::function<void(int)>& progress, float tol) {
+    const auto t_total = clk::now();
+    tm_ = Timings{};
+    if (!out || !have_ref_ || !def) return kErrInit;
+    Params p = pin;
+    const int rc0 = validate(p, w_, h_);
+    if (rc0 != 0) return rc0;
+    if (ppl_for(p.subset) < 0) return kErrInit;   // larger than the register-resident solver takes
+    if (out_capacity < 0) out_capacity = 0;
+    try {
+        const size_t npx = (size_t)w_ * h_;
+        const auto t_prep = clk::now();
+        if (!def_u8_.reserve(npx)) return kErrInit;
+        ck(hipMemcpyAsync(def_u8_.p, def, npx, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st_));
+        tm_.prep = ms_since(t_prep);
+        const int gw = p.rect_w / p.step, gh = p.rect_h / p.step, N = gw * gh;
+        if (vsg_count(p.step, p.strain_window) < 3) return kErrStrainWindow;
+        if (cancel.requested()) return kErrCancelled;
+
+        const auto t_pre = clk::now();
+        bool cached = false;
+        const int rg = ensure_grid(p, mask, cached);
+        if (rg != 0) return rg;
+        tm_.prepass = cached ? 0.0 : ms_since(t_pre);
+        const int total_valid = total_valid_;
+        if (cancel.requested()) return kErrCancelled;
+
+        Ticker ticker(progress, total_valid);
+        const size_t NN = (size_t)N;
+        if (!gplanes_.reserve(7 * NN) || !gflags_.reserve(7 * NN) || !giters_.reserve(NN) || !guess_.reserve(6 * NN) ||
+            !tmp_.reserve(6 * NN) || !strain_.reserve(3 * NN) || !bail_.reserve(NN) || !counters_.reserve(16))
+            return kErrInit;
+        GridDev g;
+        g.u = gplanes_.p;
+        g.v = g.u + NN;
+        g.ux = g.v + NN;
+        g.uy = g.ux + NN;
+        g.vx = g.uy + NN;
+        g.vy = g.vx + NN;
+        g.corr = g.vy + NN;
+        g.solved = gflags_.p;
+        g.tag = g.solved + NN;
+        g.simplex = g.tag + NN;
+        uint8_t* in_mesh = g.simplex + NN;
+        uint8_t* valid_d = in_mesh + NN;
+        uint8_t* ok_a = valid_d + NN;
+        uint8_t* ok_b = ok_a + NN;
+        (void)ok_b;
+        g.iters = giters_.p;
+        ck(launch_grid_init(g, skip_d_.p, N, st_));
+        std::vector<uint8_t> solved_h(skip_h_);   // host mirror of the solved flags
+
+        // ---- seeding: global model + anchor lattice -------------------------
+        const auto t_seed = clk::now();
+        const int L = std::min(pyramid_levels(p.rect_w, p.rect_h), pyr_ref_levels_);
+        PyrDev pd = {};
+        pd.r0 = ref_u8_.p;
+        pd.d0 = def_u8_.p;
+        pd.w[0] = w_;
+        pd.h[0] = h_;
+        {
+            int pw = w_, ph = h_;
+            for (int l = 1; l <= L; ++l) {
+                const int nw = pw / 2, nh = ph / 2;
+                if (!pyr_d_[l].reserve((size_t)nw * nh)) return kErrInit;
+                if (l == 1) ck(launch_downsample_u8(def_u8_.p, pw, ph, pyr_d_[1].p, st_));
+                else ck(launch_downsample_f32(pyr_d_[l - 1].p, pw, ph, pyr_d_[l].p, st_));
+                pd.r[l] = pyr_r_[l].p;
+                pd.d[l] = pyr_d_[l].p;
+                pd.w[l] = nw;
+                pd.h[l] = nh;
+                pw = nw;
+                ph = nh;
+            }
+        }
+        pd.levels = L;
+        std::vector<int> centres;
+        patch_centres(p.rect_x, p.rect_y, p.rect_w, p.rect_h, centres);
+        const int npatch = (int)centres.size() / 2;
+        if (!centres_d_.reserve(centres.size()) || !patch_d_.reserve(npatch)) return kErrInit;
+        ck(hipMemcpyAsync(centres_d_.p, centres.data(), sizeof(int) * centres.size(), hipMemcpyHostToDevice, st_));
+        ck(launch_patch_search(pd, centres_d_.p, npatch, patch_d_.p, st_));
+        std::vector<PatchOut> po(npatch);
+        ck(hipMemcpyAsync(po.data(), patch_d_.p, sizeof(PatchOut) * npatch, hipMemcpyDeviceToHost, st_));
+        ck(hipStreamSynchronize(st_));
+        const GlobalModel gm = fit_global_model(centres, po, p.rect_x, p.rect_y, p.rect_w, p.rect_h);
+
+        // lattice (solve_anchor_seeds, full_field_anchors.cpp:283-574)
+        int sx, sy;
+        plan_lattice(gw, gh, sx, sy);
+        std::vector<int> lx, ly;
+        for (int x = 0; x < gw; x += sx) lx.push_back(x);
+        if (lx.back() != gw - 1) lx.push_back(gw - 1);
+        for (int y = 0; y < gh; y += sy) ly.push_back(y);
+        if (ly.back() != gh - 1) ly.push_back(gh - 1);
+        const int LW = (int)lx.size(), LH = (int)ly.size();
+        struct Seeds {
+            std::vector<AnchorRec> an;
+            int quality = 0, attempted = 0, accepted = 0, solved = 0;
+            float gU = 0, gV = 0, coverage = 0;
+            bool model_rejected = false;
+            std::vector<float> ref_x, ref_y, def_x, def_y;
+        } seeds;
+        Counters st_anchor, st_a, st_b;
+        double icgn_ms_est = 0;
+        seeds.gU = (float)gm.t_u;
+        seeds.gV = (float)gm.t_v;
+        auto solve_lattice = [&](bool sim, Seeds& r) -> float {
+            r = Seeds{};
+            r.gU = (float)gm.t_u;
+            r.gV = (float)gm.t_v;
+            const int na = LW * LH;
+            r.an.assign(na, AnchorRec{});
+            std::vector<Task> tasks;
+            std::vector<int> which;
+            for (int idx = 0; idx < na; ++idx) {
+                const int gx = lx[idx % LW], gy = ly[idx / LW];
+                if (solved_h[(size_t)gy * gw + gx]) continue;
+                const int rx = p.rect_x + gx * p.step, ry = p.rect_y + gy * p.step;
+                Task t = {};
+                t.cx = rx;
+                t.cy = ry;
+                t.pool = gy * gw + gx;
+                t.mode = gm.locked ? NO_SIMPLEX : AUTO_SEARCH;
+                float gs[6] = {r.gU, r.gV, 0, 0, 0, 0};
+                if (sim) gm.at(rx, ry, gs);
+                std::copy(gs, gs + 6, t.g);
+                t.node = t.pool;
+                tasks.push_back(t);
+                which.push_back(idx);
+            }
+            r.attempted = (int)tasks.size();
+            std::vector<SolveOut> res;
+            const auto t0 = clk::now();
+            if (solve_tasks(tasks, res, p.use6, p.subset, tol) != 0) throw HipFail{hipErrorOutOfMemory};
+            icgn_ms_est += ms_since(t0);
+            for (size_t k = 0; k < tasks.size(); ++k) {
+                const SolveOut& o = res[k];
+                if (!o.init) continue;
+                st_anchor.record(o);
+                if (o.status != 0) continue;
+                AnchorRec& a = r.an[which[k]];
+                a.rx = (float)tasks[k].cx;
+                a.ry = (float)tasks[k].cy;
+                std::copy(o.p, o.p + 6, a.p);
+                a.corr = o.corr;
+                a.gx = lx[which[k] % LW];
+                a.gy = ly[which[k] / LW];
+                a.node = tasks[k].pool;
+                a.iters = o.iters;
+                a.simplex = o.simplex != 0;
+                a.vertex = o.corr <= kAnchorAcceptScore;
+                a.output = o.corr <= kCorrAccept;
+            }
+            std::vector<float> ru(na, 0.0f), rv(na, 0.0f);
+            for (int i = 0; i < na; ++i) {
+                const AnchorRec& a = r.an[i];
+                if (!a.vertex) continue;
+                float gs[6] = {0, 0, 0, 0, 0, 0};
+                if (sim) gm.at(a.rx, a.ry, gs);
+                ru[i] = a.p[0] - gs[0];
+                rv[i] = a.p[1] - gs[1];
+            }
+            mark_kept(r.an, LW, LH, ru, rv);
+            std::vector<float> us, vs;
+            std::vector<std::pair<double, double>> pts;
+            for (const AnchorRec& a : r.an) {
+                if (!a.kept) continue;
+                r.ref_x.push_back(a.rx);
+                r.ref_y.push_back(a.ry);
+                r.def_x.push_back(a.rx + a.p[0]);
+                r.def_y.push_back(a.ry + a.p[1]);
+                us.push_back(a.p[0]);
+                vs.push_back(a.p[1]);
+                pts.emplace_back(a.rx, a.ry);
+                if (a.output) r.solved++;
+            }
+            r.accepted = (int)r.ref_x.size();
+            if (r.accepted >= 3) {
+                r.gU = median_of(us.data(), us.size());
+                r.gV = median_of(vs.data(), vs.size());
+                const float area = (float)(p.rect_w * p.rect_h);
+                r.coverage = area > 0.0f ? (float)hull_area(pts) / area : 0.0f;
+            }
+            const float frac = r.attempted > 0 ? (float)r.accepted / (float)r.attempted : 0.0f;
+            if (r.accepted >= kAnchorFullMinVertices && frac >= kAnchorFullFraction &&
+                r.coverage >= kAnchorFullCoverage)
+                r.quality = 2;
+            else if (r.accepted >= 3 && r.coverage >= kAnchorSparseCoverage)
+                r.quality = 1;
+            return frac;
+        };
+        bool have_affine = false;
+        double aff[6] = {0, 0, 0, 0, 0, 0}, acx = 0, acy = 0;
+        if (LW >= 2 && LH >= 2) {
+            if (!gm.full) {
+                solve_lattice(false, seeds);
+            } else {
+                const float kept = solve_lattice(true, seeds);
+                if (seeds.quality != 2 || kept < kSimilarityKeepFraction) {
+                    solve_lattice(false, seeds);
+                    seeds.model_rejected = true;
+                }
+            }
+            bool refit = false;
+            if (seeds.accepted >= kAnchorRefitMinPoints) {
+                std::vector<double> xr, yr, ud, vd;
+                for (size_t i = 0; i < seeds.ref_x.size(); ++i) {
+                    xr.push_back(seeds.ref_x[i]);
+                    yr.push_back(seeds.ref_y[i]);
+                    ud.push_back((double)seeds.def_x[i] - seeds.ref_x[i]);
+                    vd.push_back((double)seeds.def_y[i] - seeds.ref_y[i]);
+                }
+                acx = p.rect_x + 0.5 * p.rect_w;
+                acy = p.rect_y + 0.5 * p.rect_h;
+                refit = robust_affine(xr, yr, ud, vd, acx, acy, kAnchorRefitMinPoints, 1.0, aff);
+            }
+            if (refit) {
+                have_affine = true;
+            } else if (gm.locked && !seeds.model_rejected) {
+                have_affine = true;
+                std::copy(gm.a, gm.a + 6, aff);
+                acx = gm.cx;
+                acy = gm.cy;
+            }
+        }
+        // publish_anchor_results (full_field_anchors.cpp:576-607)
+        int anchor_pts = 0;
+        {
+            std::vector<int> nodes;
+            std::vector<SolveOut> res;
+            for (const AnchorRec& a : seeds.an) {
+                if (!a.output || !a.kept) continue;
+                SolveOut o = {};
+                std::copy(a.p, a.p + 6, o.p);
+                o.corr = a.corr;
+                o.status = 0;
+                o.iters = a.iters;
+                o.simplex = a.simplex;
+                o.init = 1;
+                nodes.push_back(a.node);
+                res.push_back(o);
+                solved_h[a.node] = 1;
+                anchor_pts++;
+            }
+            if (!nodes.empty()) {
+                if (!nodes_d_.reserve(nodes.size()) || !out_d_.reserve(res.size())) return kErrInit;
+                ck(hipMemcpyAsync(nodes_d_.p, nodes.data(), sizeof(int) * nodes.size(), hipMemcpyHostToDevice, st_));
+                ck(hipMemcpyAsync(out_d_.p, res.data(), sizeof(SolveOut) * res.size(), hipMemcpyHostToDevice, st_));
+                ck(launch_apply(g, nodes_d_.p, out_d_.p, (int)nodes.size(), kMeshAnchor, false, st_));
+            }
+        }
+        ticker.set(anchor_pts);
+        tm_.seeding = ms_since(t_seed);
+        if (cancel.requested()) return kErrCancelled;
+
+        float gU = seeds.gU, gV = seeds.gV;
+        const bool have_seed = seeds.solved > 0 || seeds.accepted >= 3;
+        const bool exec_c = seeds.quality == 0 && !have_seed;
+
+        // ---- guess field (build_mesh_guess_field) --------------------------
+        const auto t_mesh = clk::now();
+        std::vector<TriDev> tris;
+        std::vector<int> cell_tri;
+        if (LW >= 2 && LH >= 2) {
+            std::vector<unsigned char> kept(seeds.an.size());
+            std::vector<float> rx(kept.size()), ry(kept.size()), uu(kept.size()), vv(kept.size());
+            for (size_t i = 0; i < kept.size(); ++i) {
+                const AnchorRec& a = seeds.an[i];
+                kept[i] = a.kept && a.rx > 0 && a.rx < w_ && a.ry > 0 && a.ry < h_;
+                rx[i] = a.rx;
+                ry[i] = a.ry;
+                uu[i] = a.p[0];
+                vv[i] = a.p[1];
+            }
+            lattice_triangles(LW, LH, kept, rx, ry, uu, vv, tris, cell_tri);
+        } else {
+            cell_tri.assign(1, 0);
+        }
+        std::vector<int> lat(lx);
+        lat.insert(lat.end(), ly.begin(), ly.end());
+        if (!tris_d_.reserve(std::max<size_t>(1, tris.size())) || !celltri_d_.reserve(cell_tri.size()) ||
+            !lat_d_.reserve(lat.size()))
+            return kErrInit;
+        if (!tris.empty())
+            ck(hipMemcpyAsync(tris_d_.p, tris.data(), sizeof(TriDev) * tris.size(), hipMemcpyHostToDevice, st_));
+        ck(hipMemcpyAsync(celltri_d_.p, cell_tri.data(), sizeof(int) * cell_tri.size(), hipMemcpyHostToDevice, st_));
+        ck(hipMemcpyAsync(lat_d_.p, lat.data(), sizeof(int) * lat.size(), hipMemcpyHostToDevice, st_));
+        GuessArgs ga = {};
+        ga.tris = tris_d_.p;
+        ga.cell_tri = celltri_d_.p;
+        ga.lat_x = lat_d_.p;
+        ga.lat_y = lat_d_.p + LW;
+        ga.LW = LW;
+        ga.LH = LH;
+        ga.quality = seeds.quality;
+        ga.have_affine = have_affine ? 1 : 0;
+        std::copy(aff, aff + 6, ga.aff);
+        ga.acx = acx;
+        ga.acy = acy;
+        ga.gU = seeds.gU;
+        ga.gV = seeds.gV;
+        ga.gw = gw;
+        ga.gh = gh;
+        ga.rect_x = p.rect_x;
+        ga.rect_y = p.rect_y;
+        ga.step = p.step;
+        float* gu = guess_.p;
+        ck(launch_guess(ga, g, in_mesh, gu, gu + NN, gu + 2 * NN, gu + 3 * NN, gu + 4 * NN, gu + 5 * NN, tmp_.p, st_));
+        ck(hipStreamSynchronize(st_));
+        tm_.mesh = ms_since(t_mesh);
+
+        // ---- Path C (full_field_path_c.cpp:27-156) --------------------------
+        int pcx = gw / 2, pcy = gh / 2;
+        if (exec_c) {
+            std::vector<int> cand;
+            if (!seeds.ref_x.empty()) {
+                const float gcx = p.rect_x + (gw / 2.f) * p.step, gcy = p.rect_y + (gh / 2.f) * p.step;
+                float md = 1e9f;
+                int bi = -1;
+                for (size_t i = 0; i < seeds.ref_x.size(); ++i) {
+                    const float dx = seeds.ref_x[i] - gcx, dy = seeds.ref_y[i] - gcy;
+                    if (dx * dx + dy * dy < md) {
+                        md = dx * dx + dy * dy;
+                        bi = (int)i;
+                    }
+                }
+                if (bi >= 0) {
+                    const int cx = std::max(0, std::min(gw - 1, (int)std::round((seeds.ref_x[bi] - p.rect_x) / p.step)));
+                    const int cy = std::max(0, std::min(gh - 1, (int)std::round((seeds.ref_y[bi] - p.rect_y) / p.step)));
+                    if (!solved_h[(size_t)cy * gw + cx]) cand.push_back(cy * gw + cx);
+                }
+            }
+            {
+                const float ccx = gw / 2.0f, ccy = gh / 2.0f;
+                std::vector<std::pair<float, int>> fr;
+                for (int y = 0; y < gh; ++y)
+                    for (int x = 0; x < gw; ++x) {
+                        if (solved_h[(size_t)y * gw + x]) continue;
+                        const float dx = x - ccx, dy = y - ccy;
+                        fr.emplace_back(dx * dx + dy * dy, y * gw + x);
+                    }
+                std::stable_sort(fr.begin(), fr.end(), [](const std::pair<float, int>& a, const std::pair<float, int>& b) {
+                    return a.first < b.first;
+                });
+                for (const auto& nd : fr) {
+                    if ((int)cand.size() >= kPathCMaxCandidates) break;
+                    if (!cand.empty() && cand.front() == nd.second) continue;
+                    cand.push_back(nd.second);
+                }
+            }
+            if (cand.empty()) return kErrRoi;
+            int rc = kErrSeed;
+            for (int flat : cand) {
+                if (cancel.requested()) return kErrCancelled;
+                Task t = {};
+                t.cx = p.rect_x + (flat % gw) * p.step;
+                t.cy = p.rect_y + (flat / gw) * p.step;
+                t.pool = flat;
+                t.mode = AUTO_SEARCH;
+                t.g[0] = gU;
+                t.g[1] = gV;
+                t.node = flat;
+                std::vector<SolveOut> r1;
+                const auto t0 = clk::now();
+                if (solve_tasks({t}, r1, p.use6, p.subset, tol) != 0) return kErrInit;
+                icgn_ms_est += ms_since(t0);
+                if (!r1[0].init) continue;
+                if (r1[0].status == 0 && r1[0].corr <= kCorrAccept) {
+                    pcx = flat % gw;
+                    pcy = flat / gw;
+                    gU = r1[0].p[0];
+                    gV = r1[0].p[1];
+                    rc = 0;
+                    break;
+                }
+            }
+            if (rc != 0) {
+                if (!gm.locked) return rc;
+                pcx = cand.front() % gw;
+                pcy = cand.front() / gw;
+            }
+        }
+
+        // ---- Path A: the grid launch ----------------------------------------
+        const auto t_a = clk::now();
+        {
+            ck(hipMemsetAsync(counters_.p, 0, sizeof(int32_t) * 8, st_));
+            SolveArgs a = {};
+            a.nitems = N;
+            a.bail = bail_.p;
+            a.nbail = counters_.p + 8;
+            a.stats = counters_.p;
+            a.grid.grid = g;
+            a.grid.in_mesh = in_mesh;
+            a.grid.gu = gu;
+            a.grid.gv = gu + NN;
+            a.grid.gux = gu + 2 * NN;
+            a.grid.guy = gu + 3 * NN;
+            a.grid.gvx = gu + 4 * NN;
+            a.grid.gvy = gu + 5 * NN;
+            a.grid.gw = gw;
+            a.grid.gh = gh;
+            a.grid.rect_x = p.rect_x;
+            a.grid.rect_y = p.rect_y;
+            a.grid.step = p.step;
+            a.grid.dim = p.subset;
+            a.grid.w = w_;
+            a.grid.h = h_;
+            a.pool = pool_.p;
+            a.refI = I_.p;
+            a.refGx = gx_.p;
+            a.refGy = gy_.p;
+            a.def = def_u8_.p;
+            a.w = w_;
+            a.h = h_;
+            a.dim = p.subset;
+            a.lm = 1;
+            a.tol = tol;
+            ck(hipEventRecord(ev0_, st_));
+            ck(launch_solve(a, p.use6, false, st_));
+            ck(hipEventRecord(ev1_, st_));
+            int sa[5];
+            ck(hipMemcpyAsync(sa, counters_.p, sizeof(sa), hipMemcpyDeviceToHost, st_));
+            ck(hipStreamSynchronize(st_));
+            float kms = 0;
+            ck(hipEventElapsedTime(&kms, ev0_, ev1_));
+            tm_.kernel_path_a = kms;
+            tm_.path_a_items = N;
+            st_a.points = sa[0];
+            st_a.iters = sa[1];
+            st_a.calls = sa[2];
+            st_a.saved = sa[3];
+            st_a.dead = sa[4];
+            tm_.path_a_iters = sa[1];
+        }
+        tm_.path_a = ms_since(t_a);
+        ticker.set(anchor_pts + st_a.points);
+        if (cancel.requested()) return kErrCancelled;
+
+        // ---- Path B (full_field_path_b.cpp:148-412) ---------------------------
+        const auto t_b = clk::now();
+        std::vector<float> hp(7 * NN);
+        std::vector<uint8_t> hsolved(NN);
+        ck(hipMemcpyAsync(hp.data(), gplanes_.p, sizeof(float) * 7 * NN, hipMemcpyDeviceToHost, st_));
+        ck(hipMemcpyAsync(hsolved.data(), g.solved, NN, hipMemcpyDeviceToHost, st_));
+        ck(hipStreamSynchronize(st_));
+        const float* hu = hp.data();
+        const float* hv = hu + NN;
+        const float* hux = hv + NN;
+        const float* huy = hux + NN;
+        const float* hvx = huy + NN;
+        const float* hvy = hvx + NN;
+        const float* hcorr = hvy + NN;
+        {
+            struct SeedNode {
+                int x, y;
+                float u, v, ux, uy, vx, vy, corr;
+            };
+            struct Rel {
+                bool operator()(const SeedNode& a, const SeedNode& b) const {
+                    if (a.corr != b.corr) return a.corr > b.corr;
+                    if (a.y != b.y) return a.y > b.y;
+                    return a.x > b.x;
+                }
+            };
+            std::vector<unsigned char> claimed(hsolved.begin(), hsolved.end());
+            const int dx4[4] = {1, -1, 0, 0}, dy4[4] = {0, 0, 1, -1};
+            std::vector<SeedNode> boundary, global;
+            const SeedNode pc{pcx, pcy, gU, gV, 0, 0, 0, 0, 0};
+            bool pc_queued = false;
+            for (int y = 0; y < gh; ++y)
+                for (int x = 0; x < gw; ++x) {
+                    const size_t i = (size_t)y * gw + x;
+                    if (!hsolved[i] || hcorr[i] < 0.f) continue;
+                    bool touch = false;
+                    for (int k = 0; k < 4; ++k) {
+                        const int nx = x + dx4[k], ny = y + dy4[k];
+                        if (nx >= 0 && nx < gw && ny >= 0 && ny < gh && !hsolved[(size_t)ny * gw + nx]) {
+                            touch = true;
+                            break;
+                        }
+                    }
+                    if (touch) boundary.push_back({x, y, hu[i], hv[i], hux[i], huy[i], hvx[i], hvy[i], hcorr[i]});
+                }
+            if (boundary.empty()) {
+                struct C {
+                    int ix, iy;
+                    float u, v, dist, mag;
+                };
+                std::vector<C> cands;
+                const float gcx = p.rect_x + (gw / 2.f) * p.step, gcy = p.rect_y + (gh / 2.f) * p.step;
+                for (size_t i = 0; i < seeds.ref_x.size(); ++i) {
+                    const float fx = seeds.ref_x[i], fy = seeds.ref_y[i];
+                    const int ix = (int)std::round((fx - p.rect_x) / (float)p.step);
+                    const int iy = (int)std::round((fy - p.rect_y) / (float)p.step);
+                    if (ix < 0 || ix >= gw || iy < 0 || iy >= gh || hsolved[(size_t)iy * gw + ix]) continue;
+                    const float du = seeds.def_x[i] - fx, dv = seeds.def_y[i] - fy;
+                    const float wx = p.rect_x + ix * p.step, wy = p.rect_y + iy * p.step;
+                    const float dc = std::sqrt((wx - gcx) * (wx - gcx) + (wy - gcy) * (wy - gcy));
+                    cands.push_back({ix, iy, du, dv, dc, std::sqrt(du * du + dv * dv)});
+                }
+                if (cands.empty()) {
+                    global.push_back(pc);
+                    pc_queued = true;
+                } else {
+                    std::sort(cands.begin(), cands.end(), [](const C& a, const C& b) {
+                        const float ma = std::floor(a.mag), mb = std::floor(b.mag);
+                        if (ma != mb) return ma < mb;
+                        if (a.dist != b.dist) return a.dist < b.dist;
+                        if (a.iy != b.iy) return a.iy < b.iy;
+                        return a.ix < b.ix;
+                    });
+                    for (size_t k = 0; k < cands.size() && k < 5; ++k)
+                        global.push_back({cands[k].ix, cands[k].iy, cands[k].u, cands[k].v, 0, 0, 0, 0, 0});
+                }
+            }
+            std::priority_queue<SeedNode, std::vector<SeedNode>, Rel> q;
+            for (const auto& b : boundary) q.push(b);
+            struct Cell {
+                Task t;
+                SolveOut r;
+            };
+            // commit one solved cell: host mirror, queue, counters
+            std::vector<int> commit_nodes;
+            std::vector<SolveOut> commit_res;
+            auto commit = [&](const Cell& c) {
+                if (!c.r.init) return;
+                st_b.record(c.r);
+                commit_nodes.push_back(c.t.node);
+                commit_res.push_back(c.r);
+                if (c.r.status == 0 && c.r.corr <= kCorrAccept) {
+                    st_b.points++;
+                    q.push({c.t.node % gw, c.t.node / gw, c.r.p[0], c.r.p[1], c.r.p[2], c.r.p[3], c.r.p[4], c.r.p[5],
+                            c.r.corr});
+                }
+            };
+            auto make_task = [&](int x, int y, const float* gs) {
+                Task t = {};
+                t.cx = p.rect_x + x * p.step;
+                t.cy = p.rect_y + y * p.step;
+                t.pool = y * gw + x;
+                t.mode = NO_SEARCH;
+                std::copy(gs, gs + 6, t.g);
+                t.node = t.pool;
+                return t;
+            };
+            auto flush_commits = [&]() {
+                if (commit_nodes.empty()) return;
+                const int n = (int)commit_nodes.size();
+                if (!nodes_d_.reserve(n) || !out_d_.reserve(n)) throw HipFail{hipErrorOutOfMemory};
+                ck(hipMemcpyAsync(nodes_d_.p, commit_nodes.data(), sizeof(int) * n, hipMemcpyHostToDevice, st_));
+                ck(hipMemcpyAsync(out_d_.p, commit_res.data(), sizeof(SolveOut) * n, hipMemcpyHostToDevice, st_));
+                ck(launch_apply(g, nodes_d_.p, out_d_.p, n, -1, true, st_));
+                ck(hipStreamSynchronize(st_));
+                commit_nodes.clear();
+                commit_res.clear();
+            };
+            for (const auto& s : global) {
+                const int flat = s.y * gw + s.x;
+                if (claimed[flat]) continue;
+                claimed[flat] = 1;
+                const float gs[6] = {s.u, s.v, 0, 0, 0, 0};
+                Cell c{make_task(s.x, s.y, gs), {}};
+                std::vector<SolveOut> r1;
+                if (solve_tasks({c.t}, r1, p.use6, p.subset, tol) != 0) return kErrInit;
+                c.r = r1[0];
+                commit(c);
+            }
+            if (q.empty() && !global.empty() && !pc_queued) {
+                const int flat = pc.y * gw + pc.x;
+                if (pc.x >= 0 && pc.x < gw && pc.y >= 0 && pc.y < gh && !claimed[flat]) {
+                    claimed[flat] = 1;
+                    const float gs[6] = {pc.u, pc.v, 0, 0, 0, 0};
+                    Cell c{make_task(pc.x, pc.y, gs), {}};
+                    std::vector<SolveOut> r1;
+                    if (solve_tasks({c.t}, r1, p.use6, p.subset, tol) != 0) return kErrInit;
+                    c.r = r1[0];
+                    commit(c);
+                }
+            }
+            struct Round {
+                std::vector<Cell> cells;
+                bool solved = false;
+            };
+            std::deque<Round> rounds;
+            auto claim = [&]() -> bool {
+                if (q.empty()) return false;
+                Round r;
+                int popped = 0;
+                while (!q.empty() && popped < kPathBBatchNodes) {
+                    const SeedNode cur = q.top();
+                    q.pop();
+                    ++popped;
+                    for (int k = 0; k < 4; ++k) {
+                        const int nx = cur.x + dx4[k], ny = cur.y + dy4[k];
+                        if (nx < 0 || nx >= gw || ny < 0 || ny >= gh) continue;
+                        const int flat = ny * gw + nx;
+                        if (claimed[flat]) continue;
+                        claimed[flat] = 1;
+                        const float ddx = (float)((nx - cur.x) * p.step), ddy = (float)((ny - cur.y) * p.step);
+                        const float gs[6] = {cur.u + cur.ux * ddx + cur.uy * ddy, cur.v + cur.vx * ddx + cur.vy * ddy,
+                                             cur.ux, cur.uy, cur.vx, cur.vy};
+                        r.cells.push_back(Cell{make_task(nx, ny, gs), {}});
+                    }
+                }
+                rounds.push_back(std::move(r));
+                return true;
+            };
+            while (rounds.size() < 2 && claim()) {
+            }
+            while (!rounds.empty()) {
+                if (cancel.requested()) return kErrCancelled;
+                if (!rounds.front().solved) {
+                    std::vector<Task> tasks;
+                    for (Round& r : rounds)
+                        if (!r.solved)
+                            for (Cell& c : r.cells) tasks.push_back(c.t);
+                    std::vector<SolveOut> res;
+                    if (solve_tasks(tasks, res, p.use6, p.subset, tol) != 0) return kErrInit;
+                    size_t k = 0;
+                    for (Round& r : rounds)
+                        if (!r.solved) {
+                            for (Cell& c : r.cells) c.r = res[k++];
+                            r.solved = true;
+                        }
+                }
+                for (const Cell& c : rounds.front().cells) commit(c);
+                rounds.pop_front();
+                while (rounds.size() < 2 && claim()) {
+                }
+                ticker.set(anchor_pts + st_a.points + st_b.points);
+            }
+            flush_commits();
+        }
+        tm_.path_b = ms_since(t_b);
+        if (cancel.requested()) return kErrCancelled;
+
+        // ---- strain and packing (pack_full_field_output) ----------------------
+        const auto t_s = clk::now();
+        ck(launch_valid(g, valid_d, N, st_));
+        if (keep_field) {
+            last_field = Field{};
+            last_field.gw = gw;
+            last_field.gh = gh;
+            last_field.rect_x = p.rect_x;
+            last_field.rect_y = p.rect_y;
+            last_field.step = p.step;
+            last_field.attempted = total_valid;
+            std::vector<float> all(7 * NN);
+            last_field.ok.resize(NN);
+            ck(hipMemcpyAsync(all.data(), gplanes_.p, sizeof(float) * 7 * NN, hipMemcpyDeviceToHost, st_));
+            ck(hipMemcpyAsync(last_field.ok.data(), valid_d, NN, hipMemcpyDeviceToHost, st_));
+            ck(hipStreamSynchronize(st_));
+            last_field.u.assign(all.begin(), all.begin() + NN);
+            last_field.v.assign(all.begin() + NN, all.begin() + 2 * NN);
+            last_field.ux.assign(all.begin() + 2 * NN, all.begin() + 3 * NN);
+            last_field.uy.assign(all.begin() + 3 * NN, all.begin() + 4 * NN);
+            last_field.vx.assign(all.begin() + 4 * NN, all.begin() + 5 * NN);
+            last_field.vy.assign(all.begin() + 5 * NN, all.begin() + 6 * NN);
+            last_field.corr.assign(all.begin() + 6 * NN, all.begin() + 7 * NN);
+        }
+        // median test: five passes, each from the previous pass's flags; a
+        // pass after one that rejected nothing changes nothing.
+        ck(hipMemsetAsync(counters_.p + 12, 0, sizeof(int32_t), st_));
+        uint8_t* src = valid_d;
+        uint8_t* dst = ok_a;
+        for (int pass = 0; pass < kOutlierMaxPasses; ++pass) {
+            ck(launch_outlier_pass(gw, gh, g.u, g.v, src, dst, counters_.p + 12, st_));
+            src = dst;
+            dst = (dst == ok_a) ? ok_b : ok_a;
+        }
+        float* exx = strain_.p;
+        float* eyy = exx + NN;
+        float* exy = eyy + NN;
+        ck(launch_vsg(gw, gh, p.step, p.strain_window, vsg_count(p.step, p.strain_window), g.u, g.v, src, exx, eyy, exy,
+                      st_));
+        std::vector<float> hs(3 * NN);
+        std::vector<uint8_t> hok(NN), hvalid(NN);
+        int dropped_outlier = 0;
+        ck(hipMemcpyAsync(hp.data(), gplanes_.p, sizeof(float) * 7 * NN, hipMemcpyDeviceToHost, st_));
+        ck(hipMemcpyAsync(hs.data(), strain_.p, sizeof(float) * 3 * NN, hipMemcpyDeviceToHost, st_));
+        ck(hipMemcpyAsync(hok.data(), src, NN, hipMemcpyDeviceToHost, st_));
+        ck(hipMemcpyAsync(&dropped_outlier, counters_.p + 12, sizeof(int), hipMemcpyDeviceToHost, st_));
+        ck(hipStreamSynchronize(st_));
+        int valid = 0, dropped_post = 0;
+        bool trunc = false;
+        for (int y = 0; y < gh && !trunc; ++y)
+            for (int x = 0; x < gw; ++x) {
+                const size_t i = (size_t)y * gw + x;
+                if (!hok[i]) continue;
+                if ((valid + 1) * 8 > out_capacity) {
+                    trunc = true;
+                    break;
+                }
+                if (hs[i] <= kStrainFail) {
+                    dropped_post++;
+                    continue;
+                }
+                float* o = out + (size_t)valid * 8;
+                o[0] = (float)(p.rect_x + x * p.step);
+                o[1] = (float)(p.rect_y + y * p.step);
+                o[2] = hp[i];
+                o[3] = hp[NN + i];
+                o[4] = hs[i];
+                o[5] = hs[NN + i];
+                o[6] = hs[2 * NN + i];
+                o[7] = hp[6 * NN + i];
+                valid++;
+            }
+        tm_.strain = ms_since(t_s);
+        ticker.stop();
+        if (progress) {
+            try {
+                progress(100);
+            } catch (...) {
+            }
+        }
+        tm_.total = ms_since(t_total);
+        tm_.icgn = icgn_ms_est + tm_.path_a;
+        // fill_engine_metrics (full_field_solver_stats.cpp:182-280)
+        const int ncopy = metrics_len < 16 ? 0 : metrics_len >= 24 ? 24 : metrics_len == 18 ? 17 : metrics_len;
+        if (metrics && ncopy > 0) {
+            const int iters = st_a.iters + st_b.iters + st_anchor.iters;
+            float m[24];
+            m[0] = (float)total_valid;
+            m[1] = (float)valid;
+            m[2] = (float)(total_valid - valid);
+            m[3] = (float)st_a.points;
+            m[4] = (float)st_b.points;
+            m[5] = (float)(st_a.calls + st_b.calls + st_anchor.calls);
+            m[6] = (float)(st_a.saved + st_b.saved + st_anchor.saved);
+            m[7] = (float)(st_a.dead + st_b.dead + st_anchor.dead);
+            m[8] = valid > 0 ? (float)iters / (float)valid : 0.0f;
+            m[9] = (float)tm_.total;
+            m[10] = (float)tm_.seeding;
+            m[11] = (float)tm_.prepass;
+            m[12] = (float)tm_.mesh;
+            m[13] = (float)tm_.strain;
+            m[14] = tm_.total > 0 ? (float)(valid / tm_.total) : 0.0f;
+            m[15] = total_valid > 0 ? ((float)valid / total_valid) * 100.0f : 0.0f;
+            m[16] = (float)seeds.quality;
+            m[17] = (float)tm_.simplex;
+            m[18] = (float)tm_.icgn;
+            m[19] = (float)anchor_pts;
+            m[20] = (float)dropped_post;
+            m[21] = gm.locked ? 1.0f : 0.0f;
+            m[22] = seeds.coverage;
+            m[23] = (float)dropped_outlier;
+            for (int i = 0; i < ncopy; ++i) metrics[i] = m[i];
+        }
+        return valid;
+    } catch (const HipFail&) {
+        return kErrInit;
+    }
+}
+
+int Engine::strain_pack(const Field& f, int window, float* out, int out_capacity, int* dropped_post,
+                        int* dropped_outlier) {
+    const int gw = f.gw, gh = f.gh, N = gw * gh;
+    const size_t NN = (size_t)N;
+    if (N <= 0) return 0;
+    try {
+        if (!gplanes_.reserve(7 * NN) || !gflags_.reserve(7 * NN) || !strain_.reserve(3 * NN) || !counters_.reserve(16))
+            return kErrInit;
+        float* u = gplanes_.p;
+        float* v = u + NN;
+        uint8_t* ok0 = gflags_.p;
+        uint8_t* ok1 = ok0 + NN;
+        uint8_t* ok2 = ok1 + NN;
+        ck(hipMemcpyAsync(u, f.u.data(), sizeof(float) * NN, hipMemcpyHostToDevice, st_));
+        ck(hipMemcpyAsync(v, f.v.data(), sizeof(float) * NN, hipMemcpyHostToDevice, st_));
+        ck(hipMemcpyAsync(ok0, f.ok.data(), NN, hipMemcpyHostToDevice, st_));
+        ck(hipMemsetAsync(counters_.p + 12, 0, sizeof(int32_t), st_));
+        uint8_t* src = ok0;
+        uint8_t* dst = ok1;
+        for (int pass = 0; pass < kOutlierMaxPasses; ++pass) {
+            ck(launch_outlier_pass(gw, gh, u, v, src, dst, counters_.p + 12, st_));
+            src = dst;
+            dst = (dst == ok1) ? ok2 : ok1;
+        }
+        float* exx = strain_.p;
+        ck(launch_vsg(gw, gh, f.step, window, vsg_count(f.step, window), u, v, src, exx, exx + NN, exx + 2 * NN, st_));
+        std::vector<float> hs(3 * NN);
+        std::vector<uint8_t> hok(NN);
+        int rej = 0;
+        ck(hipMemcpyAsync(hs.data(), exx, sizeof(float) * 3 * NN, hipMemcpyDeviceToHost, st_));
+        ck(hipMemcpyAsync(hok.data(), src, NN, hipMemcpyDeviceToHost, st_));
+        ck(hipMemcpyAsync(&rej, counters_.p + 12, sizeof(int), hipMemcpyDeviceToHost, st_));
+        ck(hipStreamSynchronize(st_));
+        int valid = 0, post = 0;
+        bool trunc = false;
+        for (int y = 0; y < gh && !trunc; ++y)
+            for (int x = 0; x < gw; ++x) {
+                const size_t i = (size_t)y * gw + x;
+                if (!hok[i]) continue;
+                if ((valid + 1) * 8 > out_capacity) {
+                    trunc = true;
+                    break;
+                }
+                if (hs[i] <= kStrainFail) {
+                    post++;
+                    continue;
+                }
+                float* o = out + (size_t)valid * 8;
+                o[0] = (float)(f.rect_x + x * f.step);
+                o[1] = (float)(f.rect_y + y * f.step);
+                o[2] = f.u[i];
+                o[3] = f.v[i];
+                o[4] = hs[i];
+                o[5] = hs[NN + i];
+                o[6] = hs[2 * NN + i];
+                o[7] = f.corr[i];
+                valid++;
+            }
+        if (dropped_post) *dropped_post = post;
+        if (dropped_outlier) *dropped_outlier = rej;
+        return valid;
+    } catch (const HipFail&) {
+        return kErrInit;
+    }
+}
+
+int validate_params(Params& p, int width, int height) { return validate(p, width, height); }
+int vsg_window_node_count(int step, int window) { return vsg_count(step, window); }
+
+}  // namespace sdic

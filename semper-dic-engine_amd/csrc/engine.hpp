@@ -1,0 +1,159 @@
+// The MI355X full-field DIC engine behind the C ABI: one instance per
+// semper_engine handle. It keeps the reference image, its gradient planes,
+// the grid scan and the Hessian pool resident in HBM (the reference's
+// ReferenceCache, include/semper/pipeline.hpp:35-64) and runs the solve of one
+// deformed frame (run_full_field_impl, src/pipeline/full_field_solver.cpp:141-586)
+// as a sequence of kernels with a thin host orchestration between them.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <functional>
+#include <vector>
+
+#include "common.hpp"
+#include "solver.hpp"
+
+namespace sdic {
+
+// Device allocation owned by the engine; grows, never shrinks.
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    bool reserve(size_t count) {
+        if (count <= n && p) return true;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        if (hipMalloc(&p, sizeof(T) * (count ? count : 1)) != hipSuccess) {
+            p = nullptr;
+            return false;
+        }
+        n = count;
+        return true;
+    }
+};
+
+struct Params {
+    int rect_x = 0, rect_y = 0, rect_w = 0, rect_h = 0;
+    int step = 0, subset = 0, strain_window = 0;
+    bool use6 = false;
+};
+
+// The cancel token of a handle (include/semper/cancel.hpp:53-80).
+struct Cancel {
+    std::atomic<bool> flag{false};
+    std::atomic<bool> window{false};
+    void request() { flag.store(true, std::memory_order_relaxed); }
+    bool requested() const { return flag.load(std::memory_order_relaxed); }
+    void begin() {
+        flag.store(false, std::memory_order_relaxed);
+        window.store(true, std::memory_order_release);
+    }
+    void end() { window.store(false, std::memory_order_release); }
+    void enter() {
+        if (!window.exchange(false, std::memory_order_acq_rel)) flag.store(false, std::memory_order_relaxed);
+    }
+};
+
+// Per-phase wall clock of the last solve (ms) and the device time of the
+// fast-path solver launch over the grid, from HIP events on the engine's stream.
+struct Timings {
+    double total = 0, prep = 0, seeding = 0, prepass = 0, mesh = 0, path_a = 0, path_b = 0, strain = 0;
+    double icgn = 0, simplex = 0;
+    double kernel_path_a = 0;     // device ms of the Path A solver launch
+    int path_a_items = 0;         // grid nodes the Path A launch covered
+    int path_a_iters = 0;         // IC-GN iterations it ran
+};
+
+// validate_params (full_field_solver.cpp:97-111) and
+// vsg_window_node_count (strain_calculator.cpp:12-25).
+int validate_params(Params& p, int width, int height);
+int vsg_window_node_count(int step, int window);
+
+class Engine {
+public:
+    Engine();
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    void reset();
+    bool has_reference() const { return have_ref_; }
+    int width() const { return w_; }
+    int height() const { return h_; }
+
+    // ReferenceCache::set_from_gray: u8 reference (host or device) and an
+    // optional ghost-wall mask (host). 0 or a negative code.
+    int set_reference(const uint8_t* gray, bool on_device, int w, int h, const uint8_t* mask);
+
+    // run_full_field on a u8 frame of the reference's size (host or device);
+    // the scan mask (host, may be null). Returns the point count or a code.
+    int run(const uint8_t* def, bool on_device, const uint8_t* mask, const Params& p, float* out,
+            int out_capacity, float* metrics, int metrics_len, Cancel& cancel,
+            const std::function<void(int)>& progress, float conv_tol = 1e-3f);
+
+    const Timings& timings() const { return tm_; }
+    hipStream_t stream() const { return st_; }
+
+    // Device planes of the last solve (for incremental sessions): per grid node.
+    struct Field {
+        int gw = 0, gh = 0, rect_x = 0, rect_y = 0, step = 0, attempted = 0;
+        std::vector<float> u, v, ux, uy, vx, vy, corr;
+        std::vector<unsigned char> ok;   // solved with corr >= 0 (before the median test)
+    };
+    bool keep_field = false;
+    Field last_field;
+
+    // Median test, VSG strain and row-major packing of a node field given on
+    // the host (f.ok marks the measured nodes; corr is reported as is). The
+    // incremental session's output path. Returns the point count.
+    int strain_pack(const Field& f, int strain_window, float* out, int out_capacity, int* dropped_post,
+                    int* dropped_outlier);
+
+private:
+    int ensure_grid(const Params& p, const uint8_t* mask, bool& cached);
+    int solve_tasks(const std::vector<Task>& tasks, std::vector<SolveOut>& out, bool use6, int dim, float tol);
+
+    hipStream_t st_ = nullptr;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    bool have_ref_ = false;
+    int w_ = 0, h_ = 0;
+    DevBuf<uint8_t> ref_u8_, def_u8_;
+    DevBuf<float> I_, gx_, gy_;
+    DevBuf<float> pyr_r_[4], pyr_d_[4];
+    int pyr_ref_levels_ = 0;
+    // grid cache (RefPrecompute)
+    bool grid_valid_ = false;
+    Params grid_p_;
+    std::vector<uint8_t> grid_mask_;
+    int gw_ = 0, gh_ = 0, total_valid_ = 0;
+    std::vector<uint8_t> skip_h_;
+    DevBuf<uint8_t> skip_d_, mask_d_;
+    DevBuf<PoolEntry> pool_;
+    // per run
+    DevBuf<float> gplanes_;        // u v ux uy vx vy corr  (7 planes)
+    DevBuf<uint8_t> gflags_;       // solved tag simplex in_mesh valid ok_a ok_b (7 planes)
+    DevBuf<int32_t> giters_;
+    DevBuf<float> guess_, tmp_;    // 6 planes each
+    DevBuf<float> strain_;         // exx eyy exy
+    DevBuf<Task> tasks_d_;
+    DevBuf<SolveOut> out_d_;
+    DevBuf<int32_t> bail_, counters_, nodes_d_;
+    DevBuf<TriDev> tris_d_;
+    DevBuf<int32_t> celltri_d_, lat_d_;
+    DevBuf<int32_t> centres_d_;
+    DevBuf<PatchOut> patch_d_;
+    Timings tm_;
+};
+
+}  // namespace sdic

@@ -1,0 +1,76 @@
+// Per-node bookkeeping kernels of the device grid (the ResultGrid as planes).
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "solver.hpp"
+
+namespace sdic {
+
+namespace {
+
+__global__ void k_grid_init(GridDev g, const uint8_t* __restrict__ skip, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    g.u[i] = g.v[i] = g.ux[i] = g.uy[i] = g.vx[i] = g.vy[i] = 0.0f;
+    g.corr[i] = kCorrInvalid;
+    g.solved[i] = skip[i];
+    g.tag[i] = kMeshNone;
+    g.iters[i] = 0;
+    g.simplex[i] = 0;
+}
+
+__global__ void k_apply(GridDev g, const int* __restrict__ nodes, const SolveOut* __restrict__ res, int n, int tag,
+                        int accepted_only) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int i = nodes[k];
+    if (i < 0) return;
+    const SolveOut o = res[k];
+    if (!o.init) return;
+    const bool acc = o.status == 0 && o.corr <= kCorrAccept;
+    g.iters[i] = o.iters;
+    g.simplex[i] = o.simplex ? 1 : 0;
+    if (!accepted_only || acc) {
+        g.u[i] = o.p[0];
+        g.v[i] = o.p[1];
+        g.ux[i] = o.p[2];
+        g.uy[i] = o.p[3];
+        g.vx[i] = o.p[4];
+        g.vy[i] = o.p[5];
+        g.corr[i] = o.corr;
+        g.solved[i] = 1;
+        if (tag >= 0) g.tag[i] = (uint8_t)tag;
+    } else {
+        g.corr[i] = kCorrInvalid;
+    }
+}
+
+__global__ void k_valid(GridDev g, uint8_t* __restrict__ valid, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    valid[i] = (g.solved[i] && g.corr[i] >= 0.0f) ? 1 : 0;
+}
+
+}  // namespace
+
+hipError_t launch_grid_init(GridDev g, const uint8_t* skip, int n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_grid_init, dim3((n + 255) / 256), dim3(256), 0, st, g, skip, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_apply(GridDev g, const int* nodes, const SolveOut* res, int n, int tag, bool accepted_only,
+                        hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_apply, dim3((n + 255) / 256), dim3(256), 0, st, g, nodes, res, n, tag,
+                       accepted_only ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_valid(GridDev g, uint8_t* valid, int n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_valid, dim3((n + 255) / 256), dim3(256), 0, st, g, valid, n);
+    return hipGetLastError();
+}
+
+}  // namespace sdic

@@ -1,0 +1,400 @@
+// Reference precompute kernels for gfx950 (HBM-bound):
+//   k_prep_ref   u8 -> float intensity plane + 4th-order gradients, Ghost Wall
+//                (image_processor.cpp:9-64, reference_cache.cpp:18-35)
+//   k_scan       grid scan: boundary buffer and the strict per-pixel mask test
+//                (full_field_solver.cpp:271-317), bit-exact integer work
+//   k_pool       Hessian pool, one wave per grid node
+//                (subset_precomputer.cpp:42-149, hessian.hpp:40-52)
+//   k_down*      2x2 box pyramid of the global-motion search
+//   k_patch      coarse-to-fine ZNCC patch search of the global-motion model
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "solver.hpp"
+#include "wave.hpp"
+
+#pragma clang fp contract(off)
+
+namespace sdic {
+
+namespace {
+
+// (-I[+2] + 8 I[+1] - 8 I[-1] + I[-2]) / 12 with the reference's evaluation
+// order; zero on the 2 px border.
+template <typename T>
+__device__ __forceinline__ void grad_at(const T* img, int w, int h, int x, int y, float& gx, float& gy) {
+    gx = 0.0f;
+    gy = 0.0f;
+    if (x < 2 || y < 2 || x >= w - 2 || y >= h - 2) return;
+    const int i = y * w + x;
+    gx = (-(float)img[i + 2] + 8.0f * (float)img[i + 1] - 8.0f * (float)img[i - 1] + (float)img[i - 2]) / 12.0f;
+    gy = (-(float)img[i + 2 * w] + 8.0f * (float)img[i + w] - 8.0f * (float)img[i - w] +
+          (float)img[i - 2 * w]) /
+         12.0f;
+}
+
+template <typename T>
+__global__ void k_prep_ref(const T* __restrict__ img, const uint8_t* __restrict__ mask, int w, int h,
+                           float* __restrict__ I, float* __restrict__ gx, float* __restrict__ gy) {
+    const int64_t n = (int64_t)w * h;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(i % w), y = (int)(i / w);
+        float a, b;
+        grad_at(img, w, h, x, y, a, b);
+        gx[i] = a;
+        gy[i] = b;
+        // The Ghost Wall is stamped after the gradients, which see the real pixels.
+        I[i] = (mask && mask[i] < 128) ? kGhostWallValue : (float)img[i];
+    }
+}
+
+// One thread per grid node: skip = inside the boundary buffer d/2 + 4 + 15,
+// or any subset pixel masked (< 128).
+__global__ void k_scan(const uint8_t* __restrict__ mask, int w, int h, int rect_x, int rect_y, int step,
+                       int gw, int gh, int dim, uint8_t* __restrict__ skip, int* count) {
+    const int node = blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= gw * gh) return;
+    const int rx = rect_x + (node % gw) * step, ry = rect_y + (node / gw) * step;
+    const int half = dim / 2, buf = half + 4 + 15;
+    bool s = rx - buf < 0 || rx + buf >= w || ry - buf < 0 || ry + buf >= h;
+    if (!s && mask) {
+        for (int dy = -half; dy <= half && !s; ++dy) {
+            const uint8_t* row = mask + (int64_t)(ry + dy) * w + rx;
+            for (int dx = -half; dx <= half; ++dx)
+                if (row[dx] < 128) {
+                    s = true;
+                    break;
+                }
+        }
+    }
+    skip[node] = s ? 1 : 0;
+    if (!s) atomicAdd(count, 1);
+}
+
+struct PoolLds {
+    float H[36];
+    float Hinv[36];
+    float Hd[36];
+    float work[144];
+};
+
+// compute_hessian_only for the subset at (cx, cy); lane-strided pixel passes,
+// the 6x6 tests and inverses by lane 0 in LDS.
+__device__ void pool_entry(const float* __restrict__ I, const float* __restrict__ gxp,
+                           const float* __restrict__ gyp, int w, int h, int cx, int cy, int dim, bool lm,
+                           PoolLds& L, PoolEntry* __restrict__ dst) {
+    const int lane = threadIdx.x & 63;
+    const int half = dim / 2, n = dim * dim;
+    if (dim < 3 || (dim & 1) == 0 || cx - half < 0 || cx + half >= w || cy - half < 0 || cy + half >= h) {
+        if (lane == 0) dst->valid = 0;
+        return;
+    }
+    float sum = 0.0f;
+    int cnt = 0;
+    for (int i = lane; i < n; i += 64) {
+        const float r = I[(cy + i / dim - half) * w + cx + i % dim - half];
+        if (r < kGhostWallThreshold) continue;
+        sum += r;
+        cnt++;
+    }
+    sum = wave_sum(sum);
+    cnt = wave_sum_i(cnt);
+    if (cnt < n * 0.5f) {
+        if (lane == 0) dst->valid = 0;
+        return;
+    }
+    const float mean = sum / (float)cnt;
+    float ss = 0.0f;
+    for (int i = lane; i < n; i += 64) {
+        const float r = I[(cy + i / dim - half) * w + cx + i % dim - half];
+        if (r < kGhostWallThreshold) continue;
+        const float d = r - mean;
+        ss += d * d;
+    }
+    ss = wave_sum(ss);
+    float sd = sqrtf(ss / (float)cnt);
+    if (sd < 1e-5f) sd = 1.0f;
+    const float inv = 1.0f / sd;
+    float hh[21];
+#pragma unroll
+    for (int k = 0; k < 21; ++k) hh[k] = 0.0f;
+    for (int i = lane; i < n; i += 64) {
+        const int oy = i / dim - half, ox = i % dim - half;
+        const int a = (cy + oy) * w + cx + ox;
+        if (I[a] < kGhostWallThreshold) continue;
+        const float g0 = gxp[a] * inv, g1 = gyp[a] * inv;
+        const float fx = (float)ox, fy = (float)oy;
+        const float s[6] = {g0, g1, g0 * fx, g0 * fy, g1 * fx, g1 * fy};
+        int k = 0;
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+#pragma unroll
+            for (int c = r; c < 6; ++c) hh[k++] += s[r] * s[c];
+    }
+#pragma unroll
+    for (int k = 0; k < 21; ++k) hh[k] = wave_sum(hh[k]);
+    if (lane == 0) {
+        int k = 0;
+        for (int r = 0; r < 6; ++r)
+            for (int c = r; c < 6; ++c) {
+                L.H[r * 6 + c] = hh[k];
+                L.H[c * 6 + r] = hh[k];
+                ++k;
+            }
+        const float* H = L.H;
+        const float det2 = H[0] * H[7] - H[6] * H[1];
+        const float nrm2 = H[0] * H[0] + H[1] * H[1] + H[6] * H[6] + H[7] * H[7];
+        const float cond2 = (fabsf(det2) > 1e-12f) ? (nrm2 / fabsf(det2)) : 1.0e13f;
+        int valid = 1;
+        if (cond2 > 1.0e12f || !well_conditioned_lds(L.H, half, L.work)) {
+            valid = 0;
+        } else {
+            gj_inverse_lds(L.H, L.Hinv, L.work);
+            const float* step = L.Hinv;
+            if (lm) {
+                // solve_icgn's LM step matrix (optimization_engine.cpp:302-315)
+                for (int q = 0; q < 36; ++q) L.Hd[q] = L.H[q];
+                L.Hd[0] += kLmAlpha;
+                L.Hd[7] += kLmAlpha;
+                if (well_conditioned_lds(L.Hd, half, L.work)) {
+                    gj_inverse_lds(L.Hd, L.Hd, L.work);
+                    step = L.Hd;
+                }
+            }
+            for (int q = 0; q < 36; ++q) dst->hstep[q] = step[q];
+        }
+        dst->mean = mean;
+        dst->inv_std = inv;
+        dst->valid = valid;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_pool(const float* __restrict__ I, const float* __restrict__ gx,
+                                             const float* __restrict__ gy, int w, int h,
+                                             const uint8_t* __restrict__ skip, int gw, int gh, int rect_x,
+                                             int rect_y, int step, int dim, int lm, PoolEntry* pool) {
+    __shared__ PoolLds L;
+    const int node = xcd_remap(blockIdx.x, gridDim.x);
+    if (node >= gw * gh) return;
+    if (skip[node]) {
+        if (threadIdx.x == 0) pool[node].valid = 0;
+        return;
+    }
+    pool_entry(I, gx, gy, w, h, rect_x + (node % gw) * step, rect_y + (node / gw) * step, dim, lm != 0, L,
+               pool + node);
+}
+
+__global__ __launch_bounds__(64) void k_pool_list(const float* __restrict__ I, const float* __restrict__ gx,
+                                                  const float* __restrict__ gy, int w, int h,
+                                                  const int* __restrict__ cxy, int n, int dim, int lm,
+                                                  PoolEntry* pool) {
+    __shared__ PoolLds L;
+    const int k = blockIdx.x;
+    if (k >= n) return;
+    pool_entry(I, gx, gy, w, h, cxy[2 * k], cxy[2 * k + 1], dim, lm != 0, L, pool + k);
+}
+
+template <typename T>
+__global__ void k_down(const T* __restrict__ src, int w, int h, float* __restrict__ dst) {
+    const int nw = w / 2, nh = h / 2;
+    const int64_t n = (int64_t)nw * nh;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(i % nw), y = (int)(i / nw);
+        const T* a = src + (int64_t)(2 * y) * w + 2 * x;
+        dst[i] = 0.25f * (((float)a[0] + (float)a[1]) + ((float)a[w] + (float)a[w + 1]));
+    }
+}
+
+// ---- global-motion patch search (the seeding front end) ---------------------
+
+constexpr int kPatchHalf = 16;
+constexpr int kCoarseRadius = 8;
+constexpr int kFineRadius = 2;
+
+__device__ __forceinline__ float lv_at(const PyrDev& p, bool def, int l, int idx) {
+    if (l == 0) return (float)(def ? p.d0[idx] : p.r0[idx]);
+    return def ? p.d[l][idx] : p.r[l][idx];
+}
+
+// ZNCC of the 32 x 32 template at (cx, cy) on level l against the window
+// shifted by (sx, sy); false if either leaves the level's image.
+__device__ bool zncc(const PyrDev& p, int l, int cx, int cy, int sx, int sy, double& out, double* tstd) {
+    const int w = p.w[l], h = p.h[l];
+    const int x0 = cx - kPatchHalf, y0 = cy - kPatchHalf, x1 = cx + kPatchHalf, y1 = cy + kPatchHalf;
+    if (x0 < 0 || y0 < 0 || x1 > w || y1 > h) return false;
+    if (x0 + sx < 0 || y0 + sy < 0 || x1 + sx > w || y1 + sy > h) return false;
+    double sr = 0, sd = 0, srr = 0, sdd = 0, srd = 0;
+    for (int y = y0; y < y1; ++y)
+        for (int x = x0; x < x1; ++x) {
+            const double a = lv_at(p, false, l, y * w + x), b = lv_at(p, true, l, (y + sy) * w + x + sx);
+            sr += a;
+            sd += b;
+            srr += a * a;
+            sdd += b * b;
+            srd += a * b;
+        }
+    const double n = (double)(4 * kPatchHalf * kPatchHalf);
+    const double vr = srr - sr * sr / n, vd = sdd - sd * sd / n;
+    if (tstd) *tstd = sqrt(fmax(0.0, vr / n));
+    if (!(vr > 1e-9) || !(vd > 1e-9)) {
+        out = -2.0;
+        return true;
+    }
+    out = (srd - sr * sd / n) / (sqrt(vr) * sqrt(vd));
+    return true;
+}
+
+struct Cand {
+    double s;
+    long r2;
+    int order;
+    int sx, sy;
+    int any;
+};
+
+// better(a, b): higher score, then smaller |shift|^2, then earlier scan order
+__device__ __forceinline__ bool better(const Cand& a, const Cand& b) {
+    if (!a.any) return false;
+    if (!b.any) return true;
+    if (a.s != b.s) return a.s > b.s;
+    if (a.r2 != b.r2) return a.r2 < b.r2;
+    return a.order < b.order;
+}
+
+__global__ __launch_bounds__(256) void k_patch(PyrDev p, const int* __restrict__ centres, PatchOut* out) {
+    __shared__ Cand red[256];
+    __shared__ int s_sx, s_sy, s_ok;
+    __shared__ double s_best;
+    const int pi = blockIdx.x, tid = threadIdx.x;
+    const int px = centres[2 * pi], py = centres[2 * pi + 1];
+    const int L = p.levels;
+    if (tid == 0) {
+        s_sx = 0;
+        s_sy = 0;
+        s_ok = 1;
+        s_best = -3.0;
+    }
+    __syncthreads();
+    for (int l = L; l >= 0; --l) {
+        if (!s_ok) break;
+        const int rad = (l == L) ? kCoarseRadius : kFineRadius;
+        const int side = 2 * rad + 1, ns = side * side;
+        const int csx = s_sx, csy = s_sy;
+        Cand best;
+        best.any = 0;
+        for (int k = tid; k < ns; k += blockDim.x) {
+            const int sy = csy - rad + k / side, sx = csx - rad + k % side;
+            double s;
+            if (!zncc(p, l, px >> l, py >> l, sx, sy, s, nullptr)) continue;
+            Cand c;
+            c.s = s;
+            c.r2 = (long)sx * sx + (long)sy * sy;
+            c.order = k;
+            c.sx = sx;
+            c.sy = sy;
+            c.any = 1;
+            if (better(c, best)) best = c;
+        }
+        red[tid] = best;
+        __syncthreads();
+        for (int m = 128; m >= 1; m >>= 1) {
+            if (tid < m && better(red[tid + m], red[tid])) red[tid] = red[tid + m];
+            __syncthreads();
+        }
+        if (tid == 0) {
+            if (!red[0].any) {
+                s_ok = 0;
+            } else {
+                s_best = red[0].s;
+                s_sx = red[0].sx * (l > 0 ? 2 : 1);
+                s_sy = red[0].sy * (l > 0 ? 2 : 1);
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        PatchOut o = {};
+        o.ok = s_ok;
+        o.sx = s_sx;
+        o.sy = s_sy;
+        o.best = s_best;
+        if (s_ok) {
+            double t0;
+            zncc(p, 0, px, py, s_sx, s_sy, t0, &o.tstd);
+            o.has_x = zncc(p, 0, px, py, s_sx - 1, s_sy, o.xm, nullptr) &&
+                      zncc(p, 0, px, py, s_sx + 1, s_sy, o.xp, nullptr);
+            o.has_y = zncc(p, 0, px, py, s_sx, s_sy - 1, o.ym, nullptr) &&
+                      zncc(p, 0, px, py, s_sx, s_sy + 1, o.yp, nullptr);
+        }
+        out[pi] = o;
+    }
+}
+
+int blocks_for(int64_t n, int bs) {
+    const int64_t b = (n + bs - 1) / bs;
+    return (int)(b < 65536 ? (b < 1 ? 1 : b) : 65536);
+}
+
+}  // namespace
+
+hipError_t launch_prep_ref_u8(const uint8_t* img, const uint8_t* mask, int w, int h, float* I, float* gx,
+                              float* gy, hipStream_t st) {
+    hipLaunchKernelGGL(k_prep_ref<uint8_t>, dim3(blocks_for((int64_t)w * h, 256)), dim3(256), 0, st, img, mask,
+                       w, h, I, gx, gy);
+    return hipGetLastError();
+}
+
+hipError_t launch_prep_ref_f32(const float* img, int w, int h, float* I, float* gx, float* gy,
+                               hipStream_t st) {
+    hipLaunchKernelGGL(k_prep_ref<float>, dim3(blocks_for((int64_t)w * h, 256)), dim3(256), 0, st, img,
+                       (const uint8_t*)nullptr, w, h, I, gx, gy);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan(const uint8_t* mask, int w, int h, int rect_x, int rect_y, int step, int gw, int gh,
+                       int dim, uint8_t* skip, int* count, hipStream_t st) {
+    const int n = gw * gh;
+    hipLaunchKernelGGL(k_scan, dim3((n + 255) / 256), dim3(256), 0, st, mask, w, h, rect_x, rect_y, step, gw,
+                       gh, dim, skip, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_pool(const float* I, const float* gx, const float* gy, int w, int h, const uint8_t* skip,
+                       int gw, int gh, int rect_x, int rect_y, int step, int dim, bool lm, PoolEntry* pool,
+                       hipStream_t st) {
+    const int n = gw * gh;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pool, dim3(n), dim3(64), 0, st, I, gx, gy, w, h, skip, gw, gh, rect_x, rect_y, step,
+                       dim, lm ? 1 : 0, pool);
+    return hipGetLastError();
+}
+
+hipError_t launch_pool_list(const float* I, const float* gx, const float* gy, int w, int h, const int* cxy,
+                            int n, int dim, bool lm, PoolEntry* pool, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pool_list, dim3(n), dim3(64), 0, st, I, gx, gy, w, h, cxy, n, dim, lm ? 1 : 0, pool);
+    return hipGetLastError();
+}
+
+hipError_t launch_downsample_u8(const uint8_t* src, int w, int h, float* dst, hipStream_t st) {
+    hipLaunchKernelGGL(k_down<uint8_t>, dim3(blocks_for((int64_t)(w / 2) * (h / 2), 256)), dim3(256), 0, st,
+                       src, w, h, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_downsample_f32(const float* src, int w, int h, float* dst, hipStream_t st) {
+    hipLaunchKernelGGL(k_down<float>, dim3(blocks_for((int64_t)(w / 2) * (h / 2), 256)), dim3(256), 0, st,
+                       src, w, h, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_patch_search(const PyrDev& p, const int* centres, int npatch, PatchOut* out,
+                               hipStream_t st) {
+    if (npatch <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_patch, dim3(npatch), dim3(256), 0, st, p, centres, out);
+    return hipGetLastError();
+}
+
+}  // namespace sdic

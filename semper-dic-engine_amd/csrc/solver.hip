@@ -1,0 +1,34 @@
+// Dispatch of the subset solver over its instantiations, and the pixels-per-lane
+// class of a subset size.
+#include "solver.hpp"
+
+namespace sdic {
+
+hipError_t launch_solve_bu8(const SolveArgs& a, int ppl, hipStream_t st);
+hipError_t launch_solve_ku8(const SolveArgs& a, int ppl, hipStream_t st);
+hipError_t launch_solve_bf32(const SolveArgs& a, int ppl, hipStream_t st);
+hipError_t launch_solve_kf32(const SolveArgs& a, int ppl, hipStream_t st);
+
+int ppl_for(int dim) {
+    if (dim < 3 || dim > kMaxSubset) return -1;
+    const int p = (dim * dim + 63) / 64;
+    static const int cls[] = {4, 8, 12, 16, 24, 32, 48, 64};
+    for (int c : cls)
+        if (p <= c) return c;
+    return -1;
+}
+
+__global__ void k_zero_int(int* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+hipError_t launch_solve(const SolveArgs& a, bool use6, bool def_f32, hipStream_t st) {
+    const int ppl = ppl_for(a.dim);
+    if (ppl < 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, a.nbail, 1);
+    if (def_f32) return use6 ? launch_solve_kf32(a, ppl, st) : launch_solve_bf32(a, ppl, st);
+    return use6 ? launch_solve_ku8(a, ppl, st) : launch_solve_bu8(a, ppl, st);
+}
+
+}  // namespace sdic

@@ -1,0 +1,116 @@
+// Launch interface of the device kernels (host side). All pointers are device
+// pointers; every launcher is asynchronous on the given stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace sdic {
+
+// Path A over the grid: node -> centre, guess planes, tag.
+struct GridArgs {
+    GridDev grid;
+    const uint8_t* in_mesh;
+    const float *gu, *gv, *gux, *guy, *gvx, *gvy;
+    int gw, gh, rect_x, rect_y, step, dim, w, h;
+};
+
+// Arguments of the subset solver kernel. Task mode when tasks != null (one
+// result per task); grid mode (Path A) otherwise.
+struct SolveArgs {
+    const Task* tasks;        // task mode: work item id indexes tasks[] and out[]
+    SolveOut* out;
+    const int* items;         // optional list of work item ids (null: 0 .. nitems-1)
+    int nitems;
+    int* bail;                // ids the fast path hands to the full solver
+    int* nbail;               // their count (device)
+    GridArgs grid;
+    int* stats;               // grid mode: points, iterations, simplex calls, saved, dead
+    const PoolEntry* pool;
+    const float* refI;
+    const float* refGx;
+    const float* refGy;
+    const void* def;          // uint8 or float plane
+    int w, h, dim, lm;
+    float tol;
+};
+
+// Pixels-per-lane class of a subset size (-1: larger than kMaxSubset).
+int ppl_for(int dim);
+
+// solver_*.hip: one translation unit per (interpolator, deformed pixel type).
+// Launches the fast path over the work items, then the full solver over the
+// items it handed on. a.bail must hold nitems ints; a.nbail is zeroed here.
+hipError_t launch_solve(const SolveArgs& a, bool use6, bool def_f32, hipStream_t st);
+
+// prep.hip
+hipError_t launch_prep_ref_u8(const uint8_t* img, const uint8_t* mask, int w, int h, float* I, float* gx,
+                              float* gy, hipStream_t st);
+hipError_t launch_prep_ref_f32(const float* img, int w, int h, float* I, float* gx, float* gy,
+                               hipStream_t st);
+hipError_t launch_scan(const uint8_t* mask, int w, int h, int rect_x, int rect_y, int step, int gw, int gh,
+                       int dim, uint8_t* skip, int* count, hipStream_t st);
+hipError_t launch_pool(const float* I, const float* gx, const float* gy, int w, int h, const uint8_t* skip,
+                       int gw, int gh, int rect_x, int rect_y, int step, int dim, bool lm, PoolEntry* pool,
+                       hipStream_t st);
+hipError_t launch_pool_list(const float* I, const float* gx, const float* gy, int w, int h, const int* cxy,
+                            int n, int dim, bool lm, PoolEntry* pool, hipStream_t st);
+hipError_t launch_downsample_u8(const uint8_t* src, int w, int h, float* dst, hipStream_t st);
+hipError_t launch_downsample_f32(const float* src, int w, int h, float* dst, hipStream_t st);
+
+// Global-motion patch search: per patch the best integer shift after the
+// coarse-to-fine search, its ZNCC, the template std and the four ZNCC values
+// around it (for the sub-pixel parabolas). Levels: 0 reads the u8 planes,
+// 1..L the float pyramids.
+struct PyrDev {
+    const uint8_t* r0;
+    const uint8_t* d0;
+    const float* r[4];
+    const float* d[4];
+    int w[4], h[4];
+    int levels;
+};
+struct PatchOut {
+    int sx, sy, ok, pad;
+    double best, tstd, xm, xp, ym, yp;
+    int has_x, has_y, pad2[2];
+};
+hipError_t launch_patch_search(const PyrDev& p, const int* centres, int npatch, PatchOut* out,
+                               hipStream_t st);
+
+// guess.hip
+struct GuessArgs {
+    const TriDev* tris;
+    const int* cell_tri;      // first triangle of lattice cell c: cell_tri[c] .. cell_tri[c + 1]
+    const int* lat_x;         // lattice grid columns / rows
+    const int* lat_y;
+    int LW, LH;
+    int quality;              // 2 = FULL (extrapolation on)
+    int have_affine;
+    double aff[6];
+    double acx, acy;
+    float gU, gV;
+    int gw, gh, rect_x, rect_y, step;
+};
+hipError_t launch_guess(const GuessArgs& a, GridDev g, uint8_t* in_mesh, float* gu, float* gv, float* gux,
+                        float* guy, float* gvx, float* gvy, float* tmp, hipStream_t st);
+
+// strain.hip
+hipError_t launch_outlier_pass(int gw, int gh, const float* u, const float* v, const uint8_t* ok_in,
+                               uint8_t* ok_out, int* nfail, hipStream_t st);
+hipError_t launch_vsg(int gw, int gh, int step, int window, int expected, const float* u, const float* v,
+                      const uint8_t* valid, float* exx, float* eyy, float* exy, hipStream_t st);
+
+// grid.hip (per-node bookkeeping)
+hipError_t launch_grid_init(GridDev g, const uint8_t* skip, int n, hipStream_t st);
+// Write solve results into the grid: with accepted_only, only results with
+// status 0 and corr <= kCorrAccept mark their node solved (Path B commits);
+// otherwise every listed result does (published anchors, with tag).
+hipError_t launch_apply(GridDev g, const int* nodes, const SolveOut* res, int n, int tag, bool accepted_only,
+                        hipStream_t st);
+// valid = solved && corr >= 0
+hipError_t launch_valid(GridDev g, uint8_t* valid, int n, hipStream_t st);
+}  // namespace sdic

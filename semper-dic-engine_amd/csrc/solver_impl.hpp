@@ -1,0 +1,1040 @@
+// IC-GN subset solver for gfx950: one 64-lane wavefront per subset.
+//
+// Reference semantics: src/math/optimization_engine.cpp (calculate_deformation
+// :55-118, estimate_initial_guess :120-179, sample_warped_subset :191-269,
+// solve_icgn :272-465, evaluate_znssd :467-528, solve_simplex :531-651) and the
+// fused reductions of include/semper/simd.hpp. Subset preparation
+// (precompute_subset_fast, src/math/subset_precomputer.cpp:159-265) is the
+// kernel prologue.
+//
+// Layout: lane l owns subset pixels i = l + 64 j (j < PPL), row-major over the
+// subset, and keeps their normalised reference value and scaled gradients in
+// registers for the whole solve. Each step warps, guards and interpolates the
+// lane's pixels (the deformed image is read through L1/L2: its bytes are shared
+// by ~(d/step)^2 neighbouring subsets) and forms the ZNSSD statistics with
+// butterfly reductions. The 6x6 Gauss-Newton step and the warp composition run
+// redundantly in every lane on wave-uniform values, so control flow never
+// diverges.
+//
+// IC-GN iterations and Nelder-Mead evaluations share one sampling site: the
+// solve is a uniform state machine (IC-GN run, Simplex, IC-GN rescue run) that
+// hands one warp per step to a single evaluator. The rarely used state (the
+// Simplex vertices, the 6x6 inverse of a masked iteration) lives in LDS.
+//
+// Arithmetic follows the reference operation by operation (multiply then add,
+// no contraction; the warp position summed in double), so a sample and its
+// interpolated value equal the CPU restatement's; only the order of the
+// reductions differs.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "solver.hpp"
+#include "wave.hpp"
+
+#pragma clang fp contract(off)
+
+namespace sdic {
+
+namespace {
+
+// Keys a = -0.5 weights (image_processor.cpp:188-195)
+__device__ __forceinline__ void keys4(float s, float* w) {
+    const float s2 = s * s;
+    const float s3 = s2 * s;
+    w[0] = -0.5f * s3 + s2 - 0.5f * s;
+    w[1] = 1.5f * s3 - 2.5f * s2 + 1.0f;
+    w[2] = -1.5f * s3 + 2.0f * s2 + 0.5f * s;
+    w[3] = 0.5f * s3 - 0.5f * s2;
+}
+// Keys 4th-order 6x6 kernel pieces (image_processor.cpp:71-82)
+__device__ __forceinline__ float k6f0(float s) {
+    return (4.0f / 3.0f) * s * s * s - (7.0f / 3.0f) * s * s + 1.0f;
+}
+__device__ __forceinline__ float k6f1(float s) {
+    return -(7.0f / 12.0f) * s * s * s + 3.0f * s * s - (59.0f / 12.0f) * s + 2.5f;
+}
+__device__ __forceinline__ float k6f2(float s) {
+    return (1.0f / 12.0f) * s * s * s - (2.0f / 3.0f) * s * s + (7.0f / 4.0f) * s - 1.5f;
+}
+
+// Interpolation inside the 4 px guard, where neither kernel demotes to
+// bilinear (the guard keeps x in [3.5, w - 4.5)).
+template <bool K6, typename TD>
+__device__ __forceinline__ float interp(const TD* __restrict__ img, int w, float x, float y) {
+    const int xi = (int)x, yi = (int)y;
+    const float dx = x - (float)xi, dy = y - (float)yi;
+    if constexpr (!K6) {
+        float wx[4], wy[4];
+        keys4(dx, wx);
+        keys4(dy, wy);
+        float val = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const TD* r = img + (yi + j - 1) * w + xi;
+            float rv = 0.0f;
+            rv += (float)r[-1] * wx[0];
+            rv += (float)r[0] * wx[1];
+            rv += (float)r[1] * wx[2];
+            rv += (float)r[2] * wx[3];
+            val += rv * wy[j];
+        }
+        return val;
+    } else {
+        const float wx[6] = {k6f2(dx + 2.0f), k6f1(dx + 1.0f), k6f0(dx),
+                             k6f0(1.0f - dx), k6f1(2.0f - dx), k6f2(3.0f - dx)};
+        const float wy[6] = {k6f2(dy + 2.0f), k6f1(dy + 1.0f), k6f0(dy),
+                             k6f0(1.0f - dy), k6f1(2.0f - dy), k6f2(3.0f - dy)};
+        float val = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            const TD* r = img + (yi + j - 2) * w + xi;
+            float rv = 0.0f;
+            rv += (float)r[-2] * wx[0];
+            rv += (float)r[-1] * wx[1];
+            rv += (float)r[0] * wx[2];
+            rv += (float)r[1] * wx[3];
+            rv += (float)r[2] * wx[4];
+            rv += (float)r[3] * wx[5];
+            val += rv * wy[j];
+        }
+        return val;
+    }
+}
+
+__device__ __forceinline__ bool inside_guard(float x, float y, int w, int h) {
+    const float rx = x + 0.5f, ry = y + 0.5f;
+    return rx >= 4.0f && rx < (float)(w - 4) && ry >= 4.0f && ry < (float)(h - 4);
+}
+
+// optimization_engine.cpp:19-21
+__device__ __forceinline__ float warped(int c, float a, float b, float t, float x, float y) {
+    return (float)((double)c + ((double)a * (double)x + (double)b * (double)y + (double)t));
+}
+
+struct Res {
+    float p[6];
+    float corr;
+    int status;
+    int iters;
+};
+
+struct WaveLds {
+    float H0[36];     // the pool's step matrix
+    float H[36];      // the step matrix of a masked iteration
+    float Hd[36];     // masked Hessian
+    float work[144];  // well_conditioned / Gauss-Jordan scratch
+    float P[7 * 6];   // Simplex vertices
+    float Y[7];
+    int idx[7];
+    int flag;
+};
+
+enum Phase : int { PH_ICGN = 0, PH_SIMPLEX = 1, PH_DONE = 2 };
+enum SStage : int { S_INIT = 0, S_REFLECT = 1, S_EXPAND = 2, S_CONTRACT = 3, S_SHRINK = 4 };
+
+template <int PPL, bool K6, typename TD>
+__device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __restrict__ pool,
+                                          const float* __restrict__ refI,
+                                          const float* __restrict__ refGx,
+                                          const float* __restrict__ refGy, const TD* __restrict__ def,
+                                          int w, int h, int dim, bool lm, float conv_tol,
+                                          WaveLds& L, SolveOut& out) {
+    const int lane = threadIdx.x & 63;
+    const PoolEntry* pe = pool + t.pool;
+    out.init = 0;
+    out.simplex = 0;
+    if (!pe->valid) return;
+    const int half = dim / 2;
+    if (t.cx - half < 0 || t.cx + half >= w || t.cy - half < 0 || t.cy + half >= h) return;
+    const int n = dim * dim;
+
+    // ---- prologue: the subset in registers --------------------------------
+    float nref[PPL], gxs[PPL], gys[PPL], val[PPL];
+    uint64_t refok = 0, inmask = 0, ok = 0;
+    const float mean_r = pe->mean, inv_r = pe->inv_std;
+    // (q, r) = divmod(lane + 64 j, dim), advanced by divmod(64, dim) per slot
+    const int q64 = 64 / dim, r64 = 64 % dim;
+    const int q0 = lane / dim, r0 = lane % dim;
+    int bad = 0;
+    {
+        int q = q0, r = r0;
+#pragma unroll
+        for (int j = 0; j < PPL; ++j) {
+            nref[j] = 0.0f;
+            gxs[j] = 0.0f;
+            gys[j] = 0.0f;
+            val[j] = 0.0f;
+            if (lane + 64 * j < n) {
+                const int a = (t.cy + q - half) * w + t.cx + r - half;
+                const float ri = refI[a];
+                nref[j] = (ri - mean_r) * inv_r;
+                gxs[j] = refGx[a] * inv_r;
+                gys[j] = refGy[a] * inv_r;
+                inmask |= (1ull << j);
+                if (!(ri < kGhostWallThreshold)) refok |= (1ull << j);
+                else bad++;
+            }
+            r += r64;
+            q += q64;
+            if (r >= dim) {
+                r -= dim;
+                q++;
+            }
+        }
+    }
+    const int invalid_ref = wave_sum_i(bad);
+    if (lane < 36) L.H0[lane] = pe->hstep[lane];
+    __syncthreads();
+    const int min_valid = (int)((float)n * kMinValidPixelFraction);
+
+    // ---- coarse search (estimate_initial_guess, :120-179) ----------------
+    float start[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) start[k] = t.g[k];
+    if (t.mode == AUTO_SEARCH && t.g[0] == 0.0f && t.g[1] == 0.0f) {
+        constexpr int SPL = 7;   // ceil(ceil(3969 / 9) / 64)
+        float rr[SPL];
+        int sxo[SPL], syo[SPL];
+        uint32_t has = 0;
+        const int sampled = (n + 8) / 9;
+#pragma unroll
+        for (int k = 0; k < SPL; ++k) {
+            const int i = 9 * (lane + 64 * k);
+            rr[k] = 0.0f;
+            sxo[k] = syo[k] = 0;
+            if (i < n) {
+                sxo[k] = i % dim - half;
+                syo[k] = i / dim - half;
+                rr[k] = refI[(t.cy + syo[k]) * w + t.cx + sxo[k]];
+                if (!(rr[k] < kGhostWallThreshold)) has |= (1u << k);
+            }
+        }
+        float best = 1e20f, bu = 0.0f, bv = 0.0f;
+        for (int s = 0; s < 256; ++s) {
+            const int u = -15 + 2 * (s & 15), v = -15 + 2 * (s >> 4);
+            float dd[SPL];
+            uint32_t use = 0;
+            float smr = 0.0f, smd = 0.0f;
+            int m = 0;
+#pragma unroll
+            for (int k = 0; k < SPL; ++k) {
+                dd[k] = 0.0f;
+                if (!((has >> k) & 1)) continue;
+                const float x = (float)(t.cx + sxo[k]) + (float)u;
+                const float y = (float)(t.cy + syo[k]) + (float)v;
+                if (!inside_guard(x, y, w, h)) continue;
+                dd[k] = interp<K6>(def, w, x, y);
+                use |= (1u << k);
+                smr += rr[k];
+                smd += dd[k];
+                m++;
+            }
+            m = wave_sum_i(m);
+            if (m < 3 || 2 * m < sampled) continue;
+            const float fm = (float)m;
+            const float mr = wave_sum(smr) / fm, md = wave_sum(smd) / fm;
+            float vr = 0.0f, vd = 0.0f;
+#pragma unroll
+            for (int k = 0; k < SPL; ++k)
+                if ((use >> k) & 1) {
+                    vr += (rr[k] - mr) * (rr[k] - mr);
+                    vd += (dd[k] - md) * (dd[k] - md);
+                }
+            vr = wave_sum(vr);
+            vd = wave_sum(vd);
+            const float sr = sqrtf(vr / fm), sdv = sqrtf(vd / fm);
+            if (!(sr > 1e-5f) || !(sdv > 1e-5f)) continue;
+            float sc = 0.0f;
+#pragma unroll
+            for (int k = 0; k < SPL; ++k)
+                if ((use >> k) & 1) {
+                    const float e = (rr[k] - mr) / sr - (dd[k] - md) / sdv;
+                    sc += e * e;
+                }
+            sc = wave_sum(sc) / fm;
+            if (sc < best) {
+                best = sc;
+                bu = (float)u;
+                bv = (float)v;
+            }
+        }
+        start[0] = bu;
+        start[1] = bv;
+    }
+
+    // ---- the solve as a uniform state machine ------------------------------
+    int phase, D = 0, np = 0, s_stage = 0, s_k = 0, s_iter = 0, s_best = 0, s_worst = 0;
+    int run = 0;            // 0: first IC-GN run; 1: IC-GN after a rescue
+    int simplex = 0;
+    float W[9];
+    int it = 0;
+    float final_score = 1.0f;
+    float q[6];             // point under evaluation (Simplex)
+    float pb[6], pr[6], pe_[6], pw[6];
+    float yr = 0.0f, yw = 0.0f, y0 = 0.0f, y2 = 0.0f;
+    Res first = {}, res = {};
+    int trig = 0;
+
+    auto begin_icgn = [&](const float* g) {
+        W[0] = 1.0f + g[2];
+        W[1] = g[3];
+        W[2] = g[0];
+        W[3] = g[4];
+        W[4] = 1.0f + g[5];
+        W[5] = g[1];
+        W[6] = 0.0f;
+        W[7] = 0.0f;
+        W[8] = 1.0f;
+        it = 0;
+        final_score = 1.0f;
+        phase = PH_ICGN;
+    };
+    auto begin_simplex = [&](const float* s0, int dims) {
+        D = dims;
+        np = dims + 1;
+        for (int k = 0; k < 42; ++k) L.P[k] = 0.0f;
+        for (int j = 0; j < 6; ++j) L.P[j] = (j < D) ? s0[j] : 0.0f;
+        s_stage = S_INIT;
+        s_k = 0;
+        s_iter = 0;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) q[j] = L.P[j];
+        phase = PH_SIMPLEX;
+        simplex = 1;
+    };
+
+    if (t.mode == AUTO_SEARCH) {
+        begin_simplex(start, 2);
+    } else {
+        begin_icgn(start);
+    }
+
+    // stable ascending order of the vertex scores: rank of each vertex
+    auto simplex_iter_begin = [&]() -> bool {
+        int* idx = L.idx;
+        for (int i = 0; i < np; ++i) {
+            int r = 0;
+            for (int k = 0; k < np; ++k) r += (L.Y[k] < L.Y[i]) || (L.Y[k] == L.Y[i] && k < i);
+            idx[r] = i;
+        }
+        s_best = idx[0];
+        s_worst = idx[np - 1];
+        y0 = L.Y[idx[0]];
+        yw = L.Y[s_worst];
+        y2 = L.Y[idx[np - 2]];
+        if (fabsf(y0 - yw) < 1e-5f) return false;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) pb[j] = 0.0f;
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int j = 0; j < 6; ++j)
+                if (j < D) pb[j] += L.P[idx[i] * 6 + j];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            if (j < D) pb[j] /= (float)D;
+            pw[j] = L.P[s_worst * 6 + j];
+            pr[j] = (j < D) ? pb[j] + 1.0f * (pb[j] - pw[j]) : 0.0f;
+            q[j] = pr[j];
+        }
+        s_stage = S_REFLECT;
+        return true;
+    };
+    auto replace_worst = [&](const float* pt, float yv) {
+#pragma unroll
+        for (int j = 0; j < 6; ++j)
+            if (j < D) L.P[s_worst * 6 + j] = pt[j];
+        L.Y[s_worst] = yv;
+    };
+    // Simplex done: best vertex, then the IC-GN run it seeds.
+    auto finish_simplex = [&]() {
+        int best = 0;
+        for (int k = 1; k < np; ++k)
+            if (L.Y[k] < L.Y[best]) best = k;
+        float p[6];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) p[j] = (j < D) ? L.P[best * 6 + j] : 0.0f;
+        begin_icgn(p);
+    };
+    // one IC-GN run ended with r
+    auto finish_icgn = [&](const Res& r) {
+        if (run == 0) {
+            res = r;
+            trig = r.iters;
+            const bool rescue = t.mode == NO_SEARCH && (r.status != 0 || r.corr > kCorrSimplexTrigger);
+            if (rescue) {
+                first = r;
+                run = 1;
+                begin_simplex(t.g, 6);
+            } else {
+                phase = PH_DONE;
+            }
+        } else {
+            const bool first_better = first.status == 0 && (r.status != 0 || first.corr < r.corr);
+            res = first_better ? first : r;
+            phase = PH_DONE;
+        }
+    };
+    auto icgn_result = [&](int status, float score, int iters) {
+        Res r;
+        r.p[0] = W[2];
+        r.p[1] = W[5];
+        r.p[2] = W[0] - 1.0f;
+        r.p[3] = W[1];
+        r.p[4] = W[3];
+        r.p[5] = W[4] - 1.0f;
+        r.corr = score;
+        r.status = status;
+        r.iters = iters;
+        return r;
+    };
+
+    while (phase != PH_DONE) {
+        const bool icgn = phase == PH_ICGN;
+        float Wc[6];
+        if (icgn) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) Wc[k] = W[k];
+        } else {
+            Wc[0] = 1.0f + q[2];
+            Wc[1] = q[3];
+            Wc[2] = q[0];
+            Wc[3] = q[4];
+            Wc[4] = 1.0f + q[5];
+            Wc[5] = q[1];
+        }
+
+        // ---- the single sampling site -------------------------------------
+        float sum = 0.0f;
+        int cnt = 0;
+        ok = 0;
+        {
+            // The slot coordinates are recomputed every step: left loop-invariant,
+            // the compiler hoists per-pixel products and doubles out of the solve
+            // loop and the register file, not the ALUs, becomes the limit.
+            int qq = q0, rq = r0;
+            asm volatile("" : "+v"(qq), "+v"(rq));
+#pragma unroll
+            for (int j = 0; j < PPL; ++j) {
+                val[j] = 0.0f;
+                if ((inmask >> j) & 1) {
+                    const float x = (float)(rq - half), y = (float)(qq - half);
+                    const float fx = warped(t.cx, Wc[0], Wc[1], Wc[2], x, y);
+                    const float fy = warped(t.cy, Wc[3], Wc[4], Wc[5], x, y);
+                    if (inside_guard(fx, fy, w, h)) {
+                        const float v = interp<K6>(def, w, fx, fy);
+                        val[j] = v;
+                        if ((refok >> j) & 1) {
+                            ok |= (1ull << j);
+                            sum += v;
+                            cnt++;
+                        }
+                    }
+                }
+                rq += r64;
+                qq += q64;
+                if (rq >= dim) {
+                    rq -= dim;
+                    qq++;
+                }
+            }
+        }
+        sum = wave_sum(sum);
+        const int nv = wave_sum_i(cnt);
+
+        if (!icgn) {
+            // ---- evaluate_znssd (:467-528) ---------------------------------
+            float score = 2.0f;
+            if (nv != 0 && nv >= min_valid) {
+                const float mean = sum / (float)nv;
+                float s2 = 0.0f;
+#pragma unroll
+                for (int j = 0; j < PPL; ++j)
+                    if ((ok >> j) & 1) {
+                        const float d = val[j] - mean;
+                        s2 += d * d;
+                    }
+                float dstd = sqrtf(wave_sum(s2) / (float)nv);
+                if (dstd < 1e-5f) dstd = 1.0f;
+                const float inv = 1.0f / dstd;
+                const bool full = nv == n;
+                float z = 0.0f;
+#pragma unroll
+                for (int j = 0; j < PPL; ++j)
+                    if ((ok >> j) & 1) {
+                        const float nd = full ? (val[j] - mean) * inv : (val[j] - mean) / dstd;
+                        const float d = nref[j] - nd;
+                        z += d * d;
+                    }
+                score = wave_sum(z) / (float)nv;
+                if (!finite_f(score)) score = 2.0f;
+            }
+            // ---- Nelder-Mead step (:531-651) --------------------------------
+            bool next_iter = false;
+            if (s_stage == S_INIT) {
+                L.Y[s_k] = score;
+                s_k++;
+                if (s_k < np) {
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) {
+                        float v = L.P[j];
+                        if (j == s_k - 1) v += (j < 2) ? 2.0f : 0.01f;
+                        L.P[s_k * 6 + j] = (j < D) ? v : 0.0f;
+                        q[j] = L.P[s_k * 6 + j];
+                    }
+                } else {
+                    if (!simplex_iter_begin()) finish_simplex();
+                }
+            } else if (s_stage == S_REFLECT) {
+                yr = score;
+                if (y0 <= yr && yr < y2) {
+                    replace_worst(pr, yr);
+                    next_iter = true;
+                } else if (yr < y0) {
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) {
+                        pe_[j] = (j < D) ? pb[j] + 2.0f * (pr[j] - pb[j]) : 0.0f;
+                        q[j] = pe_[j];
+                    }
+                    s_stage = S_EXPAND;
+                } else {
+                    const bool outside = yr < yw;
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) {
+                        pe_[j] = (j < D) ? pb[j] + 0.5f * ((outside ? pr[j] : pw[j]) - pb[j]) : 0.0f;
+                        q[j] = pe_[j];
+                    }
+                    s_stage = S_CONTRACT;
+                }
+            } else if (s_stage == S_EXPAND) {
+                if (score < yr) replace_worst(pe_, score);
+                else replace_worst(pr, yr);
+                next_iter = true;
+            } else if (s_stage == S_CONTRACT) {
+                if (score < fminf(yr, yw)) {
+                    replace_worst(pe_, score);
+                    next_iter = true;
+                } else {
+                    // shrink every vertex but the best towards it, one evaluation each
+                    s_k = (s_best == 0) ? 1 : 0;
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) {
+                        const float b = L.P[s_best * 6 + j];
+                        if (j < D) L.P[s_k * 6 + j] = b + 0.5f * (L.P[s_k * 6 + j] - b);
+                        q[j] = L.P[s_k * 6 + j];
+                    }
+                    s_stage = S_SHRINK;
+                }
+            } else {  // S_SHRINK
+                L.Y[s_k] = score;
+                int nk = s_k + 1;
+                if (nk == s_best) nk++;
+                if (nk < np) {
+                    s_k = nk;
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) {
+                        const float b = L.P[s_best * 6 + j];
+                        if (j < D) L.P[s_k * 6 + j] = b + 0.5f * (L.P[s_k * 6 + j] - b);
+                        q[j] = L.P[s_k * 6 + j];
+                    }
+                } else {
+                    next_iter = true;
+                }
+            }
+            if (next_iter) {
+                s_iter++;
+                if (s_iter >= 100 || !simplex_iter_begin()) finish_simplex();
+            }
+            continue;
+        }
+
+        // ---- IC-GN iteration (solve_icgn, :332-462) ---------------------------
+        if (nv < min_valid) {
+            finish_icgn(icgn_result(1, 2.0f, it));
+            continue;
+        }
+        const float mean = sum / (float)nv;
+        float s2 = 0.0f;
+#pragma unroll
+        for (int j = 0; j < PPL; ++j)
+            if ((ok >> j) & 1) {
+                const float d = val[j] - mean;
+                s2 += d * d;
+            }
+        float dstd = sqrtf(wave_sum(s2) / (float)nv);
+        if (dstd < 1e-5f) dstd = 1.0f;
+        const float inv = 1.0f / dstd;
+        float err = 0.0f, dp[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        const bool masked = nv != n;
+        float hh[21];
+#pragma unroll
+        for (int k = 0; k < 21; ++k) hh[k] = 0.0f;
+        {
+            int qq = q0, rq = r0;
+            asm volatile("" : "+v"(qq), "+v"(rq));
+#pragma unroll
+            for (int j = 0; j < PPL; ++j) {
+                if ((ok >> j) & 1) {
+                    const float x = (float)(rq - half), y = (float)(qq - half);
+                    const float nd = (val[j] - mean) * inv;
+                    const float d = nref[j] - nd;
+                    const float s6[6] = {gxs[j], gys[j], gxs[j] * x, gxs[j] * y, gys[j] * x, gys[j] * y};
+                    err += d * d;
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) dp[k] += s6[k] * d;
+                    if (masked) {
+                        int k = 0;
+#pragma unroll
+                        for (int a = 0; a < 6; ++a)
+#pragma unroll
+                            for (int b = a; b < 6; ++b) hh[k++] += s6[a] * s6[b];
+                    }
+                }
+                rq += r64;
+                qq += q64;
+                if (rq >= dim) {
+                    rq -= dim;
+                    qq++;
+                }
+            }
+        }
+        err = wave_sum(err);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) dp[k] = wave_sum(dp[k]);
+        const float* Hs = L.H0;
+        if (masked) {
+#pragma unroll
+            for (int k = 0; k < 21; ++k) hh[k] = wave_sum(hh[k]);
+            // masked Hessian (:373-413): 2x2 test, LM, rcond, inverse -- lane 0, in LDS
+            if (lane == 0) {
+                int k = 0;
+                for (int a = 0; a < 6; ++a)
+                    for (int b = a; b < 6; ++b) {
+                        L.Hd[a * 6 + b] = hh[k];
+                        L.Hd[b * 6 + a] = hh[k];
+                        ++k;
+                    }
+                const float* Hd = L.Hd;
+                const float det2 = Hd[0] * Hd[7] - Hd[6] * Hd[1];
+                const float nrm2 = Hd[0] * Hd[0] + Hd[1] * Hd[1] + Hd[6] * Hd[6] + Hd[7] * Hd[7];
+                const float cond2 = (fabsf(det2) > 1e-12f) ? (nrm2 / fabsf(det2)) : 1.0e13f;
+                int flag = 0;
+                if (cond2 > 1.0e12f) {
+                    flag = 1;
+                } else {
+                    if (lm) {
+                        L.Hd[0] += kLmAlpha;
+                        L.Hd[7] += kLmAlpha;
+                    }
+                    if (!well_conditioned_lds(L.Hd, half, L.work)) flag = 1;
+                    else gj_inverse_lds(L.Hd, L.H, L.work);
+                }
+                L.flag = flag;
+            }
+            __syncthreads();
+            if (L.flag) {
+                finish_icgn(icgn_result(1, 2.0f, it));
+                __syncthreads();
+                continue;
+            }
+            Hs = L.H;
+        }
+        final_score = err / (float)nv;
+        float d[6];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) acc += Hs[r * 6 + c] * dp[c];
+            d[r] = -acc;
+        }
+        if (masked) __syncthreads();
+        const float dW[9] = {1.0f + d[2], d[3], d[0], d[4], 1.0f + d[5], d[1], 0.0f, 0.0f, 1.0f};
+        const float det = dW[0] * dW[4] - dW[1] * dW[3];
+        bool fin = true;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) fin = fin && finite_f(d[k]);
+        if (!fin || !(fabsf(det) > 1e-6f)) {
+            finish_icgn(icgn_result(1, 2.0f, it + 1));
+            continue;
+        }
+        float dWi[9], Wn[9];
+        inv3(dW, dWi);
+        mul3(W, dWi, Wn);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) fin = fin && finite_f(Wn[k]);
+        if (!fin) {
+            finish_icgn(icgn_result(1, 2.0f, it + 1));
+            continue;
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) W[k] = Wn[k];
+        float nrm = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) nrm += d[k] * d[k];
+        if (sqrtf(nrm) < conv_tol) {
+            // texture check over the surviving pixels (:436-459)
+            float sx = 0.0f, sy = 0.0f;
+#pragma unroll
+            for (int j = 0; j < PPL; ++j)
+                if ((ok >> j) & 1) {
+                    float a = gxs[j], b = gys[j];
+                    asm volatile("" : "+v"(a), "+v"(b));
+                    sx += a * a;
+                    sy += b * b;
+                }
+            sx = wave_sum(sx);
+            sy = wave_sum(sy);
+            if (fminf(sx, sy) <= 0.0f || !finite_f(final_score))
+                finish_icgn(icgn_result(1, 2.0f, it + 1));
+            else
+                finish_icgn(icgn_result(0, final_score, it + 1));
+            continue;
+        }
+        it++;
+        if (it >= kIcgnMaxIter) finish_icgn(icgn_result(1, final_score, kIcgnMaxIter));
+    }
+
+    if (lm && (float)invalid_ref / (float)(dim * dim) > kGhostRejectFraction) {
+        res.corr = 2.0f;
+        res.status = 1;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) out.p[k] = res.p[k];
+    out.corr = res.corr;
+    out.status = res.status;
+    out.iters = res.iters;
+    out.trigger_iters = trig;
+    out.simplex = simplex;
+    out.init = 1;
+    out.invalid_ref = invalid_ref;
+}
+
+
+// The common case, lean: IC-GN from the guess on a subset whose every pixel
+// samples inside the image and off the Ghost Wall, with the pool's step matrix
+// (no masked Hessian, no Simplex, no coarse search). Returns false -- the work
+// item goes to the full solver, which starts it again from scratch -- as soon
+// as a masked iteration, a Simplex rescue or a coarse search would be needed,
+// so the committed results are exactly full_solve's.
+template <int PPL, bool K6, typename TD>
+__device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __restrict__ pool,
+                                           const float* __restrict__ refI,
+                                           const float* __restrict__ refGx,
+                                           const float* __restrict__ refGy,
+                                           const TD* __restrict__ def, int w, int h, int dim,
+                                           bool lm, float conv_tol, float* Hs, SolveOut& out) {
+    const int lane = threadIdx.x & 63;
+    const PoolEntry* pe = pool + t.pool;
+    out.init = 0;
+    out.simplex = 0;
+    if (!pe->valid) return true;
+    const int half = dim / 2;
+    if (t.cx - half < 0 || t.cx + half >= w || t.cy - half < 0 || t.cy + half >= h) return true;
+    if (t.mode == AUTO_SEARCH) return false;
+    const int n = dim * dim;
+    float nref[PPL], gxs[PPL], gys[PPL], val[PPL];
+    uint64_t inmask = 0;
+    const float mean_r = pe->mean, inv_r = pe->inv_std;
+    const int q64 = 64 / dim, r64 = 64 % dim;
+    const int q0 = lane / dim, r0 = lane % dim;
+    int bad = 0;
+    {
+        int q = q0, r = r0;
+#pragma unroll
+        for (int j = 0; j < PPL; ++j) {
+            nref[j] = 0.0f;
+            gxs[j] = 0.0f;
+            gys[j] = 0.0f;
+            val[j] = 0.0f;
+            if (lane + 64 * j < n) {
+                const int a = (t.cy + q - half) * w + t.cx + r - half;
+                const float ri = refI[a];
+                nref[j] = (ri - mean_r) * inv_r;
+                gxs[j] = refGx[a] * inv_r;
+                gys[j] = refGy[a] * inv_r;
+                inmask |= (1ull << j);
+                bad += (ri < kGhostWallThreshold) ? 1 : 0;
+            }
+            r += r64;
+            q += q64;
+            if (r >= dim) {
+                r -= dim;
+                q++;
+            }
+        }
+    }
+    // Any Ghost Wall pixel makes every iteration a masked one.
+    if (wave_sum_i(bad) != 0) return false;
+    if (lane < 36) Hs[lane] = pe->hstep[lane];
+    __syncthreads();
+    const int min_valid = (int)((float)n * kMinValidPixelFraction);
+    float W[9] = {1.0f + t.g[2], t.g[3], t.g[0], t.g[4], 1.0f + t.g[5], t.g[1], 0.0f, 0.0f, 1.0f};
+    float final_score = 1.0f;
+    int status = 1, iters = kIcgnMaxIter;
+    float corr = 0.0f;
+    bool done = false;
+    for (int it = 0; it < kIcgnMaxIter && !done; ++it) {
+        float sum = 0.0f;
+        int cnt = 0;
+        bool all = true;
+        {
+            int qq = q0, rq = r0;
+            asm volatile("" : "+v"(qq), "+v"(rq));
+#pragma unroll
+            for (int j = 0; j < PPL; ++j) {
+                val[j] = 0.0f;
+                if ((inmask >> j) & 1) {
+                    const float x = (float)(rq - half), y = (float)(qq - half);
+                    const float fx = warped(t.cx, W[0], W[1], W[2], x, y);
+                    const float fy = warped(t.cy, W[3], W[4], W[5], x, y);
+                    if (inside_guard(fx, fy, w, h)) {
+                        const float v = interp<K6>(def, w, fx, fy);
+                        val[j] = v;
+                        sum += v;
+                        cnt++;
+                    } else {
+                        all = false;
+                    }
+                }
+                rq += r64;
+                qq += q64;
+                if (rq >= dim) {
+                    rq -= dim;
+                    qq++;
+                }
+            }
+        }
+        sum = wave_sum(sum);
+        const int nv = wave_sum_i(cnt);
+        if (nv < min_valid) {
+            status = 1;
+            corr = 2.0f;
+            iters = it;
+            done = true;
+            break;
+        }
+        if (nv != n) return false;   // a masked iteration
+        (void)all;
+        const float mean = sum / (float)nv;
+        float s2 = 0.0f;
+#pragma unroll
+        for (int j = 0; j < PPL; ++j) {
+            const float d = val[j] - mean;
+            s2 += ((inmask >> j) & 1) ? d * d : 0.0f;
+        }
+        float dstd = sqrtf(wave_sum(s2) / (float)nv);
+        if (dstd < 1e-5f) dstd = 1.0f;
+        const float inv = 1.0f / dstd;
+        float err = 0.0f, dp[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        {
+            int qq = q0, rq = r0;
+            asm volatile("" : "+v"(qq), "+v"(rq));
+#pragma unroll
+            for (int j = 0; j < PPL; ++j) {
+                if ((inmask >> j) & 1) {
+                    const float x = (float)(rq - half), y = (float)(qq - half);
+                    const float nd = (val[j] - mean) * inv;
+                    const float d = nref[j] - nd;
+                    err += d * d;
+                    dp[0] += gxs[j] * d;
+                    dp[1] += gys[j] * d;
+                    dp[2] += (gxs[j] * x) * d;
+                    dp[3] += (gxs[j] * y) * d;
+                    dp[4] += (gys[j] * x) * d;
+                    dp[5] += (gys[j] * y) * d;
+                }
+                rq += r64;
+                qq += q64;
+                if (rq >= dim) {
+                    rq -= dim;
+                    qq++;
+                }
+            }
+        }
+        err = wave_sum(err);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) dp[k] = wave_sum(dp[k]);
+        final_score = err / (float)nv;
+        float d[6];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) acc += Hs[r * 6 + c] * dp[c];
+            d[r] = -acc;
+        }
+        const float dW[9] = {1.0f + d[2], d[3], d[0], d[4], 1.0f + d[5], d[1], 0.0f, 0.0f, 1.0f};
+        const float det = dW[0] * dW[4] - dW[1] * dW[3];
+        bool fin = true;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) fin = fin && finite_f(d[k]);
+        if (!fin || !(fabsf(det) > 1e-6f)) {
+            status = 1;
+            corr = 2.0f;
+            iters = it + 1;
+            done = true;
+            break;
+        }
+        float dWi[9], Wn[9];
+        inv3(dW, dWi);
+        mul3(W, dWi, Wn);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) fin = fin && finite_f(Wn[k]);
+        if (!fin) {
+            status = 1;
+            corr = 2.0f;
+            iters = it + 1;
+            done = true;
+            break;
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) W[k] = Wn[k];
+        float nrm = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) nrm += d[k] * d[k];
+        if (sqrtf(nrm) < conv_tol) {
+            float sx = 0.0f, sy = 0.0f;
+#pragma unroll
+            for (int j = 0; j < PPL; ++j) {
+                float a = gxs[j], b = gys[j];
+                asm volatile("" : "+v"(a), "+v"(b));
+                sx += a * a;
+                sy += b * b;
+            }
+            sx = wave_sum(sx);
+            sy = wave_sum(sy);
+            iters = it + 1;
+            if (fminf(sx, sy) <= 0.0f || !finite_f(final_score)) {
+                status = 1;
+                corr = 2.0f;
+            } else {
+                status = 0;
+                corr = final_score;
+            }
+            done = true;
+            break;
+        }
+    }
+    if (!done) {
+        status = 1;
+        corr = final_score;
+        iters = kIcgnMaxIter;
+    }
+    // the rescue this mode allows is the full solver's business
+    if (t.mode == NO_SEARCH && (status != 0 || corr > kCorrSimplexTrigger)) return false;
+    out.p[0] = W[2];
+    out.p[1] = W[5];
+    out.p[2] = W[0] - 1.0f;
+    out.p[3] = W[1];
+    out.p[4] = W[3];
+    out.p[5] = W[4] - 1.0f;
+    out.corr = corr;
+    out.status = status;
+    out.iters = iters;
+    out.trigger_iters = iters;
+    out.simplex = 0;
+    out.init = 1;
+    out.invalid_ref = 0;
+    return true;
+}
+
+}  // namespace
+
+// Work item k of a launch: a task (task mode, a.tasks != null) or a grid node
+// (Path A, full_field_path_a.cpp:28-114), through the optional index list.
+__device__ __forceinline__ bool item_task(const SolveArgs& a, int id, Task& t) {
+    if (a.tasks) {
+        t = a.tasks[id];
+        return true;
+    }
+    const GridArgs& g = a.grid;
+    if (!g.in_mesh[id] || g.grid.solved[id]) return false;
+    t.cx = g.rect_x + (id % g.gw) * g.step;
+    t.cy = g.rect_y + (id / g.gw) * g.step;
+    t.pool = id;
+    const int tag = g.grid.tag[id];
+    t.mode = (tag == kMeshExtrapolated || tag == kMeshHomography) ? NO_SIMPLEX : NO_SEARCH;
+    t.g[0] = g.gu[id];
+    t.g[1] = g.gv[id];
+    t.g[2] = g.gux[id];
+    t.g[3] = g.guy[id];
+    t.g[4] = g.gvx[id];
+    t.g[5] = g.gvy[id];
+    t.node = id;
+    return true;
+}
+
+// Lane 0 commits a work item: the result slot in task mode; in grid mode the
+// accepted node into the grid, and the Path A counters (points, iterations,
+// Simplex calls / saved / dead) with integer atomics.
+__device__ __forceinline__ void commit(const SolveArgs& a, int id, const SolveOut& o) {
+    if (a.tasks) {
+        a.out[id] = o;
+        return;
+    }
+    if (!o.init) return;
+    GridDev gd = a.grid.grid;
+    const bool acc = o.status == 0 && o.corr <= kCorrAccept;
+    atomicAdd(&a.stats[1], o.iters);
+    if (o.simplex) {
+        atomicAdd(&a.stats[2], 1);
+        atomicAdd(acc ? &a.stats[3] : &a.stats[4], 1);
+    }
+    gd.iters[id] = o.iters;
+    gd.simplex[id] = o.simplex ? 1 : 0;
+    if (acc) {
+        gd.u[id] = o.p[0];
+        gd.v[id] = o.p[1];
+        gd.ux[id] = o.p[2];
+        gd.uy[id] = o.p[3];
+        gd.vx[id] = o.p[4];
+        gd.vy[id] = o.p[5];
+        gd.corr[id] = o.corr;
+        gd.solved[id] = 1;
+        atomicAdd(&a.stats[0], 1);
+    } else {
+        gd.corr[id] = kCorrInvalid;
+    }
+}
+
+// Fast path, one wave per work item; items it cannot finish are appended to
+// a.bail for k_slow.
+template <int PPL, bool K6, typename TD>
+__global__ __launch_bounds__(64) void k_fast(SolveArgs a) {
+    __shared__ float Hs[36];
+    const int k = xcd_remap(blockIdx.x, gridDim.x);
+    if (k >= a.nitems) return;
+    const int id = a.items ? a.items[k] : k;
+    Task t;
+    if (!item_task(a, id, t)) return;
+    SolveOut o = {};
+    const bool fin = fast_solve<PPL, K6, TD>(t, a.pool, a.refI, a.refGx, a.refGy, (const TD*)a.def, a.w,
+                                             a.h, a.dim, a.lm != 0, a.tol, Hs, o);
+    if ((threadIdx.x & 63) != 0) return;
+    if (!fin) {
+        a.bail[atomicAdd(a.nbail, 1)] = id;
+        return;
+    }
+    commit(a, id, o);
+}
+
+// Full solver over the bail list: a fixed grid of waves, each striding over
+// the list (its length is read on the device, so no host round trip).
+template <int PPL, bool K6, typename TD>
+__global__ __launch_bounds__(64) void k_slow(SolveArgs a) {
+    __shared__ WaveLds L;
+    const int nb = *a.nbail;
+    for (int k = blockIdx.x; k < nb; k += gridDim.x) {
+        const int id = a.bail[k];
+        Task t;
+        if (!item_task(a, id, t)) continue;
+        SolveOut o = {};
+        full_solve<PPL, K6, TD>(t, a.pool, a.refI, a.refGx, a.refGy, (const TD*)a.def, a.w, a.h, a.dim,
+                                a.lm != 0, a.tol, L, o);
+        if ((threadIdx.x & 63) == 0) commit(a, id, o);
+        __syncthreads();
+    }
+}
+
+}  // namespace sdic

@@ -1,0 +1,146 @@
+// Wavefront-level helpers for CDNA4 (64 lanes): butterfly reductions whose
+// result is bit-identical in every lane (each butterfly step adds the same two
+// operands in every lane pair, and float addition is commutative), and a
+// small 6x6 / 3x3 algebra kit evaluated redundantly by all lanes on
+// wave-uniform values.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sdic {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);
+    return v;
+}
+
+__device__ __forceinline__ int wave_sum_i(int v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);
+    return v;
+}
+
+__device__ __forceinline__ bool finite_f(float f) {
+    return (__float_as_uint(f) & 0x7f800000u) != 0x7f800000u;
+}
+
+// Serial 6x6 algebra for one lane on (LDS) memory: Gauss-Jordan inverse with
+// partial pivoting -- the CPU oracle's sequence -- and hessian::well_conditioned.
+__device__ inline void gj_inverse_lds(const float* A, float* out, float* m /* 72 */) {
+    for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c) {
+            m[r * 12 + c] = A[r * 6 + c];
+            m[r * 12 + 6 + c] = (r == c) ? 1.0f : 0.0f;
+        }
+    for (int k = 0; k < 6; ++k) {
+        int piv = k;
+        float best = fabsf(m[k * 12 + k]);
+        for (int r = k + 1; r < 6; ++r)
+            if (fabsf(m[r * 12 + k]) > best) {
+                best = fabsf(m[r * 12 + k]);
+                piv = r;
+            }
+        if (piv != k)
+            for (int c = 0; c < 12; ++c) {
+                const float t = m[k * 12 + c];
+                m[k * 12 + c] = m[piv * 12 + c];
+                m[piv * 12 + c] = t;
+            }
+        const float ip = 1.0f / m[k * 12 + k];
+        for (int c = 0; c < 12; ++c) m[k * 12 + c] *= ip;
+        for (int r = 0; r < 6; ++r) {
+            if (r == k) continue;
+            const float f = m[r * 12 + k];
+            if (f == 0.0f) continue;
+            for (int c = 0; c < 12; ++c) m[r * 12 + c] -= f * m[k * 12 + c];
+        }
+    }
+    for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c) out[r * 6 + c] = m[r * 12 + 6 + c];
+}
+
+__device__ inline float norm1_lds(const float* m) {
+    float best = 0.0f;
+    for (int c = 0; c < 6; ++c) {
+        float s = 0.0f;
+        for (int r = 0; r < 6; ++r) s += fabsf(m[r * 6 + c]);
+        best = s > best ? s : best;
+    }
+    return best;
+}
+
+__device__ inline bool finite_lds(const float* m, int n) {
+    for (int i = 0; i < n; ++i)
+        if (!finite_f(m[i])) return false;
+    return true;
+}
+
+// hessian.hpp:40-52 on LDS scratch (work: 36 + 36 + 72 floats)
+__device__ inline bool well_conditioned_lds(const float* H, int half, float* work) {
+    const float s = half > 1 ? 1.0f / (float)half : 1.0f;
+    float* Hn = work;
+    float* Hi = work + 36;
+    float* gj = work + 72;
+    for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c) {
+            const float dr = r < 2 ? 1.0f : s, dc = c < 2 ? 1.0f : s;
+            Hn[r * 6 + c] = dr * H[r * 6 + c] * dc;
+        }
+    if (!finite_lds(Hn, 36)) return false;
+    gj_inverse_lds(Hn, Hi, gj);
+    if (!finite_lds(Hi, 36)) return false;
+    const float a = norm1_lds(Hn), b = norm1_lds(Hi);
+    return 1.0f / (a * b) > 1e-6f;
+}
+
+// 3x3 inverse by cofactors, determinant from the first column.
+__device__ inline void inv3(const float* a, float* r) {
+    auto cof = [&](int i, int j) {
+        const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+        return a[i1 * 3 + j1] * a[i2 * 3 + j2] - a[i1 * 3 + j2] * a[i2 * 3 + j1];
+    };
+    const float det = cof(0, 0) * a[0] + cof(1, 0) * a[3] + cof(2, 0) * a[6];
+    const float inv = 1.0f / det;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) r[i * 3 + j] = cof(j, i) * inv;
+}
+
+__device__ inline void mul3(const float* a, const float* b, float* r) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            float s = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) s += a[i * 3 + k] * b[k * 3 + j];
+            r[i * 3 + j] = s;
+        }
+}
+
+// blockIdx -> work index so that consecutive work items land on one XCD:
+// the dispatcher deals workgroups round-robin over the 8 XCDs, each with its
+// own L2, and neighbouring subsets share most of their pixels.
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
+    constexpr int kXcd = 8;
+    const int per = (nblocks + kXcd - 1) / kXcd;
+    const int xcd = bid % kXcd, local = bid / kXcd;
+    const int full = nblocks % kXcd == 0 ? kXcd : nblocks % kXcd;  // XCDs with `per` items
+    // XCDs [0, full) own `per` items each, the rest `per - 1`.
+    const int base = xcd < full ? xcd * per : full * per + (xcd - full) * (per - 1);
+    return base + local;
+}
+
+}  // namespace sdic

@@ -1,0 +1,146 @@
+"""ctypes bindings of the MI355X DIC engine.
+
+`Engine` mirrors the reference's Python binding `semper.Engine`
+(bindings/python/semper/__init__.py) over the C ABI of include/semper_c.h:
+set_reference(), run() -> (points[n, 8], metrics[24]), cancel(). `Dev` wraps the
+kernel-level interface of include/semper_dev.h used by the tests and the
+benchmark. The libraries are the in-tree builds next to this file; loading
+fails loudly when they are missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_C = os.path.join(HERE, "libsemper_c.so")
+LIB_DEV = os.path.join(HERE, "libsemper_dev.so")
+
+FLOATS_PER_POINT = 8
+METRICS_LEN = 24
+ERR_ROI, ERR_INIT, ERR_STRAIN_WINDOW, ERR_SEED, ERR_CANCELLED = -2, -3, -4, -5, -99
+EXPORTS = [
+    "semper_cancel", "semper_create", "semper_destroy", "semper_incremental_cancel",
+    "semper_incremental_create", "semper_incremental_destroy", "semper_incremental_init",
+    "semper_incremental_process", "semper_run", "semper_set_reference", "semper_version",
+]
+
+
+class Params(ctypes.Structure):
+    """semper_params: 8 x int32 (reference include/semper/semper_c.h:46-52)."""
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "rect_x", "rect_y", "rect_w", "rect_h", "step", "subset_size", "strain_window",
+        "use_6x6_interpolator")]
+
+
+class IncInfo(ctypes.Structure):
+    """semper_incremental_info: 6 x int32 (reference semper_c.h:155-162)."""
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "reference_frame", "reference_updated", "tracked_points", "frame0_points",
+        "taylor_fallbacks", "lost_this_frame")]
+
+
+PROGRESS_CB = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_void_p)
+_VP = ctypes.c_void_p
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_VP) if a is not None else None
+
+
+def _bind_c(lib):
+    lib.semper_create.restype = _VP
+    lib.semper_destroy.argtypes = [_VP]
+    lib.semper_set_reference.argtypes = [_VP, _VP, ctypes.c_size_t, _VP, ctypes.c_size_t, ctypes.c_int,
+                                         ctypes.c_int]
+    lib.semper_run.argtypes = [_VP, _VP, ctypes.c_size_t, _VP, ctypes.c_size_t, ctypes.POINTER(Params), _VP,
+                               ctypes.c_int, _VP, ctypes.c_int, PROGRESS_CB, _VP]
+    lib.semper_cancel.argtypes = [_VP]
+    lib.semper_version.restype = ctypes.c_char_p
+    lib.semper_incremental_create.restype = _VP
+    lib.semper_incremental_destroy.argtypes = [_VP]
+    lib.semper_incremental_init.argtypes = [_VP, _VP, ctypes.c_size_t, _VP, ctypes.c_size_t, ctypes.c_int,
+                                            ctypes.c_int, ctypes.POINTER(Params), ctypes.c_int32]
+    lib.semper_incremental_process.argtypes = [_VP, _VP, ctypes.c_size_t, _VP, ctypes.c_int, _VP, ctypes.c_int,
+                                               ctypes.POINTER(IncInfo), PROGRESS_CB, _VP]
+    lib.semper_incremental_cancel.argtypes = [_VP]
+    return lib
+
+
+def load_c(path=LIB_C):
+    if not os.path.exists(path):
+        raise RuntimeError(f"libsemper_c.so is not built ({path}); run __graft_entry__.build()")
+    return _bind_c(ctypes.CDLL(path))
+
+
+def load_dev(path=LIB_DEV):
+    if not os.path.exists(path):
+        raise RuntimeError(f"libsemper_dev.so is not built ({path}); run __graft_entry__.build()")
+    lib = _bind_c(ctypes.CDLL(path))
+    lib.sdk_device_count.restype = ctypes.c_int
+    lib.sdk_malloc.restype = _VP
+    lib.sdk_malloc.argtypes = [ctypes.c_size_t]
+    lib.sdk_free.argtypes = [_VP]
+    lib.sdk_memcpy.argtypes = [_VP, _VP, ctypes.c_size_t, ctypes.c_int]
+    lib.sdk_solve_nodes.argtypes = [_VP, _VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_float, ctypes.c_int, _VP, _VP, _VP, _VP, _VP]
+    lib.sdk_pool.argtypes = [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, _VP,
+                             _VP, _VP]
+    lib.sdk_ref_planes.argtypes = [_VP, _VP, ctypes.c_int, ctypes.c_int, _VP, _VP, _VP]
+    lib.sdk_engine_create.restype = _VP
+    lib.sdk_engine_destroy.argtypes = [_VP]
+    lib.sdk_engine_set_reference.argtypes = [_VP, _VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP]
+    lib.sdk_engine_run.argtypes = [_VP, _VP, ctypes.c_int, _VP, _VP, ctypes.c_int, _VP, ctypes.c_int,
+                                   ctypes.c_float]
+    lib.sdk_engine_timings.argtypes = [_VP, _VP]
+    lib.sdk_synth_pair.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, ctypes.c_int,
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_float, _VP, _VP, ctypes.c_int]
+    return lib
+
+
+def params(rect, step, subset, strain_window, use6=False):
+    p = Params()
+    p.rect_x, p.rect_y, p.rect_w, p.rect_h = [int(v) for v in rect]
+    p.step, p.subset_size, p.strain_window = int(step), int(subset), int(strain_window)
+    p.use_6x6_interpolator = 1 if use6 else 0
+    return p
+
+
+class Engine:
+    """One semper_engine handle (reference bindings/python/semper/__init__.py:83-230)."""
+
+    def __init__(self, lib=None):
+        self.lib = lib or load_c()
+        self.h = self.lib.semper_create()
+        if not self.h:
+            raise MemoryError("semper_create failed")
+
+    def close(self):
+        if self.h:
+            self.lib.semper_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def set_reference(self, img, mask=None):
+        img = np.ascontiguousarray(img, dtype=np.uint8)
+        h, w = img.shape
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+        return self.lib.semper_set_reference(self.h, _ptr(img), img.size, _ptr(m), 0 if m is None else m.size,
+                                             w, h)
+
+    def run(self, img, p, mask=None, capacity=None, metrics_len=METRICS_LEN, progress=None):
+        img = np.ascontiguousarray(img, dtype=np.uint8)
+        cap = capacity if capacity is not None else max(1, (p.rect_w // max(p.step, 1) + 1)
+                                                       * (p.rect_h // max(p.step, 1) + 1)) * FLOATS_PER_POINT
+        out = np.zeros(max(cap, 1), np.float32)
+        met = np.zeros(max(metrics_len, 1), np.float32)
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+        cb = PROGRESS_CB(progress) if progress else PROGRESS_CB()
+        n = self.lib.semper_run(self.h, _ptr(img), img.size, _ptr(m), 0 if m is None else m.size,
+                                ctypes.byref(p), _ptr(out), cap, _ptr(met), metrics_len, cb, None)
+        pts = out[: max(n, 0) * FLOATS_PER_POINT].reshape(-1, FLOATS_PER_POINT).copy()
+        return n, pts, met
+
+    def cancel(self):
+        self.lib.semper_cancel(self.h)

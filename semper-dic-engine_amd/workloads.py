@@ -1,0 +1,109 @@
+"""The benchmark workloads: synthetic speckle pairs and solve parameters.
+
+Speckle (all workloads): seeded Gaussian blobs, sigma U[2, 4] px, 0.5 blobs per
+px^2 over the image padded by 16 px, signs +-1, scaled to a peak of 200 with
+the reference image's range, 2% (4 grey levels) Gaussian noise, rounded to
+u8. The deformed image samples the continuous field at the analytic inverse
+of the displacement field (fixed-point iteration in FP64), never by
+resampling the reference, so the ground truth is exact. The pair is rendered
+on the GPU by sdk_synth_pair (csrc/synth.hip).
+"""
+import ctypes
+import math
+
+import numpy as np
+
+IDENTITY, AFFINE, SIN_GAUSS, BUMPS, UNIAXIAL = 0, 1, 2, 3, 4
+NOISE_SIGMA = 4.0
+
+
+def affine_params(rot_deg, exx, eyy, exy, tu, tv):
+    """Displacement t + (F - I)(p - c) with F = R(rot) [[1+exx, exy], [exy, 1+eyy]]."""
+    c, s = math.cos(math.radians(rot_deg)), math.sin(math.radians(rot_deg))
+    a11, a12, a21, a22 = 1 + exx, exy, exy, 1 + eyy
+    f00, f01 = c * a11 - s * a21, c * a12 - s * a22
+    f10, f11 = s * a11 + c * a21, s * a12 + c * a22
+    return [tu, tv, f00 - 1.0, f01, f10, f11 - 1.0]
+
+
+def bumps_params(seed, w, h, n=8):
+    """Sum of n Gaussian bumps (amplitude U[-30, 30] px per component, width
+    U[200, 800] px) plus a global rotation U[-5, 5] deg about the centre."""
+    rng = np.random.default_rng(seed)
+    p = [math.radians(rng.uniform(-5.0, 5.0)), float(n)]
+    for _ in range(n):
+        p += [rng.uniform(0, w), rng.uniform(0, h), rng.uniform(200, 800), rng.uniform(-30, 30),
+              rng.uniform(-30, 30)]
+    return p
+
+
+CONFIGS = {
+    # 512^2, u = 0.4 px rigid translation, step 10, subset 21, window 21
+    "c1_512_translate": dict(w=512, h=512, seed=1001, kind=AFFINE, p=[0.4, 0.0, 0, 0, 0, 0], step=10, subset=21,
+                             window=21, tol=1e-3),
+    # 2048^2 homogeneous affine: rot 2 deg, exx 1%, eyy -0.3%, exy 0.5%, t (12.3, -7.8)
+    "c2_2048_affine": dict(w=2048, h=2048, seed=2002, kind=AFFINE, p=affine_params(2.0, 0.01, -0.003, 0.005, 12.3, -7.8),
+                           step=5, subset=31, window=31, tol=1e-4),
+    # 4096^2 heterogeneous: u = 3 sin(2 pi x / 1024) cos(2 pi y / 1536), v = 2 exp(-r^2 / (2 600^2))
+    "c4_4096_heterogeneous": dict(w=4096, h=4096, seed=4004, kind=SIN_GAUSS, p=[3.0, 1024.0, 1536.0, 2.0, 600.0],
+                                  step=8, subset=31, window=31, tol=1e-3),
+    # 4096^2 bumps + rotation (the learned-seeding workload)
+    "c5_4096_bumps": dict(w=4096, h=4096, seed=5005, kind=BUMPS, p=None, step=8, subset=31, window=31, tol=1e-3),
+}
+
+
+def config(name):
+    c = dict(CONFIGS[name])
+    if c["p"] is None:
+        c["p"] = bumps_params(c["seed"], c["w"], c["h"])
+    return c
+
+
+def pair(dev, name, on_device=False):
+    """(ref, def) of a workload: host uint8 arrays, or device pointers
+    (allocated with sdk_malloc, to be freed by the caller) when on_device."""
+    c = config(name)
+    w, h = c["w"], c["h"]
+    prm = np.array(c["p"], np.float64)
+    if on_device:
+        r = dev.sdk_malloc(w * h)
+        d = dev.sdk_malloc(w * h)
+    else:
+        ra = np.zeros((h, w), np.uint8)
+        da = np.zeros((h, w), np.uint8)
+        r = ra.ctypes.data_as(ctypes.c_void_p)
+        d = da.ctypes.data_as(ctypes.c_void_p)
+    rc = dev.sdk_synth_pair(c["seed"], w, h, c["kind"], prm.ctypes.data_as(ctypes.c_void_p), len(prm), w / 2.0,
+                            h / 2.0, ctypes.c_float(NOISE_SIGMA), r, d, 1 if on_device else 0)
+    if rc != 0:
+        raise RuntimeError(f"sdk_synth_pair failed ({rc})")
+    return (r, d) if on_device else (ra, da)
+
+
+def solve_params(name):
+    c = config(name)
+    return (0, 0, c["w"], c["h"], c["step"], c["subset"], c["window"], 0)
+
+
+def displacement(name, x, y):
+    """Ground-truth displacement of a reference point (same fields as synth.hip)."""
+    c = config(name)
+    p, cx, cy = c["p"], c["w"] / 2.0, c["h"] / 2.0
+    dx, dy = x - cx, y - cy
+    if c["kind"] == AFFINE:
+        return p[0] + p[2] * dx + p[3] * dy, p[1] + p[4] * dx + p[5] * dy
+    if c["kind"] == SIN_GAUSS:
+        u = p[0] * np.sin(2 * np.pi * x / p[1]) * np.cos(2 * np.pi * y / p[2])
+        v = p[3] * np.exp(-(dx * dx + dy * dy) / (2 * p[4] ** 2))
+        return u, v
+    if c["kind"] == BUMPS:
+        th = p[0]
+        u = (math.cos(th) - 1) * dx - math.sin(th) * dy
+        v = math.sin(th) * dx + (math.cos(th) - 1) * dy
+        for k in range(int(p[1])):
+            bx, by, sg, au, av = p[2 + 5 * k: 7 + 5 * k]
+            g = np.exp(-((x - bx) ** 2 + (y - by) ** 2) / (2 * sg * sg))
+            u = u + au * g
+            v = v + av * g
+        return u, v
+    raise ValueError(name)

@@ -1,0 +1,197 @@
+"""Shared test plumbing: the CPU oracle (oracle/liboracle.so), the engine's
+ctypes module, the reference's golden fixtures and synthetic inputs."""
+import ctypes
+import importlib.util
+import os
+import struct
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+PKG = os.path.join(ROOT, "semper-dic-engine_amd")
+VP = ctypes.c_void_p
+
+
+def P(a):
+    return a.ctypes.data_as(VP) if a is not None else None
+
+
+def engine_module():
+    spec = importlib.util.spec_from_file_location("semper_amd", os.path.join(PKG, "semper_amd.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+_oracle = None
+
+
+def oracle():
+    """The CPU oracle (built by __graft_entry__.build() / `make -C oracle`)."""
+    global _oracle
+    if _oracle is None:
+        path = os.path.join(ROOT, "oracle", "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built")
+        lib = ctypes.CDLL(path)
+        lib.orc_engine_create.restype = VP
+        lib.orc_engine_destroy.argtypes = [VP]
+        lib.orc_engine_set_reference.argtypes = [VP, VP, ctypes.c_int, ctypes.c_int, VP]
+        lib.orc_engine_run.argtypes = [VP, VP, VP, VP, VP, ctypes.c_int, VP, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_float]
+        lib.orc_solve_nodes.argtypes = [VP, VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_float, ctypes.c_int, VP, VP, VP, VP, VP]
+        lib.orc_icgn_fp64.argtypes = [VP, VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      VP, VP, ctypes.c_double, ctypes.c_int, VP, VP, VP]
+        lib.orc_ref_speckle.argtypes = [ctypes.c_uint, ctypes.c_int, ctypes.c_int, ctypes.c_int, VP, VP]
+        lib.orc_gray8.argtypes = [VP, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint, VP]
+        lib.orc_golden_corpus.argtypes = [ctypes.c_int, ctypes.c_int, VP, VP]
+        lib.orc_pool.argtypes = [VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, VP, VP, VP, VP, VP]
+        lib.orc_gradients.argtypes = [VP, ctypes.c_int, ctypes.c_int, VP, VP]
+        lib.orc_interp.argtypes = [VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, VP, VP, VP]
+        lib.orc_outliers_and_vsg.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, VP, VP, VP, VP,
+                                             VP, VP]
+        lib.orc_vsg_window_node_count.argtypes = [ctypes.c_int, ctypes.c_int]
+        lib.orc_blob_eval.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_int, VP, VP, VP]
+        lib.orc_hash_u01.restype = ctypes.c_float
+        lib.orc_hash_u01.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+        _oracle = lib
+    return _oracle
+
+
+# ---- the reference's fixtures ------------------------------------------------
+
+CORPUS_DT = np.dtype([("x", "<i4"), ("y", "<i4"), ("status", "<i4"), ("u", "<f4"), ("v", "<f4"), ("ux", "<f4"),
+                      ("uy", "<f4"), ("vx", "<f4"), ("vy", "<f4"), ("corr", "<f4")])
+CORPUS_SCENARIOS = [(3.2, -1.7, 0.0, 0.0, 0.0, 0.0), (2.1, 1.4, 0.015, -0.008, 0.006, 0.012)]
+
+
+def golden_corpus(kind):
+    b = open(os.path.join(GOLDEN, f"golden_corpus.bin.{kind}"), "rb").read()
+    n = struct.unpack("<I", b[:4])[0]
+    a = np.frombuffer(b[4:], dtype=CORPUS_DT)
+    assert len(a) == n
+    return a
+
+
+def full_field_golden():
+    b = open(os.path.join(GOLDEN, "full_field_golden.bin"), "rb").read()
+    off = 4
+    runs = []
+    for _ in range(struct.unpack_from("<I", b, 0)[0]):
+        code = struct.unpack_from("<i", b, off)[0]
+        off += 4
+        counts = np.frombuffer(b[off:off + 60], dtype="<f4").copy()
+        off += 60
+        n = struct.unpack_from("<I", b, off)[0]
+        off += 4
+        pts = np.frombuffer(b[off:off + 4 * n], dtype="<f4").reshape(-1, 8).copy()
+        off += 4 * n
+        runs.append((code, counts, pts))
+    return runs
+
+
+FF_SLOTS = [0, 1, 2, 3, 4, 5, 6, 7, 8, 16, 19, 20, 21, 22, 23]
+FF_SCENARIOS = [(3.0, 2.0, 0.0, 0.0, 0.0, 0.0), (2.0, -1.5, 0.012, -0.006, 0.005, 0.010)]
+
+
+# ---- synthetic inputs ----------------------------------------------------------
+
+def ref_speckle(seed, w, h, blobs, aff=None):
+    """tests/framework/synthetic.h: float image of the reference's speckle
+    field; aff = (u, v, ux, uy, vx, vy, cx, cy) renders the deformed image."""
+    out = np.zeros(w * h, np.float32)
+    a = None if aff is None else np.array(aff, np.float32)
+    oracle().orc_ref_speckle(seed, w, h, blobs, P(a), P(out))
+    return out.reshape(h, w)
+
+
+def gray8(img, sigma=0.0, seed=0):
+    h, w = img.shape
+    src = np.ascontiguousarray(img, np.float32)
+    out = np.zeros(w * h, np.uint8)
+    oracle().orc_gray8(P(src), w, h, ctypes.c_float(sigma), seed, P(out))
+    return out.reshape(h, w)
+
+
+def corpus_images(scenario):
+    W = H = 512
+    ref = ref_speckle(42, W, H, 400)
+    s = CORPUS_SCENARIOS[scenario]
+    dfm = ref_speckle(42, W, H, 400, list(s) + [W / 2, H / 2])
+    return ref, dfm
+
+
+def corpus_nodes():
+    return [(x, y) for y in range(60, 453, 22) for x in range(60, 453, 22)]
+
+
+def ff_golden_images(scenario):
+    W = H = 320
+    ref = gray8(ref_speckle(11, W, H, 600))
+    s = FF_SCENARIOS[scenario]
+    dfm = gray8(ref_speckle(11, W, H, 600, list(s) + [160, 160]))
+    return ref, dfm
+
+
+class OracleEngine:
+    """The CPU restatement of semper_run (oracle/src/pipeline.cpp)."""
+
+    def __init__(self):
+        self.lib = oracle()
+        self.h = self.lib.orc_engine_create()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.orc_engine_destroy(self.h)
+
+    def set_reference(self, img, mask=None):
+        img = np.ascontiguousarray(img, np.uint8)
+        h, w = img.shape
+        m = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+        return self.lib.orc_engine_set_reference(self.h, P(img), w, h, P(m))
+
+    def run(self, img, prm, mask=None, threads=0, tol=1e-3, capacity=None):
+        img = np.ascontiguousarray(img, np.uint8)
+        pr = np.array(prm, np.int32)
+        gw, gh = pr[2] // max(pr[4], 1) + 1, pr[3] // max(pr[4], 1) + 1
+        cap = capacity if capacity is not None else gw * gh * 8
+        out = np.zeros(max(cap, 1), np.float32)
+        met = np.zeros(24, np.float32)
+        m = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+        n = self.lib.orc_engine_run(self.h, P(img), P(m), P(pr), P(out), cap, P(met), 24, threads,
+                                    ctypes.c_float(tol))
+        return n, out[: max(n, 0) * 8].reshape(-1, 8).copy(), met
+
+
+def oracle_solve_nodes(ref, dfm, dim, nodes, guesses, modes, use6=False, lm=False, tol=1e-3):
+    h, w = ref.shape
+    n = len(nodes)
+    cxy = np.array(nodes, np.int32).reshape(-1)
+    g = np.array(guesses, np.float32).reshape(-1)
+    md = np.array(modes, np.int32)
+    io = np.zeros((n, 5), np.int32)
+    fo = np.zeros((n, 7), np.float32)
+    r = np.ascontiguousarray(ref, np.float32)
+    d = np.ascontiguousarray(dfm, np.float32)
+    oracle().orc_solve_nodes(P(r), P(d), w, h, dim, int(use6), int(lm), ctypes.c_float(tol), n, P(cxy), P(g),
+                             P(md), P(io), P(fo))
+    return io, fo
+
+
+def fp64_polish(ref, dfm, dim, nodes, p0, use6=False, tol=1e-10, max_iter=200):
+    """FP64 IC-GN of the same model from per-node starts (the error yardstick)."""
+    h, w = ref.shape
+    n = len(nodes)
+    cxy = np.array(nodes, np.int32).reshape(-1)
+    start = np.ascontiguousarray(np.array(p0, np.float64).reshape(-1))
+    si = np.zeros((n, 2), np.int32)
+    po = np.zeros((n, 6), np.float64)
+    corr = np.zeros(n, np.float64)
+    r = np.ascontiguousarray(ref, np.float32)
+    d = np.ascontiguousarray(dfm, np.float32)
+    oracle().orc_icgn_fp64(P(r), P(d), w, h, dim, int(use6), n, P(cxy), P(start), ctypes.c_double(tol), max_iter,
+                           P(si), P(po), P(corr))
+    return si, po, corr

@@ -1,0 +1,227 @@
+"""GPU parity of the HIP engine against the reference's golden fixtures and the
+CPU oracle, through the C ABI (libsemper_c.so) and the kernel-level interface
+(libsemper_dev.so)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from helpers import (FF_SLOTS, P, OracleEngine, corpus_images, corpus_nodes, engine_module, ff_golden_images,
+                     full_field_golden, golden_corpus, oracle_solve_nodes, ref_speckle)
+
+pytestmark = pytest.mark.gpu
+
+sa = engine_module()
+
+
+@pytest.fixture(scope="module")
+def dev():
+    lib = sa.load_dev()
+    assert lib.sdk_device_count() >= 1, "no GPU visible"
+    return lib
+
+
+def gpu_solve_nodes(dev, ref, dfm, dim, nodes, guesses, modes, use6=False, lm=False, tol=1e-3):
+    h, w = ref.shape
+    n = len(nodes)
+    cxy = np.array(nodes, np.int32).reshape(-1)
+    g = np.array(guesses, np.float32).reshape(-1)
+    md = np.array(modes, np.int32)
+    io = np.zeros((n, 5), np.int32)
+    fo = np.zeros((n, 7), np.float32)
+    r = np.ascontiguousarray(ref, np.float32)
+    if dfm.dtype == np.uint8:
+        d, f32 = np.ascontiguousarray(dfm), 0
+    else:
+        d, f32 = np.ascontiguousarray(dfm, np.float32), 1
+    rc = dev.sdk_solve_nodes(P(r), P(d), f32, w, h, dim, int(use6), int(lm), ctypes.c_float(tol), n, P(cxy), P(g),
+                             P(md), P(io), P(fo))
+    assert rc == 0
+    return io, fo
+
+
+# ---- GoldenCorpus (tests/integration/test_golden_corpus.cpp:54-369) -------------
+
+def corpus_run(dev, use6):
+    ixy, st, vals, scen = [], [], [], []
+    for s in range(2):
+        ref, dfm = corpus_images(s)
+        nodes = corpus_nodes()
+        io, fo = gpu_solve_nodes(dev, ref, dfm, 27, nodes, [[0] * 6] * len(nodes), [2] * len(nodes), use6=use6)
+        for k, (x, y) in enumerate(nodes):
+            if not io[k, 0]:
+                continue
+            ixy.append((x, y))
+            st.append(io[k, 1])
+            vals.append(fo[k])
+            scen.append(s)
+    return np.array(ixy), np.array(st), np.array(vals), np.array(scen)
+
+
+def on_truth(x, y, u, v, s):
+    sc = [(3.2, -1.7, 0, 0, 0, 0), (2.1, 1.4, 0.015, -0.008, 0.006, 0.012)][s]
+    dx, dy = x - 256.0, y - 256.0
+    return np.hypot(u - (sc[0] + sc[2] * dx + sc[3] * dy), v - (sc[1] + sc[4] * dx + sc[5] * dy)) <= 0.1
+
+
+@pytest.mark.parametrize("kind,use6", [("bicubic", False), ("keys6x6", True)])
+def test_golden_corpus_gpu(dev, kind, use6):
+    """The reference's own acceptance rule for the corpus: no status changes
+    except slow-convergence flips at unchanged values (<= 6, net <= 2), values
+    within 1e-2 px / 1e-3 (5x at local minima)."""
+    g = golden_corpus(kind)
+    ixy, st, vals, scen = corpus_run(dev, use6)
+    assert len(ixy) == len(g)
+    assert (ixy[:, 0] == g["x"]).all() and (ixy[:, 1] == g["y"]).all()
+    gv = np.stack([g[k] for k in ("u", "v", "ux", "uy", "vx", "vy")], 1)
+    status_mis = value_mis = to_exh = to_conv = 0
+    for i in range(len(g)):
+        exact = on_truth(g["x"][i], g["y"][i], g["u"][i], g["v"][i], scen[i])
+        scale = 1.0 if exact else 5.0
+        agree = (np.abs(gv[i, :2] - vals[i, :2]) <= 1e-2 * scale).all() and \
+                (np.abs(gv[i, 2:] - vals[i, 2:6]) <= 1e-3 * scale).all()
+        if g["status"][i] != st[i]:
+            both = g["corr"][i] < 2.0 and vals[i, 6] < 2.0
+            if both and agree:
+                if st[i] != 0:
+                    to_exh += 1
+                else:
+                    to_conv += 1
+                continue
+            status_mis += 1
+            continue
+        if g["status"][i] != 0:
+            continue
+        if not agree:
+            value_mis += 1
+    assert status_mis == 0
+    assert value_mis == 0
+    assert abs(to_exh - to_conv) <= 2 and to_exh + to_conv <= 6
+
+
+@pytest.mark.parametrize("use6", [False, True])
+def test_corpus_gpu_vs_oracle(dev, use6):
+    """Node by node against the CPU restatement on the same float images:
+    identical status, |du|, |dv| <= 1e-3 px on converged nodes (LM on, as the
+    full-field engine runs it; and AUTO_SEARCH / rescue modes)."""
+    ref, dfm = corpus_images(1)
+    nodes = corpus_nodes()
+    n = len(nodes)
+    rng = np.random.default_rng(7)
+    guesses = np.zeros((n, 6), np.float32)
+    guesses[:, 0] = 2.1 + rng.uniform(-1.5, 1.5, n)
+    guesses[:, 1] = 1.4 + rng.uniform(-1.5, 1.5, n)
+    modes = np.array([1, 2, 0] * (n // 3 + 1))[:n]
+    guesses[modes == 0, :] = 0.0      # zero guess: the coarse search runs
+    io_g, fo_g = gpu_solve_nodes(dev, ref, dfm, 27, nodes, guesses, modes, use6=use6, lm=True)
+    io_o, fo_o = oracle_solve_nodes(ref, dfm, 27, nodes, guesses, modes, use6=use6, lm=True)
+    assert (io_g[:, 0] == io_o[:, 0]).all()
+    init = io_o[:, 0] == 1
+    same = io_g[:, 1] == io_o[:, 1]
+    assert same[init].mean() >= 0.98
+    conv = init & same & (io_o[:, 1] == 0)
+    assert np.abs(fo_g[conv, 0] - fo_o[conv, 0]).max() <= 1e-3
+    assert np.abs(fo_g[conv, 1] - fo_o[conv, 1]).max() <= 1e-3
+    assert np.abs(fo_g[conv, 6] - fo_o[conv, 6]).max() <= 1e-4
+
+
+# ---- FullFieldGolden through semper_run (test_full_field_golden.cpp:280-395) ---
+
+@pytest.mark.parametrize("scenario", [0, 1])
+def test_full_field_golden_gpu(scenario):
+    code, counts, gpts = full_field_golden()[scenario]
+    ref, dfm = ff_golden_images(scenario)
+    eng = sa.Engine()
+    assert eng.set_reference(ref) == 0
+    p = sa.params((0, 0, 320, 320), 12, 21, 48)
+    n, pts, met = eng.run(dfm, p)
+    assert n == code
+    assert (pts[:, :2] == gpts[:, :2]).all()
+    dev_ = np.abs(pts - gpts).max(0)
+    assert dev_[2] <= 5e-3 and dev_[3] <= 5e-3
+    assert dev_[4:7].max() <= 1e-4 and dev_[7] <= 1e-4
+    m = met[FF_SLOTS]
+    # counts that the replaced seeding front end does not move
+    for k, slot in enumerate(FF_SLOTS):
+        if slot in (0, 1, 2, 16, 19, 20, 21, 23):
+            assert m[k] == counts[k], (slot, m[k], counts[k])
+
+
+# ---- C1: 512^2 speckle, u = 0.4 px, against the oracle's semper_run ---------------
+
+def c1_images(dev):
+    import importlib.util, os
+    from helpers import PKG
+    spec = importlib.util.spec_from_file_location("workloads", os.path.join(PKG, "workloads.py"))
+    wl = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(wl)
+    return wl.pair(dev, "c1_512_translate")
+
+
+def test_c1_full_field_vs_oracle(dev):
+    ref, dfm = c1_images(dev)
+    prm = (0, 0, 512, 512, 10, 21, 21, 0)
+    oe = OracleEngine()
+    assert oe.set_reference(ref) == 0
+    n_o, pts_o, met_o = oe.run(dfm, prm)
+    eng = sa.Engine()
+    assert eng.set_reference(ref) == 0
+    n_g, pts_g, met_g = eng.run(dfm, sa.params(prm[:4], *prm[4:7]))
+    assert n_g > 0 and abs(n_g - n_o) <= max(2, n_o // 200)
+    go = {(a, b): r for a, b, *r in pts_o.tolist()}
+    gg = {(a, b): r for a, b, *r in pts_g.tolist()}
+    common = sorted(set(go) & set(gg))
+    assert len(common) >= n_o - max(2, n_o // 200)
+    d = np.array([np.abs(np.array(go[k]) - np.array(gg[k])) for k in common])
+    assert d[:, 0].max() <= 1e-3 and d[:, 1].max() <= 1e-3
+    assert d[:, 2:5].max() <= 1e-5
+    assert d[:, 5].max() <= 1e-4
+    assert met_g[0] == met_o[0]
+
+
+# ---- C ABI contract on the GPU (tests/c/contract.c) ----------------------------
+
+def test_c_abi_contract_gpu():
+    lib = sa.load_c()
+    W = H = 128
+    img = ((np.arange(W)[None, :] * 2 + np.arange(H)[:, None] * 3) & 0xFF).astype(np.uint8)
+    p = sa.params((0, 0, 112, 112), 16, 21, 33)
+    eng = sa.Engine(lib)
+    out = np.zeros(8 * 4, np.float32)
+    met = np.zeros(24, np.float32)
+    cb = sa.PROGRESS_CB()
+    # no reference yet
+    assert lib.semper_run(eng.h, P(img), img.size, None, 0, ctypes.byref(p), P(out), 32, P(met), 24, cb,
+                          None) == sa.ERR_INIT
+    assert eng.set_reference(img) == 0
+    n, _, met = eng.run(img, p)
+    assert n >= 0
+    bad = sa.params((0, 0, 15, 15), 16, 21, 33)
+    assert eng.run(img, bad)[0] == sa.ERR_ROI
+    sw = sa.params((0, 0, 112, 112), 16, 21, 16)
+    assert eng.run(img, sw)[0] == sa.ERR_STRAIN_WINDOW
+    ev = sa.params((0, 0, 112, 112), 16, 20, 33)
+    assert eng.run(img, ev)[0] == sa.ERR_INIT
+    huge = sa.params((0, 0, 0x7FFFFFFF, 0x7FFFFFFF), 16, 21, 33)
+    assert eng.run(img, huge)[0] >= 0
+    # capacity 0: metrics-only query
+    n0, _, m0 = eng.run(img, p, capacity=0)
+    assert n0 == 0 and m0[0] >= 1
+    # metrics_len 15: untouched
+    n15, _, m15 = eng.run(img, p, metrics_len=15)
+    assert n15 >= 0 and (m15 == 0).all()
+
+
+def test_speckle_translation_gpu():
+    """Analytic ground truth (test_optimization_engine.cpp:98-110 pattern):
+    pure sub-pixel translation of the reference's speckle field."""
+    W = H = 256
+    ref = ref_speckle(1234, W, H, 1600)
+    dfm = ref_speckle(1234, W, H, 1600, [2.25, -1.5, 0, 0, 0, 0, 128, 128])
+    eng = sa.Engine()
+    from helpers import gray8
+    assert eng.set_reference(gray8(ref)) == 0
+    n, pts, met = eng.run(gray8(dfm), sa.params((0, 0, W, H), 8, 21, 32))
+    assert n > 100
+    assert np.median(np.abs(pts[:, 2] - 2.25)) < 0.02
+    assert np.median(np.abs(pts[:, 3] + 1.5)) < 0.02
