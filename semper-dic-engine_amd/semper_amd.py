@@ -131,8 +131,11 @@ class Engine:
 
     def run(self, img, p, mask=None, capacity=None, metrics_len=METRICS_LEN, progress=None):
         img = np.ascontiguousarray(img, dtype=np.uint8)
-        cap = capacity if capacity is not None else max(1, (p.rect_w // max(p.step, 1) + 1)
-                                                       * (p.rect_h // max(p.step, 1) + 1)) * FLOATS_PER_POINT
+        # default capacity: every grid node of the rectangle clipped to the image
+        ih, iw = img.shape[:2]
+        st = max(p.step, 1)
+        rw, rh = min(max(p.rect_w, 0), iw), min(max(p.rect_h, 0), ih)
+        cap = capacity if capacity is not None else max(1, (rw // st + 1) * (rh // st + 1)) * FLOATS_PER_POINT
         out = np.zeros(max(cap, 1), np.float32)
         met = np.zeros(max(metrics_len, 1), np.float32)
         m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
