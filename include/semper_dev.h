@@ -47,7 +47,11 @@ SDK_API void sdk_engine_destroy(void* h);
 SDK_API int sdk_engine_set_reference(void* h, const uint8_t* img, int on_device, int w, int hh, const uint8_t* mask);
 SDK_API int sdk_engine_run(void* h, const uint8_t* def, int on_device, const int32_t* params, float* out, int cap,
                            float* metrics, int metrics_len, float tol);
-SDK_API int sdk_engine_timings(void* h, double* out11);
+/* Phase timings of the last run, 14 doubles: total, prep, seeding, prepass,
+   mesh, Path A, Path B, strain (host ms); Path A solver launch (device ms),
+   Path A items, Path A IC-GN iterations, Path B speculative solves, further
+   Path B launches, subsets those launches solved. */
+SDK_API int sdk_engine_timings(void* h, double* out14);
 
 /* Synthetic speckle pair (synth.hpp FieldSpec kinds). */
 SDK_API int sdk_synth_pair(uint64_t seed, int w, int h, int kind, const double* p, int np, double cx, double cy,

@@ -215,12 +215,14 @@ SDK_API int sdk_engine_run(void* h, const uint8_t* def, int on_device, const int
 // Last solve's timings: total, prep, seeding, prepass, mesh, path_a, path_b,
 // strain (host ms), the Path A solver launch (device ms), its grid nodes and
 // its IC-GN iterations.
-SDK_API int sdk_engine_timings(void* h, double* out11) {
+SDK_API int sdk_engine_timings(void* h, double* out14) {
     if (!h) return -3;
     const Timings& t = ((DevEngine*)h)->eng.timings();
-    const double v[11] = {t.total, t.prep,   t.seeding,       t.prepass,        t.mesh,          t.path_a,
-                          t.path_b, t.strain, t.kernel_path_a, (double)t.path_a_items, (double)t.path_a_iters};
-    std::memcpy(out11, v, sizeof(v));
+    const double v[14] = {t.total,  t.prep,   t.seeding,       t.prepass,
+                          t.mesh,   t.path_a, t.path_b,        t.strain,
+                          t.kernel_path_a,    (double)t.path_a_items, (double)t.path_a_iters,
+                          (double)t.path_b_spec, (double)t.path_b_launches, (double)t.path_b_miss};
+    std::memcpy(out14, v, sizeof(v));
     return 0;
 }
 

@@ -771,6 +771,7 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             struct Cell {
                 Task t;
                 SolveOut r;
+                int from;   // flat index of the node whose warp seeded the guess (-1: a global seed)
             };
             // commit one solved cell: host mirror, queue, counters
             std::vector<int> commit_nodes;
@@ -796,6 +797,56 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                 t.node = t.pool;
                 return t;
             };
+            // the guess a popped node hands to its neighbour: its warp carried by the grid offset
+            auto seed_task = [&](const SeedNode& cur, int nx, int ny) {
+                const float ddx = (float)((nx - cur.x) * p.step), ddy = (float)((ny - cur.y) * p.step);
+                const float gs[6] = {cur.u + cur.ux * ddx + cur.uy * ddy, cur.v + cur.vx * ddx + cur.vy * ddy,
+                                     cur.ux, cur.uy, cur.vx, cur.vy};
+                return make_task(nx, ny, gs);
+            };
+            // Speculative first claims. The queue pops the nodes solved before Path B in
+            // (corr, y, x) order, and pushes never reorder them, so an unsolved node is
+            // claimed by the first of its pre-solved neighbours in that order unless a node
+            // solved inside Path B claims it earlier. All such first claims are solved in
+            // one launch here; the round replay below takes their results when the claim
+            // matches and solves only the other claims. Each task's result depends on the
+            // task alone, so the replay commits exactly what round-by-round solving would.
+            std::vector<int> spec_from(NN, -1);
+            std::vector<SolveOut> spec_res;
+            std::vector<int> spec_slot(NN, -1);
+            {
+                std::vector<Task> spec;
+                std::vector<int> spec_node;
+                for (int y = 0; y < gh; ++y)
+                    for (int x = 0; x < gw; ++x) {
+                        const int i = y * gw + x;
+                        if (claimed[i]) continue;
+                        int best = -1;
+                        for (int k = 0; k < 4; ++k) {
+                            const int nx = x + dx4[k], ny = y + dy4[k];
+                            if (nx < 0 || nx >= gw || ny < 0 || ny >= gh) continue;
+                            const int j = ny * gw + nx;
+                            if (!hsolved[j] || hcorr[j] < 0.f) continue;
+                            if (best < 0) {
+                                best = j;
+                                continue;
+                            }
+                            const int bx = best % gw, by = best / gw;
+                            const bool better = hcorr[j] != hcorr[best] ? hcorr[j] < hcorr[best]
+                                                : ny != by           ? ny < by
+                                                                     : nx < bx;
+                            if (better) best = j;
+                        }
+                        if (best < 0) continue;
+                        const SeedNode m{best % gw, best / gw, hu[best], hv[best], hux[best], huy[best],
+                                         hvx[best], hvy[best], hcorr[best]};
+                        spec_from[i] = best;
+                        spec_slot[i] = (int)spec.size();
+                        spec.push_back(seed_task(m, x, y));
+                    }
+                if (!spec.empty() && solve_tasks(spec, spec_res, p.use6, p.subset, tol) != 0) return kErrInit;
+                tm_.path_b_spec = (int)spec.size();
+            }
             auto flush_commits = [&]() {
                 if (commit_nodes.empty()) return;
                 const int n = (int)commit_nodes.size();
@@ -812,7 +863,7 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                 if (claimed[flat]) continue;
                 claimed[flat] = 1;
                 const float gs[6] = {s.u, s.v, 0, 0, 0, 0};
-                Cell c{make_task(s.x, s.y, gs), {}};
+                Cell c{make_task(s.x, s.y, gs), {}, -1};
                 std::vector<SolveOut> r1;
                 if (solve_tasks({c.t}, r1, p.use6, p.subset, tol) != 0) return kErrInit;
                 c.r = r1[0];
@@ -823,7 +874,7 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                 if (pc.x >= 0 && pc.x < gw && pc.y >= 0 && pc.y < gh && !claimed[flat]) {
                     claimed[flat] = 1;
                     const float gs[6] = {pc.u, pc.v, 0, 0, 0, 0};
-                    Cell c{make_task(pc.x, pc.y, gs), {}};
+                    Cell c{make_task(pc.x, pc.y, gs), {}, -1};
                     std::vector<SolveOut> r1;
                     if (solve_tasks({c.t}, r1, p.use6, p.subset, tol) != 0) return kErrInit;
                     c.r = r1[0];
@@ -849,10 +900,7 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                         const int flat = ny * gw + nx;
                         if (claimed[flat]) continue;
                         claimed[flat] = 1;
-                        const float ddx = (float)((nx - cur.x) * p.step), ddy = (float)((ny - cur.y) * p.step);
-                        const float gs[6] = {cur.u + cur.ux * ddx + cur.uy * ddy, cur.v + cur.vx * ddx + cur.vy * ddy,
-                                             cur.ux, cur.uy, cur.vx, cur.vy};
-                        r.cells.push_back(Cell{make_task(nx, ny, gs), {}});
+                        r.cells.push_back(Cell{seed_task(cur, nx, ny), {}, cur.y * gw + cur.x});
                     }
                 }
                 rounds.push_back(std::move(r));
@@ -863,18 +911,28 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             while (!rounds.empty()) {
                 if (cancel.requested()) return kErrCancelled;
                 if (!rounds.front().solved) {
+                    // claims made by the speculated neighbour take its result; the rest are solved now
                     std::vector<Task> tasks;
+                    std::vector<Cell*> miss;
                     for (Round& r : rounds)
                         if (!r.solved)
-                            for (Cell& c : r.cells) tasks.push_back(c.t);
-                    std::vector<SolveOut> res;
-                    if (solve_tasks(tasks, res, p.use6, p.subset, tol) != 0) return kErrInit;
-                    size_t k = 0;
-                    for (Round& r : rounds)
-                        if (!r.solved) {
-                            for (Cell& c : r.cells) c.r = res[k++];
-                            r.solved = true;
-                        }
+                            for (Cell& c : r.cells) {
+                                const int nd = c.t.node;
+                                if (c.from >= 0 && spec_from[nd] == c.from) {
+                                    c.r = spec_res[spec_slot[nd]];
+                                } else {
+                                    tasks.push_back(c.t);
+                                    miss.push_back(&c);
+                                }
+                            }
+                    if (!tasks.empty()) {
+                        std::vector<SolveOut> res;
+                        if (solve_tasks(tasks, res, p.use6, p.subset, tol) != 0) return kErrInit;
+                        for (size_t k = 0; k < miss.size(); ++k) miss[k]->r = res[k];
+                        tm_.path_b_launches++;
+                        tm_.path_b_miss += (int)tasks.size();
+                    }
+                    for (Round& r : rounds) r.solved = true;
                 }
                 for (const Cell& c : rounds.front().cells) commit(c);
                 rounds.pop_front();

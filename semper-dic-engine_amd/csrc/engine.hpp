@@ -73,6 +73,9 @@ struct Timings {
     double kernel_path_a = 0;     // device ms of the Path A solver launch
     int path_a_items = 0;         // grid nodes the Path A launch covered
     int path_a_iters = 0;         // IC-GN iterations it ran
+    int path_b_spec = 0;          // speculative first-claim solves of Path B (one launch)
+    int path_b_launches = 0;      // further Path B solver launches (claims the speculation missed)
+    int path_b_miss = 0;          // subsets those launches solved
 };
 
 // validate_params (full_field_solver.cpp:97-111) and

@@ -6,9 +6,12 @@
 // from a cell index. The deformed image samples S at the reference point p that
 // maps to the pixel q: p + U(p) = q, solved by fixed-point iteration in double
 // for a displacement field U (affine; sin/cos + Gaussian; Gaussian bumps +
-// rotation; uniaxial stretch). Both images are scaled to a peak of 200 with the
-// reference's range, get independent Gaussian noise from a counter hash, and are
-// rounded to u8.
+// rotation; uniaxial stretch). S has mean 0 and variance
+// rho * pi * E[sigma^2] = 0.5 * pi * 28/3 (a Poisson field of signed blobs whose
+// squared profile integrates to pi sigma^2), so both images map S linearly with
+// +-3 standard deviations onto [0, 200] (peak 200, clipped beyond), get
+// independent Gaussian noise from a counter hash, and are rounded to u8. The
+// scale is analytic, so the bytes do not depend on any reduction order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -117,32 +120,10 @@ __global__ void k_field(Blobs b, FieldSpec f, int w, int h, float* __restrict__ 
     }
 }
 
-__device__ __forceinline__ unsigned f2ord(float f) {
-    const unsigned u = __float_as_uint(f);
-    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-
-__global__ void k_minmax(const float* __restrict__ s, int64_t n, unsigned* mm) {
-    unsigned lo = 0xffffffffu, hi = 0u;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const unsigned o = f2ord(s[i]);
-        lo = o < lo ? o : lo;
-        hi = o > hi ? o : hi;
-    }
-    atomicMin(&mm[0], lo);
-    atomicMax(&mm[1], hi);
-}
-
-__device__ __forceinline__ float ord2f(unsigned o) {
-    return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
-}
-
-__global__ void k_quantize(const float* __restrict__ s, int64_t n, const unsigned* mm, float peak, float sigma,
+__global__ void k_quantize(const float* __restrict__ s, int64_t n, float mid, float scale, float peak, float sigma,
                            uint64_t noise_seed, uint8_t* __restrict__ out) {
-    const float lo = ord2f(mm[0]), hi = ord2f(mm[1]);
-    const float scale = hi > lo ? peak / (hi - lo) : 0.0f;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        float v = (s[i] - lo) * scale;
+        float v = fminf(fmaxf(mid + s[i] * scale, 0.0f), peak);
         if (sigma > 0.0f) {
             const float u1 = fmaxf(hash_u01(noise_seed, 2ull * i), 1e-7f);
             const float u2 = hash_u01(noise_seed, 2ull * i + 1);
@@ -185,40 +166,38 @@ int synth_pair(uint64_t seed, int w, int h, const FieldSpec& field, float noise_
 
     float *dbx = nullptr, *dS = nullptr;
     int *dstart = nullptr, *didx = nullptr;
-    unsigned* dmm = nullptr;
     const int64_t n = (int64_t)w * h;
     hipError_t e = hipMalloc(&dbx, sizeof(float) * 4 * (size_t)nb);
     if (e == hipSuccess) e = hipMalloc(&dstart, sizeof(int) * cnt.size());
     if (e == hipSuccess) e = hipMalloc(&didx, sizeof(int) * (size_t)nb);
     if (e == hipSuccess) e = hipMalloc(&dS, sizeof(float) * 2 * (size_t)n);
-    if (e == hipSuccess) e = hipMalloc(&dmm, sizeof(unsigned) * 2);
+    if (e == hipSuccess) e = hipMemcpyAsync(dbx, bx.data(), sizeof(float) * nb, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dbx + nb, by.data(), sizeof(float) * nb, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dbx + 2 * (size_t)nb, bk.data(), sizeof(float) * nb, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dbx + 3 * (size_t)nb, ba.data(), sizeof(float) * nb, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dstart, cnt.data(), sizeof(int) * cnt.size(), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(didx, idx.data(), sizeof(int) * nb, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) {
-        hipMemcpyAsync(dbx, bx.data(), sizeof(float) * nb, hipMemcpyHostToDevice, st);
-        hipMemcpyAsync(dbx + nb, by.data(), sizeof(float) * nb, hipMemcpyHostToDevice, st);
-        hipMemcpyAsync(dbx + 2 * (size_t)nb, bk.data(), sizeof(float) * nb, hipMemcpyHostToDevice, st);
-        hipMemcpyAsync(dbx + 3 * (size_t)nb, ba.data(), sizeof(float) * nb, hipMemcpyHostToDevice, st);
-        hipMemcpyAsync(dstart, cnt.data(), sizeof(int) * cnt.size(), hipMemcpyHostToDevice, st);
-        hipMemcpyAsync(didx, idx.data(), sizeof(int) * nb, hipMemcpyHostToDevice, st);
-        const unsigned init[2] = {0xffffffffu, 0u};
-        hipMemcpyAsync(dmm, init, sizeof(init), hipMemcpyHostToDevice, st);
         Blobs b{dbx, dbx + nb, dbx + 2 * (size_t)nb, dbx + 3 * (size_t)nb, dstart, didx, x0, y0, cw, ch};
         FieldSpec ident{};
         ident.kind = kFieldIdentity;
         const int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
         hipLaunchKernelGGL(k_field, dim3(blocks), dim3(256), 0, st, b, ident, w, h, dS);
         hipLaunchKernelGGL(k_field, dim3(blocks), dim3(256), 0, st, b, field, w, h, dS + n);
-        hipLaunchKernelGGL(k_minmax, dim3(std::min(blocks, 4096)), dim3(256), 0, st, dS, n, dmm);
-        hipLaunchKernelGGL(k_quantize, dim3(blocks), dim3(256), 0, st, dS, n, dmm, 200.0f, noise_sigma,
+        const float sd = std::sqrt(density * 3.14159265f * (28.0f / 3.0f)), peak = 200.0f;
+        const float scale = 0.5f * peak / (3.0f * sd);
+        hipLaunchKernelGGL(k_quantize, dim3(blocks), dim3(256), 0, st, dS, n, 0.5f * peak, scale, peak, noise_sigma,
                            seed ^ 0x5151ull, d_ref);
-        hipLaunchKernelGGL(k_quantize, dim3(blocks), dim3(256), 0, st, dS + n, n, dmm, 200.0f, noise_sigma,
+        hipLaunchKernelGGL(k_quantize, dim3(blocks), dim3(256), 0, st, dS + n, n, 0.5f * peak, scale, peak, noise_sigma,
                            seed ^ 0xa2a2ull, d_def);
-        e = hipStreamSynchronize(st);
+        e = hipGetLastError();
+        const hipError_t es = hipStreamSynchronize(st);
+        if (e == hipSuccess) e = es;
     }
-    hipFree(dbx);
-    hipFree(dstart);
-    hipFree(didx);
-    hipFree(dS);
-    hipFree(dmm);
+    (void)hipFree(dbx);
+    (void)hipFree(dstart);
+    (void)hipFree(didx);
+    (void)hipFree(dS);
     return (int)e;
 }
 

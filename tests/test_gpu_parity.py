@@ -166,36 +166,44 @@ def test_full_field_golden_gpu(scenario):
 
 # ---- C1: 512^2 speckle, u = 0.4 px, against the oracle's semper_run ---------------
 
-def c1_images(dev):
-    import importlib.util, os
+def workload_module():
+    import importlib.util
+    import os
     from helpers import PKG
     spec = importlib.util.spec_from_file_location("workloads", os.path.join(PKG, "workloads.py"))
     wl = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(wl)
-    return wl.pair(dev, "c1_512_translate")
+    return wl
 
 
-def test_c1_full_field_vs_oracle(dev):
-    """C1 (512^2, u = 0.4 px, step 10, subset 21, window 21) through the engine
-    against the oracle's run_full_field: attempted-subset count identical,
-    point count within 0.5%, u, v within the reference's cross-toolchain bound
-    (5e-3 px) with 99% within 1e-3 px, strain and corr within 1e-4
-    (test_full_field_golden.cpp:205-225)."""
-    ref, dfm = c1_images(dev)
-    prm = (0, 0, 512, 512, 10, 21, 21, 0)
+@pytest.mark.parametrize("name", ["c1_512_translate", "c2_2048_affine", "c4_4096_heterogeneous"])
+def test_workload_full_field_vs_oracle(dev, name):
+    """Each benchmark workload at its full size through the engine against the
+    oracle's run_full_field on the same bytes and stop rule: attempted-subset
+    count identical, point counts within 0.5%, u, v within the reference's
+    cross-toolchain bound (5e-3 px) with 99% within 1e-3 px, strain and corr
+    within 1e-4 (test_full_field_golden.cpp:205-225)."""
+    wl = workload_module()
+    cfg = wl.config(name)
+    ref, dfm = wl.pair(dev, name)
+    prm = wl.solve_params(name)
+    tol = cfg["tol"]
+    print(f"workload {name} {cfg['w']}x{cfg['h']} tol {tol}", flush=True)
     oe = OracleEngine()
     assert oe.set_reference(ref) == 0
-    n_o, pts_o, met_o = oe.run(dfm, prm)
+    n_o, pts_o, met_o = oe.run(dfm, prm, tol=tol)
     h = dev.sdk_engine_create()
-    assert dev.sdk_engine_set_reference(h, P(ref), 0, 512, 512, None) == 0
-    out = np.zeros(52 * 52 * 8, np.float32)
+    assert dev.sdk_engine_set_reference(h, P(ref), 0, cfg["w"], cfg["h"], None) == 0
+    cap = (cfg["w"] // cfg["step"] + 1) * (cfg["h"] // cfg["step"] + 1) * 8
+    out = np.zeros(cap, np.float32)
     met_g = np.zeros(24, np.float32)
     pr = np.array(prm, np.int32)
-    n_g = dev.sdk_engine_run(h, P(dfm), 0, P(pr), P(out), out.size, P(met_g), 24, ctypes.c_float(1e-3))
+    n_g = dev.sdk_engine_run(h, P(dfm), 0, P(pr), P(out), out.size, P(met_g), 24, ctypes.c_float(tol))
     dev.sdk_engine_destroy(h)
     pts_g = out[: n_g * 8].reshape(-1, 8)
     assert n_g > 0 and abs(n_g - n_o) <= max(2, n_o // 200), (n_g, n_o)
     assert met_g[0] == met_o[0]
+    assert abs(float(met_g[3] + met_g[4] + met_g[19]) - float(met_o[3] + met_o[4] + met_o[19])) <= max(2, n_o // 200)
     go = {(a, b): r for a, b, *r in pts_o.tolist()}
     gg = {(a, b): r for a, b, *r in pts_g.tolist()}
     common = sorted(set(go) & set(gg))
