@@ -91,12 +91,15 @@ SDK_API int sdk_solve_nodes(const float* ref, const void* def, int def_is_f32, i
             !outd.p) {
             rc = -3;
         } else {
-            hipMemcpyAsync(dref.p, ref, sizeof(float) * npx, hipMemcpyHostToDevice, st);
-            hipMemcpyAsync(ddef.p, def, npx * (def_is_f32 ? 4 : 1), hipMemcpyHostToDevice, st);
-            hipMemcpyAsync(dcxy.p, cxy, sizeof(int32_t) * 2 * n, hipMemcpyHostToDevice, st);
-            hipMemcpyAsync(tasks.p, ht.data(), sizeof(Task) * n, hipMemcpyHostToDevice, st);
-            launch_prep_ref_f32(dref.p, w, h, I.p, gx.p, gy.p, st);
-            launch_pool_list(I.p, gx.p, gy.p, w, h, dcxy.p, n, dim, lm != 0, pool.p, st);
+            hipError_t e = hipMemcpyAsync(dref.p, ref, sizeof(float) * npx, hipMemcpyHostToDevice, st);
+            auto chk = [&e](hipError_t r) {
+                if (e == hipSuccess) e = r;
+            };
+            chk(hipMemcpyAsync(ddef.p, def, npx * (def_is_f32 ? 4 : 1), hipMemcpyHostToDevice, st));
+            chk(hipMemcpyAsync(dcxy.p, cxy, sizeof(int32_t) * 2 * n, hipMemcpyHostToDevice, st));
+            chk(hipMemcpyAsync(tasks.p, ht.data(), sizeof(Task) * n, hipMemcpyHostToDevice, st));
+            chk(launch_prep_ref_f32(dref.p, w, h, I.p, gx.p, gy.p, st));
+            chk(launch_pool_list(I.p, gx.p, gy.p, w, h, dcxy.p, n, dim, lm != 0, pool.p, st));
             SolveArgs a = {};
             a.tasks = tasks.p;
             a.out = outd.p;
@@ -113,10 +116,11 @@ SDK_API int sdk_solve_nodes(const float* ref, const void* def, int def_is_f32, i
             a.dim = dim;
             a.lm = lm;
             a.tol = tol;
-            if (launch_solve(a, use6 != 0, def_is_f32 != 0, st) != hipSuccess) rc = -3;
+            chk(launch_solve(a, use6 != 0, def_is_f32 != 0, st));
             std::vector<SolveOut> ho(n);
-            hipMemcpyAsync(ho.data(), outd.p, sizeof(SolveOut) * n, hipMemcpyDeviceToHost, st);
-            if (hipStreamSynchronize(st) != hipSuccess) rc = -3;
+            chk(hipMemcpyAsync(ho.data(), outd.p, sizeof(SolveOut) * n, hipMemcpyDeviceToHost, st));
+            chk(hipStreamSynchronize(st));
+            if (e != hipSuccess) rc = -3;
             for (int i = 0; i < n && rc == 0; ++i) {
                 const SolveOut& o = ho[i];
                 iout[5 * i] = o.init;
@@ -129,7 +133,7 @@ SDK_API int sdk_solve_nodes(const float* ref, const void* def, int def_is_f32, i
             }
         }
     }
-    hipStreamDestroy(st);
+    (void)hipStreamDestroy(st);
     return rc;
 }
 
@@ -143,10 +147,11 @@ SDK_API int sdk_pool(const float* ref, int w, int h, int dim, int lm, int n, con
     Dev<int32_t> dcxy(2 * (size_t)n);
     Dev<PoolEntry> pool(n);
     if (!dref.p || !I.p || !gx.p || !gy.p || !dcxy.p || !pool.p) return -3;
-    hipMemcpy(dref.p, ref, sizeof(float) * npx, hipMemcpyHostToDevice);
-    hipMemcpy(dcxy.p, cxy, sizeof(int32_t) * 2 * n, hipMemcpyHostToDevice);
-    launch_prep_ref_f32(dref.p, w, h, I.p, gx.p, gy.p, nullptr);
-    launch_pool_list(I.p, gx.p, gy.p, w, h, dcxy.p, n, dim, lm != 0, pool.p, nullptr);
+    if (hipMemcpy(dref.p, ref, sizeof(float) * npx, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dcxy.p, cxy, sizeof(int32_t) * 2 * n, hipMemcpyHostToDevice) != hipSuccess ||
+        launch_prep_ref_f32(dref.p, w, h, I.p, gx.p, gy.p, nullptr) != hipSuccess ||
+        launch_pool_list(I.p, gx.p, gy.p, w, h, dcxy.p, n, dim, lm != 0, pool.p, nullptr) != hipSuccess)
+        return -3;
     std::vector<PoolEntry> hp(n);
     if (hipMemcpy(hp.data(), pool.p, sizeof(PoolEntry) * n, hipMemcpyDeviceToHost) != hipSuccess) return -3;
     for (int i = 0; i < n; ++i) {
@@ -165,12 +170,14 @@ SDK_API int sdk_ref_planes(const uint8_t* img, const uint8_t* mask, int w, int h
     Dev<uint8_t> di(npx), dm(npx);
     Dev<float> dI(npx), dgx(npx), dgy(npx);
     if (!di.p || !dm.p || !dI.p || !dgx.p || !dgy.p) return -3;
-    hipMemcpy(di.p, img, npx, hipMemcpyHostToDevice);
-    if (mask) hipMemcpy(dm.p, mask, npx, hipMemcpyHostToDevice);
-    launch_prep_ref_u8(di.p, mask ? dm.p : nullptr, w, h, dI.p, dgx.p, dgy.p, nullptr);
-    hipMemcpy(I, dI.p, sizeof(float) * npx, hipMemcpyDeviceToHost);
-    hipMemcpy(gx, dgx.p, sizeof(float) * npx, hipMemcpyDeviceToHost);
-    return (int)hipMemcpy(gy, dgy.p, sizeof(float) * npx, hipMemcpyDeviceToHost);
+    if (hipMemcpy(di.p, img, npx, hipMemcpyHostToDevice) != hipSuccess) return -3;
+    if (mask && hipMemcpy(dm.p, mask, npx, hipMemcpyHostToDevice) != hipSuccess) return -3;
+    if (launch_prep_ref_u8(di.p, mask ? dm.p : nullptr, w, h, dI.p, dgx.p, dgy.p, nullptr) != hipSuccess) return -3;
+    if (hipMemcpy(I, dI.p, sizeof(float) * npx, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(gx, dgx.p, sizeof(float) * npx, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(gy, dgy.p, sizeof(float) * npx, hipMemcpyDeviceToHost) != hipSuccess)
+        return -3;
+    return 0;
 }
 
 // ---- an engine over frames already in HBM ----------------------------------
@@ -215,6 +222,17 @@ SDK_API int sdk_engine_run(void* h, const uint8_t* def, int on_device, const int
 // Last solve's timings: total, prep, seeding, prepass, mesh, path_a, path_b,
 // strain (host ms), the Path A solver launch (device ms), its grid nodes and
 // its IC-GN iterations.
+SDK_API int sdk_engine_set_flag(void* h, int flag, int value) {
+    if (!h) return -3;
+    switch (flag) {
+        case 0:
+            ((DevEngine*)h)->eng.speculate_path_b = value != 0;
+            return 0;
+        default:
+            return -3;
+    }
+}
+
 SDK_API int sdk_engine_timings(void* h, double* out14) {
     if (!h) return -3;
     const Timings& t = ((DevEngine*)h)->eng.timings();
@@ -244,8 +262,8 @@ SDK_API int sdk_synth_pair(uint64_t seed, int w, int h, int kind, const double* 
     if (!dr.p || !dd.p) return -3;
     const int rc = synth_pair(seed, w, h, f, noise_sigma, dr.p, dd.p, nullptr);
     if (rc != 0) return rc;
-    hipMemcpy(ref, dr.p, npx, hipMemcpyDeviceToHost);
-    return (int)hipMemcpy(def, dd.p, npx, hipMemcpyDeviceToHost);
+    if (hipMemcpy(ref, dr.p, npx, hipMemcpyDeviceToHost) != hipSuccess) return -3;
+    return hipMemcpy(def, dd.p, npx, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -3;
 }
 
 }  // extern "C"

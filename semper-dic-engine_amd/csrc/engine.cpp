@@ -691,256 +691,262 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
 
         // ---- Path B (full_field_path_b.cpp:148-412) ---------------------------
         const auto t_b = clk::now();
-        std::vector<float> hp(7 * NN);
-        std::vector<uint8_t> hsolved(NN);
-        ck(hipMemcpyAsync(hp.data(), gplanes_.p, sizeof(float) * 7 * NN, hipMemcpyDeviceToHost, st_));
-        ck(hipMemcpyAsync(hsolved.data(), g.solved, NN, hipMemcpyDeviceToHost, st_));
+        int open_nodes = 0;
+        ck(launch_count_open(g, N, counters_.p + 13, st_));
+        ck(hipMemcpyAsync(&open_nodes, counters_.p + 13, sizeof(int), hipMemcpyDeviceToHost, st_));
         ck(hipStreamSynchronize(st_));
-        const float* hu = hp.data();
-        const float* hv = hu + NN;
-        const float* hux = hv + NN;
-        const float* huy = hux + NN;
-        const float* hvx = huy + NN;
-        const float* hvy = hvx + NN;
-        const float* hcorr = hvy + NN;
-        {
-            struct SeedNode {
-                int x, y;
-                float u, v, ux, uy, vx, vy, corr;
-            };
-            struct Rel {
-                bool operator()(const SeedNode& a, const SeedNode& b) const {
-                    if (a.corr != b.corr) return a.corr > b.corr;
-                    if (a.y != b.y) return a.y > b.y;
-                    return a.x > b.x;
-                }
-            };
-            std::vector<unsigned char> claimed(hsolved.begin(), hsolved.end());
-            const int dx4[4] = {1, -1, 0, 0}, dy4[4] = {0, 0, 1, -1};
-            std::vector<SeedNode> boundary, global;
-            const SeedNode pc{pcx, pcy, gU, gV, 0, 0, 0, 0, 0};
-            bool pc_queued = false;
-            for (int y = 0; y < gh; ++y)
-                for (int x = 0; x < gw; ++x) {
-                    const size_t i = (size_t)y * gw + x;
-                    if (!hsolved[i] || hcorr[i] < 0.f) continue;
-                    bool touch = false;
-                    for (int k = 0; k < 4; ++k) {
-                        const int nx = x + dx4[k], ny = y + dy4[k];
-                        if (nx >= 0 && nx < gw && ny >= 0 && ny < gh && !hsolved[(size_t)ny * gw + nx]) {
-                            touch = true;
-                            break;
-                        }
-                    }
-                    if (touch) boundary.push_back({x, y, hu[i], hv[i], hux[i], huy[i], hvx[i], hvy[i], hcorr[i]});
-                }
-            if (boundary.empty()) {
-                struct C {
-                    int ix, iy;
-                    float u, v, dist, mag;
-                };
-                std::vector<C> cands;
-                const float gcx = p.rect_x + (gw / 2.f) * p.step, gcy = p.rect_y + (gh / 2.f) * p.step;
-                for (size_t i = 0; i < seeds.ref_x.size(); ++i) {
-                    const float fx = seeds.ref_x[i], fy = seeds.ref_y[i];
-                    const int ix = (int)std::round((fx - p.rect_x) / (float)p.step);
-                    const int iy = (int)std::round((fy - p.rect_y) / (float)p.step);
-                    if (ix < 0 || ix >= gw || iy < 0 || iy >= gh || hsolved[(size_t)iy * gw + ix]) continue;
-                    const float du = seeds.def_x[i] - fx, dv = seeds.def_y[i] - fy;
-                    const float wx = p.rect_x + ix * p.step, wy = p.rect_y + iy * p.step;
-                    const float dc = std::sqrt((wx - gcx) * (wx - gcx) + (wy - gcy) * (wy - gcy));
-                    cands.push_back({ix, iy, du, dv, dc, std::sqrt(du * du + dv * dv)});
-                }
-                if (cands.empty()) {
-                    global.push_back(pc);
-                    pc_queued = true;
-                } else {
-                    std::sort(cands.begin(), cands.end(), [](const C& a, const C& b) {
-                        const float ma = std::floor(a.mag), mb = std::floor(b.mag);
-                        if (ma != mb) return ma < mb;
-                        if (a.dist != b.dist) return a.dist < b.dist;
-                        if (a.iy != b.iy) return a.iy < b.iy;
-                        return a.ix < b.ix;
-                    });
-                    for (size_t k = 0; k < cands.size() && k < 5; ++k)
-                        global.push_back({cands[k].ix, cands[k].iy, cands[k].u, cands[k].v, 0, 0, 0, 0, 0});
-                }
-            }
-            std::priority_queue<SeedNode, std::vector<SeedNode>, Rel> q;
-            for (const auto& b : boundary) q.push(b);
-            struct Cell {
-                Task t;
-                SolveOut r;
-                int from;   // flat index of the node whose warp seeded the guess (-1: a global seed)
-            };
-            // commit one solved cell: host mirror, queue, counters
-            std::vector<int> commit_nodes;
-            std::vector<SolveOut> commit_res;
-            auto commit = [&](const Cell& c) {
-                if (!c.r.init) return;
-                st_b.record(c.r);
-                commit_nodes.push_back(c.t.node);
-                commit_res.push_back(c.r);
-                if (c.r.status == 0 && c.r.corr <= kCorrAccept) {
-                    st_b.points++;
-                    q.push({c.t.node % gw, c.t.node / gw, c.r.p[0], c.r.p[1], c.r.p[2], c.r.p[3], c.r.p[4], c.r.p[5],
-                            c.r.corr});
-                }
-            };
-            auto make_task = [&](int x, int y, const float* gs) {
-                Task t = {};
-                t.cx = p.rect_x + x * p.step;
-                t.cy = p.rect_y + y * p.step;
-                t.pool = y * gw + x;
-                t.mode = NO_SEARCH;
-                std::copy(gs, gs + 6, t.g);
-                t.node = t.pool;
-                return t;
-            };
-            // the guess a popped node hands to its neighbour: its warp carried by the grid offset
-            auto seed_task = [&](const SeedNode& cur, int nx, int ny) {
-                const float ddx = (float)((nx - cur.x) * p.step), ddy = (float)((ny - cur.y) * p.step);
-                const float gs[6] = {cur.u + cur.ux * ddx + cur.uy * ddy, cur.v + cur.vx * ddx + cur.vy * ddy,
-                                     cur.ux, cur.uy, cur.vx, cur.vy};
-                return make_task(nx, ny, gs);
-            };
-            // Speculative first claims. The queue pops the nodes solved before Path B in
-            // (corr, y, x) order, and pushes never reorder them, so an unsolved node is
-            // claimed by the first of its pre-solved neighbours in that order unless a node
-            // solved inside Path B claims it earlier. All such first claims are solved in
-            // one launch here; the round replay below takes their results when the claim
-            // matches and solves only the other claims. Each task's result depends on the
-            // task alone, so the replay commits exactly what round-by-round solving would.
-            std::vector<int> spec_from(NN, -1);
-            std::vector<SolveOut> spec_res;
-            std::vector<int> spec_slot(NN, -1);
+        std::vector<float> hp(7 * NN);
+        // with every node solved or skipped there is nothing to claim, and no global seed to try
+        if (open_nodes > 0) {
+            std::vector<uint8_t> hsolved(NN);
+            ck(hipMemcpyAsync(hp.data(), gplanes_.p, sizeof(float) * 7 * NN, hipMemcpyDeviceToHost, st_));
+            ck(hipMemcpyAsync(hsolved.data(), g.solved, NN, hipMemcpyDeviceToHost, st_));
+            ck(hipStreamSynchronize(st_));
+            const float* hu = hp.data();
+            const float* hv = hu + NN;
+            const float* hux = hv + NN;
+            const float* huy = hux + NN;
+            const float* hvx = huy + NN;
+            const float* hvy = hvx + NN;
+            const float* hcorr = hvy + NN;
             {
-                std::vector<Task> spec;
-                std::vector<int> spec_node;
+                struct SeedNode {
+                    int x, y;
+                    float u, v, ux, uy, vx, vy, corr;
+                };
+                struct Rel {
+                    bool operator()(const SeedNode& a, const SeedNode& b) const {
+                        if (a.corr != b.corr) return a.corr > b.corr;
+                        if (a.y != b.y) return a.y > b.y;
+                        return a.x > b.x;
+                    }
+                };
+                std::vector<unsigned char> claimed(hsolved.begin(), hsolved.end());
+                const int dx4[4] = {1, -1, 0, 0}, dy4[4] = {0, 0, 1, -1};
+                std::vector<SeedNode> boundary, global;
+                const SeedNode pc{pcx, pcy, gU, gV, 0, 0, 0, 0, 0};
+                bool pc_queued = false;
                 for (int y = 0; y < gh; ++y)
                     for (int x = 0; x < gw; ++x) {
-                        const int i = y * gw + x;
-                        if (claimed[i]) continue;
-                        int best = -1;
+                        const size_t i = (size_t)y * gw + x;
+                        if (!hsolved[i] || hcorr[i] < 0.f) continue;
+                        bool touch = false;
                         for (int k = 0; k < 4; ++k) {
                             const int nx = x + dx4[k], ny = y + dy4[k];
-                            if (nx < 0 || nx >= gw || ny < 0 || ny >= gh) continue;
-                            const int j = ny * gw + nx;
-                            if (!hsolved[j] || hcorr[j] < 0.f) continue;
-                            if (best < 0) {
-                                best = j;
-                                continue;
+                            if (nx >= 0 && nx < gw && ny >= 0 && ny < gh && !hsolved[(size_t)ny * gw + nx]) {
+                                touch = true;
+                                break;
                             }
-                            const int bx = best % gw, by = best / gw;
-                            const bool better = hcorr[j] != hcorr[best] ? hcorr[j] < hcorr[best]
-                                                : ny != by           ? ny < by
-                                                                     : nx < bx;
-                            if (better) best = j;
                         }
-                        if (best < 0) continue;
-                        const SeedNode m{best % gw, best / gw, hu[best], hv[best], hux[best], huy[best],
-                                         hvx[best], hvy[best], hcorr[best]};
-                        spec_from[i] = best;
-                        spec_slot[i] = (int)spec.size();
-                        spec.push_back(seed_task(m, x, y));
+                        if (touch) boundary.push_back({x, y, hu[i], hv[i], hux[i], huy[i], hvx[i], hvy[i], hcorr[i]});
                     }
-                if (!spec.empty() && solve_tasks(spec, spec_res, p.use6, p.subset, tol) != 0) return kErrInit;
-                tm_.path_b_spec = (int)spec.size();
-            }
-            auto flush_commits = [&]() {
-                if (commit_nodes.empty()) return;
-                const int n = (int)commit_nodes.size();
-                if (!nodes_d_.reserve(n) || !out_d_.reserve(n)) throw HipFail{hipErrorOutOfMemory};
-                ck(hipMemcpyAsync(nodes_d_.p, commit_nodes.data(), sizeof(int) * n, hipMemcpyHostToDevice, st_));
-                ck(hipMemcpyAsync(out_d_.p, commit_res.data(), sizeof(SolveOut) * n, hipMemcpyHostToDevice, st_));
-                ck(launch_apply(g, nodes_d_.p, out_d_.p, n, -1, true, st_));
-                ck(hipStreamSynchronize(st_));
-                commit_nodes.clear();
-                commit_res.clear();
-            };
-            for (const auto& s : global) {
-                const int flat = s.y * gw + s.x;
-                if (claimed[flat]) continue;
-                claimed[flat] = 1;
-                const float gs[6] = {s.u, s.v, 0, 0, 0, 0};
-                Cell c{make_task(s.x, s.y, gs), {}, -1};
-                std::vector<SolveOut> r1;
-                if (solve_tasks({c.t}, r1, p.use6, p.subset, tol) != 0) return kErrInit;
-                c.r = r1[0];
-                commit(c);
-            }
-            if (q.empty() && !global.empty() && !pc_queued) {
-                const int flat = pc.y * gw + pc.x;
-                if (pc.x >= 0 && pc.x < gw && pc.y >= 0 && pc.y < gh && !claimed[flat]) {
+                if (boundary.empty()) {
+                    struct C {
+                        int ix, iy;
+                        float u, v, dist, mag;
+                    };
+                    std::vector<C> cands;
+                    const float gcx = p.rect_x + (gw / 2.f) * p.step, gcy = p.rect_y + (gh / 2.f) * p.step;
+                    for (size_t i = 0; i < seeds.ref_x.size(); ++i) {
+                        const float fx = seeds.ref_x[i], fy = seeds.ref_y[i];
+                        const int ix = (int)std::round((fx - p.rect_x) / (float)p.step);
+                        const int iy = (int)std::round((fy - p.rect_y) / (float)p.step);
+                        if (ix < 0 || ix >= gw || iy < 0 || iy >= gh || hsolved[(size_t)iy * gw + ix]) continue;
+                        const float du = seeds.def_x[i] - fx, dv = seeds.def_y[i] - fy;
+                        const float wx = p.rect_x + ix * p.step, wy = p.rect_y + iy * p.step;
+                        const float dc = std::sqrt((wx - gcx) * (wx - gcx) + (wy - gcy) * (wy - gcy));
+                        cands.push_back({ix, iy, du, dv, dc, std::sqrt(du * du + dv * dv)});
+                    }
+                    if (cands.empty()) {
+                        global.push_back(pc);
+                        pc_queued = true;
+                    } else {
+                        std::sort(cands.begin(), cands.end(), [](const C& a, const C& b) {
+                            const float ma = std::floor(a.mag), mb = std::floor(b.mag);
+                            if (ma != mb) return ma < mb;
+                            if (a.dist != b.dist) return a.dist < b.dist;
+                            if (a.iy != b.iy) return a.iy < b.iy;
+                            return a.ix < b.ix;
+                        });
+                        for (size_t k = 0; k < cands.size() && k < 5; ++k)
+                            global.push_back({cands[k].ix, cands[k].iy, cands[k].u, cands[k].v, 0, 0, 0, 0, 0});
+                    }
+                }
+                std::priority_queue<SeedNode, std::vector<SeedNode>, Rel> q;
+                for (const auto& b : boundary) q.push(b);
+                struct Cell {
+                    Task t;
+                    SolveOut r;
+                    int from;   // flat index of the node whose warp seeded the guess (-1: a global seed)
+                };
+                // commit one solved cell: host mirror, queue, counters
+                std::vector<int> commit_nodes;
+                std::vector<SolveOut> commit_res;
+                auto commit = [&](const Cell& c) {
+                    if (!c.r.init) return;
+                    st_b.record(c.r);
+                    commit_nodes.push_back(c.t.node);
+                    commit_res.push_back(c.r);
+                    if (c.r.status == 0 && c.r.corr <= kCorrAccept) {
+                        st_b.points++;
+                        q.push({c.t.node % gw, c.t.node / gw, c.r.p[0], c.r.p[1], c.r.p[2], c.r.p[3], c.r.p[4], c.r.p[5],
+                                c.r.corr});
+                    }
+                };
+                auto make_task = [&](int x, int y, const float* gs) {
+                    Task t = {};
+                    t.cx = p.rect_x + x * p.step;
+                    t.cy = p.rect_y + y * p.step;
+                    t.pool = y * gw + x;
+                    t.mode = NO_SEARCH;
+                    std::copy(gs, gs + 6, t.g);
+                    t.node = t.pool;
+                    return t;
+                };
+                // the guess a popped node hands to its neighbour: its warp carried by the grid offset
+                auto seed_task = [&](const SeedNode& cur, int nx, int ny) {
+                    const float ddx = (float)((nx - cur.x) * p.step), ddy = (float)((ny - cur.y) * p.step);
+                    const float gs[6] = {cur.u + cur.ux * ddx + cur.uy * ddy, cur.v + cur.vx * ddx + cur.vy * ddy,
+                                         cur.ux, cur.uy, cur.vx, cur.vy};
+                    return make_task(nx, ny, gs);
+                };
+                // Speculative first claims. The queue pops the nodes solved before Path B in
+                // (corr, y, x) order, and pushes never reorder them, so an unsolved node is
+                // claimed by the first of its pre-solved neighbours in that order unless a node
+                // solved inside Path B claims it earlier. All such first claims are solved in
+                // one launch here; the round replay below takes their results when the claim
+                // matches and solves only the other claims. Each task's result depends on the
+                // task alone, so the replay commits exactly what round-by-round solving would.
+                std::vector<int> spec_from(NN, -1);
+                std::vector<SolveOut> spec_res;
+                std::vector<int> spec_slot(NN, -1);
+                {
+                    std::vector<Task> spec;
+                    for (int y = 0; y < gh && speculate_path_b; ++y)
+                        for (int x = 0; x < gw; ++x) {
+                            const int i = y * gw + x;
+                            if (claimed[i]) continue;
+                            int best = -1;
+                            for (int k = 0; k < 4; ++k) {
+                                const int nx = x + dx4[k], ny = y + dy4[k];
+                                if (nx < 0 || nx >= gw || ny < 0 || ny >= gh) continue;
+                                const int j = ny * gw + nx;
+                                if (!hsolved[j] || hcorr[j] < 0.f) continue;
+                                if (best < 0) {
+                                    best = j;
+                                    continue;
+                                }
+                                const int bx = best % gw, by = best / gw;
+                                const bool better = hcorr[j] != hcorr[best] ? hcorr[j] < hcorr[best]
+                                                    : ny != by           ? ny < by
+                                                                         : nx < bx;
+                                if (better) best = j;
+                            }
+                            if (best < 0) continue;
+                            const SeedNode m{best % gw, best / gw, hu[best], hv[best], hux[best], huy[best],
+                                             hvx[best], hvy[best], hcorr[best]};
+                            spec_from[i] = best;
+                            spec_slot[i] = (int)spec.size();
+                            spec.push_back(seed_task(m, x, y));
+                        }
+                    if (!spec.empty() && solve_tasks(spec, spec_res, p.use6, p.subset, tol) != 0) return kErrInit;
+                    tm_.path_b_spec = (int)spec.size();
+                }
+                auto flush_commits = [&]() {
+                    if (commit_nodes.empty()) return;
+                    const int n = (int)commit_nodes.size();
+                    if (!nodes_d_.reserve(n) || !out_d_.reserve(n)) throw HipFail{hipErrorOutOfMemory};
+                    ck(hipMemcpyAsync(nodes_d_.p, commit_nodes.data(), sizeof(int) * n, hipMemcpyHostToDevice, st_));
+                    ck(hipMemcpyAsync(out_d_.p, commit_res.data(), sizeof(SolveOut) * n, hipMemcpyHostToDevice, st_));
+                    ck(launch_apply(g, nodes_d_.p, out_d_.p, n, -1, true, st_));
+                    ck(hipStreamSynchronize(st_));
+                    commit_nodes.clear();
+                    commit_res.clear();
+                };
+                for (const auto& s : global) {
+                    const int flat = s.y * gw + s.x;
+                    if (claimed[flat]) continue;
                     claimed[flat] = 1;
-                    const float gs[6] = {pc.u, pc.v, 0, 0, 0, 0};
-                    Cell c{make_task(pc.x, pc.y, gs), {}, -1};
+                    const float gs[6] = {s.u, s.v, 0, 0, 0, 0};
+                    Cell c{make_task(s.x, s.y, gs), {}, -1};
                     std::vector<SolveOut> r1;
                     if (solve_tasks({c.t}, r1, p.use6, p.subset, tol) != 0) return kErrInit;
                     c.r = r1[0];
                     commit(c);
                 }
-            }
-            struct Round {
-                std::vector<Cell> cells;
-                bool solved = false;
-            };
-            std::deque<Round> rounds;
-            auto claim = [&]() -> bool {
-                if (q.empty()) return false;
-                Round r;
-                int popped = 0;
-                while (!q.empty() && popped < kPathBBatchNodes) {
-                    const SeedNode cur = q.top();
-                    q.pop();
-                    ++popped;
-                    for (int k = 0; k < 4; ++k) {
-                        const int nx = cur.x + dx4[k], ny = cur.y + dy4[k];
-                        if (nx < 0 || nx >= gw || ny < 0 || ny >= gh) continue;
-                        const int flat = ny * gw + nx;
-                        if (claimed[flat]) continue;
+                if (q.empty() && !global.empty() && !pc_queued) {
+                    const int flat = pc.y * gw + pc.x;
+                    if (pc.x >= 0 && pc.x < gw && pc.y >= 0 && pc.y < gh && !claimed[flat]) {
                         claimed[flat] = 1;
-                        r.cells.push_back(Cell{seed_task(cur, nx, ny), {}, cur.y * gw + cur.x});
+                        const float gs[6] = {pc.u, pc.v, 0, 0, 0, 0};
+                        Cell c{make_task(pc.x, pc.y, gs), {}, -1};
+                        std::vector<SolveOut> r1;
+                        if (solve_tasks({c.t}, r1, p.use6, p.subset, tol) != 0) return kErrInit;
+                        c.r = r1[0];
+                        commit(c);
                     }
                 }
-                rounds.push_back(std::move(r));
-                return true;
-            };
-            while (rounds.size() < 2 && claim()) {
-            }
-            while (!rounds.empty()) {
-                if (cancel.requested()) return kErrCancelled;
-                if (!rounds.front().solved) {
-                    // claims made by the speculated neighbour take its result; the rest are solved now
-                    std::vector<Task> tasks;
-                    std::vector<Cell*> miss;
-                    for (Round& r : rounds)
-                        if (!r.solved)
-                            for (Cell& c : r.cells) {
-                                const int nd = c.t.node;
-                                if (c.from >= 0 && spec_from[nd] == c.from) {
-                                    c.r = spec_res[spec_slot[nd]];
-                                } else {
-                                    tasks.push_back(c.t);
-                                    miss.push_back(&c);
-                                }
-                            }
-                    if (!tasks.empty()) {
-                        std::vector<SolveOut> res;
-                        if (solve_tasks(tasks, res, p.use6, p.subset, tol) != 0) return kErrInit;
-                        for (size_t k = 0; k < miss.size(); ++k) miss[k]->r = res[k];
-                        tm_.path_b_launches++;
-                        tm_.path_b_miss += (int)tasks.size();
+                struct Round {
+                    std::vector<Cell> cells;
+                    bool solved = false;
+                };
+                std::deque<Round> rounds;
+                auto claim = [&]() -> bool {
+                    if (q.empty()) return false;
+                    Round r;
+                    int popped = 0;
+                    while (!q.empty() && popped < kPathBBatchNodes) {
+                        const SeedNode cur = q.top();
+                        q.pop();
+                        ++popped;
+                        for (int k = 0; k < 4; ++k) {
+                            const int nx = cur.x + dx4[k], ny = cur.y + dy4[k];
+                            if (nx < 0 || nx >= gw || ny < 0 || ny >= gh) continue;
+                            const int flat = ny * gw + nx;
+                            if (claimed[flat]) continue;
+                            claimed[flat] = 1;
+                            r.cells.push_back(Cell{seed_task(cur, nx, ny), {}, cur.y * gw + cur.x});
+                        }
                     }
-                    for (Round& r : rounds) r.solved = true;
-                }
-                for (const Cell& c : rounds.front().cells) commit(c);
-                rounds.pop_front();
+                    rounds.push_back(std::move(r));
+                    return true;
+                };
                 while (rounds.size() < 2 && claim()) {
                 }
-                ticker.set(anchor_pts + st_a.points + st_b.points);
+                while (!rounds.empty()) {
+                    if (cancel.requested()) return kErrCancelled;
+                    if (!rounds.front().solved) {
+                        // claims made by the speculated neighbour take its result; the rest are solved now
+                        std::vector<Task> tasks;
+                        std::vector<Cell*> miss;
+                        for (Round& r : rounds)
+                            if (!r.solved)
+                                for (Cell& c : r.cells) {
+                                    const int nd = c.t.node;
+                                    if (c.from >= 0 && spec_from[nd] == c.from) {
+                                        c.r = spec_res[spec_slot[nd]];
+                                    } else {
+                                        tasks.push_back(c.t);
+                                        miss.push_back(&c);
+                                    }
+                                }
+                        if (!tasks.empty()) {
+                            std::vector<SolveOut> res;
+                            if (solve_tasks(tasks, res, p.use6, p.subset, tol) != 0) return kErrInit;
+                            for (size_t k = 0; k < miss.size(); ++k) miss[k]->r = res[k];
+                            tm_.path_b_launches++;
+                            tm_.path_b_miss += (int)tasks.size();
+                        }
+                        for (Round& r : rounds) r.solved = true;
+                    }
+                    for (const Cell& c : rounds.front().cells) commit(c);
+                    rounds.pop_front();
+                    while (rounds.size() < 2 && claim()) {
+                    }
+                    ticker.set(anchor_pts + st_a.points + st_b.points);
+                }
+                flush_commits();
             }
-            flush_commits();
         }
         tm_.path_b = ms_since(t_b);
         if (cancel.requested()) return kErrCancelled;

@@ -115,6 +115,9 @@ public:
         std::vector<unsigned char> ok;   // solved with corr >= 0 (before the median test)
     };
     bool keep_field = false;
+    // Solve Path B's first claims speculatively in one launch (off: every round
+    // is solved when it is reached; the committed field is the same either way).
+    bool speculate_path_b = true;
     Field last_field;
 
     // Median test, VSG strain and row-major packing of a node field given on

@@ -51,7 +51,23 @@ __global__ void k_valid(GridDev g, uint8_t* __restrict__ valid, int n) {
     valid[i] = (g.solved[i] && g.corr[i] >= 0.0f) ? 1 : 0;
 }
 
+// Nodes Path B can still claim (solved == 0), one atomic per wave.
+__global__ void k_count_open(GridDev g, int n, int* __restrict__ count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool open = i < n && !g.solved[i];
+    const unsigned long long m = __ballot(open);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(count, __popcll(m));
+}
+
+__global__ void k_zero1(int* p) { *p = 0; }
+
 }  // namespace
+
+hipError_t launch_count_open(GridDev g, int n, int* count, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, st, count);
+    if (n > 0) hipLaunchKernelGGL(k_count_open, dim3((n + 255) / 256), dim3(256), 0, st, g, n, count);
+    return hipGetLastError();
+}
 
 hipError_t launch_grid_init(GridDev g, const uint8_t* skip, int n, hipStream_t st) {
     if (n <= 0) return hipSuccess;

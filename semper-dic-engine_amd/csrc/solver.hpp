@@ -113,4 +113,6 @@ hipError_t launch_apply(GridDev g, const int* nodes, const SolveOut* res, int n,
                         hipStream_t st);
 // valid = solved && corr >= 0
 hipError_t launch_valid(GridDev g, uint8_t* valid, int n, hipStream_t st);
+// count of nodes with solved == 0 into *count (zeroed first by the launch)
+hipError_t launch_count_open(GridDev g, int n, int* count, hipStream_t st);
 }  // namespace sdic

@@ -176,7 +176,7 @@ def workload_module():
     return wl
 
 
-@pytest.mark.parametrize("name", ["c1_512_translate", "c2_2048_affine", "c4_4096_heterogeneous"])
+@pytest.mark.parametrize("name", ["c1_512_translate", "c2_2048_affine", "c4_4096_heterogeneous", "t_1024_noisy"])
 def test_workload_full_field_vs_oracle(dev, name):
     """Each benchmark workload at its full size through the engine against the
     oracle's run_full_field on the same bytes and stop rule: attempted-subset
@@ -199,8 +199,15 @@ def test_workload_full_field_vs_oracle(dev, name):
     met_g = np.zeros(24, np.float32)
     pr = np.array(prm, np.int32)
     n_g = dev.sdk_engine_run(h, P(dfm), 0, P(pr), P(out), out.size, P(met_g), 24, ctypes.c_float(tol))
+    tm = np.zeros(14, np.float64)
+    dev.sdk_engine_timings(h, P(tm))
     dev.sdk_engine_destroy(h)
     pts_g = out[: n_g * 8].reshape(-1, 8)
+    print(f"points gpu {n_g} oracle {n_o}; path B points {met_g[4]:.0f}/{met_o[4]:.0f}, speculative {tm[11]:.0f}, "
+          f"further launches {tm[12]:.0f} ({tm[13]:.0f} subsets)", flush=True)
+    if name == "t_1024_noisy":
+        # the flood ran, mostly on the speculative first claims
+        assert met_o[4] >= 20 and tm[11] >= 100 and tm[12] < tm[11]
     assert n_g > 0 and abs(n_g - n_o) <= max(2, n_o // 200), (n_g, n_o)
     assert met_g[0] == met_o[0]
     assert abs(float(met_g[3] + met_g[4] + met_g[19]) - float(met_o[3] + met_o[4] + met_o[19])) <= max(2, n_o // 200)
@@ -213,6 +220,37 @@ def test_workload_full_field_vs_oracle(dev, name):
     assert (d[:, :2].max(1) <= 1e-3).mean() >= 0.99
     assert d[:, 2:5].max() <= 1e-4, d[:, 2:5].max()
     assert d[:, 5].max() <= 1e-4
+
+
+def test_path_b_speculation_is_exact(dev):
+    """Path B with its first claims solved speculatively in one launch commits
+    the same field, bit for bit, as solving every round when it is reached
+    (full_field_path_b.cpp:148-412 order: rounds of 32, two in flight)."""
+    wl = workload_module()
+    name = "t_1024_noisy"
+    cfg = wl.config(name)
+    ref, dfm = wl.pair(dev, name)
+    pr = np.array(wl.solve_params(name), np.int32)
+    cap = (cfg["w"] // cfg["step"] + 1) * (cfg["h"] // cfg["step"] + 1) * 8
+    res = []
+    for spec in (1, 0):
+        h = dev.sdk_engine_create()
+        assert dev.sdk_engine_set_flag(h, 0, spec) == 0
+        assert dev.sdk_engine_set_reference(h, P(ref), 0, cfg["w"], cfg["h"], None) == 0
+        out = np.zeros(cap, np.float32)
+        met = np.zeros(24, np.float32)
+        n = dev.sdk_engine_run(h, P(dfm), 0, P(pr), P(out), out.size, P(met), 24, ctypes.c_float(cfg["tol"]))
+        tm = np.zeros(14, np.float64)
+        dev.sdk_engine_timings(h, P(tm))
+        dev.sdk_engine_destroy(h)
+        print(f"speculate={spec}: points {n}, path B points {met[4]:.0f}, speculative {tm[11]:.0f}, "
+              f"further launches {tm[12]:.0f}, path B {tm[6]:.2f} ms", flush=True)
+        res.append((n, out[: n * 8].copy(), met.copy(), tm.copy()))
+    (n1, o1, m1, t1), (n0, o0, m0, t0) = res
+    assert n1 == n0 and np.array_equal(o1, o0)
+    keep = [i for i in range(24) if i not in (9, 10, 11, 12, 13, 14, 17, 18)]   # the timing slots differ
+    assert np.array_equal(m1[keep], m0[keep])
+    assert m1[4] >= 20 and t1[11] > 0 and t0[11] == 0 and t1[12] < t0[12]
 
 
 # ---- C ABI contract on the GPU (tests/c/contract.c) ----------------------------
