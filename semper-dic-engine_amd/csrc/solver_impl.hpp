@@ -112,6 +112,75 @@ __device__ __forceinline__ float warped(int c, float a, float b, float t, float 
     return (float)((double)c + ((double)a * (double)x + (double)b * (double)y + (double)t));
 }
 
+// ---- the lean path's arithmetic ------------------------------------------
+// The Keys weights in Horner form and the stencil as fused multiply-adds: the
+// same polynomials as keys4 / k6f*, rounded once per step instead of twice.
+__device__ __forceinline__ void keys4_fma(float s, float* w) {
+    const float s2 = s * s;
+    w[0] = s * __builtin_fmaf(__builtin_fmaf(-0.5f, s, 1.0f), s, -0.5f);
+    w[1] = __builtin_fmaf(s2, __builtin_fmaf(1.5f, s, -2.5f), 1.0f);
+    w[2] = s * __builtin_fmaf(__builtin_fmaf(-1.5f, s, 2.0f), s, 0.5f);
+    w[3] = s2 * __builtin_fmaf(0.5f, s, -0.5f);
+}
+__device__ __forceinline__ float k6f0_fma(float s) {
+    return __builtin_fmaf(s * s, __builtin_fmaf(4.0f / 3.0f, s, -7.0f / 3.0f), 1.0f);
+}
+__device__ __forceinline__ float k6f1_fma(float s) {
+    return __builtin_fmaf(s, __builtin_fmaf(s, __builtin_fmaf(-7.0f / 12.0f, s, 3.0f), -59.0f / 12.0f), 2.5f);
+}
+__device__ __forceinline__ float k6f2_fma(float s) {
+    return __builtin_fmaf(s, __builtin_fmaf(s, __builtin_fmaf(1.0f / 12.0f, s, -2.0f / 3.0f), 7.0f / 4.0f), -1.5f);
+}
+
+template <bool K6, typename TD>
+__device__ __forceinline__ float interp_fma(const TD* __restrict__ img, int w, float x, float y) {
+    const int xi = (int)x, yi = (int)y;
+    const float dx = x - (float)xi, dy = y - (float)yi;
+    if constexpr (!K6) {
+        float wx[4], wy[4];
+        keys4_fma(dx, wx);
+        keys4_fma(dy, wy);
+        float val = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const TD* r = img + (yi + j - 1) * w + xi;
+            float rv = (float)r[-1] * wx[0];
+            rv = __builtin_fmaf((float)r[0], wx[1], rv);
+            rv = __builtin_fmaf((float)r[1], wx[2], rv);
+            rv = __builtin_fmaf((float)r[2], wx[3], rv);
+            val = __builtin_fmaf(rv, wy[j], val);
+        }
+        return val;
+    } else {
+        const float wx[6] = {k6f2_fma(dx + 2.0f), k6f1_fma(dx + 1.0f), k6f0_fma(dx),
+                             k6f0_fma(1.0f - dx), k6f1_fma(2.0f - dx), k6f2_fma(3.0f - dx)};
+        const float wy[6] = {k6f2_fma(dy + 2.0f), k6f1_fma(dy + 1.0f), k6f0_fma(dy),
+                             k6f0_fma(1.0f - dy), k6f1_fma(2.0f - dy), k6f2_fma(3.0f - dy)};
+        float val = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            const TD* r = img + (yi + j - 2) * w + xi;
+            float rv = (float)r[-2] * wx[0];
+            rv = __builtin_fmaf((float)r[-1], wx[1], rv);
+            rv = __builtin_fmaf((float)r[0], wx[2], rv);
+            rv = __builtin_fmaf((float)r[1], wx[3], rv);
+            rv = __builtin_fmaf((float)r[2], wx[4], rv);
+            rv = __builtin_fmaf((float)r[3], wx[5], rv);
+            val = __builtin_fmaf(rv, wy[j], val);
+        }
+        return val;
+    }
+}
+
+// The warp position of `warped` with the same roundings: a*x is exact in
+// double (a 24-bit by 6-bit product), so fma(a, x, b*y) rounds exactly as
+// a*x + b*y does.
+__device__ __forceinline__ float warped_fma(double c, double a, double b, double t, double x, double y) {
+    return (float)(c + (__builtin_fma(a, x, b * y) + t));
+}
+
+typedef _Float16 sdic_h2 __attribute__((ext_vector_type(2)));
+
 struct Res {
     float p[6];
     float corr;
@@ -734,26 +803,25 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
     if (t.mode == AUTO_SEARCH) return false;
     const int n = dim * dim;
     float nref[PPL], gxs[PPL], gys[PPL], val[PPL];
-    uint64_t inmask = 0;
+    sdic_h2 xy[PPL];   // the pixel's offset from the centre, (x, y), exact in fp16
     const float mean_r = pe->mean, inv_r = pe->inv_std;
     const int q64 = 64 / dim, r64 = 64 % dim;
-    const int q0 = lane / dim, r0 = lane % dim;
     int bad = 0;
     {
-        int q = q0, r = r0;
+        int q = lane / dim, r = lane % dim;
 #pragma unroll
         for (int j = 0; j < PPL; ++j) {
             nref[j] = 0.0f;
             gxs[j] = 0.0f;
             gys[j] = 0.0f;
             val[j] = 0.0f;
+            xy[j] = sdic_h2{(_Float16)(r - half), (_Float16)(q - half)};
             if (lane + 64 * j < n) {
                 const int a = (t.cy + q - half) * w + t.cx + r - half;
                 const float ri = refI[a];
                 nref[j] = (ri - mean_r) * inv_r;
                 gxs[j] = refGx[a] * inv_r;
                 gys[j] = refGy[a] * inv_r;
-                inmask |= (1ull << j);
                 bad += (ri < kGhostWallThreshold) ? 1 : 0;
             }
             r += r64;
@@ -769,87 +837,97 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
     if (lane < 36) Hs[lane] = pe->hstep[lane];
     __syncthreads();
     const int min_valid = (int)((float)n * kMinValidPixelFraction);
+    const double cxd = (double)t.cx, cyd = (double)t.cy, hd = (double)half;
     float W[9] = {1.0f + t.g[2], t.g[3], t.g[0], t.g[4], 1.0f + t.g[5], t.g[1], 0.0f, 0.0f, 1.0f};
     float final_score = 1.0f;
     int status = 1, iters = kIcgnMaxIter;
     float corr = 0.0f;
     bool done = false;
     for (int it = 0; it < kIcgnMaxIter && !done; ++it) {
+        const double w0 = W[0], w1 = W[1], w2 = W[2], w3 = W[3], w4 = W[4], w5 = W[5];
+        // Interior: the four warped corners lie 1 px inside the 4 px guard, so
+        // (the map being affine) every pixel does, and no pixel test is needed.
+        bool interior = true;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double x = (k & 1) ? hd : -hd, y = (k & 2) ? hd : -hd;
+            const float fx = warped_fma(cxd, w0, w1, w2, x, y), fy = warped_fma(cyd, w3, w4, w5, x, y);
+            interior = interior && fx + 0.5f >= 5.0f && fx + 0.5f < (float)(w - 5) && fy + 0.5f >= 5.0f &&
+                       fy + 0.5f < (float)(h - 5);
+        }
+        interior = __builtin_amdgcn_readfirstlane(interior ? 1 : 0) != 0;
         float sum = 0.0f;
-        int cnt = 0;
-        bool all = true;
-        {
-            int qq = q0, rq = r0;
-            asm volatile("" : "+v"(qq), "+v"(rq));
+        int nv = n;
+        if (interior) {
+#pragma unroll
+            for (int j = 0; j < PPL; ++j) {
+                if (lane + 64 * j < n) {
+                    sdic_h2 c = xy[j];
+                    asm volatile("" : "+v"(c));
+                    const double x = (double)(float)c.x, y = (double)(float)c.y;
+                    const float v = interp_fma<K6>(def, w, warped_fma(cxd, w0, w1, w2, x, y),
+                                                   warped_fma(cyd, w3, w4, w5, x, y));
+                    val[j] = v;
+                    sum += v;
+                }
+            }
+            sum = wave_sum(sum);
+        } else {
+            int cnt = 0;
 #pragma unroll
             for (int j = 0; j < PPL; ++j) {
                 val[j] = 0.0f;
-                if ((inmask >> j) & 1) {
-                    const float x = (float)(rq - half), y = (float)(qq - half);
-                    const float fx = warped(t.cx, W[0], W[1], W[2], x, y);
-                    const float fy = warped(t.cy, W[3], W[4], W[5], x, y);
+                if (lane + 64 * j < n) {
+                    sdic_h2 c = xy[j];
+                    asm volatile("" : "+v"(c));
+                    const double x = (double)(float)c.x, y = (double)(float)c.y;
+                    const float fx = warped_fma(cxd, w0, w1, w2, x, y);
+                    const float fy = warped_fma(cyd, w3, w4, w5, x, y);
                     if (inside_guard(fx, fy, w, h)) {
-                        const float v = interp<K6>(def, w, fx, fy);
+                        const float v = interp_fma<K6>(def, w, fx, fy);
                         val[j] = v;
                         sum += v;
                         cnt++;
-                    } else {
-                        all = false;
                     }
                 }
-                rq += r64;
-                qq += q64;
-                if (rq >= dim) {
-                    rq -= dim;
-                    qq++;
-                }
             }
+            sum = wave_sum(sum);
+            nv = wave_sum_i(cnt);
+            if (nv < min_valid) {
+                status = 1;
+                corr = 2.0f;
+                iters = it;
+                done = true;
+                break;
+            }
+            if (nv != n) return false;   // a masked iteration
         }
-        sum = wave_sum(sum);
-        const int nv = wave_sum_i(cnt);
-        if (nv < min_valid) {
-            status = 1;
-            corr = 2.0f;
-            iters = it;
-            done = true;
-            break;
-        }
-        if (nv != n) return false;   // a masked iteration
-        (void)all;
         const float mean = sum / (float)nv;
         float s2 = 0.0f;
 #pragma unroll
         for (int j = 0; j < PPL; ++j) {
             const float d = val[j] - mean;
-            s2 += ((inmask >> j) & 1) ? d * d : 0.0f;
+            s2 = (lane + 64 * j < n) ? __builtin_fmaf(d, d, s2) : s2;
         }
         float dstd = sqrtf(wave_sum(s2) / (float)nv);
         if (dstd < 1e-5f) dstd = 1.0f;
         const float inv = 1.0f / dstd;
         float err = 0.0f, dp[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-        {
-            int qq = q0, rq = r0;
-            asm volatile("" : "+v"(qq), "+v"(rq));
 #pragma unroll
-            for (int j = 0; j < PPL; ++j) {
-                if ((inmask >> j) & 1) {
-                    const float x = (float)(rq - half), y = (float)(qq - half);
-                    const float nd = (val[j] - mean) * inv;
-                    const float d = nref[j] - nd;
-                    err += d * d;
-                    dp[0] += gxs[j] * d;
-                    dp[1] += gys[j] * d;
-                    dp[2] += (gxs[j] * x) * d;
-                    dp[3] += (gxs[j] * y) * d;
-                    dp[4] += (gys[j] * x) * d;
-                    dp[5] += (gys[j] * y) * d;
-                }
-                rq += r64;
-                qq += q64;
-                if (rq >= dim) {
-                    rq -= dim;
-                    qq++;
-                }
+        for (int j = 0; j < PPL; ++j) {
+            if (lane + 64 * j < n) {
+                sdic_h2 c = xy[j];
+                asm volatile("" : "+v"(c));
+                const float x = (float)c.x, y = (float)c.y;
+                const float d = nref[j] - (val[j] - mean) * inv;
+                err = __builtin_fmaf(d, d, err);
+                const float gd = gxs[j] * d, hdd = gys[j] * d;
+                dp[0] += gd;
+                dp[1] += hdd;
+                dp[2] = __builtin_fmaf(gd, x, dp[2]);
+                dp[3] = __builtin_fmaf(gd, y, dp[3]);
+                dp[4] = __builtin_fmaf(hdd, x, dp[4]);
+                dp[5] = __builtin_fmaf(hdd, y, dp[5]);
             }
         }
         err = wave_sum(err);
