@@ -47,7 +47,8 @@ SDK_API void sdk_engine_destroy(void* h);
 SDK_API int sdk_engine_set_reference(void* h, const uint8_t* img, int on_device, int w, int hh, const uint8_t* mask);
 SDK_API int sdk_engine_run(void* h, const uint8_t* def, int on_device, const int32_t* params, float* out, int cap,
                            float* metrics, int metrics_len, float tol);
-/* Engine switches: flag 0 = speculative Path B first claims (default 1). */
+/* Engine switches: flag 0 = speculative Path B first claims (default 1);
+   flag 1 = `out` of sdk_engine_run is device memory (default 0). */
 SDK_API int sdk_engine_set_flag(void* h, int flag, int value);
 
 /* Phase timings of the last run, 14 doubles: total, prep, seeding, prepass,

@@ -228,6 +228,9 @@ SDK_API int sdk_engine_set_flag(void* h, int flag, int value) {
         case 0:
             ((DevEngine*)h)->eng.speculate_path_b = value != 0;
             return 0;
+        case 1:
+            ((DevEngine*)h)->eng.out_on_device = value != 0;
+            return 0;
         default:
             return -3;
     }

@@ -192,7 +192,7 @@ int Engine::ensure_grid(const Params& p, const uint8_t* mask, bool& cached) {
     if (cached) return 0;
     grid_valid_ = false;
     const int N = gw * gh;
-    if (!skip_d_.reserve(N) || !pool_.reserve(N) || !counters_.reserve(16)) return kErrInit;
+    if (!skip_d_.reserve(N) || !pool_.reserve(N) || !counters_.reserve(32)) return kErrInit;
     const uint8_t* md = nullptr;
     if (mask) {
         if (!mask_d_.reserve(n)) return kErrInit;
@@ -223,7 +223,7 @@ int Engine::solve_tasks(const std::vector<Task>& tasks, std::vector<SolveOut>& o
     const int n = (int)tasks.size();
     out.assign(n, SolveOut{});
     if (n == 0) return 0;
-    if (!tasks_d_.reserve(n) || !out_d_.reserve(n) || !bail_.reserve(n) || !counters_.reserve(16)) return kErrInit;
+    if (!tasks_d_.reserve(n) || !out_d_.reserve(n) || !bail_.reserve(n) || !counters_.reserve(32)) return kErrInit;
     ck(hipMemcpyAsync(tasks_d_.p, tasks.data(), sizeof(Task) * n, hipMemcpyHostToDevice, st_));
     SolveArgs a = {};
     a.tasks = tasks_d_.p;
@@ -279,7 +279,7 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
         Ticker ticker(progress, total_valid);
         const size_t NN = (size_t)N;
         if (!gplanes_.reserve(7 * NN) || !gflags_.reserve(7 * NN) || !giters_.reserve(NN) || !guess_.reserve(6 * NN) ||
-            !tmp_.reserve(6 * NN) || !strain_.reserve(3 * NN) || !bail_.reserve(NN) || !counters_.reserve(16))
+            !tmp_.reserve(6 * NN) || !strain_.reserve(3 * NN) || !bail_.reserve(NN) || !counters_.reserve(32))
             return kErrInit;
         GridDev g;
         g.u = gplanes_.p;
@@ -695,9 +695,9 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
         ck(launch_count_open(g, N, counters_.p + 13, st_));
         ck(hipMemcpyAsync(&open_nodes, counters_.p + 13, sizeof(int), hipMemcpyDeviceToHost, st_));
         ck(hipStreamSynchronize(st_));
-        std::vector<float> hp(7 * NN);
         // with every node solved or skipped there is nothing to claim, and no global seed to try
         if (open_nodes > 0) {
+            std::vector<float> hp(7 * NN);
             std::vector<uint8_t> hsolved(NN);
             ck(hipMemcpyAsync(hp.data(), gplanes_.p, sizeof(float) * 7 * NN, hipMemcpyDeviceToHost, st_));
             ck(hipMemcpyAsync(hsolved.data(), g.solved, NN, hipMemcpyDeviceToHost, st_));
@@ -976,12 +976,16 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             last_field.corr.assign(all.begin() + 6 * NN, all.begin() + 7 * NN);
         }
         // median test: five passes, each from the previous pass's flags; a
-        // pass after one that rejected nothing changes nothing.
-        ck(hipMemsetAsync(counters_.p + 12, 0, sizeof(int32_t), st_));
+        // pass after one that rejected nothing changes nothing, and its kernel
+        // only copies the flags.
+        int* pass_fail = counters_.p + 16;   // one rejection count per pass
+        int* pack_counts = counters_.p + 24; // written before the cut, dropped before the cut
+        ck(launch_zero(pass_fail, 16, st_));
         uint8_t* src = valid_d;
         uint8_t* dst = ok_a;
         for (int pass = 0; pass < kOutlierMaxPasses; ++pass) {
-            ck(launch_outlier_pass(gw, gh, g.u, g.v, src, dst, counters_.p + 12, st_));
+            ck(launch_outlier_pass(gw, gh, g.u, g.v, src, dst, pass > 0 ? pass_fail + pass - 1 : nullptr,
+                                   pass_fail + pass, st_));
             src = dst;
             dst = (dst == ok_a) ? ok_b : ok_a;
         }
@@ -990,39 +994,23 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
         float* exy = eyy + NN;
         ck(launch_vsg(gw, gh, p.step, p.strain_window, vsg_count(p.step, p.strain_window), g.u, g.v, src, exx, eyy, exy,
                       st_));
-        std::vector<float> hs(3 * NN);
-        std::vector<uint8_t> hok(NN), hvalid(NN);
-        int dropped_outlier = 0;
-        ck(hipMemcpyAsync(hp.data(), gplanes_.p, sizeof(float) * 7 * NN, hipMemcpyDeviceToHost, st_));
-        ck(hipMemcpyAsync(hs.data(), strain_.p, sizeof(float) * 3 * NN, hipMemcpyDeviceToHost, st_));
-        ck(hipMemcpyAsync(hok.data(), src, NN, hipMemcpyDeviceToHost, st_));
-        ck(hipMemcpyAsync(&dropped_outlier, counters_.p + 12, sizeof(int), hipMemcpyDeviceToHost, st_));
+        // row-major pack on the device, then one copy of the packed points
+        const int cap_points = out_capacity / 8;
+        if (!pack_out_.reserve(8 * NN) || !pack_blocks_.reserve((size_t)pack_blocks(N))) throw HipFail{hipErrorOutOfMemory};
+        ck(launch_pack(gw, gh, p.rect_x, p.rect_y, p.step, cap_points, src, g.u, g.v, exx, eyy, exy, g.corr,
+                       pack_blocks_.p, pack_counts, pack_out_.p, st_));
+        int hc[32];
+        ck(hipMemcpyAsync(hc, counters_.p, sizeof(hc), hipMemcpyDeviceToHost, st_));
         ck(hipStreamSynchronize(st_));
-        int valid = 0, dropped_post = 0;
-        bool trunc = false;
-        for (int y = 0; y < gh && !trunc; ++y)
-            for (int x = 0; x < gw; ++x) {
-                const size_t i = (size_t)y * gw + x;
-                if (!hok[i]) continue;
-                if ((valid + 1) * 8 > out_capacity) {
-                    trunc = true;
-                    break;
-                }
-                if (hs[i] <= kStrainFail) {
-                    dropped_post++;
-                    continue;
-                }
-                float* o = out + (size_t)valid * 8;
-                o[0] = (float)(p.rect_x + x * p.step);
-                o[1] = (float)(p.rect_y + y * p.step);
-                o[2] = hp[i];
-                o[3] = hp[NN + i];
-                o[4] = hs[i];
-                o[5] = hs[NN + i];
-                o[6] = hs[2 * NN + i];
-                o[7] = hp[6 * NN + i];
-                valid++;
-            }
+        int dropped_outlier = 0;
+        for (int pass = 0; pass < kOutlierMaxPasses; ++pass) dropped_outlier += hc[16 + pass];
+        const int valid = std::min(hc[24], cap_points);
+        const int dropped_post = hc[25];
+        if (valid > 0) {
+            ck(hipMemcpyAsync(out, pack_out_.p, sizeof(float) * 8 * (size_t)valid,
+                              out_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st_));
+            ck(hipStreamSynchronize(st_));
+        }
         tm_.strain = ms_since(t_s);
         ticker.stop();
         if (progress) {
@@ -1076,7 +1064,7 @@ int Engine::strain_pack(const Field& f, int window, float* out, int out_capacity
     const size_t NN = (size_t)N;
     if (N <= 0) return 0;
     try {
-        if (!gplanes_.reserve(7 * NN) || !gflags_.reserve(7 * NN) || !strain_.reserve(3 * NN) || !counters_.reserve(16))
+        if (!gplanes_.reserve(7 * NN) || !gflags_.reserve(7 * NN) || !strain_.reserve(3 * NN) || !counters_.reserve(32))
             return kErrInit;
         float* u = gplanes_.p;
         float* v = u + NN;
@@ -1090,7 +1078,7 @@ int Engine::strain_pack(const Field& f, int window, float* out, int out_capacity
         uint8_t* src = ok0;
         uint8_t* dst = ok1;
         for (int pass = 0; pass < kOutlierMaxPasses; ++pass) {
-            ck(launch_outlier_pass(gw, gh, u, v, src, dst, counters_.p + 12, st_));
+            ck(launch_outlier_pass(gw, gh, u, v, src, dst, nullptr, counters_.p + 12, st_));
             src = dst;
             dst = (dst == ok1) ? ok2 : ok1;
         }

@@ -118,6 +118,8 @@ public:
     // Solve Path B's first claims speculatively in one launch (off: every round
     // is solved when it is reached; the committed field is the same either way).
     bool speculate_path_b = true;
+    // `out` of run() is device memory (the packed points stay in HBM)
+    bool out_on_device = false;
     Field last_field;
 
     // Median test, VSG strain and row-major packing of a node field given on
@@ -152,6 +154,8 @@ private:
     DevBuf<int32_t> giters_;
     DevBuf<float> guess_, tmp_;    // 6 planes each
     DevBuf<float> strain_;         // exx eyy exy
+    DevBuf<float> pack_out_;       // packed output points, 8 floats each
+    DevBuf<int32_t> pack_blocks_;  // per-block write counts of the pack
     DevBuf<Task> tasks_d_;
     DevBuf<SolveOut> out_d_;
     DevBuf<int32_t> bail_, counters_, nodes_d_;

@@ -61,7 +61,18 @@ __global__ void k_count_open(GridDev g, int n, int* __restrict__ count) {
 
 __global__ void k_zero1(int* p) { *p = 0; }
 
+__global__ void k_zero_n(int* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 }  // namespace
+
+hipError_t launch_zero(int* p, int n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_n, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+    return hipGetLastError();
+}
 
 hipError_t launch_count_open(GridDev g, int n, int* count, hipStream_t st) {
     hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, st, count);

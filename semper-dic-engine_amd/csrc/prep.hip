@@ -219,23 +219,33 @@ __device__ __forceinline__ float lv_at(const PyrDev& p, bool def, int l, int idx
 }
 
 // ZNCC of the 32 x 32 template at (cx, cy) on level l against the window
-// shifted by (sx, sy); false if either leaves the level's image.
+// shifted by (sx, sy), evaluated by one wave (lanes stride over the template);
+// false if either leaves the level's image. Every pyramid value is a dyadic
+// rational with at most 6 fractional bits and at most 8 integer bits, so the
+// five sums are exact in double in any order: the score does not depend on
+// how the lanes split the work.
 __device__ bool zncc(const PyrDev& p, int l, int cx, int cy, int sx, int sy, double& out, double* tstd) {
     const int w = p.w[l], h = p.h[l];
     const int x0 = cx - kPatchHalf, y0 = cy - kPatchHalf, x1 = cx + kPatchHalf, y1 = cy + kPatchHalf;
     if (x0 < 0 || y0 < 0 || x1 > w || y1 > h) return false;
     if (x0 + sx < 0 || y0 + sy < 0 || x1 + sx > w || y1 + sy > h) return false;
+    const int lane = threadIdx.x & 63, side = 2 * kPatchHalf;
     double sr = 0, sd = 0, srr = 0, sdd = 0, srd = 0;
-    for (int y = y0; y < y1; ++y)
-        for (int x = x0; x < x1; ++x) {
-            const double a = lv_at(p, false, l, y * w + x), b = lv_at(p, true, l, (y + sy) * w + x + sx);
-            sr += a;
-            sd += b;
-            srr += a * a;
-            sdd += b * b;
-            srd += a * b;
-        }
-    const double n = (double)(4 * kPatchHalf * kPatchHalf);
+    for (int k = lane; k < side * side; k += 64) {
+        const int y = y0 + k / side, x = x0 + k % side;
+        const double a = lv_at(p, false, l, y * w + x), b = lv_at(p, true, l, (y + sy) * w + x + sx);
+        sr += a;
+        sd += b;
+        srr += a * a;
+        sdd += b * b;
+        srd += a * b;
+    }
+    sr = wave_sum_d(sr);
+    sd = wave_sum_d(sd);
+    srr = wave_sum_d(srr);
+    sdd = wave_sum_d(sdd);
+    srd = wave_sum_d(srd);
+    const double n = (double)(side * side);
     const double vr = srr - sr * sr / n, vd = sdd - sd * sd / n;
     if (tstd) *tstd = sqrt(fmax(0.0, vr / n));
     if (!(vr > 1e-9) || !(vd > 1e-9)) {
@@ -263,14 +273,21 @@ __device__ __forceinline__ bool better(const Cand& a, const Cand& b) {
     return a.order < b.order;
 }
 
-__global__ __launch_bounds__(256) void k_patch(PyrDev p, const int* __restrict__ centres, PatchOut* out) {
-    __shared__ Cand red[256];
+// One block of 16 waves per patch. Each level's candidates are spread over the
+// waves (one candidate per wave at a time); the best by better() is reduced
+// over the waves, and better() is a total order, so the choice is the serial
+// scan's.
+constexpr int kPatchWaves = 16;
+
+__global__ __launch_bounds__(64 * kPatchWaves) void k_patch(PyrDev p, const int* __restrict__ centres,
+                                                            PatchOut* out) {
+    __shared__ Cand red[kPatchWaves];
     __shared__ int s_sx, s_sy, s_ok;
     __shared__ double s_best;
-    const int pi = blockIdx.x, tid = threadIdx.x;
+    const int pi = blockIdx.x, lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int px = centres[2 * pi], py = centres[2 * pi + 1];
     const int L = p.levels;
-    if (tid == 0) {
+    if (threadIdx.x == 0) {
         s_sx = 0;
         s_sy = 0;
         s_ok = 1;
@@ -284,12 +301,12 @@ __global__ __launch_bounds__(256) void k_patch(PyrDev p, const int* __restrict__
         const int csx = s_sx, csy = s_sy;
         Cand best;
         best.any = 0;
-        for (int k = tid; k < ns; k += blockDim.x) {
+        for (int k = wid; k < ns; k += kPatchWaves) {
             const int sy = csy - rad + k / side, sx = csx - rad + k % side;
-            double s;
-            if (!zncc(p, l, px >> l, py >> l, sx, sy, s, nullptr)) continue;
+            double sc;
+            if (!zncc(p, l, px >> l, py >> l, sx, sy, sc, nullptr)) continue;
             Cand c;
-            c.s = s;
+            c.s = sc;
             c.r2 = (long)sx * sx + (long)sy * sy;
             c.order = k;
             c.sx = sx;
@@ -297,36 +314,49 @@ __global__ __launch_bounds__(256) void k_patch(PyrDev p, const int* __restrict__
             c.any = 1;
             if (better(c, best)) best = c;
         }
-        red[tid] = best;
+        if (lane == 0) red[wid] = best;
         __syncthreads();
-        for (int m = 128; m >= 1; m >>= 1) {
-            if (tid < m && better(red[tid + m], red[tid])) red[tid] = red[tid + m];
-            __syncthreads();
-        }
-        if (tid == 0) {
-            if (!red[0].any) {
+        if (threadIdx.x == 0) {
+            Cand b = red[0];
+            for (int k = 1; k < kPatchWaves; ++k)
+                if (better(red[k], b)) b = red[k];
+            if (!b.any) {
                 s_ok = 0;
             } else {
-                s_best = red[0].s;
-                s_sx = red[0].sx * (l > 0 ? 2 : 1);
-                s_sy = red[0].sy * (l > 0 ? 2 : 1);
+                s_best = b.s;
+                s_sx = b.sx * (l > 0 ? 2 : 1);
+                s_sy = b.sy * (l > 0 ? 2 : 1);
             }
         }
         __syncthreads();
     }
-    if (tid == 0) {
+    // level-0 statistics and the four neighbours of the best shift, one wave each
+    __shared__ double s_vals[5];
+    __shared__ int s_has[5];
+    if (s_ok && wid < 5) {
+        const int dx[5] = {0, -1, 1, 0, 0}, dy[5] = {0, 0, 0, -1, 1};
+        double v = 0.0, t = 0.0;
+        const bool ok = zncc(p, 0, px, py, s_sx + dx[wid], s_sy + dy[wid], v, wid == 0 ? &t : nullptr);
+        if (lane == 0) {
+            s_vals[wid] = wid == 0 ? t : v;
+            s_has[wid] = ok ? 1 : 0;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
         PatchOut o = {};
         o.ok = s_ok;
         o.sx = s_sx;
         o.sy = s_sy;
         o.best = s_best;
         if (s_ok) {
-            double t0;
-            zncc(p, 0, px, py, s_sx, s_sy, t0, &o.tstd);
-            o.has_x = zncc(p, 0, px, py, s_sx - 1, s_sy, o.xm, nullptr) &&
-                      zncc(p, 0, px, py, s_sx + 1, s_sy, o.xp, nullptr);
-            o.has_y = zncc(p, 0, px, py, s_sx, s_sy - 1, o.ym, nullptr) &&
-                      zncc(p, 0, px, py, s_sx, s_sy + 1, o.yp, nullptr);
+            o.tstd = s_vals[0];
+            o.xm = s_vals[1];
+            o.xp = s_vals[2];
+            o.ym = s_vals[3];
+            o.yp = s_vals[4];
+            o.has_x = s_has[1] && s_has[2];
+            o.has_y = s_has[3] && s_has[4];
         }
         out[pi] = o;
     }
@@ -393,7 +423,7 @@ hipError_t launch_downsample_f32(const float* src, int w, int h, float* dst, hip
 hipError_t launch_patch_search(const PyrDev& p, const int* centres, int npatch, PatchOut* out,
                                hipStream_t st) {
     if (npatch <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_patch, dim3(npatch), dim3(256), 0, st, p, centres, out);
+    hipLaunchKernelGGL(k_patch, dim3(npatch), dim3(64 * kPatchWaves), 0, st, p, centres, out);
     return hipGetLastError();
 }
 

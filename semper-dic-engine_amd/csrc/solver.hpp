@@ -99,8 +99,18 @@ hipError_t launch_guess(const GuessArgs& a, GridDev g, uint8_t* in_mesh, float* 
                         float* guy, float* gvx, float* gvy, float* tmp, hipStream_t st);
 
 // strain.hip
+// one median-test pass; with prev_fail non-null and *prev_fail == 0 the pass
+// copies the flags (a pass after one that rejected nothing changes nothing)
 hipError_t launch_outlier_pass(int gw, int gh, const float* u, const float* v, const uint8_t* ok_in,
-                               uint8_t* ok_out, int* nfail, hipStream_t st);
+                               uint8_t* ok_out, const int* prev_fail, int* nfail, hipStream_t st);
+// Row-major pack of the surviving nodes into out (8 floats per point) with the
+// capacity rule; counts[0] = points to write before the capacity cut (all
+// strain-valid nodes), counts[1] += strain failures before the cut (counts[1]
+// must be zero on entry). block_w holds pack_blocks(gw * gh) ints.
+int pack_blocks(int n);
+hipError_t launch_pack(int gw, int gh, int rect_x, int rect_y, int step, int cap_points, const uint8_t* ok,
+                       const float* u, const float* v, const float* exx, const float* eyy, const float* exy,
+                       const float* corr, int* block_w, int* counts, float* out, hipStream_t st);
 hipError_t launch_vsg(int gw, int gh, int step, int window, int expected, const float* u, const float* v,
                       const uint8_t* valid, float* exx, float* eyy, float* exy, hipStream_t st);
 
@@ -113,6 +123,8 @@ hipError_t launch_apply(GridDev g, const int* nodes, const SolveOut* res, int n,
                         hipStream_t st);
 // valid = solved && corr >= 0
 hipError_t launch_valid(GridDev g, uint8_t* valid, int n, hipStream_t st);
+// n ints set to zero by a fill kernel (no memset node in a captured graph)
+hipError_t launch_zero(int* p, int n, hipStream_t st);
 // count of nodes with solved == 0 into *count (zeroed first by the launch)
 hipError_t launch_count_open(GridDev g, int n, int* count, hipStream_t st);
 }  // namespace sdic

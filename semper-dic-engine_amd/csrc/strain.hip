@@ -45,9 +45,15 @@ __device__ __forceinline__ float norm_residual(float c, const float (&nb)[kNb], 
 }
 
 __global__ void k_outlier(int gw, int gh, const float* __restrict__ u, const float* __restrict__ v,
-                          const uint8_t* __restrict__ ok_in, uint8_t* __restrict__ ok_out, int* nfail) {
+                          const uint8_t* __restrict__ ok_in, uint8_t* __restrict__ ok_out,
+                          const int* __restrict__ prev_fail, int* nfail) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= gw * gh) return;
+    // a pass after one that rejected nothing sees the same flags: it is the identity
+    if (prev_fail && *prev_fail == 0) {
+        ok_out[idx] = ok_in[idx];
+        return;
+    }
     if (!ok_in[idx]) {
         ok_out[idx] = 0;
         return;
@@ -141,13 +147,137 @@ __global__ void k_vsg(int gw, int gh, int step, int window, int expected, const 
     exy[idx] = (float)(0.5 * (uy + vx + ux * uy + vx * vy));
 }
 
+// ---- row-major pack (full_field_solver_stats.cpp:21-91) ---------------------
+// A node that passed the median test is written when its strain succeeded
+// (exx > kStrainFail) and dropped (slot 20) otherwise. Packing stops at the
+// first such node that does not fit: with pos = the number of written nodes
+// before a node and cap = out_capacity / 8, a node is written iff pos < cap,
+// and a dropped node is counted iff pos < cap.
+constexpr int kPackChunk = 1024;   // nodes per block (256 threads x 4)
+
+__device__ __forceinline__ void pack_flags(int i, int n, const uint8_t* ok, const float* exx, int& wr, int& dr) {
+    wr = 0;
+    dr = 0;
+    if (i < n && ok[i]) {
+        if (exx[i] > kStrainFail) wr = 1;
+        else dr = 1;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pack_count(int n, const uint8_t* __restrict__ ok,
+                                                    const float* __restrict__ exx, int* __restrict__ block_w) {
+    __shared__ int part[4];
+    const int base = blockIdx.x * kPackChunk;
+    int c = 0;
+    for (int k = threadIdx.x; k < kPackChunk; k += 256) {
+        int wr, dr;
+        pack_flags(base + k, n, ok, exx, wr, dr);
+        c += wr;
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) c += __shfl_xor(c, m, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) block_w[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+// exclusive scan of the block counts in place, total into *total (one block)
+__global__ __launch_bounds__(1024) void k_pack_scan(int nb, int* __restrict__ block_w, int* __restrict__ total) {
+    __shared__ int carry;
+    __shared__ int wsum[16];
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int base = 0; base < nb; base += 1024) {
+        const int i = base + threadIdx.x;
+        const int v = i < nb ? block_w[i] : 0;
+        int incl = v;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int t = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += t;
+        }
+        if (lane == 63) wsum[wid] = incl;
+        __syncthreads();
+        int before = carry;
+        for (int k = 0; k < wid; ++k) before += wsum[k];
+        if (i < nb) block_w[i] = before + incl - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry = before + incl;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(256) void k_pack_write(int n, int gw, int rect_x, int rect_y, int step, int cap,
+                                                    const uint8_t* __restrict__ ok, const float* __restrict__ u,
+                                                    const float* __restrict__ v, const float* __restrict__ exx,
+                                                    const float* __restrict__ eyy, const float* __restrict__ exy,
+                                                    const float* __restrict__ corr,
+                                                    const int* __restrict__ block_w, float* __restrict__ out,
+                                                    int* __restrict__ dropped) {
+    __shared__ int wsum[4];
+    __shared__ int running;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int base = blockIdx.x * kPackChunk;
+    if (threadIdx.x == 0) running = block_w[blockIdx.x];
+    __syncthreads();
+    int drops = 0;
+    // four rounds of 256 consecutive nodes, in row-major order
+    for (int r = 0; r < kPackChunk / 256; ++r) {
+        const int i = base + r * 256 + threadIdx.x;
+        int wr, dr;
+        pack_flags(i, n, ok, exx, wr, dr);
+        const unsigned long long m = __ballot(wr);
+        const int in_wave = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[wid] = __popcll(m);
+        __syncthreads();
+        int pos = running + in_wave;
+        for (int k = 0; k < wid; ++k) pos += wsum[k];
+        if (wr && pos < cap) {
+            float* o = out + (size_t)pos * 8;
+            o[0] = (float)(rect_x + (i % gw) * step);
+            o[1] = (float)(rect_y + (i / gw) * step);
+            o[2] = u[i];
+            o[3] = v[i];
+            o[4] = exx[i];
+            o[5] = eyy[i];
+            o[6] = exy[i];
+            o[7] = corr[i];
+        }
+        if (dr && pos < cap) drops++;
+        __syncthreads();
+        if (threadIdx.x == 0) running += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        __syncthreads();
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) drops += __shfl_xor(drops, m, 64);
+    if (lane == 0 && drops) atomicAdd(dropped, drops);
+}
+
 }  // namespace
 
+int pack_blocks(int n) { return (n + kPackChunk - 1) / kPackChunk; }
+
+hipError_t launch_pack(int gw, int gh, int rect_x, int rect_y, int step, int cap_points, const uint8_t* ok,
+                       const float* u, const float* v, const float* exx, const float* eyy, const float* exy,
+                       const float* corr, int* block_w, int* counts, float* out, hipStream_t st) {
+    const int n = gw * gh;
+    const int nb = pack_blocks(n);
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_count, dim3(nb), dim3(256), 0, st, n, ok, exx, block_w);
+    hipLaunchKernelGGL(k_pack_scan, dim3(1), dim3(1024), 0, st, nb, block_w, counts);
+    hipLaunchKernelGGL(k_pack_write, dim3(nb), dim3(256), 0, st, n, gw, rect_x, rect_y, step, cap_points, ok, u, v,
+                       exx, eyy, exy, corr, block_w, out, counts + 1);
+    return hipGetLastError();
+}
+
 hipError_t launch_outlier_pass(int gw, int gh, const float* u, const float* v, const uint8_t* ok_in,
-                               uint8_t* ok_out, int* nfail, hipStream_t st) {
+                               uint8_t* ok_out, const int* prev_fail, int* nfail, hipStream_t st) {
     const int n = gw * gh;
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_outlier, dim3((n + 255) / 256), dim3(256), 0, st, gw, gh, u, v, ok_in, ok_out, nfail);
+    hipLaunchKernelGGL(k_outlier, dim3((n + 255) / 256), dim3(256), 0, st, gw, gh, u, v, ok_in, ok_out, prev_fail,
+                       nfail);
     return hipGetLastError();
 }
 
