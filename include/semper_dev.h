@@ -47,6 +47,11 @@ SDK_API void sdk_engine_destroy(void* h);
 SDK_API int sdk_engine_set_reference(void* h, const uint8_t* img, int on_device, int w, int hh, const uint8_t* mask);
 SDK_API int sdk_engine_run(void* h, const uint8_t* def, int on_device, const int32_t* params, float* out, int cap,
                            float* metrics, int metrics_len, float tol);
+/* The solver's wave reductions on nw x 64 floats: every lane's float sum into
+   out and integer sum of (int)in into iout (a self-check of the cross-lane
+   network). */
+SDK_API int sdk_wave_sum_probe(const float* in, int nw, float* out, int32_t* iout);
+
 /* Engine switches: flag 0 = speculative Path B first claims (default 1);
    flag 1 = `out` of sdk_engine_run is device memory (default 0). */
 SDK_API int sdk_engine_set_flag(void* h, int flag, int value);

@@ -222,6 +222,22 @@ SDK_API int sdk_engine_run(void* h, const uint8_t* def, int on_device, const int
 // Last solve's timings: total, prep, seeding, prepass, mesh, path_a, path_b,
 // strain (host ms), the Path A solver launch (device ms), its grid nodes and
 // its IC-GN iterations.
+// The wave reductions of the solver on nw x 64 host floats: out receives every
+// lane's float sum, iout every lane's integer sum of (int)in.
+SDK_API int sdk_wave_sum_probe(const float* in, int nw, float* out, int32_t* iout) {
+    if (!in || !out || !iout || nw <= 0) return -3;
+    const size_t n = (size_t)nw * 64;
+    Dev<float> din(n), dout(n);
+    Dev<int32_t> di(n);
+    if (!din.p || !dout.p || !di.p) return -3;
+    if (hipMemcpy(din.p, in, sizeof(float) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        launch_wave_sum_probe(din.p, nw, dout.p, di.p, nullptr) != hipSuccess ||
+        hipMemcpy(out, dout.p, sizeof(float) * n, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(iout, di.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost) != hipSuccess)
+        return -3;
+    return 0;
+}
+
 SDK_API int sdk_engine_set_flag(void* h, int flag, int value) {
     if (!h) return -3;
     switch (flag) {

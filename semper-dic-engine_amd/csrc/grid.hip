@@ -3,6 +3,7 @@
 
 #include "common.hpp"
 #include "solver.hpp"
+#include "wave.hpp"
 
 namespace sdic {
 
@@ -66,7 +67,23 @@ __global__ void k_zero_n(int* p, int n) {
     if (i < n) p[i] = 0;
 }
 
+// wave_sum / wave_sum_i over one wave per 64 inputs; every lane writes its result
+__global__ void k_wave_sum_probe(const float* __restrict__ in, int nw, float* __restrict__ out,
+                                 int* __restrict__ iout) {
+    const int wv = blockIdx.x;
+    if (wv >= nw) return;
+    const int i = wv * 64 + threadIdx.x;
+    out[i] = wave_sum(in[i]);
+    iout[i] = wave_sum_i((int)in[i]);
+}
+
 }  // namespace
+
+hipError_t launch_wave_sum_probe(const float* in, int nw, float* out, int* iout, hipStream_t st) {
+    if (nw <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_wave_sum_probe, dim3(nw), dim3(64), 0, st, in, nw, out, iout);
+    return hipGetLastError();
+}
 
 hipError_t launch_zero(int* p, int n, hipStream_t st) {
     if (n <= 0) return hipSuccess;

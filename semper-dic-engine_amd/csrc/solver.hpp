@@ -123,6 +123,8 @@ hipError_t launch_apply(GridDev g, const int* nodes, const SolveOut* res, int n,
                         hipStream_t st);
 // valid = solved && corr >= 0
 hipError_t launch_valid(GridDev g, uint8_t* valid, int n, hipStream_t st);
+// wave_sum and wave_sum_i of nw waves of 64 inputs, every lane's result
+hipError_t launch_wave_sum_probe(const float* in, int nw, float* out, int* iout, hipStream_t st);
 // n ints set to zero by a fill kernel (no memset node in a captured graph)
 hipError_t launch_zero(int* p, int n, hipStream_t st);
 // count of nodes with solved == 0 into *count (zeroed first by the launch)

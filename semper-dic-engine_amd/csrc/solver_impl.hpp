@@ -172,6 +172,69 @@ __device__ __forceinline__ float interp_fma(const TD* __restrict__ img, int w, f
     }
 }
 
+// The same interpolation through a buffer resource over the deformed plane:
+// one 32-bit byte offset per sample (yi - 1) * w + xi - 1 in a VGPR, the row
+// step j * w in the wave-uniform SGPR offset, and one load per stencil row
+// (4 bytes / 4 floats for Keys 4x4, 8 bytes / 6 floats for Keys 6x6). The
+// sample lies inside the 4 px guard, so every row read is inside the plane.
+__device__ __forceinline__ float ub(uint32_t v, int k) { return (float)((v >> (8 * k)) & 0xFFu); }
+
+template <bool K6, typename TD>
+__device__ __forceinline__ float interp_buf(__amdgpu_buffer_rsrc_t rs, int w, float x, float y) {
+    const int xi = (int)x, yi = (int)y;
+    const float dx = __builtin_amdgcn_fractf(x), dy = __builtin_amdgcn_fractf(y);
+    constexpr int R = K6 ? 6 : 4, H = K6 ? 2 : 1;
+    const int base = ((yi - H) * w + xi - H) * (int)sizeof(TD);
+    float wx[R], wy[R];
+    if constexpr (!K6) {
+        keys4_fma(dx, wx);
+        keys4_fma(dy, wy);
+    } else {
+        const float ax[6] = {k6f2_fma(dx + 2.0f), k6f1_fma(dx + 1.0f), k6f0_fma(dx),
+                             k6f0_fma(1.0f - dx), k6f1_fma(2.0f - dx), k6f2_fma(3.0f - dx)};
+        const float ay[6] = {k6f2_fma(dy + 2.0f), k6f1_fma(dy + 1.0f), k6f0_fma(dy),
+                             k6f0_fma(1.0f - dy), k6f1_fma(2.0f - dy), k6f2_fma(3.0f - dy)};
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            wx[k] = ax[k];
+            wy[k] = ay[k];
+        }
+    }
+    float val = 0.0f;
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const int so = j * w * (int)sizeof(TD);
+        float px[R];
+        if constexpr (sizeof(TD) == 1) {
+            if constexpr (!K6) {
+                const uint32_t b = __builtin_amdgcn_raw_buffer_load_b32(rs, base, so, 0);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) px[k] = ub(b, k);
+            } else {
+                const auto b = __builtin_amdgcn_raw_buffer_load_b64(rs, base, so, 0);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) px[k] = ub((uint32_t)b[0], k);
+                px[4] = ub((uint32_t)b[1], 0);
+                px[5] = ub((uint32_t)b[1], 1);
+            }
+        } else {
+            const auto b = __builtin_amdgcn_raw_buffer_load_b128(rs, base, so, 0);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) px[k] = __uint_as_float((uint32_t)b[k]);
+            if constexpr (K6) {
+                const auto c = __builtin_amdgcn_raw_buffer_load_b64(rs, base + 16, so, 0);
+                px[4] = __uint_as_float((uint32_t)c[0]);
+                px[5] = __uint_as_float((uint32_t)c[1]);
+            }
+        }
+        float rv = px[0] * wx[0];
+#pragma unroll
+        for (int k = 1; k < R; ++k) rv = __builtin_fmaf(px[k], wx[k], rv);
+        val = __builtin_fmaf(rv, wy[j], val);
+    }
+    return val;
+}
+
 // The warp position of `warped` with the same roundings: a*x is exact in
 // double (a 24-bit by 6-bit product), so fma(a, x, b*y) rounds exactly as
 // a*x + b*y does.
@@ -792,9 +855,9 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
                                            const float* __restrict__ refGx,
                                            const float* __restrict__ refGy,
                                            const TD* __restrict__ def, int w, int h, int dim,
-                                           bool lm, float conv_tol, float* Hs, SolveOut& out) {
+                                           bool lm, float conv_tol, SolveOut& out) {
     const int lane = threadIdx.x & 63;
-    const PoolEntry* pe = pool + t.pool;
+    const PoolEntry* pe = pool + __builtin_amdgcn_readfirstlane(t.pool);
     out.init = 0;
     out.simplex = 0;
     if (!pe->valid) return true;
@@ -834,10 +897,20 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
     }
     // Any Ghost Wall pixel makes every iteration a masked one.
     if (wave_sum_i(bad) != 0) return false;
-    if (lane < 36) Hs[lane] = pe->hstep[lane];
-    __syncthreads();
+    const float* __restrict__ hs = pe->hstep;   // wave-uniform: scalar loads
+    // the texture test at convergence: min(sum gx^2, sum gy^2) > 0 over the subset
+    float tgx = 0.0f, tgy = 0.0f;
+#pragma unroll
+    for (int j = 0; j < PPL; ++j) {
+        tgx = __builtin_fmaf(gxs[j], gxs[j], tgx);
+        tgy = __builtin_fmaf(gys[j], gys[j], tgy);
+    }
+    const bool textured = fminf(wave_sum(tgx), wave_sum(tgy)) > 0.0f;
     const int min_valid = (int)((float)n * kMinValidPixelFraction);
     const double cxd = (double)t.cx, cyd = (double)t.cy, hd = (double)half;
+    const float cxf = (float)t.cx, cyf = (float)t.cy;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void*)def, (short)0, (int)((size_t)w * h * sizeof(TD)), 0x00020000);
     float W[9] = {1.0f + t.g[2], t.g[3], t.g[0], t.g[4], 1.0f + t.g[5], t.g[1], 0.0f, 0.0f, 1.0f};
     float final_score = 1.0f;
     int status = 1, iters = kIcgnMaxIter;
@@ -859,19 +932,21 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
         float sum = 0.0f;
         int nv = n;
         if (interior) {
+            // Inside the margin the warp is evaluated in FP32: it can differ from
+            // the FP64 position by one float ulp, far inside the 1 px margin.
 #pragma unroll
             for (int j = 0; j < PPL; ++j) {
                 if (lane + 64 * j < n) {
                     sdic_h2 c = xy[j];
                     asm volatile("" : "+v"(c));
-                    const double x = (double)(float)c.x, y = (double)(float)c.y;
-                    const float v = interp_fma<K6>(def, w, warped_fma(cxd, w0, w1, w2, x, y),
-                                                   warped_fma(cyd, w3, w4, w5, x, y));
+                    const float x = (float)c.x, y = (float)c.y;
+                    const float fx = cxf + __builtin_fmaf(W[0], x, __builtin_fmaf(W[1], y, W[2]));
+                    const float fy = cyf + __builtin_fmaf(W[3], x, __builtin_fmaf(W[4], y, W[5]));
+                    const float v = interp_buf<K6, TD>(rs, w, fx, fy);
                     val[j] = v;
                     sum += v;
                 }
             }
-            sum = wave_sum(sum);
         } else {
             int cnt = 0;
 #pragma unroll
@@ -891,7 +966,6 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
                     }
                 }
             }
-            sum = wave_sum(sum);
             nv = wave_sum_i(cnt);
             if (nv < min_valid) {
                 status = 1;
@@ -902,6 +976,7 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
             }
             if (nv != n) return false;   // a masked iteration
         }
+        sum = wave_sum(sum);
         const float mean = sum / (float)nv;
         float s2 = 0.0f;
 #pragma unroll
@@ -939,7 +1014,7 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
         for (int r = 0; r < 6; ++r) {
             float acc = 0.0f;
 #pragma unroll
-            for (int c = 0; c < 6; ++c) acc += Hs[r * 6 + c] * dp[c];
+            for (int c = 0; c < 6; ++c) acc += hs[r * 6 + c] * dp[c];
             d[r] = -acc;
         }
         const float dW[9] = {1.0f + d[2], d[3], d[0], d[4], 1.0f + d[5], d[1], 0.0f, 0.0f, 1.0f};
@@ -972,18 +1047,8 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
 #pragma unroll
         for (int k = 0; k < 6; ++k) nrm += d[k] * d[k];
         if (sqrtf(nrm) < conv_tol) {
-            float sx = 0.0f, sy = 0.0f;
-#pragma unroll
-            for (int j = 0; j < PPL; ++j) {
-                float a = gxs[j], b = gys[j];
-                asm volatile("" : "+v"(a), "+v"(b));
-                sx += a * a;
-                sy += b * b;
-            }
-            sx = wave_sum(sx);
-            sy = wave_sum(sy);
             iters = it + 1;
-            if (fminf(sx, sy) <= 0.0f || !finite_f(final_score)) {
+            if (!textured || !finite_f(final_score)) {
                 status = 1;
                 corr = 2.0f;
             } else {
@@ -1080,7 +1145,6 @@ __device__ __forceinline__ void commit(const SolveArgs& a, int id, const SolveOu
 // a.bail for k_slow.
 template <int PPL, bool K6, typename TD>
 __global__ __launch_bounds__(64) void k_fast(SolveArgs a) {
-    __shared__ float Hs[36];
     const int k = xcd_remap(blockIdx.x, gridDim.x);
     if (k >= a.nitems) return;
     const int id = a.items ? a.items[k] : k;
@@ -1088,7 +1152,7 @@ __global__ __launch_bounds__(64) void k_fast(SolveArgs a) {
     if (!item_task(a, id, t)) return;
     SolveOut o = {};
     const bool fin = fast_solve<PPL, K6, TD>(t, a.pool, a.refI, a.refGx, a.refGy, (const TD*)a.def, a.w,
-                                             a.h, a.dim, a.lm != 0, a.tol, Hs, o);
+                                             a.h, a.dim, a.lm != 0, a.tol, o);
     if ((threadIdx.x & 63) != 0) return;
     if (!fin) {
         a.bail[atomicAdd(a.nbail, 1)] = id;

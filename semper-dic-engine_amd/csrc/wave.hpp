@@ -13,16 +13,38 @@ namespace sdic {
 
 constexpr int kWave = 64;
 
+// Cross-lane moves without LDS: DPP within rows of 16 (quad_perm [1,0,3,2],
+// quad_perm [2,3,0,1], row_half_mirror, row_mirror), then gfx950's
+// v_permlane16_swap / v_permlane32_swap across rows and halves. After the two
+// quad steps every lane of a quad holds the quad's sum, so the mirrors pair
+// equal partial sums exactly as xor 4 and xor 8 would; each swap step adds
+// (low half + high half) in the same operand order in both halves. The sum is
+// therefore the same bits in all 64 lanes.
+template <int Ctrl>
+__device__ __forceinline__ uint32_t dpp_u(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, Ctrl, 0xF, 0xF, false);
+}
+
 __device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);
-    return v;
+    v += __uint_as_float(dpp_u<0xB1>(__float_as_uint(v)));    // quad_perm [1,0,3,2]
+    v += __uint_as_float(dpp_u<0x4E>(__float_as_uint(v)));    // quad_perm [2,3,0,1]
+    v += __uint_as_float(dpp_u<0x141>(__float_as_uint(v)));   // row_half_mirror
+    v += __uint_as_float(dpp_u<0x140>(__float_as_uint(v)));   // row_mirror
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    v = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    const auto h = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(h[0]) + __uint_as_float(h[1]);
 }
 
 __device__ __forceinline__ int wave_sum_i(int v) {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);
-    return v;
+    v += (int)dpp_u<0xB1>((uint32_t)v);
+    v += (int)dpp_u<0x4E>((uint32_t)v);
+    v += (int)dpp_u<0x141>((uint32_t)v);
+    v += (int)dpp_u<0x140>((uint32_t)v);
+    const auto r = __builtin_amdgcn_permlane16_swap((uint32_t)v, (uint32_t)v, false, false);
+    v = (int)r[0] + (int)r[1];
+    const auto h = __builtin_amdgcn_permlane32_swap((uint32_t)v, (uint32_t)v, false, false);
+    return (int)h[0] + (int)h[1];
 }
 
 __device__ __forceinline__ double wave_sum_d(double v) {

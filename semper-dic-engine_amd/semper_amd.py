@@ -94,6 +94,7 @@ def load_dev(path=LIB_DEV):
                                    ctypes.c_float]
     lib.sdk_engine_timings.argtypes = [_VP, _VP]
     lib.sdk_engine_set_flag.argtypes = [_VP, ctypes.c_int, ctypes.c_int]
+    lib.sdk_wave_sum_probe.argtypes = [_VP, ctypes.c_int, _VP, _VP]
     lib.sdk_synth_pair.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, ctypes.c_int,
                                    ctypes.c_double, ctypes.c_double, ctypes.c_float, _VP, _VP, ctypes.c_int]
     return lib
