@@ -65,6 +65,10 @@ SDK_API int sdk_engine_timings(void* h, double* out14);
 /* Synthetic speckle pair (synth.hpp FieldSpec kinds). */
 SDK_API int sdk_synth_pair(uint64_t seed, int w, int h, int kind, const double* p, int np, double cx, double cy,
                            float noise_sigma, uint8_t* ref, uint8_t* def, int on_device);
+/* One deformed frame of a sequence: the speckle of `seed` under the field, with
+   its own noise drawn from noise_seed; w x h bytes to def (host or device). */
+SDK_API int sdk_synth_frame(uint64_t seed, uint64_t noise_seed, int w, int h, int kind, const double* p, int np,
+                            double cx, double cy, float noise_sigma, uint8_t* def, int on_device);
 
 #ifdef __cplusplus
 }

@@ -252,4 +252,52 @@ int orc_engine_run(orc_engine* e, const uint8_t* def, const uint8_t* mask, const
     }
 }
 
+// IncrementalSession: params as orc_engine_run; info receives 6 ints
+// (reference_frame, reference_updated, tracked_points, frame0_points,
+// taylor_fallbacks, lost_this_frame).
+void* orc_inc_create(void) { return inc_create(); }
+void orc_inc_destroy(void* s) { inc_destroy((IncSession*)s); }
+
+int orc_inc_init(void* s, const uint8_t* ref, int w, int h, const uint8_t* mask, const int32_t* params, int interval,
+                 int threads, float conv_tol) {
+    if (!s || !params) return kErrInit;
+    Params p;
+    p.rect_x = params[0];
+    p.rect_y = params[1];
+    p.rect_w = params[2];
+    p.rect_h = params[3];
+    p.step = params[4];
+    p.subset = params[5];
+    p.strain_window = params[6];
+    p.use6 = params[7] != 0;
+    RunOptions o;
+    o.threads = threads;
+    o.conv_tol = conv_tol;
+    try {
+        return inc_init((IncSession*)s, ref, w, h, mask, p, interval, o);
+    } catch (...) {
+        return kErrInit;
+    }
+}
+
+int orc_inc_process(void* s, const uint8_t* def, float* out, int out_capacity, float* metrics, int metrics_len,
+                    int32_t* info) {
+    if (!s) return kErrInit;
+    try {
+        IncInfo fi;
+        const int rc = inc_process((IncSession*)s, def, out, out_capacity, metrics, metrics_len, &fi);
+        if (info && rc >= 0) {
+            info[0] = fi.reference_frame;
+            info[1] = fi.reference_updated;
+            info[2] = fi.tracked_points;
+            info[3] = fi.frame0_points;
+            info[4] = fi.taylor_fallbacks;
+            info[5] = fi.lost_this_frame;
+        }
+        return rc;
+    } catch (...) {
+        return kErrInit;
+    }
+}
+
 }  // extern "C"

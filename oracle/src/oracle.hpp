@@ -210,13 +210,42 @@ struct RefCache {
 };
 void set_reference(RefCache& c, int w, int h, const uint8_t* gray, const uint8_t* mask);
 
+// The solved node field of a run (FieldCapture, full_field_internal.hpp:247-268),
+// in the form the incremental session samples: u .. corr of solved nodes with
+// corr >= 0 (zero / kCorrInvalid elsewhere) and `valid` = those nodes after the
+// displacement median test (node_field_from_capture, incremental_math.cpp:19-59).
+struct Capture {
+    int gw = 0, gh = 0, rect_x = 0, rect_y = 0, step = 0, attempted = 0;
+    std::vector<float> u, v, ux, uy, vx, vy, corr;
+    std::vector<unsigned char> valid;
+};
+
 struct RunOptions {
     int threads = 0;             // 0: hardware_concurrency
     float conv_tol = 1e-3f;
+    Capture* capture = nullptr;  // filled when set
 };
 // Full-field solve. Returns point count or a negative code; metrics[24].
 int run_full_field(RefCache& c, const uint8_t* def_gray, const uint8_t* mask, const Params& p,
                    float* out, int out_capacity, float* metrics, int metrics_len,
                    const RunOptions& opt);
+
+// pack_full_field_output (full_field_solver_stats.cpp:21-91) of a node field
+// on its own grid: the median test over `ok`, VSG strain with `window`, row-major
+// pack with the capacity rule. Returns the point count.
+int pack_field(const Capture& f, int window, float* out, int out_capacity);
+
+// IncrementalSession (include/semper/incremental.hpp, src/pipeline/incremental_session.cpp).
+struct IncInfo {
+    int reference_frame = 0, reference_updated = 0, tracked_points = 0, frame0_points = 0,
+        taylor_fallbacks = 0, lost_this_frame = 0;
+};
+struct IncSession;
+IncSession* inc_create();
+void inc_destroy(IncSession* s);
+int inc_init(IncSession* s, const uint8_t* ref, int w, int h, const uint8_t* mask, const Params& p, int interval,
+             const RunOptions& opt);
+int inc_process(IncSession* s, const uint8_t* def, float* out, int out_capacity, float* metrics, int metrics_len,
+                IncInfo* info);
 
 }  // namespace orc

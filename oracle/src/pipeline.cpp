@@ -973,6 +973,34 @@ int run_full_field(RefCache& c, const uint8_t* def_gray, const uint8_t* mask, co
             d.valid[i] = 1;
         }
     const int dropped_outlier = reject_displacement_outliers(d);
+    if (opt.capture) {
+        Capture& cp = *opt.capture;
+        const size_t n = (size_t)gw * gh;
+        cp.gw = gw;
+        cp.gh = gh;
+        cp.rect_x = p.rect_x;
+        cp.rect_y = p.rect_y;
+        cp.step = p.step;
+        cp.attempted = total_valid;
+        cp.u.assign(n, 0.0f);
+        cp.v.assign(n, 0.0f);
+        cp.ux.assign(n, 0.0f);
+        cp.uy.assign(n, 0.0f);
+        cp.vx.assign(n, 0.0f);
+        cp.vy.assign(n, 0.0f);
+        cp.corr.assign(n, kCorrInvalid);
+        cp.valid.assign(d.valid.begin(), d.valid.end());
+        for (size_t i = 0; i < n; ++i)
+            if (grid[i].solved && grid[i].corr >= 0.0f) {
+                cp.u[i] = grid[i].u;
+                cp.v[i] = grid[i].v;
+                cp.ux[i] = grid[i].ux;
+                cp.uy[i] = grid[i].uy;
+                cp.vx[i] = grid[i].vx;
+                cp.vy[i] = grid[i].vy;
+                cp.corr[i] = grid[i].corr;
+            }
+    }
     const StrainField sf = compute_vsg_strain(d, p.strain_window);
     int valid = 0, dropped_post = 0;
     bool trunc = false;
@@ -1033,6 +1061,37 @@ int run_full_field(RefCache& c, const uint8_t* def_gray, const uint8_t* mask, co
         m[23] = (float)dropped_outlier;
         for (int i = 0; i < ncopy; ++i) metrics[i] = m[i];
     }
+    return valid;
+}
+
+int pack_field(const Capture& f, int window, float* out, int out_capacity) {
+    DispField d;
+    d.w = f.gw;
+    d.h = f.gh;
+    d.step = f.step;
+    d.u = f.u;
+    d.v = f.v;
+    d.valid = f.valid;
+    reject_displacement_outliers(d);
+    const StrainField sf = compute_vsg_strain(d, window);
+    int valid = 0;
+    for (int y = 0; y < f.gh; ++y)
+        for (int x = 0; x < f.gw; ++x) {
+            const int i = y * f.gw + x;
+            if (!d.valid[i]) continue;
+            if ((valid + 1) * 8 > out_capacity) return valid;
+            if (sf.exx[i] <= kStrainFail) continue;
+            float* o = out + (size_t)valid * 8;
+            o[0] = (float)(f.rect_x + x * f.step);
+            o[1] = (float)(f.rect_y + y * f.step);
+            o[2] = d.u[i];
+            o[3] = d.v[i];
+            o[4] = sf.exx[i];
+            o[5] = sf.eyy[i];
+            o[6] = sf.exy[i];
+            o[7] = f.corr[i];
+            valid++;
+        }
     return valid;
 }
 

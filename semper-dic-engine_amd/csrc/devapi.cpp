@@ -276,13 +276,34 @@ SDK_API int sdk_synth_pair(uint64_t seed, int w, int h, int kind, const double* 
     f.cy = cy;
     for (int i = 0; i < np; ++i) f.p[i] = p[i];
     const size_t npx = (size_t)w * h;
-    if (on_device) return synth_pair(seed, w, h, f, noise_sigma, ref, def, nullptr);
+    if (on_device) return synth_pair(seed, seed, w, h, f, noise_sigma, ref, def, nullptr);
     Dev<uint8_t> dr(npx), dd(npx);
     if (!dr.p || !dd.p) return -3;
-    const int rc = synth_pair(seed, w, h, f, noise_sigma, dr.p, dd.p, nullptr);
+    const int rc = synth_pair(seed, seed, w, h, f, noise_sigma, dr.p, dd.p, nullptr);
     if (rc != 0) return rc;
     if (hipMemcpy(ref, dr.p, npx, hipMemcpyDeviceToHost) != hipSuccess) return -3;
     return hipMemcpy(def, dd.p, npx, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -3;
+}
+
+// One deformed frame of a sequence: the speckle of `seed` under the field,
+// with its own noise from noise_seed (frames of a sequence do not share a
+// noise pattern). Writes w x h bytes to def (host, or device when on_device).
+SDK_API int sdk_synth_frame(uint64_t seed, uint64_t noise_seed, int w, int h, int kind, const double* p, int np,
+                            double cx, double cy, float noise_sigma, uint8_t* def, int on_device) {
+    if (w <= 0 || h <= 0 || np < 0 || np > 48 || !def) return -3;
+    FieldSpec f = {};
+    f.kind = kind;
+    f.cx = cx;
+    f.cy = cy;
+    for (int i = 0; i < np; ++i) f.p[i] = p[i];
+    const size_t npx = (size_t)w * h;
+    Dev<uint8_t> dr(npx), dd(on_device ? 1 : npx);
+    if (!dr.p || !dd.p) return -3;
+    uint8_t* out = on_device ? def : dd.p;
+    const int rc = synth_pair(seed, noise_seed, w, h, f, noise_sigma, dr.p, out, nullptr);
+    if (rc != 0) return rc;
+    if (!on_device && hipMemcpy(def, dd.p, npx, hipMemcpyDeviceToHost) != hipSuccess) return -3;
+    return 0;
 }
 
 }  // extern "C"

@@ -247,6 +247,104 @@ int Engine::solve_tasks(const std::vector<Task>& tasks, std::vector<SolveOut>& o
     return 0;
 }
 
+// match_anchor (src/pipeline/incremental_math.cpp:237-272) for a batch of
+// stored subsets. The patches are laid side by side on a canvas as wide as the
+// frame, so the subset solver reads the canvas as its reference plane (the
+// subset pixels lie at least the anchor margin inside their patch, so their
+// gradients are the patch's own) and the last run's frame as its deformed
+// plane; a subset centred at (cx, cy) on the canvas starts at the node's guess
+// less its offset from the patch centre.
+int Engine::match_anchors(std::vector<AnchorMatch>& reqs, int subset, bool use6, float tol) {
+    for (auto& r : reqs) r.ok = false;
+    if (reqs.empty()) return 0;
+    if (w_ <= 0 || h_ <= 0 || !def_u8_.p) return kErrInit;
+    const int P = reqs[0].side;
+    for (const auto& r : reqs)
+        if (r.side != P || !r.patch) return kErrInit;
+    const int cols = w_ / P, rows_max = h_ / P;
+    if (cols < 1 || rows_max < 1 || ppl_for(subset) < 0) return 0;   // no patch fits the frame: nothing matches
+    const int per_batch = cols * rows_max;
+    try {
+        for (size_t first = 0; first < reqs.size(); first += per_batch) {
+            const int n = (int)std::min<size_t>(per_batch, reqs.size() - first);
+            const int rows = (n + cols - 1) / cols, Hc = rows * P;
+            const size_t npx = (size_t)w_ * Hc;
+            std::vector<uint8_t> canvas(npx, 0);
+            std::vector<int32_t> cxy(2 * (size_t)n);
+            std::vector<Task> tasks(n);
+            for (int k = 0; k < n; ++k) {
+                const AnchorMatch& r = reqs[first + k];
+                const int ox = (k % cols) * P, oy = (k / cols) * P;
+                for (int y = 0; y < P; ++y)
+                    std::memcpy(&canvas[(size_t)(oy + y) * w_ + ox], r.patch + (size_t)y * P, (size_t)P);
+                Task t = {};
+                t.cx = ox + P / 2;
+                t.cy = oy + P / 2;
+                t.pool = k;
+                t.mode = NO_SEARCH;
+                t.g[0] = r.guess_x - r.dx - (float)t.cx;
+                t.g[1] = r.guess_y - r.dy - (float)t.cy;
+                t.node = k;
+                tasks[k] = t;
+                cxy[2 * k] = t.cx;
+                cxy[2 * k + 1] = t.cy;
+            }
+            DevBuf<uint8_t> dcan;
+            DevBuf<float> planes;
+            DevBuf<int32_t> dcxy;
+            DevBuf<PoolEntry> pool;
+            if (!dcan.reserve(npx) || !planes.reserve(3 * npx) || !dcxy.reserve(2 * (size_t)n) || !pool.reserve(n) ||
+                !tasks_d_.reserve(n) || !out_d_.reserve(n) || !bail_.reserve(n) || !counters_.reserve(32))
+                return kErrInit;
+            float* cI = planes.p;
+            float* cgx = cI + npx;
+            float* cgy = cgx + npx;
+            ck(hipMemcpyAsync(dcan.p, canvas.data(), npx, hipMemcpyHostToDevice, st_));
+            ck(hipMemcpyAsync(dcxy.p, cxy.data(), sizeof(int32_t) * 2 * n, hipMemcpyHostToDevice, st_));
+            ck(hipMemcpyAsync(tasks_d_.p, tasks.data(), sizeof(Task) * n, hipMemcpyHostToDevice, st_));
+            ck(launch_prep_ref_u8(dcan.p, nullptr, w_, Hc, cI, cgx, cgy, st_));
+            ck(launch_pool_list(cI, cgx, cgy, w_, Hc, dcxy.p, n, subset, true, pool.p, st_));
+            SolveArgs a = {};
+            a.tasks = tasks_d_.p;
+            a.out = out_d_.p;
+            a.nitems = n;
+            a.bail = bail_.p;
+            a.nbail = counters_.p + 8;
+            a.pool = pool.p;
+            a.refI = cI;
+            a.refGx = cgx;
+            a.refGy = cgy;
+            a.def = def_u8_.p;
+            a.w = w_;
+            a.h = h_;
+            a.dim = subset;
+            a.lm = 1;
+            a.tol = tol;
+            ck(launch_solve(a, use6, false, st_));
+            std::vector<SolveOut> res(n);
+            ck(hipMemcpyAsync(res.data(), out_d_.p, sizeof(SolveOut) * n, hipMemcpyDeviceToHost, st_));
+            ck(hipStreamSynchronize(st_));
+            for (int k = 0; k < n; ++k) {
+                AnchorMatch& r = reqs[first + k];
+                const SolveOut& o = res[k];
+                if (!o.init || o.status != 0 || !(o.corr <= kCorrAccept)) continue;
+                // the node, (dx, dy) from the patch centre, through the matched map
+                const float u = o.p[0] + (float)(tasks[k].cx - r.ax), v = o.p[1] + (float)(tasks[k].cy - r.ay);
+                const float nx = (float)r.ax + u + (1.0f + o.p[2]) * r.dx + o.p[3] * r.dy;
+                const float ny = (float)r.ay + v + o.p[4] * r.dx + (1.0f + o.p[5]) * r.dy;
+                if (!(std::hypot(nx - r.guess_x, ny - r.guess_y) <= r.max_shift)) continue;
+                r.ok = true;
+                r.x = nx;
+                r.y = ny;
+                r.corr = o.corr;
+            }
+        }
+        return 0;
+    } catch (const HipFail&) {
+        return kErrInit;
+    }
+}
+
 int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const Params& pin, float* out,
                 int out_capacity, float* metrics, int metrics_len, Cancel& cancel,
                 const std::function<void(int)>& progress, float tol) {
@@ -988,6 +1086,10 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                                    pass_fail + pass, st_));
             src = dst;
             dst = (dst == ok_a) ? ok_b : ok_a;
+        }
+        if (keep_field) {
+            last_field.med.resize(NN);
+            ck(hipMemcpyAsync(last_field.med.data(), src, NN, hipMemcpyDeviceToHost, st_));
         }
         float* exx = strain_.p;
         float* eyy = exx + NN;

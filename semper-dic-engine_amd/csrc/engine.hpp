@@ -113,7 +113,24 @@ public:
         int gw = 0, gh = 0, rect_x = 0, rect_y = 0, step = 0, attempted = 0;
         std::vector<float> u, v, ux, uy, vx, vy, corr;
         std::vector<unsigned char> ok;   // solved with corr >= 0 (before the median test)
+        std::vector<unsigned char> med;  // ok and kept by the displacement median test
     };
+    // A stored subset to match on the last run's deformed frame
+    // (match_anchor, src/pipeline/incremental_math.cpp:237-272): a gray patch
+    // of side 2 (subset / 2 + margin) + 1 centred on (ax, ay) of the frame it
+    // was cut from, the node at (ax + dx, ay + dy) there, and the guess of
+    // where the node is now.
+    struct AnchorMatch {
+        const uint8_t* patch = nullptr;
+        int side = 0, ax = 0, ay = 0;
+        float dx = 0, dy = 0, guess_x = 0, guess_y = 0, max_shift = 0;
+        bool ok = false;          // out: converged, ZNSSD <= 0.15, within max_shift
+        float x = 0, y = 0, corr = 0;
+    };
+    // Matches every request with the subset solver (NO_SEARCH with the Simplex
+    // rescue, LM damping) on the deformed frame of the last run. Returns 0 or
+    // a negative code.
+    int match_anchors(std::vector<AnchorMatch>& reqs, int subset, bool use6, float tol);
     bool keep_field = false;
     // Solve Path B's first claims speculatively in one launch (off: every round
     // is solved when it is reached; the committed field is the same either way).

@@ -1,207 +1,721 @@
-// Incremental DIC session; see incremental.hpp. Composition rule:
-// src/pipeline/incremental_session.cpp:289-549 and incremental_math.cpp:61-123.
+// Incremental DIC session; see incremental.hpp. Session rules:
+// src/pipeline/incremental_session.cpp:215-549; sampling, coasting fits,
+// anchors and the mask warp: src/pipeline/incremental_math.cpp:19-447;
+// constants: include/semper/tuning.hpp:186-281.
 #include "incremental.hpp"
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <limits>
+
+#include "maskwarp.hpp"
 
 namespace sdic {
 
 namespace {
+
 constexpr int kErrRoi = -2, kErrInit = -3, kErrStrainWindow = -4;
-constexpr float kTaylorMaxCells = 1.5f;     // tuning::kIncTaylorMaxCells
-constexpr float kUpdateMinReported = 0.5f;  // tuning::kIncUpdateMinReported
+constexpr float kCorrInvalidF = -1.0f;
+constexpr float kTaylorMaxCells = 1.5f;
+constexpr int kMaskMaxSamplesPerPx = 8;
+constexpr int kCoastRadiusCells = 8;
+constexpr int kCoastMinNodes = 6;
+constexpr float kCoastMaxTravelSubsets = 2.0f;
+constexpr float kCoastTrustedTravelSubsets = 0.25f;
+constexpr float kReanchorMaxShiftFraction = 0.3f;
+constexpr float kReanchorMinShiftPx = 1.0f;
+constexpr int kAnchorMarginPx = 10;
+constexpr float kEarlyUpdateDrop = 0.05f;
+constexpr float kUpdateMinReported = 0.5f;
+constexpr int kMaskExtrapCells = 3;
+
+// A solved lattice as sampling wants it (NodeField, incremental_internal.hpp:18-28):
+// valid = solved with corr >= 0 and kept by the displacement median test.
+struct NodeField {
+    int gw = 0, gh = 0, rx = 0, ry = 0, step = 0;
+    const std::vector<float>*u, *v, *ux, *uy, *vx, *vy, *corr;
+    const std::vector<unsigned char>* valid;
+    float nx(int i) const { return (float)(rx + i * step); }
+    float ny(int j) const { return (float)(ry + j * step); }
+    size_t idx(int i, int j) const { return (size_t)j * gw + i; }
+    bool in(int i, int j) const { return i >= 0 && i < gw && j >= 0 && j < gh; }
+};
+
+NodeField node_field(const Engine::Field& f) {
+    NodeField n;
+    n.gw = f.gw;
+    n.gh = f.gh;
+    n.rx = f.rect_x;
+    n.ry = f.rect_y;
+    n.step = f.step;
+    n.u = &f.u;
+    n.v = &f.v;
+    n.ux = &f.ux;
+    n.uy = &f.uy;
+    n.vx = &f.vx;
+    n.vy = &f.vy;
+    n.corr = &f.corr;
+    n.valid = &f.med;
+    return n;
+}
+
+struct Sample {
+    bool ok = false, taylor = false;
+    float u = 0, v = 0, corr = 0;
+};
+
+// sample_increment (incremental_math.cpp:61-123)
+Sample sample_increment(const NodeField& f, float x, float y) {
+    Sample s;
+    if (f.gw <= 0 || f.gh <= 0 || f.step <= 0) return s;
+    const float gx = (x - (float)f.rx) / (float)f.step, gy = (y - (float)f.ry) / (float)f.step;
+    const int i0 = (int)std::floor(gx), j0 = (int)std::floor(gy);
+    const float fx = gx - (float)i0, fy = gy - (float)j0;
+    const auto& V = *f.valid;
+    if (fx == 0.0f && fy == 0.0f && f.in(i0, j0) && V[f.idx(i0, j0)]) {
+        const size_t k = f.idx(i0, j0);
+        s.ok = true;
+        s.u = (*f.u)[k];
+        s.v = (*f.v)[k];
+        s.corr = (*f.corr)[k];
+        return s;
+    }
+    if (f.in(i0, j0) && f.in(i0 + 1, j0 + 1)) {
+        const size_t a = f.idx(i0, j0), b = f.idx(i0 + 1, j0), c = f.idx(i0, j0 + 1), d = f.idx(i0 + 1, j0 + 1);
+        if (V[a] && V[b] && V[c] && V[d]) {
+            const float wa = (1 - fx) * (1 - fy), wb = fx * (1 - fy), wc = (1 - fx) * fy, wd = fx * fy;
+            s.ok = true;
+            s.u = wa * (*f.u)[a] + wb * (*f.u)[b] + wc * (*f.u)[c] + wd * (*f.u)[d];
+            s.v = wa * (*f.v)[a] + wb * (*f.v)[b] + wc * (*f.v)[c] + wd * (*f.v)[d];
+            s.corr = std::max(std::max((*f.corr)[a], (*f.corr)[b]), std::max((*f.corr)[c], (*f.corr)[d]));
+            return s;
+        }
+    }
+    // first-order expansion from the nearest valid node of the 4 x 4 block (ties: row-major first)
+    const float max_d = kTaylorMaxCells * (float)f.step;
+    float best = std::numeric_limits<float>::infinity();
+    int bi = -1, bj = -1;
+    for (int j = j0 - 1; j <= j0 + 2; ++j)
+        for (int i = i0 - 1; i <= i0 + 2; ++i) {
+            if (!f.in(i, j) || !V[f.idx(i, j)]) continue;
+            const float dx = x - f.nx(i), dy = y - f.ny(j);
+            const float d2 = dx * dx + dy * dy;
+            if (d2 < best) {
+                best = d2;
+                bi = i;
+                bj = j;
+            }
+        }
+    if (bi < 0 || best > max_d * max_d) return s;
+    const size_t k = f.idx(bi, bj);
+    const float dx = x - f.nx(bi), dy = y - f.ny(bj);
+    s.ok = true;
+    s.taylor = true;
+    s.u = (*f.u)[k] + (*f.ux)[k] * dx + (*f.uy)[k] * dy;
+    s.v = (*f.v)[k] + (*f.vx)[k] * dx + (*f.vy)[k] * dy;
+    s.corr = (*f.corr)[k];
+    return s;
+}
+
+double det3(const double m[3][3]) {
+    return m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+           m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+}
+
+// fit_increment (incremental_math.cpp:125-188): weighted plane through the
+// valid nodes within radius_px, weights 1 / (1 + (d / step)^2), offsets in steps.
+Sample fit_increment(const NodeField& f, float x, float y, float radius_px) {
+    Sample s;
+    if (f.gw <= 0 || f.gh <= 0 || f.step <= 0 || radius_px <= 0.0f) return s;
+    const double step = f.step;
+    const int r = (int)std::ceil(radius_px / step);
+    const int ic = (int)std::lround((x - f.rx) / step), jc = (int)std::lround((y - f.ry) / step);
+    const double r2max = (double)radius_px * radius_px;
+    double S[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, bu[3] = {0, 0, 0}, bv[3] = {0, 0, 0};
+    int n = 0;
+    float worst = 0.0f;
+    for (int j = std::max(0, jc - r); j <= std::min(f.gh - 1, jc + r); ++j)
+        for (int i = std::max(0, ic - r); i <= std::min(f.gw - 1, ic + r); ++i) {
+            const size_t k = f.idx(i, j);
+            if (!(*f.valid)[k]) continue;
+            const double dx = f.nx(i) - x, dy = f.ny(j) - y;
+            const double d2 = dx * dx + dy * dy;
+            if (d2 > r2max) continue;
+            const double wgt = 1.0 / (1.0 + d2 / (step * step));
+            const double p[3] = {1.0, dx / step, dy / step};
+            for (int a = 0; a < 3; ++a) {
+                for (int b = 0; b < 3; ++b) S[a][b] += wgt * p[a] * p[b];
+                bu[a] += wgt * p[a] * (*f.u)[k];
+                bv[a] += wgt * p[a] * (*f.v)[k];
+            }
+            worst = std::max(worst, (*f.corr)[k]);
+            ++n;
+        }
+    if (n < kCoastMinNodes) return s;
+    const double W = S[0][0];
+    const double mx = S[0][1] / W, my = S[0][2] / W;
+    const double cxx = S[1][1] / W - mx * mx, cyy = S[2][2] / W - my * my, cxy = S[1][2] / W - mx * my;
+    const double tr = cxx + cyy, dt = cxx * cyy - cxy * cxy;
+    const double lmin = 0.5 * (tr - std::sqrt(std::max(0.0, tr * tr - 4.0 * dt)));
+    if (lmin < 1.0) return s;
+    const double D = det3(S);
+    if (std::abs(D) < 1e-12) return s;
+    auto first = [&](const double b[3]) {
+        double M[3][3];
+        for (int a = 0; a < 3; ++a) {
+            M[a][0] = b[a];
+            M[a][1] = S[a][1];
+            M[a][2] = S[a][2];
+        }
+        return det3(M) / D;
+    };
+    s.ok = true;
+    s.u = (float)first(bu);
+    s.v = (float)first(bv);
+    s.corr = worst;
+    return s;
+}
+
+// Exact Euclidean distance transform (to the nearest zero cell) of a w x h
+// grid of 0/1, in cells: the lower envelope of parabolas per row, then per
+// column (Felzenszwalb and Huttenlocher). Cells with no zero anywhere get +inf.
+void edt_1d(const std::vector<double>& f, std::vector<double>& d, std::vector<int>& v, std::vector<double>& z,
+            int n) {
+    const double inf = std::numeric_limits<double>::infinity();
+    int k = -1;
+    for (int q = 0; q < n; ++q) {
+        if (f[q] == inf) continue;
+        if (k < 0) {
+            k = 0;
+            v[0] = q;
+            z[0] = -inf;
+            z[1] = inf;
+            continue;
+        }
+        double s;
+        while (true) {
+            s = ((f[q] + (double)q * q) - (f[v[k]] + (double)v[k] * v[k])) / (2.0 * q - 2.0 * v[k]);
+            if (s <= z[k] && k > 0) {
+                --k;
+                continue;
+            }
+            break;
+        }
+        if (s <= z[k]) {   // k == 0 and the new parabola hides it everywhere
+            v[0] = q;
+            z[0] = -inf;
+            z[1] = inf;
+            continue;
+        }
+        ++k;
+        v[k] = q;
+        z[k] = s;
+        z[k + 1] = inf;
+    }
+    if (k < 0) {
+        for (int q = 0; q < n; ++q) d[q] = inf;
+        return;
+    }
+    int j = 0;
+    for (int q = 0; q < n; ++q) {
+        while (z[j + 1] < q) ++j;
+        const double dq = (double)q - v[j];
+        d[q] = dq * dq + f[v[j]];
+    }
+}
+
+std::vector<float> edt(const std::vector<uint8_t>& one, int w, int h) {
+    const double inf = std::numeric_limits<double>::infinity();
+    const int m = std::max(w, h);
+    std::vector<double> g((size_t)w * h), f(m), d(m), z(m + 1);
+    std::vector<int> v(m);
+    for (int x = 0; x < w; ++x) {
+        for (int y = 0; y < h; ++y) f[y] = one[(size_t)y * w + x] ? inf : 0.0;
+        edt_1d(f, d, v, z, h);
+        for (int y = 0; y < h; ++y) g[(size_t)y * w + x] = d[y];
+    }
+    std::vector<float> out((size_t)w * h);
+    for (int y = 0; y < h; ++y) {
+        for (int x = 0; x < w; ++x) f[x] = g[(size_t)y * w + x];
+        edt_1d(f, d, v, z, w);
+        for (int x = 0; x < w; ++x) out[(size_t)y * w + x] = (float)std::sqrt(d[x]);
+    }
+    return out;
+}
+
+// gap_distance (incremental_math.cpp:190-210): px to the nearest unsolved
+// node (hole; off-grid counts as solved) and to the nearest unsolved or
+// off-grid position (gap).
+struct GapDistance {
+    std::vector<float> hole, gap;
+};
+
+GapDistance gap_distance(const NodeField& f) {
+    GapDistance g;
+    if (f.gw <= 0 || f.gh <= 0 || f.step <= 0) return g;
+    const int W = f.gw + 2, H = f.gh + 2;
+    std::vector<uint8_t> solved((size_t)W * H, 0), hole_in((size_t)W * H, 1);
+    for (int j = 0; j < f.gh; ++j)
+        for (int i = 0; i < f.gw; ++i) {
+            const uint8_t v = (*f.valid)[f.idx(i, j)] ? 1 : 0;
+            solved[(size_t)(j + 1) * W + i + 1] = v;
+            hole_in[(size_t)(j + 1) * W + i + 1] = v;
+        }
+    const std::vector<float> dg = edt(solved, W, H), dh = edt(hole_in, W, H);
+    g.gap.resize((size_t)f.gw * f.gh);
+    g.hole.resize(g.gap.size());
+    for (int j = 0; j < f.gh; ++j)
+        for (int i = 0; i < f.gw; ++i) {
+            g.gap[f.idx(i, j)] = dg[(size_t)(j + 1) * W + i + 1] * (float)f.step;
+            g.hole[f.idx(i, j)] = dh[(size_t)(j + 1) * W + i + 1] * (float)f.step;
+        }
+    return g;
+}
+
+enum class AnchorNeed { kDrop, kRefresh, kKeep };
+
+// anchor_need (incremental_math.cpp:212-222)
+AnchorNeed anchor_need(const NodeField& f, const GapDistance& g, float x, float y, int subset) {
+    if (g.gap.empty()) return AnchorNeed::kKeep;
+    const int i = std::min(f.gw - 1, std::max(0, (int)std::lround((x - f.rx) / f.step)));
+    const int j = std::min(f.gh - 1, std::max(0, (int)std::lround((y - f.ry) / f.step)));
+    if (g.hole[f.idx(i, j)] <= (float)(subset / 2 + f.step)) return AnchorNeed::kKeep;
+    if (g.gap[f.idx(i, j)] <= (float)(2 * subset + 2 * f.step)) return AnchorNeed::kRefresh;
+    return AnchorNeed::kDrop;
+}
+
+// fill_field (incremental_math.cpp:278-319): every empty node with a filled
+// 4-neighbour takes their mean, pass by pass from the previous pass's state.
+bool fill_field(int w, int h, std::vector<float>& U, std::vector<float>& V, const std::vector<uint8_t>& valid) {
+    const size_t n = (size_t)w * h;
+    std::vector<uint8_t> filled(valid.begin(), valid.end());
+    bool any = false;
+    for (size_t i = 0; i < n; ++i) {
+        if (filled[i]) any = true;
+        else U[i] = V[i] = 0.0f;
+    }
+    if (!any) return false;
+    std::vector<uint8_t> next;
+    std::vector<float> nu(U), nv(V);
+    for (int pass = 0; pass < w + h + 2; ++pass) {
+        next = filled;
+        bool changed = false;
+        for (int y = 0; y < h; ++y)
+            for (int x = 0; x < w; ++x) {
+                const size_t i = (size_t)y * w + x;
+                if (filled[i]) continue;
+                float su = 0, sv = 0;
+                int cnt = 0;
+                const int qx[4] = {x - 1, x + 1, x, x}, qy[4] = {y, y, y - 1, y + 1};
+                for (int k = 0; k < 4; ++k) {
+                    if (qx[k] < 0 || qx[k] >= w || qy[k] < 0 || qy[k] >= h) continue;
+                    const size_t j = (size_t)qy[k] * w + qx[k];
+                    if (!filled[j]) continue;
+                    su += U[j];
+                    sv += V[j];
+                    ++cnt;
+                }
+                if (cnt == 0) continue;
+                nu[i] = su / (float)cnt;
+                nv[i] = sv / (float)cnt;
+                next[i] = 1;
+                changed = true;
+            }
+        U = nu;
+        V = nv;
+        filled.swap(next);
+        if (!changed) break;
+    }
+    return true;
+}
+
+int prefix_len(int metrics_len) {
+    return metrics_len < 16 ? 0 : metrics_len >= 24 ? 24 : metrics_len == 18 ? 17 : metrics_len;
+}
+
 }  // namespace
 
 void IncrementalSession::reset() {
     eng_.reset();
     init_ = false;
-    frame_ = ref_frame_ = 0;
-    alive_.clear();
+    w_ = h_ = interval_ = gw_ = gh_ = 0;
+    user_mask0_.clear();
+    material_mask0_.clear();
+    frame0_points_ = -1;
     ub_.clear();
     vb_.clear();
+    coast_travel_.clear();
+    alive_.clear();
+    anchors_.clear();
+    ref_index_ = frames_since_update_ = frame_count_ = 0;
+    ref_reported_ = adopted_reported_ = -1;
+    pk_ = Params{};
+    mask_k_.clear();
+    scratch_.clear();
 }
 
+int IncrementalSession::alive_count() const {
+    int n = 0;
+    for (uint8_t a : alive_) n += a ? 1 : 0;
+    return n;
+}
+
+int IncrementalSession::nothing_to_follow(FrameInfo fi, float* metrics, int metrics_len,
+                                          const std::function<void(int)>& progress, FrameInfo* info) {
+    frame_count_++;
+    fi.frame0_points = std::max(frame0_points_, 0);
+    if (info) *info = fi;
+    if (metrics) std::fill_n(metrics, prefix_len(metrics_len), 0.0f);
+    if (progress) {
+        try {
+            progress(100);
+        } catch (...) {
+        }
+    }
+    return 0;
+}
+
+// IncrementalSession::init (incremental_session.cpp:215-265)
 int IncrementalSession::init(const uint8_t* ref, int w, int h, const uint8_t* mask, const Params& pin,
                              int update_interval) {
     reset();
-    if (update_interval < 0 || !ref) return kErrInit;
+    if (!ref || w <= 0 || h <= 0 || update_interval < 0) return kErrInit;
     Params p = pin;
     const int rc = validate_params(p, w, h);
     if (rc != 0) return rc;
     if (vsg_window_node_count(p.step, p.strain_window) < 3) return kErrStrainWindow;
-    eng_ = std::make_unique<Engine>();
-    if (eng_->set_reference(ref, false, w, h, mask) != 0) {
-        eng_.reset();
-        return kErrInit;
-    }
     w_ = w;
     h_ = h;
-    p_ = p;
+    p0_ = p;
     interval_ = update_interval;
     gw_ = p.rect_w / p.step;
     gh_ = p.rect_h / p.step;
-    mask0_ = mask ? std::vector<uint8_t>(mask, mask + (size_t)w * h) : std::vector<uint8_t>();
-    // frame-0 boundary and mask scan (full_field_solver.cpp:271-317)
-    alive_.assign((size_t)gw_ * gh_, 0);
-    const int half = p.subset / 2, buf = half + 4 + 15;
-    frame0_points_ = 0;
-    for (int y = 0; y < gh_; ++y)
-        for (int x = 0; x < gw_; ++x) {
-            const int rx = p.rect_x + x * p.step, ry = p.rect_y + y * p.step;
-            bool skip = rx - buf < 0 || rx + buf >= w || ry - buf < 0 || ry + buf >= h;
-            if (!skip && mask)
-                for (int dy = -half; dy <= half && !skip; ++dy)
-                    for (int dx = -half; dx <= half; ++dx)
-                        if (mask[(size_t)(ry + dy) * w + rx + dx] < 128) {
-                            skip = true;
-                            break;
-                        }
-            alive_[(size_t)y * gw_ + x] = skip ? 0 : 1;
-            frame0_points_ += skip ? 0 : 1;
-        }
-    if (frame0_points_ == 0) {
-        eng_.reset();
-        return kErrRoi;
+    const size_t npx = (size_t)w * h;
+    material_mask0_.assign(npx, 0);
+    if (mask) {
+        user_mask0_.assign(mask, mask + npx);
+        for (size_t i = 0; i < npx; ++i) material_mask0_[i] = mask[i] >= 128 ? 255 : 0;
+    } else {
+        // the ROI rect plus the reach of its edge nodes' subsets at 45 degrees
+        const int half = (int)std::ceil((p.subset / 2) * std::sqrt(2.0)) + 1;
+        const int x0 = std::max(0, p.rect_x - half), y0 = std::max(0, p.rect_y - half);
+        const int x1 = std::min(w, p.rect_x + p.rect_w + half), y1 = std::min(h, p.rect_y + p.rect_h + half);
+        for (int y = y0; y < y1; ++y) std::memset(&material_mask0_[(size_t)y * w + x0], 255, (size_t)std::max(0, x1 - x0));
     }
-    ub_.assign(alive_.size(), 0.0);
-    vb_.assign(alive_.size(), 0.0);
-    ref_first_count_ = -1;
-    ref_own_count_ = frame0_points_;
+    eng_ = std::make_unique<Engine>();
+    if (eng_->set_reference(ref, false, w, h, mask) != 0) {
+        reset();
+        return kErrInit;
+    }
+    const size_t n = (size_t)gw_ * gh_;
+    ub_.assign(n, 0.0f);
+    vb_.assign(n, 0.0f);
+    alive_.assign(n, 1);
+    coast_travel_.assign(n, 0.0f);
     init_ = true;
     return 0;
 }
 
+// IncrementalSession::Impl::process (incremental_session.cpp:289-549)
 int IncrementalSession::process(const uint8_t* def, float* out, int out_capacity, float* metrics, int metrics_len,
                                 Cancel& cancel, const std::function<void(int)>& progress, FrameInfo* info) {
-    if (!init_ || !eng_) return kErrInit;
-    eng_->keep_field = true;
-    const uint8_t* mask = ref_frame_ == 0 && !mask0_.empty() ? mask0_.data() : nullptr;
-    const int frame = frame_ + 1;
+    if (!init_ || !eng_ || !out || !def) return kErrInit;
     FrameInfo fi;
-    fi.reference_frame = ref_frame_;
-    fi.frame0_points = frame0_points_;
-    const size_t N = alive_.size();
-    std::vector<double> tu(N, 0.0), tv(N, 0.0);
-    std::vector<unsigned char> meas(N, 0);
-    std::vector<float> tc(N, kCorrInvalid);
-    int count;
-    if (ref_frame_ == 0) {
-        // against the first image: the solve's own output, on the frame-0 grid
-        count = eng_->run(def, false, mask, p_, out, out_capacity, metrics, metrics_len, cancel, progress);
-        if (count < 0) return count;
-        const Engine::Field& f = eng_->last_field;
-        for (size_t i = 0; i < N; ++i)
-            if (alive_[i] && f.ok[i]) {
-                meas[i] = 1;
-                tu[i] = f.u[i];
-                tv[i] = f.v[i];
-                tc[i] = f.corr[i];
-            }
-    } else {
-        std::vector<float> scratch((size_t)std::max(8, 8 * gw_ * gh_));
-        const int rc = eng_->run(def, false, mask, p_, scratch.data(), (int)scratch.size(), metrics, metrics_len,
-                                 cancel, progress);
+    fi.reference_frame = ref_index_;
+    const int N = interval_;
+    bool will_update = N > 0 && frames_since_update_ + 1 >= N;
+    const size_t n0 = (size_t)gw_ * gh_;
+    if (ref_index_ > 0 && alive_count() == 0) return nothing_to_follow(fi, metrics, metrics_len, progress, info);
+
+    std::vector<float> U(n0, 0.0f), V(n0, 0.0f), coast_step(n0, 0.0f);
+    std::vector<uint8_t> valid(n0, 0), carried(n0, 0);
+    std::vector<AnchorNeed> need(n0, AnchorNeed::kKeep);
+    int coasting = 0, written = 0;
+    int frame0_points = frame0_points_;
+    const int subset = p0_.subset;
+    const float max_travel = kCoastMaxTravelSubsets * (float)subset;
+    const float trusted_travel = kCoastTrustedTravelSubsets * (float)subset;
+    auto carry = [&](size_t i, float du, float dv) {
+        const float travel = std::hypot(du, dv);
+        if (coast_travel_[i] + travel > max_travel) return;
+        U[i] = ub_[i] + du;
+        V[i] = vb_[i] + dv;
+        carried[i] = 1;
+        coast_step[i] = travel;
+        coasting++;
+    };
+    auto coast = [&](size_t i, const NodeField& inc, float px, float py) {
+        if (px < 0.0f || py < 0.0f || px > (float)(w_ - 1) || py > (float)(h_ - 1)) return;
+        const Sample fit = fit_increment(inc, px, py, (float)(kCoastRadiusCells * inc.step));
+        if (fit.ok) carry(i, fit.u, fit.v);
+    };
+
+    eng_->keep_field = true;
+    if (ref_index_ == 0) {
+        // the engine's own run on frame 0, straight into the caller's buffer
+        const int rc = eng_->run(def, false, user_mask0_.empty() ? nullptr : user_mask0_.data(), p0_, out,
+                                 out_capacity, metrics, metrics_len, cancel, progress, tol);
         if (rc < 0) return rc;
+        written = rc;
         const Engine::Field& f = eng_->last_field;
-        auto ok = [&](int i, int j) { return i >= 0 && j >= 0 && i < f.gw && j < f.gh && f.ok[(size_t)j * f.gw + i]; };
-        for (int y = 0; y < gh_; ++y)
-            for (int x = 0; x < gw_; ++x) {
-                const size_t n0 = (size_t)y * gw_ + x;
-                if (!alive_[n0]) continue;
-                // the material point in the current reference
-                const double X = p_.rect_x + (double)x * p_.step + ub_[n0];
-                const double Y = p_.rect_y + (double)y * p_.step + vb_[n0];
-                const double fx = (X - f.rect_x) / f.step, fy = (Y - f.rect_y) / f.step;
-                const int i0 = (int)std::floor(fx), j0 = (int)std::floor(fy);
-                double du = 0, dv = 0;
-                float corr = kCorrInvalid;
-                bool have = false;
-                if (ok(i0, j0) && ok(i0 + 1, j0) && ok(i0, j0 + 1) && ok(i0 + 1, j0 + 1)) {
-                    const double ax = fx - i0, ay = fy - j0;
-                    auto at = [&](const std::vector<float>& a, int i, int j) { return (double)a[(size_t)j * f.gw + i]; };
-                    auto bil = [&](const std::vector<float>& a) {
-                        return (1 - ay) * ((1 - ax) * at(a, i0, j0) + ax * at(a, i0 + 1, j0)) +
-                               ay * ((1 - ax) * at(a, i0, j0 + 1) + ax * at(a, i0 + 1, j0 + 1));
-                    };
-                    du = bil(f.u);
-                    dv = bil(f.v);
-                    const int ni = std::lround(fx), nj = std::lround(fy);
-                    corr = f.corr[(size_t)std::min(std::max(nj, j0), j0 + 1) * f.gw + std::min(std::max(ni, i0), i0 + 1)];
-                    have = true;
-                } else {
-                    // first-order expansion of the nearest solved node within 1.5 cells
-                    double best = 1e30;
-                    int bi = -1, bj = -1;
-                    for (int j = j0 - 1; j <= j0 + 2; ++j)
-                        for (int i = i0 - 1; i <= i0 + 2; ++i) {
-                            if (!ok(i, j)) continue;
-                            const double d = std::hypot(fx - i, fy - j);
-                            if (d <= kTaylorMaxCells && d < best) {
-                                best = d;
-                                bi = i;
-                                bj = j;
-                            }
-                        }
-                    if (bi >= 0) {
-                        const size_t k = (size_t)bj * f.gw + bi;
-                        const double ddx = X - (f.rect_x + (double)bi * f.step), ddy = Y - (f.rect_y + (double)bj * f.step);
-                        du = f.u[k] + f.ux[k] * ddx + f.uy[k] * ddy;
-                        dv = f.v[k] + f.vx[k] * ddx + f.vy[k] * ddy;
-                        corr = f.corr[k];
-                        have = true;
-                        fi.taylor_fallbacks++;
-                    }
-                }
-                if (!have) continue;
-                meas[n0] = 1;
-                tu[n0] = ub_[n0] + du;
-                tv[n0] = vb_[n0] + dv;
-                tc[n0] = corr;
-            }
+        const NodeField nf = node_field(f);
+        U = f.u;
+        V = f.v;
+        valid = f.med;
+        carried = valid;
+        if (frame0_points < 0) frame0_points = f.attempted;
+        const GapDistance gd = gap_distance(nf);
+        for (size_t i = 0; i < n0; ++i)
+            if (valid[i])
+                need[i] = anchor_need(nf, gd, nf.nx((int)(i % gw_)), nf.ny((int)(i / gw_)), subset);
+    } else {
+        const int gw = pk_.rect_w / pk_.step, gh = pk_.rect_h / pk_.step;
+        scratch_.resize((size_t)std::max(gw * gh, 1) * 8);
+        const int rc = eng_->run(def, false, mask_k_.data(), pk_, scratch_.data(), (int)scratch_.size(), metrics,
+                                 metrics_len, cancel, progress, tol);
+        if (rc == kErrRoi) {
+            // nothing on this reference's grid can be solved, now or on any later frame
+            fi.lost_this_frame = alive_count();
+            std::fill(alive_.begin(), alive_.end(), 0);
+            anchors_.clear();
+            return nothing_to_follow(fi, metrics, metrics_len, progress, info);
+        }
+        if (rc < 0) return rc;
+        const Engine::Field inc_field = eng_->last_field;
+        const NodeField inc = node_field(inc_field);
+        const GapDistance gd = gap_distance(inc);
         Engine::Field tot;
         tot.gw = gw_;
         tot.gh = gh_;
-        tot.rect_x = p_.rect_x;
-        tot.rect_y = p_.rect_y;
-        tot.step = p_.step;
-        tot.u.resize(N);
-        tot.v.resize(N);
-        tot.corr.assign(tc.begin(), tc.end());
-        tot.ok.assign(meas.begin(), meas.end());
-        for (size_t i = 0; i < N; ++i) {
-            tot.u[i] = (float)tu[i];
-            tot.v[i] = (float)tv[i];
-        }
-        count = eng_->strain_pack(tot, p_.strain_window, out, out_capacity, nullptr, nullptr);
-        if (count < 0) return count;
-    }
-    // reference update: every `interval` frames, never onto a failed solve
-    bool update = interval_ > 0 && (frame - ref_frame_) % interval_ == 0;
-    const int floor_base = (ref_first_count_ >= 0 && interval_ != 1) ? ref_first_count_ : ref_own_count_;
-    if (update && (float)count < kUpdateMinReported * (float)floor_base) update = false;
-    if (ref_first_count_ < 0) ref_first_count_ = count;
-    if (update) {
-        for (size_t i = 0; i < N; ++i) {
-            if (!alive_[i]) continue;
-            if (!meas[i]) {
-                alive_[i] = 0;
-                fi.lost_this_frame++;
-                continue;
+        tot.rect_x = p0_.rect_x;
+        tot.rect_y = p0_.rect_y;
+        tot.step = p0_.step;
+        tot.u.assign(n0, 0.0f);
+        tot.v.assign(n0, 0.0f);
+        tot.corr.assign(n0, kCorrInvalidF);
+        tot.ok.assign(n0, 0);
+        // re-anchoring requests of nodes that coasted far, matched in one batch
+        std::vector<size_t> ra_node;
+        std::vector<uint8_t> ra_taylor;
+        std::vector<Engine::AnchorMatch> ra;
+        std::vector<float> corr_of(n0, 0.0f);
+        for (int gy = 0; gy < gh_; ++gy)
+            for (int gx = 0; gx < gw_; ++gx) {
+                const size_t i = (size_t)gy * gw_ + gx;
+                if (!alive_[i]) continue;
+                const float x0 = (float)(p0_.rect_x + gx * p0_.step), y0 = (float)(p0_.rect_y + gy * p0_.step);
+                const float px = x0 + ub_[i], py = y0 + vb_[i];
+                const Sample smp = sample_increment(inc, px, py);
+                if (!smp.ok) {
+                    coast(i, inc, px, py);
+                    continue;
+                }
+                U[i] = ub_[i] + smp.u;
+                V[i] = vb_[i] + smp.v;
+                corr_of[i] = smp.corr;
+                need[i] = anchor_need(inc, gd, px, py, subset);
+                if (coast_travel_[i] > trusted_travel) {
+                    const auto a = anchors_.find(i);
+                    if (need[i] == AnchorNeed::kKeep || a == anchors_.end()) {
+                        carried[i] = 1;   // unconfirmed: moves on, unreported
+                        coasting++;
+                        continue;
+                    }
+                    Engine::AnchorMatch m;
+                    m.patch = a->second.patch.data();
+                    m.side = a->second.side;
+                    m.ax = a->second.ax;
+                    m.ay = a->second.ay;
+                    m.dx = a->second.dx;
+                    m.dy = a->second.dy;
+                    m.guess_x = x0 + U[i];
+                    m.guess_y = y0 + V[i];
+                    m.max_shift = std::max(kReanchorMinShiftPx, kReanchorMaxShiftFraction * coast_travel_[i]);
+                    ra_node.push_back(i);
+                    ra_taylor.push_back(smp.taylor ? 1 : 0);
+                    ra.push_back(m);
+                    continue;
+                }
+                valid[i] = 1;
+                carried[i] = 1;
+                if (smp.taylor) fi.taylor_fallbacks++;
             }
-            ub_[i] = tu[i];
-            vb_[i] = tv[i];
+        if (!ra.empty()) {
+            if (eng_->match_anchors(ra, subset, p0_.use6, tol) != 0) return kErrInit;
+            for (size_t k = 0; k < ra.size(); ++k) {
+                const size_t i = ra_node[k];
+                if (!ra[k].ok) {
+                    carried[i] = 1;
+                    coasting++;
+                    continue;
+                }
+                const float x0 = (float)(p0_.rect_x + (int)(i % gw_) * p0_.step);
+                const float y0 = (float)(p0_.rect_y + (int)(i / gw_) * p0_.step);
+                U[i] = ra[k].x - x0;
+                V[i] = ra[k].y - y0;
+                corr_of[i] = ra[k].corr;
+                valid[i] = 1;
+                carried[i] = 1;
+                if (ra_taylor[k]) fi.taylor_fallbacks++;
+            }
         }
-        if (eng_->set_reference(def, false, w_, h_, nullptr) != 0) return kErrInit;
-        ref_frame_ = frame;
-        ref_own_count_ = count;
-        ref_first_count_ = -1;
+        for (size_t i = 0; i < n0; ++i)
+            if (valid[i]) {
+                tot.u[i] = U[i];
+                tot.v[i] = V[i];
+                tot.corr[i] = corr_of[i];
+                tot.ok[i] = 1;
+            }
+        // the median test runs on the total field and drops from this frame's
+        // output only; it is not fed back into `valid`
+        const int rcp = eng_->strain_pack(tot, p0_.strain_window, out, out_capacity, nullptr, nullptr);
+        if (rcp < 0) return rcp;
+        written = rcp;
     }
-    frame_ = frame;
-    fi.reference_updated = update;
-    fi.tracked_points = (int)std::count(alive_.begin(), alive_.end(), (uint8_t)1);
+
+    for (uint8_t c : carried) fi.tracked_points += c ? 1 : 0;
+    fi.frame0_points = std::max(frame0_points, 0);
+
+    // the update floor and the early update on a drop in the points reported
+    int ref_reported = ref_reported_;
+    const int base = ref_reported >= 0 ? ref_reported : adopted_reported_ >= 0 ? adopted_reported_ : frame0_points;
+    if (base > 0 && (float)written < kUpdateMinReported * (float)base) {
+        will_update = false;
+    } else if (N > 1) {
+        if (ref_reported < 0) ref_reported = written;
+        else if ((float)written < (1.0f - kEarlyUpdateDrop) * (float)ref_reported) will_update = true;
+    }
+
+    if (will_update) {
+        const int before = (ref_index_ == 0) ? fi.frame0_points : alive_count();
+        std::vector<float> coast_travel = coast_travel_;
+        auto anchors = anchors_;
+        for (size_t i = 0; i < n0; ++i) {
+            coast_travel[i] = (carried[i] && !valid[i]) ? coast_travel[i] + coast_step[i] : 0.0f;
+            if (!carried[i] || (valid[i] && need[i] == AnchorNeed::kDrop)) {
+                anchors.erase(i);
+            } else if (valid[i] && need[i] == AnchorNeed::kRefresh) {
+                // make_anchor (incremental_math.cpp:224-235) on the frame becoming the reference
+                const float x = (float)(p0_.rect_x + (int)(i % gw_) * p0_.step) + U[i];
+                const float y = (float)(p0_.rect_y + (int)(i / gw_) * p0_.step) + V[i];
+                const int ax = (int)std::lround(x), ay = (int)std::lround(y);
+                const int r = subset / 2 + kAnchorMarginPx;
+                if (ax - r >= 0 && ay - r >= 0 && ax + r < w_ && ay + r < h_) {
+                    Anchor an;
+                    an.side = 2 * r + 1;
+                    an.ax = ax;
+                    an.ay = ay;
+                    an.dx = x - (float)ax;
+                    an.dy = y - (float)ay;
+                    an.patch.resize((size_t)an.side * an.side);
+                    for (int yy = 0; yy < an.side; ++yy)
+                        std::memcpy(&an.patch[(size_t)yy * an.side], def + (size_t)(ay - r + yy) * w_ + (ax - r),
+                                    (size_t)an.side);
+                    anchors[i] = std::move(an);
+                }
+            }
+        }
+        // the next reference's mask (warp_material_mask) and grid (prepare_reference)
+        std::vector<float> Ub = U, Vb = V;
+        std::vector<uint8_t> alive = carried;
+        const int half = p0_.subset / 2;
+        const int margin = (kMaskExtrapCells + 1) * p0_.step + half;
+        std::vector<uint8_t> mask_k((size_t)w_ * h_, 0);
+        {
+            std::vector<float> Uf = Ub, Vf = Vb;
+            if (fill_field(gw_, gh_, Uf, Vf, alive)) {
+                double stretch = 1.0;
+                for (int gy = 0; gy < gh_; ++gy)
+                    for (int gx = 0; gx < gw_; ++gx) {
+                        const size_t i = (size_t)gy * gw_ + gx;
+                        const size_t ix = gx + 1 < gw_ ? i + 1 : i, iy = gy + 1 < gh_ ? i + gw_ : i;
+                        const double ux = (Uf[ix] - Uf[i]) / p0_.step, uy = (Uf[iy] - Uf[i]) / p0_.step;
+                        const double vx = (Vf[ix] - Vf[i]) / p0_.step, vy = (Vf[iy] - Vf[i]) / p0_.step;
+                        stretch = std::max(stretch, std::max(std::abs(1 + ux) + std::abs(uy),
+                                                             std::abs(vx) + std::abs(1 + vy)));
+                    }
+                const int per_px = std::min(kMaskMaxSamplesPerPx, std::max(2, (int)std::ceil(stretch / 0.7)));
+                const int X0 = std::max(0, p0_.rect_x - margin);
+                const int X1 = std::min(w_ - 1, p0_.rect_x + (gw_ - 1) * p0_.step + margin);
+                const int Y0 = std::max(0, p0_.rect_y - margin);
+                const int Y1 = std::min(h_ - 1, p0_.rect_y + (gh_ - 1) * p0_.step + margin);
+                const MaskGrid mg{gw_, gh_, p0_.rect_x, p0_.rect_y, p0_.step};
+                if (mask_warp(material_mask0_.data(), w_, h_, mg, Uf, Vf, X0, Y0, X1, Y1, per_px, mask_k, nullptr) != 0)
+                    return kErrInit;
+            }
+        }
+        int bx0 = w_, by0 = h_, bx1 = -1, by1 = -1;
+        for (int y = 0; y < h_; ++y)
+            for (int x = 0; x < w_; ++x)
+                if (mask_k[(size_t)y * w_ + x]) {
+                    bx0 = std::min(bx0, x);
+                    bx1 = std::max(bx1, x);
+                    by0 = std::min(by0, y);
+                    by1 = std::max(by1, y);
+                }
+        Params pk = p0_;
+        float mnx = 1e30f, mny = 1e30f, mxx = -1e30f, mxy = -1e30f;
+        for (int gy = 0; gy < gh_; ++gy)
+            for (int gx = 0; gx < gw_; ++gx) {
+                const size_t i = (size_t)gy * gw_ + gx;
+                if (!alive[i]) continue;
+                const float x = (float)(p0_.rect_x + gx * p0_.step) + Ub[i];
+                const float y = (float)(p0_.rect_y + gy * p0_.step) + Vb[i];
+                mnx = std::min(mnx, x);
+                mxx = std::max(mxx, x);
+                mny = std::min(mny, y);
+                mxy = std::max(mxy, y);
+            }
+        int rx = 0, ry = 0, rw = 0, rh = 0;
+        if (mnx <= mxx) {
+            const int x0 = (int)std::floor(mnx), y0 = (int)std::floor(mny);
+            // that rect, widened by one step on the far sides, clipped to the image
+            const int x1 = std::min(w_, x0 + ((int)std::ceil(mxx) - x0 + p0_.step));
+            const int y1 = std::min(h_, y0 + ((int)std::ceil(mxy) - y0 + p0_.step));
+            rx = std::max(0, x0);
+            ry = std::max(0, y0);
+            rw = std::max(0, x1 - rx);
+            rh = std::max(0, y1 - ry);
+        }
+        pk.rect_x = rx;
+        pk.rect_y = ry;
+        pk.rect_w = rw;
+        pk.rect_h = rh;
+        // the increment solve's strain is never reported; its window only has to pass validation
+        if (pk.step > 0) {
+            const int64_t gmin = std::min(rw / pk.step, rh / pk.step);
+            pk.strain_window = (int)std::min<int64_t>(pk.strain_window, 2 * (gmin + 1) * (int64_t)pk.step);
+        }
+        if (bx1 < 0) std::fill(alive.begin(), alive.end(), 0);
+        // adopt the frame just solved, with the warped mask as its Ghost Wall
+        if (eng_->set_reference(def, false, w_, h_, mask_k.data()) != 0) {
+            init_ = false;   // the engine lost its reference: the session cannot go on
+            return kErrInit;
+        }
+        coast_travel_.swap(coast_travel);
+        anchors_.swap(anchors);
+        ub_.swap(Ub);
+        vb_.swap(Vb);
+        alive_.swap(alive);
+        mask_k_.swap(mask_k);
+        pk_ = pk;
+        frame_count_++;
+        ref_index_ = frame_count_;
+        frames_since_update_ = 0;
+        ref_reported_ = -1;
+        adopted_reported_ = written;
+        frame0_points_ = frame0_points;
+        fi.reference_updated = true;
+        fi.lost_this_frame = std::max(0, before - alive_count());
+    } else {
+        frame_count_++;
+        frames_since_update_++;
+        ref_reported_ = ref_reported;
+        frame0_points_ = frame0_points;
+    }
     if (info) *info = fi;
-    return count;
+    return written;
 }
 
 }  // namespace sdic

@@ -1,25 +1,31 @@
 // Incremental (updated-reference) DIC over a frame sequence
-// (include/semper/incremental.hpp:74-117, src/pipeline/incremental_session.cpp).
+// (include/semper/incremental.hpp:74-117, src/pipeline/incremental_session.cpp,
+// src/pipeline/incremental_math.cpp).
 //
-// Each frame is solved on the GPU against the current reference on that
-// reference's grid; the increment is composed onto the running total at the
-// frame-0 grid points in FP64,
+// Every frame is solved by the full-field engine on the GPU against the
+// current reference, on that reference's own grid; each frame-0 node's
+// increment is sampled where the node sits on the reference and added to its
+// total,
 //     U(X0, t) = U(X0, k) + u_k(X0 + U(X0, k)),
-// with u_k sampled bilinearly between solved nodes, or by the first-order
-// expansion of the nearest solved node within 1.5 cells near an edge. Every
-// `update_interval` frames (0: never) the frame just solved becomes the
-// reference; a frame-0 point that could not be sampled at an update is lost.
-// Output: the frozen 8-float layout at the frame-0 grid, Green-Lagrange strain
-// of the total displacement.
-//
-// Covered: the composition, the Taylor fallback, the update interval and the
-// update floor (no move to a frame reporting under half the points of the
-// reference's first frame). Not covered yet: coasting of unsampled points
-// across frames, re-anchoring, the early update on a 5% drop, and the mask warp.
+// bilinearly between solved nodes, or by the first-order expansion of the
+// nearest solved node within 1.5 cells. A node the solve does not reach coasts,
+// unreported, on a weighted plane through the solved increments within 8 cells
+// (at most 2 subsets of travel); a node that coasted further than a quarter
+// subset is reported again only once its subset, stored where it was last
+// clean, matches the frame. The reference moves to the frame just solved every
+// `update_interval` frames (0: never), early when the points reported fall 5%
+// below the first frame on the current reference, and never to a frame
+// reporting under half of that count (or of the reference's own, or of the
+// frame-0 scan). At an update the frame-0 material mask is warped forward into
+// the new reference and the new grid spans where the tracked nodes sit. Output:
+// the frozen 8-float layout at the frame-0 grid, Green-Lagrange strain of the
+// total displacement, corr of the increment. While the reference is frame 0,
+// a frame is exactly the engine's run on it.
 #pragma once
 
 #include <cstdint>
 #include <functional>
+#include <map>
 #include <memory>
 #include <vector>
 
@@ -40,21 +46,39 @@ class IncrementalSession {
 public:
     void reset();
     bool initialized() const { return init_; }
+    int frames_processed() const { return frame_count_; }
     int init(const uint8_t* ref, int w, int h, const uint8_t* mask, const Params& p, int update_interval);
     int process(const uint8_t* def, float* out, int out_capacity, float* metrics, int metrics_len, Cancel& cancel,
                 const std::function<void(int)>& progress, FrameInfo* info);
 
+    // The solver's stop rule (the reference's fixed 1e-3 through the C ABI).
+    float tol = 1e-3f;
+
 private:
+    struct Anchor {
+        std::vector<uint8_t> patch;   // side x side gray around (ax, ay)
+        int side = 0, ax = 0, ay = 0;
+        float dx = 0, dy = 0;
+    };
+
     std::unique_ptr<Engine> eng_;
     bool init_ = false;
-    int w_ = 0, h_ = 0, interval_ = 0, frame_ = 0, ref_frame_ = 0;
-    int ref_first_count_ = -1;     // points reported by the first frame on the current reference
-    int ref_own_count_ = 0;        // points the reference reported when it was solved (frame 0: scan count)
-    Params p_;
-    std::vector<uint8_t> mask0_;
-    int gw_ = 0, gh_ = 0, frame0_points_ = 0;
+    int w_ = 0, h_ = 0, interval_ = 0;
+    Params p0_;                       // the frame-0 grid (validated)
+    int gw_ = 0, gh_ = 0;
+    std::vector<uint8_t> user_mask0_, material_mask0_;
+    int frame0_points_ = -1;
+    std::vector<float> ub_, vb_, coast_travel_;
     std::vector<uint8_t> alive_;
-    std::vector<double> ub_, vb_;  // total displacement of each frame-0 node at the current reference
+    std::map<size_t, Anchor> anchors_;
+    int ref_index_ = 0, frames_since_update_ = 0, ref_reported_ = -1, adopted_reported_ = -1, frame_count_ = 0;
+    Params pk_;                       // the grid on the current reference (k > 0)
+    std::vector<uint8_t> mask_k_;
+    std::vector<float> scratch_;
+
+    int alive_count() const;
+    int nothing_to_follow(FrameInfo fi, float* metrics, int metrics_len, const std::function<void(int)>& progress,
+                          FrameInfo* info);
 };
 
 }  // namespace sdic

@@ -139,8 +139,8 @@ __global__ void k_quantize(const float* __restrict__ s, int64_t n, float mid, fl
 // Builds the blob field for a w x h image (host), uploads it, renders the
 // reference (identity field) and the deformed image, and writes both as u8
 // device planes. Returns a HIP error code (0 on success).
-int synth_pair(uint64_t seed, int w, int h, const FieldSpec& field, float noise_sigma, uint8_t* d_ref,
-               uint8_t* d_def, hipStream_t st) {
+int synth_pair(uint64_t seed, uint64_t noise_seed, int w, int h, const FieldSpec& field, float noise_sigma,
+               uint8_t* d_ref, uint8_t* d_def, hipStream_t st) {
     const float pad = 16.0f, density = 0.5f;
     const int nb = (int)std::lround((double)density * (w + 2 * pad) * (h + 2 * pad));
     std::vector<float> bx(nb), by(nb), bk(nb), ba(nb);
@@ -187,9 +187,9 @@ int synth_pair(uint64_t seed, int w, int h, const FieldSpec& field, float noise_
         const float sd = std::sqrt(density * 3.14159265f * (28.0f / 3.0f)), peak = 200.0f;
         const float scale = 0.5f * peak / (3.0f * sd);
         hipLaunchKernelGGL(k_quantize, dim3(blocks), dim3(256), 0, st, dS, n, 0.5f * peak, scale, peak, noise_sigma,
-                           seed ^ 0x5151ull, d_ref);
+                           noise_seed ^ 0x5151ull, d_ref);
         hipLaunchKernelGGL(k_quantize, dim3(blocks), dim3(256), 0, st, dS + n, n, 0.5f * peak, scale, peak, noise_sigma,
-                           seed ^ 0xa2a2ull, d_def);
+                           noise_seed ^ 0xa2a2ull, d_def);
         e = hipGetLastError();
         const hipError_t es = hipStreamSynchronize(st);
         if (e == hipSuccess) e = es;

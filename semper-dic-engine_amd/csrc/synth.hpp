@@ -22,7 +22,7 @@ struct FieldSpec {
     double p[48];
 };
 
-int synth_pair(uint64_t seed, int w, int h, const FieldSpec& field, float noise_sigma, uint8_t* d_ref,
-               uint8_t* d_def, hipStream_t st);
+int synth_pair(uint64_t seed, uint64_t noise_seed, int w, int h, const FieldSpec& field, float noise_sigma,
+               uint8_t* d_ref, uint8_t* d_def, hipStream_t st);
 
 }  // namespace sdic

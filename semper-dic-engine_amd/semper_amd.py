@@ -95,6 +95,8 @@ def load_dev(path=LIB_DEV):
     lib.sdk_engine_timings.argtypes = [_VP, _VP]
     lib.sdk_engine_set_flag.argtypes = [_VP, ctypes.c_int, ctypes.c_int]
     lib.sdk_wave_sum_probe.argtypes = [_VP, ctypes.c_int, _VP, _VP]
+    lib.sdk_synth_frame.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP,
+                                    ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_float, _VP, ctypes.c_int]
     lib.sdk_synth_pair.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, ctypes.c_int,
                                    ctypes.c_double, ctypes.c_double, ctypes.c_float, _VP, _VP, ctypes.c_int]
     return lib
@@ -149,3 +151,44 @@ class Engine:
 
     def cancel(self):
         self.lib.semper_cancel(self.h)
+
+
+class Incremental:
+    """One semper_incremental handle (reference include/semper/incremental.hpp,
+    bindings/python/semper/__init__.py): init(), process() -> (points[n, 8],
+    metrics[24], info)."""
+
+    def __init__(self, lib=None):
+        self.lib = lib or load_c()
+        self.h = self.lib.semper_incremental_create()
+        if not self.h:
+            raise MemoryError("semper_incremental_create failed")
+
+    def close(self):
+        if self.h:
+            self.lib.semper_incremental_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def init(self, img, p, update_interval, mask=None):
+        img = np.ascontiguousarray(img, dtype=np.uint8)
+        h, w = img.shape
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+        self.cap = max(1, (min(p.rect_w, w) // max(p.step, 1) + 1) * (min(p.rect_h, h) // max(p.step, 1) + 1)) * 8
+        return self.lib.semper_incremental_init(self.h, _ptr(img), img.size, _ptr(m), 0 if m is None else m.size,
+                                                w, h, ctypes.byref(p), update_interval)
+
+    def process(self, img, capacity=None, metrics_len=METRICS_LEN):
+        img = np.ascontiguousarray(img, dtype=np.uint8)
+        cap = capacity if capacity is not None else self.cap
+        out = np.zeros(max(cap, 1), np.float32)
+        met = np.zeros(max(metrics_len, 1), np.float32)
+        info = IncInfo()
+        n = self.lib.semper_incremental_process(self.h, _ptr(img), img.size, _ptr(out), cap, _ptr(met), metrics_len,
+                                                ctypes.byref(info), PROGRESS_CB(), None)
+        pts = out[: max(n, 0) * FLOATS_PER_POINT].reshape(-1, FLOATS_PER_POINT).copy()
+        return n, pts, met, info
+
+    def cancel(self):
+        self.lib.semper_incremental_cancel(self.h)

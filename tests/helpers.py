@@ -35,6 +35,11 @@ def oracle():
         if not os.path.exists(path):
             raise RuntimeError("oracle/liboracle.so is not built")
         lib = ctypes.CDLL(path)
+        lib.orc_inc_create.restype = VP
+        lib.orc_inc_destroy.argtypes = [VP]
+        lib.orc_inc_init.argtypes = [VP, VP, ctypes.c_int, ctypes.c_int, VP, VP, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_float]
+        lib.orc_inc_process.argtypes = [VP, VP, VP, ctypes.c_int, VP, ctypes.c_int, VP]
         lib.orc_engine_create.restype = VP
         lib.orc_engine_destroy.argtypes = [VP]
         lib.orc_engine_set_reference.argtypes = [VP, VP, ctypes.c_int, ctypes.c_int, VP]
@@ -195,3 +200,73 @@ def fp64_polish(ref, dfm, dim, nodes, p0, use6=False, tol=1e-10, max_iter=200):
     oracle().orc_icgn_fp64(P(r), P(d), w, h, dim, int(use6), n, P(cxy), P(start), ctypes.c_double(tol), max_iter,
                            P(si), P(po), P(corr))
     return si, po, corr
+
+
+# ---- frame sequences with exact ground truth (tests/framework/sequence.h) ----
+
+def stretch_rotate_aff(e, nu, theta_deg, tx, ty, cx, cy):
+    """x = c + F (X - c) + t with F = R(theta) diag(1 + e, 1 - nu e), as the
+    (u, v, ux, uy, vx, vy, cx, cy) affine the speckle renderer takes."""
+    th = np.radians(theta_deg)
+    c, s_ = np.cos(th), np.sin(th)
+    a, d = 1.0 + e, 1.0 - nu * e
+    F = np.array([[c * a, -s_ * d], [s_ * a, c * d]])
+    return [tx, ty, F[0, 0] - 1.0, F[0, 1], F[1, 0], F[1, 1] - 1.0, cx, cy]
+
+
+def aff_truth(aff, x, y):
+    """Displacement and Green-Lagrange strain of a reference point under aff."""
+    u0, v0, ux, uy, vx, vy, cx, cy = aff
+    dx, dy = np.asarray(x, np.float64) - cx, np.asarray(y, np.float64) - cy
+    u = u0 + ux * dx + uy * dy
+    v = v0 + vx * dx + vy * dy
+    exx = 0.5 * (2 * ux + ux * ux + vx * vx)
+    eyy = 0.5 * (2 * vy + uy * uy + vy * vy)
+    exy = 0.5 * (uy + vx + ux * uy + vx * vy)
+    return u, v, exx, eyy, exy
+
+
+def speckle_sequence(T, w=256, h=256, e_max=0.1, theta_max=4.0, nu=0.3, tx_max=0.0, ty_max=0.0, seed=31,
+                     blobs=2400):
+    """Frames 1..T of stretch e_max t / T with rotation theta_max t / T and a
+    drift (tx, ty) t / T about the image centre, rendered from one speckle."""
+    ref = gray8(ref_speckle(seed, w, h, blobs))
+    frames, affs = [], []
+    for t in range(1, T + 1):
+        f = t / T
+        aff = stretch_rotate_aff(e_max * f, nu, theta_max * f, tx_max * f, ty_max * f, w / 2.0, h / 2.0)
+        frames.append(gray8(ref_speckle(seed, w, h, blobs, aff)))
+        affs.append(aff)
+    return ref, frames, affs
+
+
+def centred_params(w, h, roi, step=8, subset=21, window=None, use6=False):
+    return ((w - roi) // 2, (h - roi) // 2, roi, roi, step, subset, window or 2 * step + 1, 1 if use6 else 0)
+
+
+class OracleIncremental:
+    """The CPU restatement of IncrementalSession (oracle/src/incremental.cpp)."""
+
+    def __init__(self):
+        self.lib = oracle()
+        self.h = self.lib.orc_inc_create()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.orc_inc_destroy(self.h)
+
+    def init(self, ref, prm, interval, mask=None, threads=0, tol=1e-3):
+        ref = np.ascontiguousarray(ref, np.uint8)
+        h, w = ref.shape
+        m = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+        pr = np.array(prm, np.int32)
+        self.cap = (prm[2] // prm[4] + 1) * (prm[3] // prm[4] + 1) * 8
+        return self.lib.orc_inc_init(self.h, P(ref), w, h, P(m), P(pr), interval, threads, ctypes.c_float(tol))
+
+    def process(self, img):
+        img = np.ascontiguousarray(img, np.uint8)
+        out = np.zeros(self.cap, np.float32)
+        met = np.zeros(24, np.float32)
+        info = np.zeros(6, np.int32)
+        n = self.lib.orc_inc_process(self.h, P(img), P(out), self.cap, P(met), 24, P(info))
+        return n, out[: max(n, 0) * 8].reshape(-1, 8).copy(), met, info

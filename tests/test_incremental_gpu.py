@@ -1,0 +1,147 @@
+"""The MI355X IncrementalSession through the C ABI (semper_incremental_*):
+the reference's session contracts (tests/integration/test_incremental.cpp)
+and frame-by-frame parity with the oracle's session on the same sequence."""
+import numpy as np
+import pytest
+
+from helpers import (OracleIncremental, aff_truth, centred_params, engine_module, speckle_sequence)
+
+pytestmark = pytest.mark.gpu
+W = H = 256
+
+
+@pytest.fixture(scope="module")
+def sa():
+    m = engine_module()
+    if m.load_dev().sdk_device_count() < 1:
+        pytest.skip("no GPU")
+    return m
+
+
+@pytest.fixture(scope="module")
+def seq():
+    return speckle_sequence(8, W, H, e_max=0.16, theta_max=8.0)
+
+
+def params(sa, prm):
+    return sa.params(prm[:4], prm[4], prm[5], prm[6], bool(prm[7]))
+
+
+def run_gpu(sa, ref, frames, prm, interval, mask=None):
+    s = sa.Incremental()
+    assert s.init(ref, params(sa, prm), interval, mask) == 0
+    out = []
+    for f in frames:
+        n, pts, met, info = s.process(f)
+        out.append((n, pts, met, [info.reference_frame, info.reference_updated, info.tracked_points,
+                                  info.frame0_points, info.taylor_fallbacks, info.lost_this_frame]))
+    return out
+
+
+def test_interval_zero_and_first_frame_are_semper_run(sa, seq):
+    """IntervalZero_BitIdenticalToRunFullField and FirstFrame_BitIdenticalToRunFullField
+    (test_incremental.cpp:202-228), against this library's own semper_run."""
+    ref, frames, _ = seq
+    prm = centred_params(W, H, 128)
+    eng = sa.Engine()
+    assert eng.set_reference(ref) == 0
+    fixed = [eng.run(f, params(sa, prm)) for f in frames[:3]]
+    for (n, pts, _, info), (n_f, pts_f, _) in zip(run_gpu(sa, ref, frames[:3], prm, 0), fixed):
+        assert n == n_f > 0 and np.array_equal(pts, pts_f) and info[0] == 0 and info[1] == 0
+    for interval in (1, 3):
+        n, pts, _, _ = run_gpu(sa, ref, frames[:1], prm, interval)[0]
+        assert n == fixed[0][0] and np.array_equal(pts, fixed[0][1])
+
+
+def test_update_schedule(sa, seq):
+    ref, frames, _ = seq
+    prm = centred_params(W, H, 128)
+    n1 = run_gpu(sa, ref, frames[:6], prm, 1)
+    n3 = run_gpu(sa, ref, frames[:6], prm, 3)
+    for t in range(6):
+        assert n1[t][3][:2] == [t, 1] and n1[t][0] > 0
+        assert n3[t][3][:2] == [(t // 3) * 3, int((t + 1) % 3 == 0)] and n3[t][0] > 0
+
+
+@pytest.mark.parametrize("interval", [1, 3])
+def test_session_vs_oracle(sa, seq, interval):
+    """Frame by frame against the oracle's session: the same reference
+    schedule, point counts within 1%, tracked counts within 1%, u, v within the
+    reference's cross-toolchain bound (5e-3 px) on the common points, strain
+    within 1e-4; and both within 0.1 px rms of the truth."""
+    ref, frames, affs = seq
+    prm = centred_params(W, H, 128)
+    g = run_gpu(sa, ref, frames, prm, interval)
+    o = OracleIncremental()
+    assert o.init(ref, prm, interval) == 0
+    for t, ((n_g, p_g, _, i_g), f, aff) in enumerate(zip(g, frames, affs)):
+        n_o, p_o, _, i_o = o.process(f)
+        i_o = i_o.tolist()
+        print(f"frame {t + 1}: gpu {n_g} pts {i_g}, oracle {n_o} pts {i_o}", flush=True)
+        assert i_g[:2] == i_o[:2] and i_g[3] == i_o[3]
+        assert abs(n_g - n_o) <= max(2, n_o // 100) and abs(i_g[2] - i_o[2]) <= max(2, i_o[2] // 100)
+        go = {(a, b): r for a, b, *r in p_o.tolist()}
+        gg = {(a, b): r for a, b, *r in p_g.tolist()}
+        common = sorted(set(go) & set(gg))
+        assert len(common) >= n_o - max(2, n_o // 100)
+        d = np.array([np.abs(np.array(go[k]) - np.array(gg[k])) for k in common])
+        assert d[:, :2].max() <= 5e-3, d[:, :2].max()
+        assert d[:, 2:5].max() <= 1e-4, d[:, 2:5].max()
+        u, v, *_ = aff_truth(aff, p_g[:, 0], p_g[:, 1])
+        assert np.sqrt(np.mean((p_g[:, 2] - u) ** 2 + (p_g[:, 3] - v) ** 2)) <= 0.1
+
+
+def test_lost_nodes_never_return(sa):
+    ref, frames, _ = speckle_sequence(8, W, H, e_max=0.02, theta_max=1.0, tx_max=100.0)
+    inc = run_gpu(sa, ref, frames, centred_params(W, H, 160), 1)
+    for t in range(1, len(inc)):
+        assert inc[t][3][2] <= inc[t - 1][3][2] and inc[t][3][5] >= 0
+    assert inc[-1][3][2] < inc[0][3][3]
+
+
+def test_uniaxial_ramp_1024_64_frames(sa):
+    """The batched/incremental workload at its size: 64 frames of 1024^2
+    speckle under a uniaxial stretch ramping 0 -> 20% exx with Poisson
+    contraction 0.3, update_interval 1, step 8, subset 21, strain window 21.
+    Bounds of the reference's long-horizon test (test_incremental.cpp:552-611,
+    which accumulates interpolation bias over every update the same way):
+    total displacement within 0.45 px rms of the truth, strain within 3.5e-2
+    rms, >= 95% of frame-0 nodes tracked, >= 75% of frame 1's points. Each frame
+    has its own noise: a fixed noise pattern shared by consecutive frames would
+    pull every increment towards zero."""
+    import importlib.util
+    import os
+    from helpers import PKG
+    spec = importlib.util.spec_from_file_location("workloads", os.path.join(PKG, "workloads.py"))
+    wl = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(wl)
+    dev = sa.load_dev()
+    w = h = 1024
+    T = 64
+    frames = []
+    for t in range(0, T + 1):
+        e = 0.2 * t / T
+        prm = np.array([e, 0.3], np.float64)
+        f = np.zeros((h, w), np.uint8)
+        # each frame carries its own sensor noise
+        rc = dev.sdk_synth_frame(7007, 100 + t, w, h, wl.UNIAXIAL, prm.ctypes.data_as(sa.ctypes.c_void_p), 2,
+                                 w / 2.0, h / 2.0, sa.ctypes.c_float(wl.NOISE_SIGMA),
+                                 f.ctypes.data_as(sa.ctypes.c_void_p), 0)
+        assert rc == 0
+        frames.append(f)
+    prm = (w // 4, h // 4, w // 2, h // 2, 8, 21, 21, 0)
+    res = run_gpu(sa, frames[0], frames[1:], prm, 1)
+    n_last, pts, _, info = res[-1]
+    e = 0.2
+    u_true = e * (pts[:, 0] - w / 2.0)
+    v_true = -0.3 * e * (pts[:, 1] - h / 2.0)
+    rms = np.sqrt(np.mean((pts[:, 2] - u_true) ** 2 + (pts[:, 3] - v_true) ** 2))
+    exx = 0.5 * (2 * e + e * e)
+    vy = -0.3 * e
+    eyy = 0.5 * (2 * vy + vy * vy)
+    rms_e = np.sqrt(np.mean((pts[:, 4] - exx) ** 2 + (pts[:, 5] - eyy) ** 2 + pts[:, 6] ** 2) / 3)
+    print(f"uniaxial ramp: frame 64 {n_last} pts (frame 1 {res[0][0]}), tracked {info[2]}/{info[3]}, "
+          f"rms u {rms:.4f} px, rms E {rms_e:.2e}", flush=True)
+    assert all(r[0] > 0 for r in res)
+    assert info[2] >= 0.95 * info[3] and n_last >= 0.75 * res[0][0]
+    assert rms <= 0.45 and rms_e <= 3.5e-2
