@@ -70,6 +70,22 @@ SDK_API int sdk_synth_pair(uint64_t seed, int w, int h, int kind, const double* 
 SDK_API int sdk_synth_frame(uint64_t seed, uint64_t noise_seed, int w, int h, int kind, const double* p, int np,
                             double cx, double cy, float noise_sigma, uint8_t* def, int on_device);
 
+/* The boundary's image decode (host only, no GPU).
+   sdk_decode: io::decode_gray (mask = 0) or io::decode_mask (mask = 1)
+   (src/io/image_codec.cpp:129-161,254-262): writes the decoded size to *w, *h
+   and, when out is not NULL and cap >= w * h, the plane. Returns 0, or -3 when
+   nothing decodes (or cap is short).
+   sdk_image_dimensions: io::image_dimensions (image_codec.cpp:265-287).
+   sdk_to_gray8 / sdk_to_mask8: io::to_gray8 / to_mask8 (image_codec.cpp:221-252)
+   of a w x h image of cn = 1 or 3 interleaved channels (R, G, B) of
+   depth 0 = u8, 1 = u16, 2 = s16, 3 = s32, 4 = f32, 5 = f64; w * h bytes to
+   out. Returns 0 or -3. */
+SDK_API int sdk_decode(const uint8_t* data, size_t len, int expected_w, int expected_h, int mask, uint8_t* out,
+                       size_t cap, int32_t* w, int32_t* h);
+SDK_API void sdk_image_dimensions(const uint8_t* data, size_t len, int32_t* w, int32_t* h);
+SDK_API int sdk_to_gray8(const void* px, int w, int h, int cn, int depth, uint8_t* out);
+SDK_API int sdk_to_mask8(const void* px, int w, int h, int cn, int depth, uint8_t* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,9 +1,9 @@
 // The drop-in C ABI: the eleven semper_* entry points of the reference
 // (include/semper/semper_c.h:83-197, adapters/c/semper_c.cpp) over the MI355X
 // engine. Handles, the one-call-per-handle rule, the cancel window and the
-// exception barrier behave as the reference's; raw gray / RGBA buffers are
-// decoded here (encoded PNG / TIFF / JPEG need OpenCV's codecs and return
-// SEMPER_ERR_INIT).
+// exception barrier behave as the reference's. Image bytes are raw gray,
+// raw RGBA or an encoded PNG / TIFF / BMP / PNM file (codec.hpp); JPEG and
+// WebP do not decode here and return SEMPER_ERR_INIT.
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -12,6 +12,7 @@
 #include <memory>
 #include <vector>
 
+#include "codec.hpp"
 #include "engine.hpp"
 #include "incremental.hpp"
 #include "semper_c.h"
@@ -64,34 +65,24 @@ R guarded(R on_error, F&& body) noexcept {
     }
 }
 
-// Whether bytes start like an image file (src/io/image_codec.cpp:31-40).
-bool has_signature(const uint8_t* d, size_t len) {
-    auto starts = [&](const char* sig, size_t n) { return len >= n && std::memcmp(d, sig, n) == 0; };
-    return starts("\x89PNG\r\n\x1a\n", 8) || starts("\xff\xd8\xff", 3) || starts("II*\0", 4) ||
-           starts("MM\0*", 4) || starts("BM", 2) ||
-           (starts("RIFF", 4) && len >= 12 && std::memcmp(d + 8, "WEBP", 4) == 0) ||
-           (len >= 2 && d[0] == 'P' && d[1] >= '1' && d[1] <= '6');
+// Image bytes to the engine's 8-bit gray plane: raw gray / RGBA by length,
+// else an encoded file (io::decode_gray, src/io/image_codec.cpp:129-161).
+bool decode_image(const uint8_t* data, size_t len, int ew, int eh, std::vector<uint8_t>& out, int& w, int& h) {
+    return sdic::codec::decode_gray(data, len, ew, eh, out, w, h);
 }
 
-// decode_any (src/io/image_codec.cpp:129-161) for raw buffers: w*h bytes are
-// gray, 4*w*h RGBA (ITU-R 601 luma in 14-bit fixed point, as
-// cv::COLOR_RGBA2GRAY); anything else would need an image codec.
-bool decode_raw(const uint8_t* data, size_t len, int w, int h, std::vector<uint8_t>& out) {
-    if (!data || len == 0 || w <= 0 || h <= 0) return false;
-    const uint64_t px = (uint64_t)w * (uint64_t)h;
-    if (px > (uint64_t(1) << 28)) return false;
-    const bool rgba = (uint64_t)len == px * 4, gray = (uint64_t)len == px;
-    if (!rgba && !gray) return false;
-    if (has_signature(data, len)) return false;   // an encoded file: no codec here
-    out.resize(px);
-    if (gray) {
-        std::memcpy(out.data(), data, px);
-    } else {
-        for (uint64_t i = 0; i < px; ++i) {
-            const uint8_t* q = data + 4 * i;
-            out[i] = (uint8_t)((q[0] * 4899u + q[1] * 9617u + q[2] * 1868u + (1u << 13)) >> 14);
-        }
-    }
+// No mask bytes is no mask; bytes that do not decode are an error
+// (adapters/c/semper_c.cpp:70-79). A mask of another size is brought to the
+// image's by nearest neighbour (full_field_solver.cpp:190-194,
+// incremental_session.cpp:238-242).
+bool decode_mask(const uint8_t* mask, size_t len, int w, int h, std::vector<uint8_t>& out, bool& have) {
+    have = mask != nullptr && len != 0;
+    if (!have) return true;
+    std::vector<uint8_t> m;
+    int mw = 0, mh = 0;
+    if (!sdic::codec::decode_mask(mask, len, w, h, m, mw, mh)) return false;
+    if (mw == w && mh == h) out.swap(m);
+    else sdic::codec::resize_nearest(m, mw, mh, w, h, out);
     return true;
 }
 
@@ -138,10 +129,11 @@ SEMPER_C_API int semper_set_reference(semper_engine* eng, const uint8_t* img, si
     if (img == nullptr || img_len == 0) return SEMPER_ERR_INIT;
     return guarded(SEMPER_ERR_INIT, [&] {
         std::vector<uint8_t> ref, m;
-        if (!decode_raw(img, img_len, expected_w, expected_h, ref)) return SEMPER_ERR_INIT;
-        const bool have_mask = mask != nullptr && mask_len != 0;
-        if (have_mask && !decode_raw(mask, mask_len, expected_w, expected_h, m)) return SEMPER_ERR_INIT;
-        return eng->eng.set_reference(ref.data(), false, expected_w, expected_h, have_mask ? m.data() : nullptr);
+        int w = 0, h = 0;
+        bool have_mask = false;
+        if (!decode_image(img, img_len, expected_w, expected_h, ref, w, h)) return SEMPER_ERR_INIT;
+        if (!decode_mask(mask, mask_len, w, h, m, have_mask)) return SEMPER_ERR_INIT;
+        return eng->eng.set_reference(ref.data(), false, w, h, have_mask ? m.data() : nullptr);
     });
 }
 
@@ -157,9 +149,12 @@ SEMPER_C_API int semper_run(semper_engine* eng, const uint8_t* def, size_t def_l
         if (!eng->eng.has_reference()) return SEMPER_ERR_INIT;
         const int w = eng->eng.width(), h = eng->eng.height();
         std::vector<uint8_t> d, m;
-        if (!decode_raw(def, def_len, w, h, d)) return SEMPER_ERR_INIT;
-        const bool have_mask = mask != nullptr && mask_len != 0;
-        if (have_mask && !decode_raw(mask, mask_len, w, h, m)) return SEMPER_ERR_INIT;
+        int dw = 0, dh = 0;
+        bool have_mask = false;
+        // The frame is correlated pixel for pixel with the reference: another
+        // size is an error (full_field_solver.cpp:168-174).
+        if (!decode_image(def, def_len, w, h, d, dw, dh) || dw != w || dh != h) return SEMPER_ERR_INIT;
+        if (!decode_mask(mask, mask_len, w, h, m, have_mask)) return SEMPER_ERR_INIT;
         eng->cancel.enter();
         return eng->eng.run(d.data(), false, have_mask ? m.data() : nullptr, to_params(params), out, out_capacity,
                             metrics, metrics_len, eng->cancel, to_progress(cb, user));
@@ -194,13 +189,14 @@ SEMPER_C_API int semper_incremental_init(semper_incremental* inc, const uint8_t*
     if (ref == nullptr || ref_len == 0 || params == nullptr) return SEMPER_ERR_INIT;
     return guarded(SEMPER_ERR_INIT, [&] {
         std::vector<uint8_t> r, m;
-        if (!decode_raw(ref, ref_len, expected_w, expected_h, r)) return SEMPER_ERR_INIT;
-        const bool have_mask = mask != nullptr && mask_len != 0;
-        if (have_mask && !decode_raw(mask, mask_len, expected_w, expected_h, m)) return SEMPER_ERR_INIT;
-        const int rc = inc->session.init(r.data(), expected_w, expected_h, have_mask ? m.data() : nullptr,
-                                         to_params(params), update_interval);
-        inc->width = rc == 0 ? expected_w : 0;
-        inc->height = rc == 0 ? expected_h : 0;
+        int w = 0, h = 0;
+        bool have_mask = false;
+        if (!decode_image(ref, ref_len, expected_w, expected_h, r, w, h)) return SEMPER_ERR_INIT;
+        if (!decode_mask(mask, mask_len, w, h, m, have_mask)) return SEMPER_ERR_INIT;
+        const int rc = inc->session.init(r.data(), w, h, have_mask ? m.data() : nullptr, to_params(params),
+                                         update_interval);
+        inc->width = rc == 0 ? w : 0;
+        inc->height = rc == 0 ? h : 0;
         return rc;
     });
 }
@@ -216,7 +212,10 @@ SEMPER_C_API int semper_incremental_process(semper_incremental* inc, const uint8
     return guarded(SEMPER_ERR_INIT, [&] {
         CancelScope window(inc->cancel);
         std::vector<uint8_t> d;
-        if (!decode_raw(def, def_len, inc->width, inc->height, d)) return SEMPER_ERR_INIT;
+        int dw = 0, dh = 0;
+        if (!decode_image(def, def_len, inc->width, inc->height, d, dw, dh) || dw != inc->width ||
+            dh != inc->height)
+            return SEMPER_ERR_INIT;
         inc->cancel.enter();
         sdic::FrameInfo fi;
         const int rc = inc->session.process(d.data(), out, out_capacity, metrics, metrics_len, inc->cancel,

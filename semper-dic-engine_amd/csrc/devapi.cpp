@@ -7,6 +7,7 @@
 #include <memory>
 #include <vector>
 
+#include "codec.hpp"
 #include "engine.hpp"
 #include "semper_dev.h"
 #include "synth.hpp"
@@ -304,6 +305,68 @@ SDK_API int sdk_synth_frame(uint64_t seed, uint64_t noise_seed, int w, int h, in
     if (rc != 0) return rc;
     if (!on_device && hipMemcpy(def, dd.p, npx, hipMemcpyDeviceToHost) != hipSuccess) return -3;
     return 0;
+}
+
+// The boundary's image decode (codec.hpp), for tests of the host side.
+SDK_API int sdk_decode(const uint8_t* data, size_t len, int expected_w, int expected_h, int mask, uint8_t* out,
+                       size_t cap, int32_t* w, int32_t* h) {
+    try {
+        std::vector<uint8_t> plane;
+        int dw = 0, dh = 0;
+        const bool ok = mask ? sdic::codec::decode_mask(data, len, expected_w, expected_h, plane, dw, dh)
+                             : sdic::codec::decode_gray(data, len, expected_w, expected_h, plane, dw, dh);
+        if (w) *w = ok ? dw : 0;
+        if (h) *h = ok ? dh : 0;
+        if (!ok) return -3;
+        if (out != nullptr) {
+            if (cap < plane.size()) return -3;
+            std::memcpy(out, plane.data(), plane.size());
+        }
+        return 0;
+    } catch (...) {
+        return -3;
+    }
+}
+
+SDK_API void sdk_image_dimensions(const uint8_t* data, size_t len, int32_t* w, int32_t* h) {
+    int dw = 0, dh = 0;
+    try {
+        sdic::codec::image_dimensions(data, len, dw, dh);
+    } catch (...) {
+        dw = dh = 0;
+    }
+    if (w) *w = dw;
+    if (h) *h = dh;
+}
+
+static int raster_convert(const void* px, int w, int h, int cn, int depth, uint8_t* out, bool mask) {
+    using sdic::codec::Depth;
+    static const Depth kDepth[6] = {Depth::U8, Depth::U16, Depth::S16, Depth::S32, Depth::F32, Depth::F64};
+    static const size_t kBytes[6] = {1, 2, 2, 4, 4, 8};
+    if (!px || !out || w <= 0 || h <= 0 || (cn != 1 && cn != 3) || depth < 0 || depth > 5) return -3;
+    try {
+        sdic::codec::Raster r;
+        r.w = w;
+        r.h = h;
+        r.cn = cn;
+        r.depth = kDepth[depth];
+        const size_t bytes = (size_t)w * h * cn * kBytes[depth];
+        r.px.assign((const uint8_t*)px, (const uint8_t*)px + bytes);
+        std::vector<uint8_t> o;
+        if (!(mask ? sdic::codec::to_mask8(r, o) : sdic::codec::to_gray8(r, o))) return -3;
+        std::memcpy(out, o.data(), o.size());
+        return 0;
+    } catch (...) {
+        return -3;
+    }
+}
+
+SDK_API int sdk_to_gray8(const void* px, int w, int h, int cn, int depth, uint8_t* out) {
+    return raster_convert(px, w, h, cn, depth, out, false);
+}
+
+SDK_API int sdk_to_mask8(const void* px, int w, int h, int cn, int depth, uint8_t* out) {
+    return raster_convert(px, w, h, cn, depth, out, true);
 }
 
 }  // extern "C"

@@ -99,7 +99,84 @@ def load_dev(path=LIB_DEV):
                                     ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_float, _VP, ctypes.c_int]
     lib.sdk_synth_pair.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, ctypes.c_int,
                                    ctypes.c_double, ctypes.c_double, ctypes.c_float, _VP, _VP, ctypes.c_int]
+    lib.sdk_decode.argtypes = [_VP, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, ctypes.c_size_t,
+                               _VP, _VP]
+    lib.sdk_image_dimensions.argtypes = [_VP, ctypes.c_size_t, _VP, _VP]
+    lib.sdk_image_dimensions.restype = None
+    lib.sdk_to_gray8.argtypes = [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP]
+    lib.sdk_to_mask8.argtypes = [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP]
     return lib
+
+
+_DEPTHS = {np.dtype(np.uint8): 0, np.dtype(np.uint16): 1, np.dtype(np.int16): 2, np.dtype(np.int32): 3,
+           np.dtype(np.float32): 4, np.dtype(np.float64): 5}
+
+
+def decode(data, expected_w=0, expected_h=0, mask=False, lib=None):
+    """The boundary's decode of image bytes (io::decode_gray, or decode_mask
+    with mask=True): a 2-D uint8 array, or None when nothing decodes."""
+    lib = lib or load_dev()
+    buf = np.frombuffer(bytes(data), np.uint8) if not isinstance(data, np.ndarray) else data
+    w, h = ctypes.c_int32(0), ctypes.c_int32(0)
+    if lib.sdk_decode(_ptr(buf), buf.size, expected_w, expected_h, int(mask), None, 0, ctypes.byref(w),
+                      ctypes.byref(h)) != 0:
+        return None
+    out = np.empty((h.value, w.value), np.uint8)
+    rc = lib.sdk_decode(_ptr(buf), buf.size, expected_w, expected_h, int(mask), _ptr(out), out.size,
+                        ctypes.byref(w), ctypes.byref(h))
+    return out if rc == 0 else None
+
+
+def image_dimensions(data, lib=None):
+    lib = lib or load_dev()
+    buf = np.frombuffer(bytes(data), np.uint8) if data is not None else None
+    w, h = ctypes.c_int32(-1), ctypes.c_int32(-1)
+    lib.sdk_image_dimensions(_ptr(buf), 0 if buf is None else buf.size, ctypes.byref(w), ctypes.byref(h))
+    return w.value, h.value
+
+
+def _convert(fn, img):
+    a = np.ascontiguousarray(img)
+    if a.dtype not in _DEPTHS or a.ndim not in (2, 3) or (a.ndim == 3 and a.shape[2] != 3):
+        raise ValueError("expected an (h, w) or (h, w, 3) array of u8/u16/s16/s32/f32/f64")
+    out = np.empty(a.shape[:2], np.uint8)
+    if fn(_ptr(a), a.shape[1], a.shape[0], 1 if a.ndim == 2 else 3, _DEPTHS[a.dtype], _ptr(out)) != 0:
+        return None
+    return out
+
+
+def to_gray8(img, lib=None):
+    """io::to_gray8 of an (h, w) or (h, w, 3 = R, G, B) array."""
+    return _convert((lib or load_dev()).sdk_to_gray8, img)
+
+
+def to_mask8(img, lib=None):
+    """io::to_mask8 of an (h, w) or (h, w, 3 = R, G, B) array."""
+    return _convert((lib or load_dev()).sdk_to_mask8, img)
+
+
+def _image_buffer(img):
+    """(contiguous uint8 buffer, (h, w) or None): an array is a gray plane,
+    bytes are passed through for the engine to decode."""
+    if isinstance(img, (bytes, bytearray, memoryview)):
+        return np.frombuffer(img, np.uint8), None
+    a = np.ascontiguousarray(img, dtype=np.uint8)
+    return a, a.shape[:2]
+
+
+def _mask_buffer(mask):
+    """Mask bytes for the C ABI (reference bindings/python/semper/__init__.py:
+    Engine docstring): bytes pass through; a bool or non-uint8 array, or a
+    uint8 array of only 0 and 1, is boolean (nonzero included, as 255); any
+    other uint8 array excludes the pixels below 128."""
+    if mask is None:
+        return None
+    if isinstance(mask, (bytes, bytearray, memoryview)):
+        return np.frombuffer(mask, np.uint8)
+    m = np.asarray(mask)
+    if m.dtype != np.uint8 or (m.size and int(m.max()) <= 1):
+        m = np.where(m != 0, 255, 0).astype(np.uint8)
+    return np.ascontiguousarray(m, dtype=np.uint8)
 
 
 def params(rect, step, subset, strain_window, use6=False):
@@ -126,25 +203,30 @@ class Engine:
 
     __del__ = close
 
-    def set_reference(self, img, mask=None):
-        img = np.ascontiguousarray(img, dtype=np.uint8)
-        h, w = img.shape
-        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
-        return self.lib.semper_set_reference(self.h, _ptr(img), img.size, _ptr(m), 0 if m is None else m.size,
-                                             w, h)
+    def set_reference(self, img, mask=None, expected_w=0, expected_h=0):
+        """Cache the reference: a 2-D uint8 array, or image bytes (raw gray or
+        RGBA of expected_w x expected_h, or an encoded file)."""
+        buf, shape = _image_buffer(img)
+        if shape is not None:
+            expected_h, expected_w = shape
+        m = _mask_buffer(mask)
+        rc = self.lib.semper_set_reference(self.h, _ptr(buf), buf.size, _ptr(m), 0 if m is None else m.size,
+                                           int(expected_w), int(expected_h))
+        self.shape = shape if shape is not None else None
+        return rc
 
     def run(self, img, p, mask=None, capacity=None, metrics_len=METRICS_LEN, progress=None):
-        img = np.ascontiguousarray(img, dtype=np.uint8)
+        buf, shape = _image_buffer(img)
         # default capacity: every grid node of the rectangle clipped to the image
-        ih, iw = img.shape[:2]
+        ih, iw = shape if shape is not None else (max(p.rect_h, 0), max(p.rect_w, 0))
         st = max(p.step, 1)
         rw, rh = min(max(p.rect_w, 0), iw), min(max(p.rect_h, 0), ih)
         cap = capacity if capacity is not None else max(1, (rw // st + 1) * (rh // st + 1)) * FLOATS_PER_POINT
         out = np.zeros(max(cap, 1), np.float32)
         met = np.zeros(max(metrics_len, 1), np.float32)
-        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+        m = _mask_buffer(mask)
         cb = PROGRESS_CB(progress) if progress else PROGRESS_CB()
-        n = self.lib.semper_run(self.h, _ptr(img), img.size, _ptr(m), 0 if m is None else m.size,
+        n = self.lib.semper_run(self.h, _ptr(buf), buf.size, _ptr(m), 0 if m is None else m.size,
                                 ctypes.byref(p), _ptr(out), cap, _ptr(met), metrics_len, cb, None)
         pts = out[: max(n, 0) * FLOATS_PER_POINT].reshape(-1, FLOATS_PER_POINT).copy()
         return n, pts, met
@@ -171,16 +253,19 @@ class Incremental:
 
     __del__ = close
 
-    def init(self, img, p, update_interval, mask=None):
-        img = np.ascontiguousarray(img, dtype=np.uint8)
-        h, w = img.shape
-        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+    def init(self, img, p, update_interval, mask=None, expected_w=0, expected_h=0):
+        buf, shape = _image_buffer(img)
+        if shape is not None:
+            expected_h, expected_w = shape
+        h, w = (expected_h, expected_w) if shape is None else shape
+        w, h = (w or p.rect_w), (h or p.rect_h)
+        m = _mask_buffer(mask)
         self.cap = max(1, (min(p.rect_w, w) // max(p.step, 1) + 1) * (min(p.rect_h, h) // max(p.step, 1) + 1)) * 8
-        return self.lib.semper_incremental_init(self.h, _ptr(img), img.size, _ptr(m), 0 if m is None else m.size,
-                                                w, h, ctypes.byref(p), update_interval)
+        return self.lib.semper_incremental_init(self.h, _ptr(buf), buf.size, _ptr(m), 0 if m is None else m.size,
+                                                int(expected_w), int(expected_h), ctypes.byref(p), update_interval)
 
     def process(self, img, capacity=None, metrics_len=METRICS_LEN):
-        img = np.ascontiguousarray(img, dtype=np.uint8)
+        img, _ = _image_buffer(img)
         cap = capacity if capacity is not None else self.cap
         out = np.zeros(max(cap, 1), np.float32)
         met = np.zeros(max(metrics_len, 1), np.float32)

@@ -270,3 +270,59 @@ class OracleIncremental:
         info = np.zeros(6, np.int32)
         n = self.lib.orc_inc_process(self.h, P(img), P(out), self.cap, P(met), 24, P(info))
         return n, out[: max(n, 0) * 8].reshape(-1, 8).copy(), met, info
+
+
+# ---- DICe open-hole-tension fixtures (tests/fixtures/dice, LICENSE.DICe) ----
+
+DICE_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "dice")
+# (image, DICe solution) per load step, each against oht_cfrp_00.tiff
+DICE_LADDER = [("oht_cfrp_01.tiff", "DICe_solution_01.txt"), ("oht_cfrp_03.tiff", "DICe_solution_03.txt"),
+               ("oht_cfrp_06.tiff", "DICe_solution_06.txt"), ("oht_cfrp_11.tiff", "DICe_solution_11.txt")]
+
+
+def dice_bytes(name):
+    with open(os.path.join(DICE_DIR, name), "rb") as f:
+        return f.read()
+
+
+def dice_image(name):
+    """A fixture decoded by PIL (independent of the engine's codec)."""
+    import io
+
+    from PIL import Image
+
+    return np.array(Image.open(io.BytesIO(dice_bytes(name))).convert("L"))
+
+
+def dice_gold(name):
+    """DICe's solved field: rows (x, y, u, v); the '***' header and the column
+    line skipped, x and y rounded (test_field_agreement.cpp:63-81)."""
+    rows = []
+    with open(os.path.join(DICE_DIR, name)) as f:
+        for line in f:
+            if not line.strip() or line.startswith("***") or line.startswith("COORDINATE_X"):
+                continue
+            c = line.split(",")
+            if len(c) < 4:
+                continue
+            rows.append((round(float(c[0])), round(float(c[1])), float(c[2]), float(c[3])))
+    return np.array(rows)
+
+
+def dice_grid_params():
+    """The step-7 grid whose nodes include every DICe point two steps inside
+    the ROI (test_incremental_oht.cpp:86-98)."""
+    step = 7
+    return (61 - 2 * step, 61 - 2 * step, (341 - 61) + 4 * step, (971 - 61) + 4 * step, step, 27, 21, 1)
+
+
+def dice_agreement(pts, gold):
+    """(compared fraction, rms du, rms dv, mean du, max |d|) of an (n, 8) point
+    list at DICe's coordinates (test_incremental_oht.cpp:104-135)."""
+    at = {(int(round(r[0])), int(round(r[1]))): (float(r[2]), float(r[3])) for r in pts}
+    d = np.array([(at[(int(x), int(y))][0] - u, at[(int(x), int(y))][1] - v)
+                  for x, y, u, v in gold if (int(x), int(y)) in at])
+    if len(d) == 0:
+        return 0.0, np.inf, np.inf, np.inf, np.inf
+    rms = np.sqrt((d ** 2).mean(0))
+    return len(d) / len(gold), float(rms[0]), float(rms[1]), float(d[:, 0].mean()), float(np.abs(d).max())
