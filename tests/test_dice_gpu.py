@@ -28,7 +28,7 @@ def sa():
     return m
 
 
-def solve_nodes(dev, ref, dfm, nodes, mode):
+def solve_nodes(dev, ref, dfm, nodes, mode, use6=True):
     n = len(nodes)
     cxy = np.ascontiguousarray(nodes, np.int32).reshape(-1)
     g = np.zeros(6 * n, np.float32)
@@ -37,7 +37,7 @@ def solve_nodes(dev, ref, dfm, nodes, mode):
     fout = np.zeros((n, 7), np.float32)
     r = np.ascontiguousarray(ref, np.float32)
     d = np.ascontiguousarray(dfm, np.uint8)
-    assert dev.sdk_solve_nodes(P(r), P(d), 0, ref.shape[1], ref.shape[0], 27, 1, 0, ctypes.c_float(1e-3), n, P(cxy),
+    assert dev.sdk_solve_nodes(P(r), P(d), 0, ref.shape[1], ref.shape[0], 27, int(use6), 0, ctypes.c_float(1e-3), n, P(cxy),
                                P(g), P(md), P(iout), P(fout)) == 0
     return iout, fout
 
@@ -74,6 +74,22 @@ def test_subsets_agree_with_dice_and_oracle(sa, step):
         assert np.abs(du).max() < MAX_TOL and np.abs(dv).max() < MAX_TOL
     else:
         assert ok.mean() >= LADDER_MIN_FRACTION and rms_u < LADDER_RMS_TOL and rms_v < LADDER_RMS_TOL
+
+
+def test_translation_real_image(sa):
+    """DiceTranslationReal.CustomApp_0p4px_RealSpeckle
+    (tests/dice/test_translation_real_image.cpp) on the TIFF bytes: four
+    subsets, subset 27, bicubic, INIT_NO_SIMPLEX, each converged with
+    |u - 0.4| <= 0.1 px; per subset within 5e-3 px of the oracle."""
+    dev = sa.load_dev()
+    ref, dfm = sa.decode(dice_bytes("ref.tif"), lib=dev), sa.decode(dice_bytes("def.tif"), lib=dev)
+    nodes = np.array([(100, 100), (200, 200), (300, 300), (400, 400)], np.int32)
+    io_g, fo_g = solve_nodes(dev, ref, dfm, nodes, 2, use6=False)
+    io_o, fo_o = oracle_solve_nodes(ref.astype(np.float32), dfm.astype(np.float32), 27, nodes, [[0] * 6] * 4, [2] * 4)
+    print("u", fo_g[:, 0], "v", fo_g[:, 1], flush=True)
+    assert (io_g[:, 0] == 1).all() and (io_g[:, 1] == 0).all() and np.array_equal(io_g[:, :2], io_o[:, :2])
+    assert (np.abs(fo_g[:, 0] - 0.4) <= 0.1).all()
+    assert np.abs(fo_g[:, :2] - fo_o[:, :2]).max() <= 5e-3
 
 
 def full_field_common(pts_g, pts_o, n_o):

@@ -74,3 +74,17 @@ def test_incremental_ladder_totals_agree_with_dice():
         assert af[0] >= INC_MIN_FRACTION and max(af[1], af[2]) < 0.0035
     # step 11: the spread the reference measured
     assert 0.045 < a[1] < 0.07 and 0.018 < a[2] < 0.034 and -0.065 < a[3] < -0.045
+
+
+def test_translation_real_image():
+    """DiceTranslationReal.CustomApp_0p4px_RealSpeckle
+    (tests/dice/test_translation_real_image.cpp): DICe's ref.tif / def.tif, the
+    four subsets of its custom_app, subset 27, bicubic, INIT_NO_SIMPLEX: every
+    subset converges with |u - 0.4| <= 0.1 px (DICe's published contract)."""
+    ref, dfm = dice_image("ref.tif"), dice_image("def.tif")
+    assert ref.shape == dfm.shape == (512, 512)
+    nodes = [(100, 100), (200, 200), (300, 300), (400, 400)]
+    io, fo = oracle_solve_nodes(ref.astype(np.float32), dfm.astype(np.float32), 27, nodes, [[0] * 6] * 4, [2] * 4)
+    print("u", fo[:, 0], "v", fo[:, 1])
+    assert (io[:, 0] == 1).all() and (io[:, 1] == 0).all()
+    assert (np.abs(fo[:, 0] - 0.4) <= 0.1).all()
