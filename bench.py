@@ -43,7 +43,7 @@ PEAK_FP32_TFLOPS = 157.3
 # HBM bytes per Path A launch from two rocprofv3 --pmc runs (FETCH_SIZE x 2.0,
 # WRITE_SIZE x 1.0, the factors a 256 MiB copy kernel reads under the same
 # counters); measured per workload, recorded with the kernel it was measured on
-TRAFFIC_BYTES = {"c4_4096_heterogeneous": 151908 * 1024 * 2.0 + 24357.5 * 1024 * 1.0}
+TRAFFIC_BYTES = {"c4_4096_heterogeneous": 155575.8 * 1024 * 2.0 + 24357.5 * 1024 * 1.0}
 
 
 def fp64_accuracy(ref, dfm, subset, pts, use6, k=2000, seed=0):
@@ -65,7 +65,47 @@ def fp64_accuracy(ref, dfm, subset, pts, use6, k=2000, seed=0):
     dv = np.abs(po[ok, 1] - sel[ok, 3])
     frac = float(np.mean((du <= 1e-3) & (dv <= 1e-3))) if ok.any() else 0.0
     return {"sampled": int(k), "fp64_converged": int(ok.sum()), "within_1e-3px": frac,
-            "max_du": float(du.max()) if ok.any() else None, "max_dv": float(dv.max()) if ok.any() else None}
+            "max_du": float(du.max()) if ok.any() else None, "max_dv": float(dv.max()) if ok.any() else None,
+            "rms_du": float(np.sqrt((du ** 2).mean())) if ok.any() else None,
+            "rms_dv": float(np.sqrt((dv ** 2).mean())) if ok.any() else None}
+
+
+PRECISION_VARIANTS = [("fp64_cholesky_step", (2,)), ("fp16_gradients", (3,)), ("mfma_jtj_pool", (4,)),
+                      ("all_three", (2, 3, 4))]
+
+
+def precision_study(dev, dref, ddef, prm, cap, tol, ref, dfm, subset, use6, steps):
+    """The precision variants (FP64 Cholesky Gauss-Newton step, fp16 gradient
+    storage, J^T J on MFMA): per variant the first run's Hessian pre-pass (the
+    pool build), the step time, the converged throughput and the error of
+    every output point's (u, v) against the FP64 IC-GN optimum on the same
+    2000-node sample as the plain method."""
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    w = ref.shape[1]
+    res = {}
+    for name, flags in PRECISION_VARIANTS:
+        eng = dev.sdk_engine_create()
+        for f in flags:
+            assert dev.sdk_engine_set_flag(eng, f, 1) == 0
+        assert dev.sdk_engine_set_reference(eng, dref, 1, w, ref.shape[0], None) == 0
+        out = np.zeros(cap, np.float32)
+        met = np.zeros(24, np.float32)
+        tm = np.zeros(14, np.float64)
+        n = dev.sdk_engine_run(eng, ddef, 1, P(prm), P(out), cap, P(met), 24, tol)
+        dev.sdk_engine_timings(eng, P(tm))
+        prepass = float(tm[3])
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            n = dev.sdk_engine_run(eng, ddef, 1, P(prm), P(out), cap, P(met), 24, tol)
+        ms = (time.perf_counter() - t0) * 1000.0 / steps
+        dev.sdk_engine_destroy(eng)
+        pts = out[: int(n) * 8].reshape(-1, 8).copy()
+        acc = fp64_accuracy(ref, dfm, subset, pts, use6)
+        solved = float(met[3] + met[4] + met[19])
+        res[name] = {"ms_per_step": ms, "prepass_ms": prepass, "output_points": int(n),
+                     "value": solved * acc["within_1e-3px"] / (ms / 1000.0), "iterations_per_point": float(met[8]),
+                     "accuracy_vs_fp64": acc}
+    return res
 
 
 def cpu_baseline(ref, dfm, prm, tol, gpu_pts, use6, threads):
@@ -103,6 +143,19 @@ def cpu_baseline(ref, dfm, prm, tol, gpu_pts, use6, threads):
     }, parity
 
 
+def copy_probe():
+    """A copy of known size for the FETCH_SIZE / WRITE_SIZE counters: 64 Mi
+    fp32 read and written (256 MiB each way) per dispatch."""
+    import torch
+
+    x = torch.randn(64 << 20, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print("copy probe: 256 MiB read, 256 MiB written per dispatch", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,11 +164,18 @@ def main():
     ap.add_argument("--workload", default="c4_4096_heterogeneous")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and FP64 check")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-precision", action="store_true", help="skip the precision-variant study")
     ap.add_argument("--tol", type=float, default=None, help="IC-GN stop rule ||dp|| < tol (default: the workload's)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 256 MiB copy kernel three times (the calibration of the HBM traffic "
+                         "counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
 
     sa = _load("semper_amd", os.path.join(PKG, "semper_amd.py"))
     wl = _load("workloads", os.path.join(PKG, "workloads.py"))
@@ -139,9 +199,13 @@ def main():
     P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
     tol_v = args.tol if args.tol is not None else cfg["tol"]
     tol = ctypes.c_float(tol_v)
+    prepass_ms = None
     for _ in range(args.warmup):
         n = dev.sdk_engine_run(eng, ddef, 1, P(prm), P(out), cap, P(met), 24, tol)
         assert n > 0, n
+        if prepass_ms is None:
+            dev.sdk_engine_timings(eng, P(tm))
+            prepass_ms = float(tm[3])
     kern_ms, kern_iters, kern_nodes = [], [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -176,6 +240,7 @@ def main():
     cpu = None
     parity = None
     acc = None
+    precision = None
     if not args.no_cpu:
         ref = np.zeros((h, w), np.uint8)
         dfm = np.zeros((h, w), np.uint8)
@@ -184,6 +249,9 @@ def main():
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         acc = fp64_accuracy(ref, dfm, cfg["subset"], pts, use6)
         cpu, parity = cpu_baseline(ref, dfm, prm, tol_v, pts, use6, threads)
+        if not args.no_precision:
+            precision = precision_study(dev, dref, ddef, prm, cap, tol, ref, dfm, cfg["subset"], use6,
+                                        max(2, args.steps // 2))
     frac = acc["within_1e-3px"] if acc else 1.0
     value = float(solved * frac / (ms_per_step / 1000.0))
     line = {
@@ -219,6 +287,8 @@ def main():
         "phases_ms": {k: float(v) for k, v in zip(
             ["total", "prep", "seeding", "prepass", "mesh", "path_a", "path_b", "strain"], tm[:8])},
         "path_b": {"speculative_solves": int(tm[11]), "further_launches": int(tm[12]), "further_solves": int(tm[13])},
+        "prepass_ms": prepass_ms,
+        "precision_study": precision,
     }
     dev.sdk_engine_destroy(eng)
     dev.sdk_free(dref)

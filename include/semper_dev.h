@@ -53,7 +53,12 @@ SDK_API int sdk_engine_run(void* h, const uint8_t* def, int on_device, const int
 SDK_API int sdk_wave_sum_probe(const float* in, int nw, float* out, int32_t* iout);
 
 /* Engine switches: flag 0 = speculative Path B first claims (default 1);
-   flag 1 = `out` of sdk_engine_run is device memory (default 0). */
+   flag 1 = `out` of sdk_engine_run is device memory (default 0);
+   precision variants with no reference counterpart (default 0, a change
+   rebuilds the Hessian pool on the next run): flag 2 = the Gauss-Newton step
+   by FP64 Cholesky of each pool entry, flag 3 = fp16 storage of the reference
+   gradient planes, flag 4 = J^T J of the pool on MFMA (fp16 inputs, FP32
+   accumulation). */
 SDK_API int sdk_engine_set_flag(void* h, int flag, int value);
 
 /* Phase timings of the last run, 14 doubles: total, prep, seeding, prepass,

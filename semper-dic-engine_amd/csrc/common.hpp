@@ -62,6 +62,9 @@ struct alignas(16) PoolEntry {
     int32_t pad;
 };
 static_assert(sizeof(PoolEntry) == 160, "pool entry layout");
+// Doubles per pool entry of the optional FP64 Cholesky factors (21 used: the
+// lower triangle, its diagonal as reciprocals).
+constexpr int kCholStride = 24;
 
 // One solve request: node centre, its pool slot, a 6-DOF guess and the
 // initialisation mode (calculate_deformation's arguments).

@@ -248,6 +248,15 @@ SDK_API int sdk_engine_set_flag(void* h, int flag, int value) {
         case 1:
             ((DevEngine*)h)->eng.out_on_device = value != 0;
             return 0;
+        case 2:
+            ((DevEngine*)h)->eng.prec.chol64 = value != 0;
+            return 0;
+        case 3:
+            ((DevEngine*)h)->eng.prec.grad16 = value != 0;
+            return 0;
+        case 4:
+            ((DevEngine*)h)->eng.prec.mfma_pool = value != 0;
+            return 0;
         default:
             return -3;
     }

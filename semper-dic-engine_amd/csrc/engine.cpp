@@ -144,6 +144,7 @@ Engine::~Engine() {
 void Engine::reset() {
     have_ref_ = false;
     grid_valid_ = false;
+    g16_ready_ = false;
     w_ = h_ = 0;
 }
 
@@ -188,7 +189,7 @@ int Engine::ensure_grid(const Params& p, const uint8_t* mask, bool& cached) {
                            (mask && grid_mask_.size() == n && std::memcmp(grid_mask_.data(), mask, n) == 0);
     cached = grid_valid_ && grid_p_.rect_x == p.rect_x && grid_p_.rect_y == p.rect_y &&
              grid_p_.rect_w == p.rect_w && grid_p_.rect_h == p.rect_h && grid_p_.step == p.step &&
-             grid_p_.subset == p.subset && mask_same && gw_ == gw && gh_ == gh;
+             grid_p_.subset == p.subset && mask_same && gw_ == gw && gh_ == gh && grid_prec_ == prec;
     if (cached) return 0;
     grid_valid_ = false;
     const int N = gw * gh;
@@ -208,15 +209,41 @@ int Engine::ensure_grid(const Params& p, const uint8_t* mask, bool& cached) {
     ck(hipStreamSynchronize(st_));
     total_valid_ = total;
     if (total == 0) return kErrRoi;
+    PoolOpts po;
+    po.mfma = prec.mfma_pool;
+    if (prec.grad16) {
+        if (!g16_ready_) {
+            if (!gx16_.reserve(n) || !gy16_.reserve(n)) return kErrInit;
+            ck(launch_to_f16(gx_.p, n, gx16_.p, st_));
+            ck(launch_to_f16(gy_.p, n, gy16_.p, st_));
+            g16_ready_ = true;
+        }
+        po.gx16 = gx16_.p;
+        po.gy16 = gy16_.p;
+    }
+    if (prec.chol64) {
+        if (!chol_.reserve((size_t)N * kCholStride)) return kErrInit;
+        po.chol = chol_.p;
+    }
+    if (!pool_raw_.reserve(pool_raw_bytes(N))) return kErrInit;
     ck(launch_pool(I_.p, gx_.p, gy_.p, w_, h_, skip_d_.p, gw, gh, p.rect_x, p.rect_y, p.step, p.subset, true,
-                   pool_.p, st_));
+                   pool_.p, pool_raw_.p, st_, po));
     ck(hipStreamSynchronize(st_));
+    grid_prec_ = prec;
     grid_p_ = p;
     grid_mask_ = mask ? std::vector<uint8_t>(mask, mask + n) : std::vector<uint8_t>();
     gw_ = gw;
     gh_ = gh;
     grid_valid_ = true;
     return 0;
+}
+
+void Engine::set_precision_args(SolveArgs& a) const {
+    if (grid_prec_.grad16) {
+        a.refGx16 = gx16_.p;
+        a.refGy16 = gy16_.p;
+    }
+    if (grid_prec_.chol64) a.chol = chol_.p;
 }
 
 int Engine::solve_tasks(const std::vector<Task>& tasks, std::vector<SolveOut>& out, bool use6, int dim, float tol) {
@@ -235,6 +262,7 @@ int Engine::solve_tasks(const std::vector<Task>& tasks, std::vector<SolveOut>& o
     a.refI = I_.p;
     a.refGx = gx_.p;
     a.refGy = gy_.p;
+    set_precision_args(a);
     a.def = def_u8_.p;
     a.w = w_;
     a.h = h_;
@@ -760,6 +788,7 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             a.refI = I_.p;
             a.refGx = gx_.p;
             a.refGy = gy_.p;
+            set_precision_args(a);
             a.def = def_u8_.p;
             a.w = w_;
             a.h = h_;

@@ -137,6 +137,19 @@ public:
     bool speculate_path_b = true;
     // `out` of run() is device memory (the packed points stay in HBM)
     bool out_on_device = false;
+    // Precision variants (no reference counterpart): the Gauss-Newton step by
+    // FP64 Cholesky of each pool entry, fp16 storage of the reference gradient
+    // planes the pool and the solver read, and J^T J on MFMA in the pool.
+    // A change rebuilds the grid cache on the next run.
+    struct Precision {
+        bool chol64 = false;
+        bool grad16 = false;
+        bool mfma_pool = false;
+        bool operator==(const Precision& o) const {
+            return chol64 == o.chol64 && grad16 == o.grad16 && mfma_pool == o.mfma_pool;
+        }
+    };
+    Precision prec;
     Field last_field;
 
     // Median test, VSG strain and row-major packing of a node field given on
@@ -165,6 +178,13 @@ private:
     std::vector<uint8_t> skip_h_;
     DevBuf<uint8_t> skip_d_, mask_d_;
     DevBuf<PoolEntry> pool_;
+    DevBuf<uint8_t> pool_raw_;      // per-node sums between the two pool launches
+    Precision grid_prec_;
+    DevBuf<uint16_t> gx16_, gy16_;  // fp16 gradient planes (Precision::grad16)
+    bool g16_ready_ = false;
+    DevBuf<double> chol_;           // FP64 Cholesky factors (Precision::chol64)
+    // the precision-variant pointers of a solver launch on the grid pool
+    void set_precision_args(SolveArgs& a) const;
     // per run
     DevBuf<float> gplanes_;        // u v ux uy vx vy corr  (7 planes)
     DevBuf<uint8_t> gflags_;       // solved tag simplex in_mesh valid ok_a ok_b (7 planes)

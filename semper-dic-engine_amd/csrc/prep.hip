@@ -72,24 +72,36 @@ __global__ void k_scan(const uint8_t* __restrict__ mask, int w, int h, int rect_
     if (!s) atomicAdd(count, 1);
 }
 
-struct PoolLds {
-    float H[36];
-    float Hinv[36];
-    float Hd[36];
-    float work[144];
+// The reference planes a pool pass reads: float gradients, or their fp16
+// copies (bits in uint16_t) when the gradient storage is fp16.
+struct GradRead {
+    const float* gx;
+    const float* gy;
+    const uint16_t* gx16;
+    const uint16_t* gy16;
+    __device__ __forceinline__ float x(int a) const {
+        return gx16 ? (float)__builtin_bit_cast(_Float16, gx16[a]) : gx[a];
+    }
+    __device__ __forceinline__ float y(int a) const {
+        return gy16 ? (float)__builtin_bit_cast(_Float16, gy16[a]) : gy[a];
+    }
 };
 
-// compute_hessian_only for the subset at (cx, cy); lane-strided pixel passes,
-// the 6x6 tests and inverses by lane 0 in LDS.
-__device__ void pool_entry(const float* __restrict__ I, const float* __restrict__ gxp,
-                           const float* __restrict__ gyp, int w, int h, int cx, int cy, int dim, bool lm,
-                           PoolLds& L, PoolEntry* __restrict__ dst) {
+typedef _Float16 sdic_half4 __attribute__((ext_vector_type(4)));
+typedef float sdic_float4 __attribute__((ext_vector_type(4)));
+
+// compute_hessian_only's pixel passes for the subset at (cx, cy): the mean
+// and the standard deviation over the valid pixels, then the 21 Hessian sums,
+// either as FP32 lane sums with wave reductions or (MFMA) as one 16x16x16 fp16
+// MFMA per 16 pixels with FP32 accumulation. False when the subset is invalid
+// (outside the image, or under half of it valid). Results are wave-uniform.
+template <bool MFMA>
+__device__ bool pool_sums(const float* __restrict__ I, GradRead G, int w, int h, int cx, int cy, int dim,
+                          float* lds36, float* hh, float& mean, float& inv) {
     const int lane = threadIdx.x & 63;
     const int half = dim / 2, n = dim * dim;
-    if (dim < 3 || (dim & 1) == 0 || cx - half < 0 || cx + half >= w || cy - half < 0 || cy + half >= h) {
-        if (lane == 0) dst->valid = 0;
-        return;
-    }
+    if (dim < 3 || (dim & 1) == 0 || cx - half < 0 || cx + half >= w || cy - half < 0 || cy + half >= h)
+        return false;
     float sum = 0.0f;
     int cnt = 0;
     for (int i = lane; i < n; i += 64) {
@@ -100,11 +112,8 @@ __device__ void pool_entry(const float* __restrict__ I, const float* __restrict_
     }
     sum = wave_sum(sum);
     cnt = wave_sum_i(cnt);
-    if (cnt < n * 0.5f) {
-        if (lane == 0) dst->valid = 0;
-        return;
-    }
-    const float mean = sum / (float)cnt;
+    if (cnt < n * 0.5f) return false;
+    mean = sum / (float)cnt;
     float ss = 0.0f;
     for (int i = lane; i < n; i += 64) {
         const float r = I[(cy + i / dim - half) * w + cx + i % dim - half];
@@ -115,84 +124,216 @@ __device__ void pool_entry(const float* __restrict__ I, const float* __restrict_
     ss = wave_sum(ss);
     float sd = sqrtf(ss / (float)cnt);
     if (sd < 1e-5f) sd = 1.0f;
-    const float inv = 1.0f / sd;
-    float hh[21];
+    inv = 1.0f / sd;
+    if constexpr (!MFMA) {
 #pragma unroll
-    for (int k = 0; k < 21; ++k) hh[k] = 0.0f;
-    for (int i = lane; i < n; i += 64) {
-        const int oy = i / dim - half, ox = i % dim - half;
-        const int a = (cy + oy) * w + cx + ox;
-        if (I[a] < kGhostWallThreshold) continue;
-        const float g0 = gxp[a] * inv, g1 = gyp[a] * inv;
-        const float fx = (float)ox, fy = (float)oy;
-        const float s[6] = {g0, g1, g0 * fx, g0 * fy, g1 * fx, g1 * fy};
+        for (int k = 0; k < 21; ++k) hh[k] = 0.0f;
+        for (int i = lane; i < n; i += 64) {
+            const int oy = i / dim - half, ox = i % dim - half;
+            const int a = (cy + oy) * w + cx + ox;
+            if (I[a] < kGhostWallThreshold) continue;
+            const float g0 = G.x(a) * inv, g1 = G.y(a) * inv;
+            const float fx = (float)ox, fy = (float)oy;
+            const float s[6] = {g0, g1, g0 * fx, g0 * fy, g1 * fx, g1 * fy};
+            int k = 0;
+#pragma unroll
+            for (int r = 0; r < 6; ++r)
+#pragma unroll
+                for (int c = r; c < 6; ++c) hh[k++] += s[r] * s[c];
+        }
+#pragma unroll
+        for (int k = 0; k < 21; ++k) hh[k] = wave_sum(hh[k]);
+    } else {
+        // A (16 x 16, rows = the six steepest-descent components, columns = 16
+        // pixels) times its transpose: lane l supplies component l % 16 of the
+        // pixels 4 (l / 16) .. 4 (l / 16) + 3 of the chunk, which is its share
+        // of both operands; D[r][c] comes back in lane 16 (r / 4) + c, slot r % 4.
+        const int comp = lane & 15, grp = lane >> 4;
+        sdic_float4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int c0 = 0; c0 < n; c0 += 16) {
+            sdic_half4 av;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int p = c0 + 4 * grp + j;
+                float s = 0.0f;
+                if (comp < 6 && p < n) {
+                    const int oy = p / dim - half, ox = p % dim - half;
+                    const int a = (cy + oy) * w + cx + ox;
+                    if (!(I[a] < kGhostWallThreshold)) {
+                        const float g0 = G.x(a) * inv, g1 = G.y(a) * inv;
+                        const float gs = comp == 0 || comp == 2 || comp == 3 ? g0 : g1;
+                        const float f = (comp == 2 || comp == 4) ? (float)ox : (float)oy;
+                        s = comp < 2 ? gs : gs * f;
+                    }
+                }
+                av[j] = (_Float16)s;
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x16f16(av, av, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int r = 4 * grp + j;
+            if (r < 6 && comp < 6) lds36[r * 6 + comp] = acc[j];
+        }
+        __syncthreads();
         int k = 0;
 #pragma unroll
         for (int r = 0; r < 6; ++r)
 #pragma unroll
-            for (int c = r; c < 6; ++c) hh[k++] += s[r] * s[c];
+            for (int c = r; c < 6; ++c) hh[k++] = lds36[r * 6 + c];
+        __syncthreads();
     }
-#pragma unroll
-    for (int k = 0; k < 21; ++k) hh[k] = wave_sum(hh[k]);
-    if (lane == 0) {
+    return true;
+}
+
+// compute_hessian_only's 6x6 part from the 21 sums (subset_precomputer.cpp:
+// 104-149): the 2x2 condition test, hessian::well_conditioned, the inverse,
+// and under LM the damped matrix's test and inverse (optimization_engine.cpp:
+// 302-315). step receives the step matrix; Lp (optional) the FP64 Cholesky
+// factor of the step matrix's source, its first entry NaN when it has none.
+// Returns valid.
+template <bool CHOL>
+__device__ __forceinline__ int pool_tail(const float* hh, int half, bool lm, float* step, double* Lp) {
+    float H[36];
+    {
         int k = 0;
+#pragma unroll
         for (int r = 0; r < 6; ++r)
+#pragma unroll
             for (int c = r; c < 6; ++c) {
-                L.H[r * 6 + c] = hh[k];
-                L.H[c * 6 + r] = hh[k];
+                H[r * 6 + c] = hh[k];
+                H[c * 6 + r] = hh[k];
                 ++k;
             }
-        const float* H = L.H;
-        const float det2 = H[0] * H[7] - H[6] * H[1];
-        const float nrm2 = H[0] * H[0] + H[1] * H[1] + H[6] * H[6] + H[7] * H[7];
-        const float cond2 = (fabsf(det2) > 1e-12f) ? (nrm2 / fabsf(det2)) : 1.0e13f;
-        int valid = 1;
-        if (cond2 > 1.0e12f || !well_conditioned_lds(L.H, half, L.work)) {
-            valid = 0;
-        } else {
-            gj_inverse_lds(L.H, L.Hinv, L.work);
-            const float* step = L.Hinv;
-            if (lm) {
-                // solve_icgn's LM step matrix (optimization_engine.cpp:302-315)
-                for (int q = 0; q < 36; ++q) L.Hd[q] = L.H[q];
-                L.Hd[0] += kLmAlpha;
-                L.Hd[7] += kLmAlpha;
-                if (well_conditioned_lds(L.Hd, half, L.work)) {
-                    gj_inverse_lds(L.Hd, L.Hd, L.work);
-                    step = L.Hd;
-                }
+    }
+    const float det2 = H[0] * H[7] - H[6] * H[1];
+    const float nrm2 = H[0] * H[0] + H[1] * H[1] + H[6] * H[6] + H[7] * H[7];
+    const float cond2 = (fabsf(det2) > 1e-12f) ? (nrm2 / fabsf(det2)) : 1.0e13f;
+    int valid = 1;
+    bool damped = false;
+    if (cond2 > 1.0e12f || !well_conditioned_reg(H, half)) {
+        valid = 0;
+    } else {
+        gj_inverse_reg(H, step);
+        if (lm) {
+            float Hd[36];
+#pragma unroll
+            for (int q = 0; q < 36; ++q) Hd[q] = H[q];
+            Hd[0] += kLmAlpha;
+            Hd[7] += kLmAlpha;
+            if (well_conditioned_reg(Hd, half)) {
+                gj_inverse_reg(Hd, step);
+                damped = true;
             }
-            for (int q = 0; q < 36; ++q) dst->hstep[q] = step[q];
         }
+    }
+    if constexpr (CHOL) {
+#pragma unroll
+        for (int q = 0; q < kCholStride; ++q) Lp[q] = 0.0;
+        bool ok = false;
+        if (valid) {
+            if (damped) {
+                H[0] += kLmAlpha;
+                H[7] += kLmAlpha;
+            }
+            ok = chol64_factor(H, Lp);
+        }
+        if (!ok) Lp[0] = __builtin_nan("");
+    }
+    return valid;
+}
+
+// The grid pool in two launches: k_pool_sums, one wave per node, writes the
+// sums and statistics; k_pool_tail, one thread per node, does the serial 6x6
+// part, so a wave finishes 64 nodes' algebra at once.
+struct alignas(16) PoolRaw {
+    float hh[21];
+    float mean, inv;
+    int32_t ok;
+};
+
+template <bool MFMA>
+__global__ __launch_bounds__(64) void k_pool_sums(const float* __restrict__ I, GradRead G, int w, int h,
+                                                  const uint8_t* __restrict__ skip, int gw, int gh, int rect_x,
+                                                  int rect_y, int step, int dim, PoolRaw* __restrict__ raw) {
+    __shared__ float lds36[36];
+    const int node = xcd_remap(blockIdx.x, gridDim.x);
+    if (node >= gw * gh) return;
+    const int lane = threadIdx.x & 63;
+    float hh[21], mean = 0.0f, inv = 1.0f;
+    const bool ok = !skip[node] && pool_sums<MFMA>(I, G, w, h, rect_x + (node % gw) * step,
+                                                   rect_y + (node / gw) * step, dim, lds36, hh, mean, inv);
+    float v = 0.0f;
+    if (ok) {
+#pragma unroll
+        for (int k = 0; k < 21; ++k) v = lane == k ? hh[k] : v;
+        v = lane == 21 ? mean : v;
+        v = lane == 22 ? inv : v;
+    }
+    float* dst = reinterpret_cast<float*>(raw + node);
+    if (lane < 23) dst[lane] = v;
+    if (lane == 23) reinterpret_cast<int32_t*>(dst)[23] = ok ? 1 : 0;
+}
+
+template <bool CHOL>
+__global__ __launch_bounds__(256) void k_pool_tail(const PoolRaw* __restrict__ raw, int n, int half, int lm,
+                                                   PoolEntry* __restrict__ pool, double* __restrict__ chol) {
+    const int node = blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= n) return;
+    PoolEntry* dst = pool + node;
+    if (!raw[node].ok) {
+        dst->valid = 0;
+        if constexpr (CHOL) chol[(size_t)node * kCholStride] = __builtin_nan("");
+        return;
+    }
+    float hh[21];
+#pragma unroll
+    for (int k = 0; k < 21; ++k) hh[k] = raw[node].hh[k];
+    float step[36];
+    double Lp[kCholStride];
+    const int valid = pool_tail<CHOL>(hh, half, lm != 0, step, Lp);
+    if (valid) {
+#pragma unroll
+        for (int q = 0; q < 36; ++q) dst->hstep[q] = step[q];
+    }
+    dst->mean = raw[node].mean;
+    dst->inv_std = raw[node].inv;
+    dst->valid = valid;
+    if constexpr (CHOL) {
+#pragma unroll
+        for (int q = 0; q < kCholStride; ++q) chol[(size_t)node * kCholStride + q] = Lp[q];
+    }
+}
+
+// One launch per node list (anchor canvases, the node solver's pool): the
+// sums and the 6x6 part in one wave, the latter evaluated redundantly by every
+// lane, lane q writing entry q.
+__global__ __launch_bounds__(64) void k_pool_list(const float* __restrict__ I, GradRead G, int w, int h,
+                                                  const int* __restrict__ cxy, int n, int dim, int lm,
+                                                  PoolEntry* pool) {
+    __shared__ float lds36[36];
+    const int k = blockIdx.x;
+    if (k >= n) return;
+    const int lane = threadIdx.x & 63;
+    PoolEntry* dst = pool + k;
+    float hh[21], mean = 0.0f, inv = 1.0f;
+    if (!pool_sums<false>(I, G, w, h, cxy[2 * k], cxy[2 * k + 1], dim, lds36, hh, mean, inv)) {
+        if (lane == 0) dst->valid = 0;
+        return;
+    }
+    float step[36];
+    const int valid = pool_tail<false>(hh, dim / 2, lm != 0, step, nullptr);
+    if (valid) {
+        float v = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 36; ++q) v = lane == q ? step[q] : v;
+        if (lane < 36) dst->hstep[lane] = v;
+    }
+    if (lane == 0) {
         dst->mean = mean;
         dst->inv_std = inv;
         dst->valid = valid;
     }
-}
-
-__global__ __launch_bounds__(64) void k_pool(const float* __restrict__ I, const float* __restrict__ gx,
-                                             const float* __restrict__ gy, int w, int h,
-                                             const uint8_t* __restrict__ skip, int gw, int gh, int rect_x,
-                                             int rect_y, int step, int dim, int lm, PoolEntry* pool) {
-    __shared__ PoolLds L;
-    const int node = xcd_remap(blockIdx.x, gridDim.x);
-    if (node >= gw * gh) return;
-    if (skip[node]) {
-        if (threadIdx.x == 0) pool[node].valid = 0;
-        return;
-    }
-    pool_entry(I, gx, gy, w, h, rect_x + (node % gw) * step, rect_y + (node / gw) * step, dim, lm != 0, L,
-               pool + node);
-}
-
-__global__ __launch_bounds__(64) void k_pool_list(const float* __restrict__ I, const float* __restrict__ gx,
-                                                  const float* __restrict__ gy, int w, int h,
-                                                  const int* __restrict__ cxy, int n, int dim, int lm,
-                                                  PoolEntry* pool) {
-    __shared__ PoolLds L;
-    const int k = blockIdx.x;
-    if (k >= n) return;
-    pool_entry(I, gx, gy, w, h, cxy[2 * k], cxy[2 * k + 1], dim, lm != 0, L, pool + k);
 }
 
 template <typename T>
@@ -393,18 +534,44 @@ hipError_t launch_scan(const uint8_t* mask, int w, int h, int rect_x, int rect_y
 
 hipError_t launch_pool(const float* I, const float* gx, const float* gy, int w, int h, const uint8_t* skip,
                        int gw, int gh, int rect_x, int rect_y, int step, int dim, bool lm, PoolEntry* pool,
-                       hipStream_t st) {
+                       void* raw, hipStream_t st, const PoolOpts& o) {
     const int n = gw * gh;
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_pool, dim3(n), dim3(64), 0, st, I, gx, gy, w, h, skip, gw, gh, rect_x, rect_y, step,
-                       dim, lm ? 1 : 0, pool);
+    const GradRead G{gx, gy, o.gx16, o.gy16};
+    PoolRaw* r = static_cast<PoolRaw*>(raw);
+    if (o.mfma)
+        hipLaunchKernelGGL(k_pool_sums<true>, dim3(n), dim3(64), 0, st, I, G, w, h, skip, gw, gh, rect_x, rect_y,
+                           step, dim, r);
+    else
+        hipLaunchKernelGGL(k_pool_sums<false>, dim3(n), dim3(64), 0, st, I, G, w, h, skip, gw, gh, rect_x, rect_y,
+                           step, dim, r);
+    if (o.chol)
+        hipLaunchKernelGGL(k_pool_tail<true>, dim3((n + 255) / 256), dim3(256), 0, st, r, n, dim / 2, lm ? 1 : 0,
+                           pool, o.chol);
+    else
+        hipLaunchKernelGGL(k_pool_tail<false>, dim3((n + 255) / 256), dim3(256), 0, st, r, n, dim / 2, lm ? 1 : 0,
+                           pool, nullptr);
+    return hipGetLastError();
+}
+
+size_t pool_raw_bytes(int n) { return (size_t)n * sizeof(PoolRaw); }
+
+__global__ void k_to_f16(const float* __restrict__ src, size_t n, uint16_t* __restrict__ dst) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = __builtin_bit_cast(uint16_t, (_Float16)src[i]);
+}
+
+hipError_t launch_to_f16(const float* src, size_t n, uint16_t* dst, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_to_f16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, n, dst);
     return hipGetLastError();
 }
 
 hipError_t launch_pool_list(const float* I, const float* gx, const float* gy, int w, int h, const int* cxy,
                             int n, int dim, bool lm, PoolEntry* pool, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_pool_list, dim3(n), dim3(64), 0, st, I, gx, gy, w, h, cxy, n, dim, lm ? 1 : 0, pool);
+    const GradRead G{gx, gy, nullptr, nullptr};
+    hipLaunchKernelGGL(k_pool_list, dim3(n), dim3(64), 0, st, I, G, w, h, cxy, n, dim, lm ? 1 : 0, pool);
     return hipGetLastError();
 }
 

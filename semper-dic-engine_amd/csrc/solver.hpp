@@ -18,6 +18,17 @@ struct GridArgs {
     int gw, gh, rect_x, rect_y, step, dim, w, h;
 };
 
+// Optional precision variants of the Hessian pool and the solver: J^T J on
+// MFMA (fp16 steepest-descent values, FP32 accumulation), fp16 storage of the
+// reference gradient planes (half bits in uint16_t), and the FP64 Cholesky
+// factor of each pool entry for the Gauss-Newton step.
+struct PoolOpts {
+    bool mfma = false;
+    const uint16_t* gx16 = nullptr;
+    const uint16_t* gy16 = nullptr;
+    double* chol = nullptr;
+};
+
 // Arguments of the subset solver kernel. Task mode when tasks != null (one
 // result per task); grid mode (Path A) otherwise.
 struct SolveArgs {
@@ -33,6 +44,9 @@ struct SolveArgs {
     const float* refI;
     const float* refGx;
     const float* refGy;
+    const uint16_t* refGx16;  // fp16 gradient planes (read instead of refGx / refGy when set)
+    const uint16_t* refGy16;
+    const double* chol;       // FP64 Cholesky factors per pool entry (the step when set)
     const void* def;          // uint8 or float plane
     int w, h, dim, lm;
     float tol;
@@ -55,10 +69,14 @@ hipError_t launch_scan(const uint8_t* mask, int w, int h, int rect_x, int rect_y
                        int dim, uint8_t* skip, int* count, hipStream_t st);
 hipError_t launch_pool(const float* I, const float* gx, const float* gy, int w, int h, const uint8_t* skip,
                        int gw, int gh, int rect_x, int rect_y, int step, int dim, bool lm, PoolEntry* pool,
-                       hipStream_t st);
+                       void* raw, hipStream_t st, const PoolOpts& opts = PoolOpts());
+// Bytes of the scratch launch_pool needs for n grid nodes (the per-node sums).
+size_t pool_raw_bytes(int n);
 hipError_t launch_pool_list(const float* I, const float* gx, const float* gy, int w, int h, const int* cxy,
                             int n, int dim, bool lm, PoolEntry* pool, hipStream_t st);
 hipError_t launch_downsample_u8(const uint8_t* src, int w, int h, float* dst, hipStream_t st);
+// fp16 copy of a float plane (round to nearest), as half bits
+hipError_t launch_to_f16(const float* src, size_t n, uint16_t* dst, hipStream_t st);
 hipError_t launch_downsample_f32(const float* src, int w, int h, float* dst, hipStream_t st);
 
 // Global-motion patch search: per patch the best integer shift after the

@@ -152,6 +152,138 @@ __device__ inline void mul3(const float* a, const float* b, float* r) {
         }
 }
 
+// The same Gauss-Jordan sequence as gj_inverse_lds on a matrix held in
+// registers (every index is a compile-time constant once unrolled; the pivot
+// row moves by selects). Each element sees the same operations in the same
+// order, so the result is the same bits.
+__device__ __forceinline__ void gj_inverse_reg(const float* A, float* out) {
+    float m[6][12];
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            m[r][c] = A[r * 6 + c];
+            m[r][6 + c] = (r == c) ? 1.0f : 0.0f;
+        }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        int piv = k;
+        float best = fabsf(m[k][k]);
+#pragma unroll
+        for (int r = k + 1; r < 6; ++r)
+            if (fabsf(m[r][k]) > best) {
+                best = fabsf(m[r][k]);
+                piv = r;
+            }
+#pragma unroll
+        for (int r = k + 1; r < 6; ++r) {
+            const bool sw = piv == r;
+#pragma unroll
+            for (int c = 0; c < 12; ++c) {
+                const float t = m[k][c];
+                m[k][c] = sw ? m[r][c] : t;
+                m[r][c] = sw ? t : m[r][c];
+            }
+        }
+        const float ip = 1.0f / m[k][k];
+#pragma unroll
+        for (int c = 0; c < 12; ++c) m[k][c] *= ip;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            if (r == k) continue;
+            const float f = m[r][k];
+#pragma unroll
+            for (int c = 0; c < 12; ++c) m[r][c] = (f == 0.0f) ? m[r][c] : m[r][c] - f * m[k][c];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) out[r * 6 + c] = m[r][6 + c];
+}
+
+__device__ __forceinline__ float norm1_reg(const float* m) {
+    float best = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        float s = 0.0f;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) s += fabsf(m[r * 6 + c]);
+        best = s > best ? s : best;
+    }
+    return best;
+}
+
+// hessian.hpp:40-52 in registers (the sequence of well_conditioned_lds).
+__device__ __forceinline__ bool well_conditioned_reg(const float* H, int half) {
+    const float s = half > 1 ? 1.0f / (float)half : 1.0f;
+    float Hn[36], Hi[36];
+    bool fin = true;
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            const float dr = r < 2 ? 1.0f : s, dc = c < 2 ? 1.0f : s;
+            Hn[r * 6 + c] = dr * H[r * 6 + c] * dc;
+            fin = fin && finite_f(Hn[r * 6 + c]);
+        }
+    if (!fin) return false;
+    gj_inverse_reg(Hn, Hi);
+#pragma unroll
+    for (int q = 0; q < 36; ++q) fin = fin && finite_f(Hi[q]);
+    if (!fin) return false;
+    const float a = norm1_reg(Hn), b = norm1_reg(Hi);
+    return 1.0f / (a * b) > 1e-6f;
+}
+
+// FP64 Cholesky factor of a symmetric positive definite 6x6 (float input,
+// converted exactly): L row-major lower triangle at i (i + 1) / 2 + j, its
+// diagonal stored as the reciprocal. False when a pivot is not positive.
+__device__ __forceinline__ bool chol64_factor(const float* A, double* Lp) {
+    double L[6][6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j) {
+            double s = (double)A[i * 6 + j];
+#pragma unroll
+            for (int k = 0; k < j; ++k) s -= L[i][k] * L[j][k];
+            if (i == j) {
+                if (!(s > 0.0)) return false;
+                L[i][i] = sqrt(s);
+            } else {
+                L[i][j] = s / L[j][j];
+            }
+        }
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j) Lp[i * (i + 1) / 2 + j] = (i == j) ? 1.0 / L[i][i] : L[i][j];
+    return true;
+}
+
+// The Gauss-Newton step d = -(L L^T)^-1 g by FP64 forward and back
+// substitution on a factor from chol64_factor.
+__device__ __forceinline__ void chol64_step(const double* __restrict__ Lp, const float* g, float* d) {
+    double y[6], x[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        double s = (double)g[i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) s -= Lp[i * (i + 1) / 2 + k] * y[k];
+        y[i] = s * Lp[i * (i + 1) / 2 + i];
+    }
+#pragma unroll
+    for (int i = 5; i >= 0; --i) {
+        double s = y[i];
+#pragma unroll
+        for (int k = i + 1; k < 6; ++k) s -= Lp[k * (k + 1) / 2 + i] * x[k];
+        x[i] = s * Lp[i * (i + 1) / 2 + i];
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) d[i] = -(float)x[i];
+}
+
 // blockIdx -> work index so that consecutive work items land on one XCD:
 // the dispatcher deals workgroups round-robin over the 8 XCDs, each with its
 // own L2, and neighbouring subsets share most of their pixels.
