@@ -156,6 +156,69 @@ def copy_probe():
     print("copy probe: 256 MiB read, 256 MiB written per dispatch", flush=True)
 
 
+def learned_seed_study(dev, sa, wl, steps, threads):
+    """The learned-seeding workload (4096^2, eight seeded Gaussian bumps of up
+    to +-30 px plus a global rotation of up to 5 degrees, step 8, subset 31):
+    converged subsets per second of the plain GPU method, of the GPU method with
+    the learned seeder (the committed weights), and of the CPU oracle (the
+    plain method restated), each scaled by its fraction within 1e-3 px of the
+    FP64 optimum on a 2000-node sample."""
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    name = "c5_4096_bumps"
+    cfg = wl.config(name)
+    w, h = cfg["w"], cfg["h"]
+    ref, dfm = wl.pair(dev, name)
+    dref, ddef = dev.sdk_malloc(w * h), dev.sdk_malloc(w * h)
+    dev.sdk_memcpy(dref, P(ref), w * h, 0)
+    dev.sdk_memcpy(ddef, P(dfm), w * h, 0)
+    prm = np.array(wl.solve_params(name), np.int32)
+    cap = (w // cfg["step"] + 1) * (h // cfg["step"] + 1) * 8
+    tol = ctypes.c_float(cfg["tol"])
+    res = {"workload": f"{name}: {w}x{h} speckle, eight Gaussian bumps (+-30 px, width 200-800 px) and a rotation "
+                       f"of up to 5 deg, step {cfg['step']}, subset {cfg['subset']}", "weights": "seeder_weights.bin"}
+    for mode in ("plain_gpu", "learned_seed_gpu"):
+        eng = dev.sdk_engine_create()
+        if mode == "learned_seed_gpu":
+            if dev.sdk_engine_load_seeder(eng, sa.SEEDER_WEIGHTS.encode()) != 0:
+                raise RuntimeError("seeder weights missing: " + sa.SEEDER_WEIGHTS)
+            assert dev.sdk_engine_set_flag(eng, 5, 1) == 0
+        assert dev.sdk_engine_set_reference(eng, dref, 1, w, h, None) == 0
+        out = np.zeros(cap, np.float32)
+        met = np.zeros(24, np.float32)
+        n = dev.sdk_engine_run(eng, ddef, 1, P(prm), P(out), cap, P(met), 24, tol)   # warm (pool, pyramids)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            n = dev.sdk_engine_run(eng, ddef, 1, P(prm), P(out), cap, P(met), 24, tol)
+        ms = (time.perf_counter() - t0) * 1000.0 / steps
+        ss = np.zeros(3, np.int32)
+        dev.sdk_engine_seed_stats(eng, P(ss))
+        dev.sdk_engine_destroy(eng)
+        n = int(n)
+        pts = out[: max(n, 0) * 8].reshape(-1, 8).copy()
+        acc = fp64_accuracy(ref, dfm, cfg["subset"], pts, False) if n > 0 else None
+        solved = float(met[3] + met[4] + met[19]) if n > 0 else 0.0
+        frac = acc["within_1e-3px"] if acc else 0.0
+        res[mode] = {"ms_per_step": ms, "output_points": n, "value": solved * frac / (ms / 1000.0),
+                     "accuracy_vs_fp64": acc, "anchors_cnn_seeded": int(ss[0]), "anchors_cnn_held": int(ss[1]),
+                     "anchors_plain_fallback": int(ss[2])}
+    dev.sdk_free(dref)
+    dev.sdk_free(ddef)
+    if threads:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from helpers import OracleEngine
+
+        oe = OracleEngine()
+        oe.set_reference(ref)
+        t0 = time.perf_counter()
+        n_o, _, met_o = oe.run(dfm, list(prm), threads=threads, tol=cfg["tol"])
+        dt = time.perf_counter() - t0
+        res["cpu_oracle"] = {"seconds": dt, "output_points": int(n_o), "cores": threads,
+                             "value": float(met_o[3] + met_o[4] + met_o[19]) / dt if n_o > 0 else 0.0}
+    lv, pv = res["learned_seed_gpu"]["value"], res["plain_gpu"]["value"]
+    res["learned_over_plain"] = (lv / pv) if pv > 0 else None
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,6 +228,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and FP64 check")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-precision", action="store_true", help="skip the precision-variant study")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-seeding comparison")
     ap.add_argument("--tol", type=float, default=None, help="IC-GN stop rule ||dp|| < tol (default: the workload's)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB copy kernel three times (the calibration of the HBM traffic "
@@ -241,6 +305,7 @@ def main():
     parity = None
     acc = None
     precision = None
+    learned = None
     if not args.no_cpu:
         ref = np.zeros((h, w), np.uint8)
         dfm = np.zeros((h, w), np.uint8)
@@ -252,6 +317,8 @@ def main():
         if not args.no_precision:
             precision = precision_study(dev, dref, ddef, prm, cap, tol, ref, dfm, cfg["subset"], use6,
                                         max(2, args.steps // 2))
+        if not args.no_learned:
+            learned = learned_seed_study(dev, sa, wl, 3, threads)
     frac = acc["within_1e-3px"] if acc else 1.0
     value = float(solved * frac / (ms_per_step / 1000.0))
     line = {
@@ -289,6 +356,7 @@ def main():
         "path_b": {"speculative_solves": int(tm[11]), "further_launches": int(tm[12]), "further_solves": int(tm[13])},
         "prepass_ms": prepass_ms,
         "precision_study": precision,
+        "learned_seed": learned,
     }
     dev.sdk_engine_destroy(eng)
     dev.sdk_free(dref)

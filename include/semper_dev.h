@@ -58,7 +58,8 @@ SDK_API int sdk_wave_sum_probe(const float* in, int nw, float* out, int32_t* iou
    rebuilds the Hessian pool on the next run): flag 2 = the Gauss-Newton step
    by FP64 Cholesky of each pool entry, flag 3 = fp16 storage of the reference
    gradient planes, flag 4 = J^T J of the pool on MFMA (fp16 inputs, FP32
-   accumulation). */
+   accumulation); flag 5 = learned anchor seeding (needs a seeder loaded by
+   sdk_engine_load_seeder; -3 otherwise). */
 SDK_API int sdk_engine_set_flag(void* h, int flag, int value);
 
 /* Phase timings of the last run, 14 doubles: total, prep, seeding, prepass,
@@ -90,6 +91,29 @@ SDK_API int sdk_decode(const uint8_t* data, size_t len, int expected_w, int expe
 SDK_API void sdk_image_dimensions(const uint8_t* data, size_t len, int32_t* w, int32_t* h);
 SDK_API int sdk_to_gray8(const void* px, int w, int h, int cn, int depth, uint8_t* out);
 SDK_API int sdk_to_mask8(const void* px, int w, int h, int cn, int depth, uint8_t* out);
+
+/* The learned seeder (no reference counterpart): a CNN from a level-2 patch
+   pair (2 x 64 x 64, each plane normalised) to the scaled warp (u, v) / 16,
+   (ux, uy, vx, vy) / 0.1 of the patch centre; see csrc/seeder.hpp.
+   sdk_seeder_param_count: the flat parameter count.
+   sdk_seeder_grad / sdk_seeder_forward: one forward (and backward) pass on the
+   GPU with the given parameters on a host batch; the loss is the mean squared
+   error over B x 6, grads its gradient.
+   sdk_seeder_train: renders a 2048 x 2048 speckle reference from `seed`,
+   trains from a He initialisation (steps x batch pairs, Adam, cosine decay
+   from lr), writes the weights to out_path; returns the count of mean losses
+   (one per 100 steps) written to losses (at most nloss), or -3.
+   sdk_engine_load_seeder: weights for an engine (0 or -3).
+   sdk_engine_seed_stats: anchors given CNN guesses, those whose refined
+   solve held, those solved again the plain way, of the last run. */
+SDK_API int sdk_seeder_param_count(void);
+SDK_API int sdk_seeder_grad(const float* params, const float* x, const float* t, int batch, float* grads,
+                            float* loss);
+SDK_API int sdk_seeder_forward(const float* params, const float* x, int batch, float* y);
+SDK_API int sdk_seeder_train(uint64_t seed, int steps, int batch, float lr, const char* out_path, float* losses,
+                             int nloss);
+SDK_API int sdk_engine_load_seeder(void* h, const char* path);
+SDK_API int sdk_engine_seed_stats(void* h, int32_t* out3);
 
 #ifdef __cplusplus
 }

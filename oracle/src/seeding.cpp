@@ -50,10 +50,18 @@ namespace {
 
 constexpr int kPatchHalf = 16;     // 32 x 32 template at every level
 constexpr int kCoarseRadius = 8;   // shifts at the coarsest level
+constexpr int kCoarseRadiusWide = 16;   // the second pass when the first does not lock
 constexpr int kFineRadius = 2;     // around twice the coarser estimate
-constexpr int kPatchGrid = 5;      // 5 x 5 patch centres over the ROI
+// patch centres per side: 5, or 8 on ROIs of 2048 px and more
+int patch_grid(int rect_w, int rect_h) { return std::min(rect_w, rect_h) >= 2048 ? 8 : 5; }
 constexpr double kMinTemplateStd = 2.0;
 constexpr double kMinScore = 0.5;
+// consensus fit over random triples when the least squares does not hold half
+// the patches
+constexpr int kConsensusTrials = 600;
+constexpr double kConsensusTol = 3.0;
+constexpr int kConsensusMinInliers = 6;
+constexpr double kConsensusMinArea = 1.0e4;
 
 // ZNCC of the template at (cx, cy) against the window shifted by (sx, sy) at
 // one pyramid level; false if either window leaves the image.
@@ -172,22 +180,93 @@ bool robust_affine(const std::vector<double>& xr, const std::vector<double>& yr,
     return have;
 }
 
+// Deterministic consensus affine: triples drawn by a fixed LCG; the affine
+// through the triple holding the most patches within kConsensusTol (ties: the
+// smaller summed residual); refined by robust_affine on that set.
+bool consensus_affine(const std::vector<double>& x, const std::vector<double>& y,
+                      const std::vector<double>& u, const std::vector<double>& v, double cx, double cy,
+                      double a[6]) {
+    const int n = (int)x.size();
+    const int need = std::max(kConsensusMinInliers, n / 4);
+    if (n < need || n < 3) return false;
+    uint64_t lcg = 0x9E3779B97F4A7C15ull;
+    auto draw = [&](int m) {
+        lcg = lcg * 6364136223846793005ull + 1442695040888963407ull;
+        return (int)((lcg >> 33) % (uint64_t)m);
+    };
+    int best_in = 0;
+    double best_res = 0.0, best[6] = {0, 0, 0, 0, 0, 0};
+    for (int t = 0; t < kConsensusTrials; ++t) {
+        const int i = draw(n), j = draw(n), k = draw(n);
+        if (i == j || j == k || i == k) continue;
+        const double A[9] = {1.0, x[i] - cx, y[i] - cy, 1.0, x[j] - cx, y[j] - cy, 1.0, x[k] - cx, y[k] - cy};
+        const double det = A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) +
+                           A[2] * (A[3] * A[7] - A[4] * A[6]);
+        if (!(std::fabs(det) > kConsensusMinArea)) continue;
+        const double bu[3] = {u[i], u[j], u[k]}, bv[3] = {v[i], v[j], v[k]};
+        double su[3], sv[3];
+        if (!solve3(A, bu, su) || !solve3(A, bv, sv)) continue;
+        int in = 0;
+        double res = 0.0;
+        for (int q = 0; q < n; ++q) {
+            const double dx = x[q] - cx, dy = y[q] - cy;
+            const double e = std::hypot(su[0] + su[1] * dx + su[2] * dy - u[q], sv[0] + sv[1] * dx + sv[2] * dy - v[q]);
+            if (e <= kConsensusTol) {
+                ++in;
+                res += e;
+            }
+        }
+        if (in > best_in || (in == best_in && in > 0 && res < best_res)) {
+            best_in = in;
+            best_res = res;
+            for (int q = 0; q < 3; ++q) {
+                best[q] = su[q];
+                best[3 + q] = sv[q];
+            }
+        }
+    }
+    if (best_in < need) return false;
+    std::vector<double> xi, yi, ui, vi;
+    for (int q = 0; q < n; ++q) {
+        const double dx = x[q] - cx, dy = y[q] - cy;
+        const double e = std::hypot(best[0] + best[1] * dx + best[2] * dy - u[q], best[3] + best[4] * dx + best[5] * dy - v[q]);
+        if (e <= kConsensusTol) {
+            xi.push_back(x[q]);
+            yi.push_back(y[q]);
+            ui.push_back(u[q]);
+            vi.push_back(v[q]);
+        }
+    }
+    return robust_affine(xi, yi, ui, vi, cx, cy, need, kConsensusTol, a);
+}
+
+GlobalModel global_motion_pass(const Pyramid& ref, const Pyramid& def, int rect_x, int rect_y, int rect_w,
+                               int rect_h, int coarse_radius);
+
 GlobalModel global_motion(const Pyramid& ref, const Pyramid& def, int rect_x, int rect_y,
                           int rect_w, int rect_h) {
+    GlobalModel gm = global_motion_pass(ref, def, rect_x, rect_y, rect_w, rect_h, kCoarseRadius);
+    if (!gm.locked) gm = global_motion_pass(ref, def, rect_x, rect_y, rect_w, rect_h, kCoarseRadiusWide);
+    return gm;
+}
+
+GlobalModel global_motion_pass(const Pyramid& ref, const Pyramid& def, int rect_x, int rect_y, int rect_w,
+                               int rect_h, int coarse_radius) {
     GlobalModel gm;
     gm.cx = rect_x + 0.5 * rect_w;
     gm.cy = rect_y + 0.5 * rect_h;
     const int L = (int)ref.lv.size() - 1;
     std::vector<double> xs, ys, us, vs;
-    for (int j = 0; j < kPatchGrid; ++j)
-        for (int i = 0; i < kPatchGrid; ++i) {
-            const int px = rect_x + (int)std::lround((i + 0.5) * rect_w / kPatchGrid);
-            const int py = rect_y + (int)std::lround((j + 0.5) * rect_h / kPatchGrid);
+    const int grid = patch_grid(rect_w, rect_h);
+    for (int j = 0; j < grid; ++j)
+        for (int i = 0; i < grid; ++i) {
+            const int px = rect_x + (int)std::lround((i + 0.5) * rect_w / grid);
+            const int py = rect_y + (int)std::lround((j + 0.5) * rect_h / grid);
             int sx = 0, sy = 0;
             double best = -3.0;
             bool ok = true;
             for (int l = L; l >= 0 && ok; --l) {
-                const int rad = (l == L) ? kCoarseRadius : kFineRadius;
+                const int rad = (l == L) ? coarse_radius : kFineRadius;
                 int bx = 0, by = 0;
                 ok = search(ref, def, l, px >> l, py >> l, sx, sy, rad, bx, by, best);
                 sx = bx;
@@ -220,7 +299,9 @@ GlobalModel global_motion(const Pyramid& ref, const Pyramid& def, int rect_x, in
     if (gm.patches < 3) return gm;
     double a[6];
     const int need = std::max(3, (gm.patches + 1) / 2);
-    if (!robust_affine(xs, ys, us, vs, gm.cx, gm.cy, need, 2.0, a)) return gm;
+    if (!robust_affine(xs, ys, us, vs, gm.cx, gm.cy, need, 2.0, a) &&
+        !consensus_affine(xs, ys, us, vs, gm.cx, gm.cy, a))
+        return gm;
     for (int k = 0; k < 6; ++k) gm.a[k] = a[k];
     gm.locked = true;
     gm.t_u = a[0];

@@ -257,6 +257,10 @@ SDK_API int sdk_engine_set_flag(void* h, int flag, int value) {
         case 4:
             ((DevEngine*)h)->eng.prec.mfma_pool = value != 0;
             return 0;
+        case 5:
+            if (value && !((DevEngine*)h)->eng.seeder_ready()) return -3;
+            ((DevEngine*)h)->eng.learned_seed = value != 0;
+            return 0;
         default:
             return -3;
     }
@@ -376,6 +380,81 @@ SDK_API int sdk_to_gray8(const void* px, int w, int h, int cn, int depth, uint8_
 
 SDK_API int sdk_to_mask8(const void* px, int w, int h, int cn, int depth, uint8_t* out) {
     return raster_convert(px, w, h, cn, depth, out, true);
+}
+
+// ---- the learned seeder ------------------------------------------------------
+
+SDK_API int sdk_seeder_param_count(void) { return Seeder::param_count(); }
+
+SDK_API int sdk_seeder_grad(const float* params, const float* x, const float* t, int batch, float* grads,
+                            float* loss) {
+    if (!params || !x || !t || !grads || !loss) return -3;
+    try {
+        Seeder s;
+        return s.grad(params, x, t, batch, grads, loss);
+    } catch (...) {
+        return -3;
+    }
+}
+
+SDK_API int sdk_seeder_forward(const float* params, const float* x, int batch, float* y) {
+    if (!params || !x || !y) return -3;
+    try {
+        Seeder s;
+        return s.forward(params, x, batch, y);
+    } catch (...) {
+        return -3;
+    }
+}
+
+SDK_API int sdk_seeder_train(uint64_t seed, int steps, int batch, float lr, const char* out_path, float* losses,
+                             int nloss) {
+    if (!out_path || steps < 1 || batch < 1) return -3;
+    try {
+        // the reference the pairs are cut from: the engine's own speckle
+        // (the workloads' statistics) on 2048 x 2048, noise 4 grey levels
+        const int w = 2048, h = 2048;
+        Dev<uint8_t> ref((size_t)w * h), tmp((size_t)w * h);
+        if (!ref.p || !tmp.p) return -3;
+        FieldSpec f = {};
+        f.kind = kFieldIdentity;
+        if (synth_pair(seed, seed + 1, w, h, f, 4.0f, ref.p, tmp.p, nullptr) != 0) return -3;
+        if (hipDeviceSynchronize() != hipSuccess) return -3;
+        Seeder s;
+        if (s.init(seed) != 0) return -3;
+        SeederTrainConfig cfg;
+        cfg.steps = steps;
+        cfg.batch = batch;
+        cfg.lr = lr;
+        cfg.seed = seed;
+        std::vector<float> hist;
+        if (s.train(ref.p, w, h, cfg, &hist) != 0) return -3;
+        if (s.save(out_path) != 0) return -3;
+        const int k = std::min((int)hist.size(), std::max(nloss, 0));
+        if (losses)
+            for (int i = 0; i < k; ++i) losses[i] = hist[i];
+        return k;
+    } catch (...) {
+        return -3;
+    }
+}
+
+SDK_API int sdk_engine_load_seeder(void* h, const char* path) {
+    if (!h || !path) return -3;
+    try {
+        return ((DevEngine*)h)->eng.load_seeder(path);
+    } catch (...) {
+        return -3;
+    }
+}
+
+SDK_API int sdk_engine_seed_stats(void* h, int32_t* out3) {
+    if (!h || !out3) return -3;
+    const auto& st = ((DevEngine*)h)->eng.seed_stats;
+    out3[0] = st.cnn;
+    out3[1] = st.cnn_ok;
+    out3[2] = st.fallback;
+    return 0;
 }
 
 }  // extern "C"

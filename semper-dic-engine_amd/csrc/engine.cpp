@@ -238,6 +238,11 @@ int Engine::ensure_grid(const Params& p, const uint8_t* mask, bool& cached) {
     return 0;
 }
 
+int Engine::load_seeder(const std::string& path) {
+    if (!seeder_) seeder_ = std::make_unique<Seeder>();
+    return seeder_->load(path);
+}
+
 void Engine::set_precision_args(SolveArgs& a) const {
     if (grid_prec_.grad16) {
         a.refGx16 = gx16_.p;
@@ -430,14 +435,19 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
         // ---- seeding: global model + anchor lattice -------------------------
         const auto t_seed = clk::now();
         const int L = std::min(pyramid_levels(p.rect_w, p.rect_h), pyr_ref_levels_);
+        // the learned seeder reads level 2 of both pyramids
+        const bool use_cnn = learned_seed && seeder_ready() && pyr_ref_levels_ >= SeederShape::kLevel;
+        const int Lb = use_cnn ? std::max(L, (int)SeederShape::kLevel) : L;
+        seed_stats = SeedStats{};
         PyrDev pd = {};
         pd.r0 = ref_u8_.p;
         pd.d0 = def_u8_.p;
         pd.w[0] = w_;
         pd.h[0] = h_;
+        int w2 = 0, h2 = 0;
         {
             int pw = w_, ph = h_;
-            for (int l = 1; l <= L; ++l) {
+            for (int l = 1; l <= Lb; ++l) {
                 const int nw = pw / 2, nh = ph / 2;
                 if (!pyr_d_[l].reserve((size_t)nw * nh)) return kErrInit;
                 if (l == 1) ck(launch_downsample_u8(def_u8_.p, pw, ph, pyr_d_[1].p, st_));
@@ -449,6 +459,10 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                 pw = nw;
                 ph = nh;
             }
+            if (use_cnn) {
+                w2 = pd.w[SeederShape::kLevel];
+                h2 = pd.h[SeederShape::kLevel];
+            }
         }
         pd.levels = L;
         std::vector<int> centres;
@@ -456,15 +470,24 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
         const int npatch = (int)centres.size() / 2;
         if (!centres_d_.reserve(centres.size()) || !patch_d_.reserve(npatch)) return kErrInit;
         ck(hipMemcpyAsync(centres_d_.p, centres.data(), sizeof(int) * centres.size(), hipMemcpyHostToDevice, st_));
-        ck(launch_patch_search(pd, centres_d_.p, npatch, patch_d_.p, st_));
         std::vector<PatchOut> po(npatch);
-        ck(hipMemcpyAsync(po.data(), patch_d_.p, sizeof(PatchOut) * npatch, hipMemcpyDeviceToHost, st_));
-        ck(hipStreamSynchronize(st_));
-        const GlobalModel gm = fit_global_model(centres, po, p.rect_x, p.rect_y, p.rect_w, p.rect_h);
+        GlobalModel gm;
+        for (const int rad : {kCoarseRadius, kCoarseRadiusWide}) {
+            pd.coarse_radius = rad;
+            ck(launch_patch_search(pd, centres_d_.p, npatch, patch_d_.p, st_));
+            ck(hipMemcpyAsync(po.data(), patch_d_.p, sizeof(PatchOut) * npatch, hipMemcpyDeviceToHost, st_));
+            ck(hipStreamSynchronize(st_));
+            gm = fit_global_model(centres, po, p.rect_x, p.rect_y, p.rect_w, p.rect_h);
+            if (gm.locked) break;
+        }
 
         // lattice (solve_anchor_seeds, full_field_anchors.cpp:283-574)
         int sx, sy;
-        plan_lattice(gw, gh, sx, sy);
+        // the learned seeder's lattice is 16 times denser (anchors 4 times
+        // closer): its guesses cost one CNN pass each instead of a coarse
+        // search, and the 8-neighbour median test (+-1 px on residuals) holds
+        // where a field curves between anchors of the plain spacing
+        plan_lattice(gw, gh, sx, sy, use_cnn ? 16 * kAnchorTarget : kAnchorTarget);
         std::vector<int> lx, ly;
         for (int x = 0; x < gw; x += sx) lx.push_back(x);
         if (lx.back() != gw - 1) lx.push_back(gw - 1);
@@ -509,7 +532,62 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             r.attempted = (int)tasks.size();
             std::vector<SolveOut> res;
             const auto t0 = clk::now();
+            std::vector<Task> plain;
+            if (use_cnn && !tasks.empty()) {
+                // the CNN's warp of each anchor's patch pair, the deformed patch
+                // placed by the global model's shift at the patch centre
+                plain = tasks;
+                const int n = (int)tasks.size(), S2 = SeederShape::kScale;
+                std::vector<int> c2(2 * n), off(2 * n);
+                std::vector<float> y(6 * (size_t)n);
+                std::vector<double> px(n), py(n);
+                for (int k = 0; k < n; ++k) {
+                    c2[2 * k] = (int)std::lround((tasks[k].cx + 0.5) / S2);
+                    c2[2 * k + 1] = (int)std::lround((tasks[k].cy + 0.5) / S2);
+                    px[k] = (double)S2 * c2[2 * k] - 0.5;
+                    py[k] = (double)S2 * c2[2 * k + 1] - 0.5;
+                    float bg[6] = {0, 0, 0, 0, 0, 0};
+                    if (gm.locked) gm.at(px[k], py[k], bg);
+                    off[2 * k] = (int)std::lround(bg[0] / S2);
+                    off[2 * k + 1] = (int)std::lround(bg[1] / S2);
+                }
+                if (seeder_->infer(pyr_r_[SeederShape::kLevel].p, pyr_d_[SeederShape::kLevel].p, w2, h2, c2.data(),
+                                   off.data(), n, y.data(), st_) != 0)
+                    throw HipFail{hipErrorLaunchFailure};
+                for (int k = 0; k < n; ++k) {
+                    const float* q = &y[6 * (size_t)k];
+                    const double U = (double)S2 * off[2 * k] + q[0], V = (double)S2 * off[2 * k + 1] + q[1];
+                    const double dx = tasks[k].cx - px[k], dy = tasks[k].cy - py[k];
+                    tasks[k].g[0] = (float)(U + q[2] * dx + q[3] * dy);
+                    tasks[k].g[1] = (float)(V + q[4] * dx + q[5] * dy);
+                    for (int j = 2; j < 6; ++j) tasks[k].g[j] = q[j];
+                    // IC-GN from the CNN's warp, with the Simplex rescue
+                    tasks[k].mode = NO_SEARCH;
+                }
+            }
             if (solve_tasks(tasks, res, p.use6, p.subset, tol) != 0) throw HipFail{hipErrorOutOfMemory};
+            if (!plain.empty()) {
+                // the plain anchor solve wherever the refined CNN guess failed
+                std::vector<Task> retry;
+                std::vector<int> ridx;
+                for (size_t k = 0; k < tasks.size(); ++k) {
+                    const SolveOut& o = res[k];
+                    if (o.init && o.status == 0 && o.corr <= kAnchorAcceptScore) continue;
+                    retry.push_back(plain[k]);
+                    ridx.push_back((int)k);
+                }
+                seed_stats.cnn += (int)tasks.size();
+                seed_stats.cnn_ok += (int)(tasks.size() - retry.size());
+                seed_stats.fallback += (int)retry.size();
+                if (!retry.empty()) {
+                    std::vector<SolveOut> rr;
+                    if (solve_tasks(retry, rr, p.use6, p.subset, tol) != 0) throw HipFail{hipErrorOutOfMemory};
+                    for (size_t i = 0; i < retry.size(); ++i) {
+                        res[ridx[i]] = rr[i];
+                        tasks[ridx[i]] = retry[i];
+                    }
+                }
+            }
             icgn_ms_est += ms_since(t0);
             for (size_t k = 0; k < tasks.size(); ++k) {
                 const SolveOut& o = res[k];

@@ -11,9 +11,12 @@
 #include <atomic>
 #include <cstdint>
 #include <functional>
+#include <memory>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
+#include "seeder.hpp"
 #include "solver.hpp"
 
 namespace sdic {
@@ -150,6 +153,18 @@ public:
         }
     };
     Precision prec;
+    // Learned anchor seeding (no reference counterpart): with a seeder loaded,
+    // each lattice anchor's guess comes from the CNN (relative to the global
+    // model's shift), IC-GN refines it (NO_SEARCH: the Simplex rescue when it
+    // fails), and an anchor whose refined solve fails is solved again the
+    // plain way.
+    bool learned_seed = false;
+    int load_seeder(const std::string& path);
+    bool seeder_ready() const { return seeder_ && seeder_->ready(); }
+    struct SeedStats {
+        int cnn = 0, cnn_ok = 0, fallback = 0;
+    };
+    SeedStats seed_stats;
     Field last_field;
 
     // Median test, VSG strain and row-major packing of a node field given on
@@ -180,6 +195,7 @@ private:
     DevBuf<PoolEntry> pool_;
     DevBuf<uint8_t> pool_raw_;      // per-node sums between the two pool launches
     Precision grid_prec_;
+    std::unique_ptr<Seeder> seeder_;
     DevBuf<uint16_t> gx16_, gy16_;  // fp16 gradient planes (Precision::grad16)
     bool g16_ready_ = false;
     DevBuf<double> chol_;           // FP64 Cholesky factors (Precision::chol64)

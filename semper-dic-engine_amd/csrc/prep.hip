@@ -351,7 +351,6 @@ __global__ void k_down(const T* __restrict__ src, int w, int h, float* __restric
 // ---- global-motion patch search (the seeding front end) ---------------------
 
 constexpr int kPatchHalf = 16;
-constexpr int kCoarseRadius = 8;
 constexpr int kFineRadius = 2;
 
 __device__ __forceinline__ float lv_at(const PyrDev& p, bool def, int l, int idx) {
@@ -437,7 +436,7 @@ __global__ __launch_bounds__(64 * kPatchWaves) void k_patch(PyrDev p, const int*
     __syncthreads();
     for (int l = L; l >= 0; --l) {
         if (!s_ok) break;
-        const int rad = (l == L) ? kCoarseRadius : kFineRadius;
+        const int rad = (l == L) ? p.coarse_radius : kFineRadius;
         const int side = 2 * rad + 1, ns = side * side;
         const int csx = s_sx, csy = s_sy;
         Cand best;

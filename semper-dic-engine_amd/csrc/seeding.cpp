@@ -32,9 +32,18 @@ bool solve3(const double A[9], const double b[3], double x[3]) {
     return true;
 }
 
-constexpr int kPatchGrid = 5;
+// patch centres per side: 5, or 8 on ROIs of 2048 px and more (where local
+// motion has room to stray from one affine)
+int patch_grid(int rect_w, int rect_h) { return std::min(rect_w, rect_h) >= 2048 ? 8 : 5; }
 constexpr double kMinTemplateStd = 2.0;
 constexpr double kMinScore = 0.5;
+// The consensus fit when the plain least squares does not hold half the
+// patches (local motion beyond an affine): deterministic random triples, the
+// affine through the one with the most patches within kConsensusTol px.
+constexpr int kConsensusTrials = 600;
+constexpr double kConsensusTol = 3.0;
+constexpr int kConsensusMinInliers = 6;
+constexpr double kConsensusMinArea = 1.0e4;   // |det| of the triple's design matrix, px^2
 
 }  // namespace
 
@@ -45,8 +54,8 @@ int lattice_count(int n, int stride) {
     return c;
 }
 
-void plan_lattice(int gw, int gh, int& sx, int& sy) {
-    int stride = (int)std::lround(std::sqrt((double)(gw * gh) / (double)kAnchorTarget));
+void plan_lattice(int gw, int gh, int& sx, int& sy, int target) {
+    int stride = (int)std::lround(std::sqrt((double)(gw * gh) / (double)target));
     stride = std::max(1, stride);
     const int budget = lattice_count(gw, stride) * lattice_count(gh, stride);
     sx = sy = stride;
@@ -173,6 +182,67 @@ bool robust_affine(const std::vector<double>& x, const std::vector<double>& y, c
     return have;
 }
 
+bool consensus_affine(const std::vector<double>& x, const std::vector<double>& y, const std::vector<double>& u,
+                      const std::vector<double>& v, double cx, double cy, double a[6]) {
+    const int n = (int)x.size();
+    const int need = std::max(kConsensusMinInliers, n / 4);
+    if (n < need || n < 3) return false;
+    uint64_t lcg = 0x9E3779B97F4A7C15ull;
+    auto draw = [&](int m) {
+        lcg = lcg * 6364136223846793005ull + 1442695040888963407ull;
+        return (int)((lcg >> 33) % (uint64_t)m);
+    };
+    int best_in = 0;
+    double best_res = 0.0, best[6] = {0, 0, 0, 0, 0, 0};
+    for (int t = 0; t < kConsensusTrials; ++t) {
+        const int i = draw(n), j = draw(n), k = draw(n);
+        if (i == j || j == k || i == k) continue;
+        const double A[9] = {1.0, x[i] - cx, y[i] - cy, 1.0, x[j] - cx, y[j] - cy, 1.0, x[k] - cx, y[k] - cy};
+        const double det = A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) +
+                           A[2] * (A[3] * A[7] - A[4] * A[6]);
+        // a triple spanning too little area fixes no gradient
+        if (!(std::fabs(det) > kConsensusMinArea)) continue;
+        const double bu[3] = {u[i], u[j], u[k]}, bv[3] = {v[i], v[j], v[k]};
+        double su[3], sv[3];
+        if (!solve3(A, bu, su) || !solve3(A, bv, sv)) continue;
+        int in = 0;
+        double res = 0.0;
+        for (int q = 0; q < n; ++q) {
+            const double dx = x[q] - cx, dy = y[q] - cy;
+            const double e = std::hypot(su[0] + su[1] * dx + su[2] * dy - u[q], sv[0] + sv[1] * dx + sv[2] * dy - v[q]);
+            if (e <= kConsensusTol) {
+                ++in;
+                res += e;
+            }
+        }
+        if (in > best_in || (in == best_in && in > 0 && res < best_res)) {
+            best_in = in;
+            best_res = res;
+            best[0] = su[0];
+            best[1] = su[1];
+            best[2] = su[2];
+            best[3] = sv[0];
+            best[4] = sv[1];
+            best[5] = sv[2];
+        }
+    }
+    if (best_in < need) return false;
+    // least squares on the consensus set, then the iterated rejection
+    std::vector<double> xi, yi, ui, vi;
+    for (int q = 0; q < n; ++q) {
+        const double dx = x[q] - cx, dy = y[q] - cy;
+        const double e = std::hypot(best[0] + best[1] * dx + best[2] * dy - u[q],
+                                    best[3] + best[4] * dx + best[5] * dy - v[q]);
+        if (e <= kConsensusTol) {
+            xi.push_back(x[q]);
+            yi.push_back(y[q]);
+            ui.push_back(u[q]);
+            vi.push_back(v[q]);
+        }
+    }
+    return robust_affine(xi, yi, ui, vi, cx, cy, need, kConsensusTol, a);
+}
+
 void GlobalModel::at(double x, double y, float g[6]) const {
     const double dx = x - cx, dy = y - cy;
     g[0] = (float)(a[0] + a[1] * dx + a[2] * dy);
@@ -192,10 +262,11 @@ int pyramid_levels(int rect_w, int rect_h) {
 
 void patch_centres(int rect_x, int rect_y, int rect_w, int rect_h, std::vector<int>& cxy) {
     cxy.clear();
-    for (int j = 0; j < kPatchGrid; ++j)
-        for (int i = 0; i < kPatchGrid; ++i) {
-            cxy.push_back(rect_x + (int)std::lround((i + 0.5) * rect_w / kPatchGrid));
-            cxy.push_back(rect_y + (int)std::lround((j + 0.5) * rect_h / kPatchGrid));
+    const int g = patch_grid(rect_w, rect_h);
+    for (int j = 0; j < g; ++j)
+        for (int i = 0; i < g; ++i) {
+            cxy.push_back(rect_x + (int)std::lround((i + 0.5) * rect_w / g));
+            cxy.push_back(rect_y + (int)std::lround((j + 0.5) * rect_h / g));
         }
 }
 
@@ -226,7 +297,9 @@ GlobalModel fit_global_model(const std::vector<int>& cxy, const std::vector<Patc
     gm.patches = (int)xs.size();
     if (gm.patches < 3) return gm;
     double a[6];
-    if (!robust_affine(xs, ys, us, vs, gm.cx, gm.cy, std::max(3, (gm.patches + 1) / 2), 2.0, a)) return gm;
+    if (!robust_affine(xs, ys, us, vs, gm.cx, gm.cy, std::max(3, (gm.patches + 1) / 2), 2.0, a) &&
+        !consensus_affine(xs, ys, us, vs, gm.cx, gm.cy, a))
+        return gm;
     std::copy(a, a + 6, gm.a);
     gm.locked = true;
     gm.t_u = a[0];

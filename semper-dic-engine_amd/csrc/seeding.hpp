@@ -14,8 +14,10 @@
 namespace sdic {
 
 int lattice_count(int n, int stride);
-// plan_anchor_lattice (full_field_anchors.cpp:73-120)
-void plan_lattice(int gw, int gh, int& sx, int& sy);
+// plan_anchor_lattice (full_field_anchors.cpp:73-120) for about `target`
+// anchors (the reference's kAnchorTarget, 256, unless the learned seeder asks
+// for a denser lattice)
+void plan_lattice(int gw, int gh, int& sx, int& sy, int target = kAnchorTarget);
 
 // median_inplace (src/util/median.hpp:21-30)
 float median_of(float* v, size_t n);
@@ -36,6 +38,11 @@ double hull_area(std::vector<std::pair<double, double>> p);
 // least squares with iterated rejection at `tol` px; false below min_pts.
 bool robust_affine(const std::vector<double>& x, const std::vector<double>& y, const std::vector<double>& u,
                    const std::vector<double>& v, double cx, double cy, int min_pts, double tol, double a[6]);
+
+// The same model from a consensus of random triples (tolerance 3 px, at least
+// max(6, n / 4) patches), refined by robust_affine on the consensus set.
+bool consensus_affine(const std::vector<double>& x, const std::vector<double>& y, const std::vector<double>& u,
+                      const std::vector<double>& v, double cx, double cy, double a[6]);
 
 struct GlobalModel {
     bool locked = false, full = false;

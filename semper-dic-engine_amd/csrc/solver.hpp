@@ -90,7 +90,11 @@ struct PyrDev {
     const float* d[4];
     int w[4], h[4];
     int levels;
+    int coarse_radius;   // shifts searched at the coarsest level
 };
+// Coarse search radii of the global-motion patch search: the first pass, and
+// the wider pass run when the first one's model does not lock.
+constexpr int kCoarseRadius = 8, kCoarseRadiusWide = 16;
 struct PatchOut {
     int sx, sy, ok, pad;
     double best, tstd, xm, xp, ym, yp;

@@ -15,6 +15,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_C = os.path.join(HERE, "libsemper_c.so")
 LIB_DEV = os.path.join(HERE, "libsemper_dev.so")
+# the learned seeder's trained weights (train_seeder.py)
+SEEDER_WEIGHTS = os.path.join(HERE, "seeder_weights.bin")
 
 FLOATS_PER_POINT = 8
 METRICS_LEN = 24
@@ -105,6 +107,12 @@ def load_dev(path=LIB_DEV):
     lib.sdk_image_dimensions.restype = None
     lib.sdk_to_gray8.argtypes = [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP]
     lib.sdk_to_mask8.argtypes = [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP]
+    lib.sdk_seeder_grad.argtypes = [_VP, _VP, _VP, ctypes.c_int, _VP, _VP]
+    lib.sdk_seeder_forward.argtypes = [_VP, _VP, ctypes.c_int, _VP]
+    lib.sdk_seeder_train.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_char_p, _VP,
+                                     ctypes.c_int]
+    lib.sdk_engine_load_seeder.argtypes = [_VP, ctypes.c_char_p]
+    lib.sdk_engine_seed_stats.argtypes = [_VP, _VP]
     return lib
 
 

@@ -326,3 +326,46 @@ def dice_agreement(pts, gold):
         return 0.0, np.inf, np.inf, np.inf, np.inf
     rms = np.sqrt((d ** 2).mean(0))
     return len(d) / len(gold), float(rms[0]), float(rms[1]), float(d[:, 0].mean()), float(np.abs(d).max())
+
+
+# ---- the learned seeder's CPU restatement (oracle/src/seeder.cpp) ---------------
+
+def seeder_params(seed=0):
+    """He-normal conv weights and a 1/sqrt(512) linear layer, small random
+    biases, in the flat layout of csrc/seeder.hpp."""
+    lib = oracle()
+    lib.orc_seeder_param_count.restype = ctypes.c_int
+    n = lib.orc_seeder_param_count()
+    rng = np.random.default_rng(seed)
+    p = np.zeros(n, np.float32)
+    o = 0
+    for l in range(4):
+        cin = 2 if l == 0 else 32
+        nw = 32 * cin * 9
+        p[o:o + nw] = rng.normal(0, np.sqrt(2.0 / (cin * 9)), nw)
+        o += nw
+        p[o:o + 32] = rng.normal(0, 0.05, 32)
+        o += 32
+    p[o:o + 6 * 512] = rng.normal(0, np.sqrt(1.0 / 512), 6 * 512)
+    o += 6 * 512
+    p[o:o + 6] = rng.normal(0, 0.05, 6)
+    return p
+
+
+def seeder_batch(B, seed=0):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(0, 1, (B, 2, 64, 64)).astype(np.float32)
+    t = rng.normal(0, 1, (B, 6)).astype(np.float32)
+    return x, t
+
+
+def oracle_seeder(params, x, t=None):
+    """(loss, y, grads) of the double-precision restatement."""
+    lib = oracle()
+    lib.orc_seeder_pass.restype = ctypes.c_double
+    B = x.shape[0]
+    y = np.zeros((B, 6), np.float32)
+    g = np.zeros_like(params)
+    loss = lib.orc_seeder_pass(P(params), P(np.ascontiguousarray(x)), P(t) if t is not None else None, B, P(y),
+                               P(g) if t is not None else None)
+    return loss, y, g

@@ -1,0 +1,120 @@
+"""The learned seeder on the GPU (no reference counterpart): its forward pass
+and gradient against the double-precision CPU restatement, a short training
+run that must lower the loss, and what the trained network gains over the
+plain method on the learned-seeding workload (a 4096^2 field of eight seeded
+Gaussian bumps of up to +-30 px plus a global rotation of up to 5 degrees)."""
+import ctypes
+import importlib.util
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from helpers import PKG, P, oracle_seeder, seeder_batch, seeder_params
+
+pytestmark = pytest.mark.gpu
+
+
+def _load(name, path):
+    spec = importlib.util.spec_from_file_location(name, path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+@pytest.fixture(scope="module")
+def sa():
+    m = _load("semper_amd", os.path.join(PKG, "semper_amd.py"))
+    if m.load_dev().sdk_device_count() < 1:
+        pytest.skip("no GPU")
+    return m
+
+
+def test_forward_and_gradient_match_the_cpu_restatement(sa):
+    dev = sa.load_dev()
+    params = seeder_params(7)
+    x, t = seeder_batch(8, 9)
+    loss_o, y_o, g_o = oracle_seeder(params, x, t)
+    y = np.zeros((8, 6), np.float32)
+    assert dev.sdk_seeder_forward(P(params), P(x), 8, P(y)) == 0
+    g = np.zeros_like(params)
+    loss = ctypes.c_float(0)
+    assert dev.sdk_seeder_grad(P(params), P(x), P(t), 8, P(g), ctypes.byref(loss)) == 0
+    print(f"loss gpu {loss.value:.6f} cpu {loss_o:.6f}; max |dy| {np.abs(y - y_o).max():.2e}", flush=True)
+    assert np.abs(y - y_o).max() <= 1e-4 * max(1.0, np.abs(y_o).max())
+    assert abs(loss.value - loss_o) <= 1e-5 * max(1.0, loss_o)
+    # per parameter block, relative to the block's largest gradient
+    o = 0
+    for l in range(4):
+        cin = 2 if l == 0 else 32
+        for n in (32 * cin * 9, 32):
+            blk = slice(o, o + n)
+            scale = np.abs(g_o[blk]).max()
+            err = np.abs(g[blk] - g_o[blk]).max()
+            print(f"  layer {l} block of {n}: max |dg| {err:.2e} of {scale:.2e}", flush=True)
+            assert err <= 1e-4 * scale + 1e-7
+            o += n
+    for n in (6 * 512, 6):
+        blk = slice(o, o + n)
+        assert np.abs(g[blk] - g_o[blk]).max() <= 1e-4 * np.abs(g_o[blk]).max() + 1e-7
+        o += n
+
+
+def test_short_training_lowers_the_loss(sa):
+    dev = sa.load_dev()
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "w.bin").encode()
+        losses = (ctypes.c_float * 8)()
+        k = dev.sdk_seeder_train(11, 600, 64, ctypes.c_float(1e-3), path, losses, 8)
+        assert k == 6
+        ls = list(losses)[:k]
+        print("losses", [round(v, 4) for v in ls], flush=True)
+        assert ls[-1] < 0.5 * ls[0]
+        h = dev.sdk_engine_create()
+        assert dev.sdk_engine_load_seeder(h, path) == 0
+        assert dev.sdk_engine_set_flag(h, 5, 1) == 0
+        dev.sdk_engine_destroy(h)
+    h = dev.sdk_engine_create()
+    assert dev.sdk_engine_set_flag(h, 5, 1) == -3   # no seeder loaded
+    dev.sdk_engine_destroy(h)
+
+
+def test_trained_seeder_gains_on_the_learned_seeding_workload(sa):
+    """The committed weights (train_seeder.py's default run: 8000 x 256 pairs)
+    on the learned-seeding workload at full size. The plain method's anchors
+    all fail (the bumps put each anchor's motion beyond IC-GN's reach from the
+    global model, and beyond the +-15 px coarse search) and it reports no
+    point; with the CNN's guesses refined by IC-GN the same solve reports
+    >= 95% of the attempted subsets, within 0.06 px rms of the analytic field,
+    and >= 85% of the CNN-seeded anchors hold without the plain fallback."""
+    dev = sa.load_dev()
+    wl = _load("workloads", os.path.join(PKG, "workloads.py"))
+    name = "c5_4096_bumps"
+    cfg = wl.config(name)
+    ref, dfm = wl.pair(dev, name)
+    prm = np.array(wl.solve_params(name), np.int32)
+    cap = (cfg["w"] // cfg["step"] + 1) * (cfg["h"] // cfg["step"] + 1) * 8
+    res = {}
+    for learned in (0, 1):
+        h = dev.sdk_engine_create()
+        if learned:
+            assert dev.sdk_engine_load_seeder(h, sa.SEEDER_WEIGHTS.encode()) == 0
+            assert dev.sdk_engine_set_flag(h, 5, 1) == 0
+        assert dev.sdk_engine_set_reference(h, P(ref), 0, cfg["w"], cfg["h"], None) == 0
+        out = np.zeros(cap, np.float32)
+        met = np.zeros(24, np.float32)
+        ss = np.zeros(3, np.int32)
+        n = dev.sdk_engine_run(h, P(dfm), 0, P(prm), P(out), cap, P(met), 24, ctypes.c_float(1e-3))
+        dev.sdk_engine_seed_stats(h, P(ss))
+        dev.sdk_engine_destroy(h)
+        res[learned] = (n, out[: max(n, 0) * 8].reshape(-1, 8), met, ss)
+        print(f"learned {learned}: {n} points of {int(met[0])} attempted, anchors {ss.tolist()}", flush=True)
+    assert res[0][0] <= 0.01 * res[0][2][0]
+    n, pts, met, ss = res[1]
+    assert n >= 0.95 * met[0]
+    u, v = wl.displacement(name, pts[:, 0].astype(np.float64), pts[:, 1].astype(np.float64))
+    rms = float(np.sqrt(np.mean((pts[:, 2] - u) ** 2 + (pts[:, 3] - v) ** 2)))
+    print(f"  rms error against the field {rms:.4f} px", flush=True)
+    assert rms <= 0.06
+    assert ss[0] > 0 and ss[1] >= 0.85 * ss[0]
