@@ -68,6 +68,13 @@ SDK_API int sdk_engine_set_flag(void* h, int flag, int value);
    Path B launches, subsets those launches solved. */
 SDK_API int sdk_engine_timings(void* h, double* out14);
 
+/* Mixed subset sizes (no reference counterpart): the subset size of each node
+   of a gw x gh grid (row-major, odd, 3 .. 63) for the engine's runs on a grid
+   of that shape; dims NULL clears the map. Each node is solved with its own
+   size (scan border, Hessian pool, IC-GN); one solver launch per size.
+   0 or -3. */
+SDK_API int sdk_engine_set_subset_map(void* h, int gw, int gh, const int32_t* dims);
+
 /* Synthetic speckle pair (synth.hpp FieldSpec kinds). */
 SDK_API int sdk_synth_pair(uint64_t seed, int w, int h, int kind, const double* p, int np, double cx, double cy,
                            float noise_sigma, uint8_t* ref, uint8_t* def, int on_device);

@@ -439,6 +439,17 @@ SDK_API int sdk_seeder_train(uint64_t seed, int steps, int batch, float lr, cons
     }
 }
 
+SDK_API int sdk_engine_set_subset_map(void* h, int gw, int gh, const int32_t* dims) {
+    if (!h) return -3;
+    try {
+        std::vector<int> d;
+        if (dims && gw > 0 && gh > 0) d.assign(dims, dims + (size_t)gw * gh);
+        return ((DevEngine*)h)->eng.set_subset_map(gw, gh, d);
+    } catch (...) {
+        return -3;
+    }
+}
+
 SDK_API int sdk_engine_load_seeder(void* h, const char* path) {
     if (!h || !path) return -3;
     try {

@@ -182,14 +182,30 @@ int Engine::set_reference(const uint8_t* gray, bool on_device, int w, int h, con
 }
 
 // Grid scan and Hessian pool, cached while rect / step / subset / mask hold.
-int Engine::ensure_grid(const Params& p, const uint8_t* mask, bool& cached) {
+int Engine::set_subset_map(int gw, int gh, const std::vector<int>& dims) {
+    if (dims.empty()) {
+        subset_map_.clear();
+        map_gw_ = map_gh_ = 0;
+        return 0;
+    }
+    if (gw <= 0 || gh <= 0 || dims.size() != (size_t)gw * gh) return kErrInit;
+    for (int d : dims)
+        if (d < 3 || d % 2 == 0 || ppl_for(d) < 0) return kErrInit;
+    subset_map_ = dims;
+    map_gw_ = gw;
+    map_gh_ = gh;
+    return 0;
+}
+
+int Engine::ensure_grid(const Params& p, const uint8_t* mask, const std::vector<int>* dims, bool& cached) {
     const int gw = p.rect_w / p.step, gh = p.rect_h / p.step;
     const size_t n = (size_t)w_ * h_;
     const bool mask_same = (!mask && grid_mask_.empty()) ||
                            (mask && grid_mask_.size() == n && std::memcmp(grid_mask_.data(), mask, n) == 0);
     cached = grid_valid_ && grid_p_.rect_x == p.rect_x && grid_p_.rect_y == p.rect_y &&
              grid_p_.rect_w == p.rect_w && grid_p_.rect_h == p.rect_h && grid_p_.step == p.step &&
-             grid_p_.subset == p.subset && mask_same && gw_ == gw && gh_ == gh && grid_prec_ == prec;
+             grid_p_.subset == p.subset && mask_same && gw_ == gw && gh_ == gh && grid_prec_ == prec &&
+             (dims ? grid_dims_ == *dims : grid_dims_.empty());
     if (cached) return 0;
     grid_valid_ = false;
     const int N = gw * gh;
@@ -200,8 +216,31 @@ int Engine::ensure_grid(const Params& p, const uint8_t* mask, bool& cached) {
         ck(hipMemcpyAsync(mask_d_.p, mask, n, hipMemcpyHostToDevice, st_));
         md = mask_d_.p;
     }
+    const int32_t* dd = nullptr;
+    classes_.clear();
+    if (dims) {
+        // per-node sizes, and the node lists of each size for Path A
+        if (!dims_d_.reserve(N)) return kErrInit;
+        ck(hipMemcpyAsync(dims_d_.p, dims->data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, st_));
+        dd = dims_d_.p;
+        std::vector<int> sizes(dims->begin(), dims->end());
+        std::sort(sizes.begin(), sizes.end());
+        sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
+        std::vector<int32_t> lists;
+        for (int d : sizes) {
+            const int off = (int)lists.size();
+            for (int i = 0; i < N; ++i)
+                if ((*dims)[i] == d) lists.push_back(i);
+            classes_.push_back(DimClass{d, off, (int)lists.size() - off});
+        }
+        if (!class_nodes_d_.reserve(lists.size())) return kErrInit;
+        ck(hipMemcpyAsync(class_nodes_d_.p, lists.data(), sizeof(int32_t) * lists.size(), hipMemcpyHostToDevice,
+                          st_));
+    } else {
+        classes_.push_back(DimClass{p.subset, -1, N});
+    }
     ck(hipMemsetAsync(counters_.p, 0, sizeof(int32_t), st_));
-    ck(launch_scan(md, w_, h_, p.rect_x, p.rect_y, p.step, gw, gh, p.subset, skip_d_.p, counters_.p, st_));
+    ck(launch_scan(md, w_, h_, p.rect_x, p.rect_y, p.step, gw, gh, p.subset, skip_d_.p, counters_.p, st_, dd));
     int total = 0;
     skip_h_.resize(N);
     ck(hipMemcpyAsync(&total, counters_.p, sizeof(int), hipMemcpyDeviceToHost, st_));
@@ -211,6 +250,7 @@ int Engine::ensure_grid(const Params& p, const uint8_t* mask, bool& cached) {
     if (total == 0) return kErrRoi;
     PoolOpts po;
     po.mfma = prec.mfma_pool;
+    po.dims = dd;
     if (prec.grad16) {
         if (!g16_ready_) {
             if (!gx16_.reserve(n) || !gy16_.reserve(n)) return kErrInit;
@@ -230,6 +270,7 @@ int Engine::ensure_grid(const Params& p, const uint8_t* mask, bool& cached) {
                    pool_.p, pool_raw_.p, st_, po));
     ck(hipStreamSynchronize(st_));
     grid_prec_ = prec;
+    grid_dims_ = dims ? *dims : std::vector<int>();
     grid_p_ = p;
     grid_mask_ = mask ? std::vector<uint8_t>(mask, mask + n) : std::vector<uint8_t>();
     gw_ = gw;
@@ -252,6 +293,28 @@ void Engine::set_precision_args(SolveArgs& a) const {
 }
 
 int Engine::solve_tasks(const std::vector<Task>& tasks, std::vector<SolveOut>& out, bool use6, int dim, float tol) {
+    if (grid_dims_.empty()) return solve_tasks_dim(tasks, out, use6, dim, tol);
+    // mixed sizes: one launch per size among the tasks' nodes
+    out.assign(tasks.size(), SolveOut{});
+    for (const DimClass& c : classes_) {
+        std::vector<Task> sub;
+        std::vector<size_t> at;
+        for (size_t k = 0; k < tasks.size(); ++k)
+            if (tasks[k].pool >= 0 && (size_t)tasks[k].pool < grid_dims_.size() && grid_dims_[tasks[k].pool] == c.dim) {
+                sub.push_back(tasks[k]);
+                at.push_back(k);
+            }
+        if (sub.empty()) continue;
+        std::vector<SolveOut> r;
+        const int rc = solve_tasks_dim(sub, r, use6, c.dim, tol);
+        if (rc != 0) return rc;
+        for (size_t i = 0; i < at.size(); ++i) out[at[i]] = r[i];
+    }
+    return 0;
+}
+
+int Engine::solve_tasks_dim(const std::vector<Task>& tasks, std::vector<SolveOut>& out, bool use6, int dim,
+                            float tol) {
     const int n = (int)tasks.size();
     out.assign(n, SolveOut{});
     if (n == 0) return 0;
@@ -401,7 +464,8 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
 
         const auto t_pre = clk::now();
         bool cached = false;
-        const int rg = ensure_grid(p, mask, cached);
+        const bool mixed = !subset_map_.empty() && map_gw_ == gw && map_gh_ == gh;
+        const int rg = ensure_grid(p, mask, mixed ? &subset_map_ : nullptr, cached);
         if (rg != 0) return rg;
         tm_.prepass = cached ? 0.0 : ms_since(t_pre);
         const int total_valid = total_valid_;
@@ -859,7 +923,6 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             a.grid.rect_x = p.rect_x;
             a.grid.rect_y = p.rect_y;
             a.grid.step = p.step;
-            a.grid.dim = p.subset;
             a.grid.w = w_;
             a.grid.h = h_;
             a.pool = pool_.p;
@@ -870,11 +933,17 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             a.def = def_u8_.p;
             a.w = w_;
             a.h = h_;
-            a.dim = p.subset;
             a.lm = 1;
             a.tol = tol;
             ck(hipEventRecord(ev0_, st_));
-            ck(launch_solve(a, p.use6, false, st_));
+            // one launch per subset size (one in all unless sizes are mixed)
+            for (const DimClass& c : classes_) {
+                a.items = c.off >= 0 ? class_nodes_d_.p + c.off : nullptr;
+                a.nitems = c.n;
+                a.dim = c.dim;
+                a.grid.dim = c.dim;
+                ck(launch_solve(a, p.use6, false, st_));
+            }
             ck(hipEventRecord(ev1_, st_));
             int sa[5];
             ck(hipMemcpyAsync(sa, counters_.p, sizeof(sa), hipMemcpyDeviceToHost, st_));

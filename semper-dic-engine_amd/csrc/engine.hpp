@@ -159,6 +159,12 @@ public:
     // fails), and an anchor whose refined solve fails is solved again the
     // plain way.
     bool learned_seed = false;
+    // Mixed subset sizes (no reference counterpart): the subset size of each
+    // node of a gw x gh grid (row-major), used by runs on a grid of that shape
+    // (any other shape, or an empty map: the run's subset everywhere). Each
+    // size odd, 3 .. 63. The reference's semantics apply node by node with
+    // the node's own size. 0 or -3.
+    int set_subset_map(int gw, int gh, const std::vector<int>& dims);
     int load_seeder(const std::string& path);
     bool seeder_ready() const { return seeder_ && seeder_->ready(); }
     struct SeedStats {
@@ -174,7 +180,7 @@ public:
                     int* dropped_outlier);
 
 private:
-    int ensure_grid(const Params& p, const uint8_t* mask, bool& cached);
+    int ensure_grid(const Params& p, const uint8_t* mask, const std::vector<int>* dims, bool& cached);
     int solve_tasks(const std::vector<Task>& tasks, std::vector<SolveOut>& out, bool use6, int dim, float tol);
 
     hipStream_t st_ = nullptr;
@@ -196,6 +202,16 @@ private:
     DevBuf<uint8_t> pool_raw_;      // per-node sums between the two pool launches
     Precision grid_prec_;
     std::unique_ptr<Seeder> seeder_;
+    std::vector<int> subset_map_;
+    int map_gw_ = 0, map_gh_ = 0;
+    std::vector<int> grid_dims_;        // the grid cache's per-node sizes (empty: uniform)
+    DevBuf<int32_t> dims_d_;
+    struct DimClass {
+        int dim, off, n;
+    };
+    std::vector<DimClass> classes_;     // Path A launches: nodes of one size each
+    DevBuf<int32_t> class_nodes_d_;
+    int solve_tasks_dim(const std::vector<Task>& tasks, std::vector<SolveOut>& out, bool use6, int dim, float tol);
     DevBuf<uint16_t> gx16_, gy16_;  // fp16 gradient planes (Precision::grad16)
     bool g16_ready_ = false;
     DevBuf<double> chol_;           // FP64 Cholesky factors (Precision::chol64)

@@ -52,11 +52,12 @@ __global__ void k_prep_ref(const T* __restrict__ img, const uint8_t* __restrict_
 // One thread per grid node: skip = inside the boundary buffer d/2 + 4 + 15,
 // or any subset pixel masked (< 128).
 __global__ void k_scan(const uint8_t* __restrict__ mask, int w, int h, int rect_x, int rect_y, int step,
-                       int gw, int gh, int dim, uint8_t* __restrict__ skip, int* count) {
+                       int gw, int gh, int dim, const int* __restrict__ dims, uint8_t* __restrict__ skip,
+                       int* count) {
     const int node = blockIdx.x * blockDim.x + threadIdx.x;
     if (node >= gw * gh) return;
     const int rx = rect_x + (node % gw) * step, ry = rect_y + (node / gw) * step;
-    const int half = dim / 2, buf = half + 4 + 15;
+    const int half = (dims ? dims[node] : dim) / 2, buf = half + 4 + 15;
     bool s = rx - buf < 0 || rx + buf >= w || ry - buf < 0 || ry + buf >= h;
     if (!s && mask) {
         for (int dy = -half; dy <= half && !s; ++dy) {
@@ -255,14 +256,16 @@ struct alignas(16) PoolRaw {
 template <bool MFMA>
 __global__ __launch_bounds__(64) void k_pool_sums(const float* __restrict__ I, GradRead G, int w, int h,
                                                   const uint8_t* __restrict__ skip, int gw, int gh, int rect_x,
-                                                  int rect_y, int step, int dim, PoolRaw* __restrict__ raw) {
+                                                  int rect_y, int step, int dim, const int* __restrict__ dims,
+                                                  PoolRaw* __restrict__ raw) {
     __shared__ float lds36[36];
     const int node = xcd_remap(blockIdx.x, gridDim.x);
     if (node >= gw * gh) return;
     const int lane = threadIdx.x & 63;
     float hh[21], mean = 0.0f, inv = 1.0f;
     const bool ok = !skip[node] && pool_sums<MFMA>(I, G, w, h, rect_x + (node % gw) * step,
-                                                   rect_y + (node / gw) * step, dim, lds36, hh, mean, inv);
+                                                   rect_y + (node / gw) * step, dims ? dims[node] : dim, lds36, hh,
+                                                   mean, inv);
     float v = 0.0f;
     if (ok) {
 #pragma unroll
@@ -276,10 +279,12 @@ __global__ __launch_bounds__(64) void k_pool_sums(const float* __restrict__ I, G
 }
 
 template <bool CHOL>
-__global__ __launch_bounds__(256) void k_pool_tail(const PoolRaw* __restrict__ raw, int n, int half, int lm,
-                                                   PoolEntry* __restrict__ pool, double* __restrict__ chol) {
+__global__ __launch_bounds__(256) void k_pool_tail(const PoolRaw* __restrict__ raw, int n, int dim,
+                                                   const int* __restrict__ dims, int lm, PoolEntry* __restrict__ pool,
+                                                   double* __restrict__ chol) {
     const int node = blockIdx.x * blockDim.x + threadIdx.x;
     if (node >= n) return;
+    const int half = (dims ? dims[node] : dim) / 2;
     PoolEntry* dst = pool + node;
     if (!raw[node].ok) {
         dst->valid = 0;
@@ -524,10 +529,10 @@ hipError_t launch_prep_ref_f32(const float* img, int w, int h, float* I, float* 
 }
 
 hipError_t launch_scan(const uint8_t* mask, int w, int h, int rect_x, int rect_y, int step, int gw, int gh,
-                       int dim, uint8_t* skip, int* count, hipStream_t st) {
+                       int dim, uint8_t* skip, int* count, hipStream_t st, const int* dims) {
     const int n = gw * gh;
     hipLaunchKernelGGL(k_scan, dim3((n + 255) / 256), dim3(256), 0, st, mask, w, h, rect_x, rect_y, step, gw,
-                       gh, dim, skip, count);
+                       gh, dim, dims, skip, count);
     return hipGetLastError();
 }
 
@@ -540,16 +545,16 @@ hipError_t launch_pool(const float* I, const float* gx, const float* gy, int w, 
     PoolRaw* r = static_cast<PoolRaw*>(raw);
     if (o.mfma)
         hipLaunchKernelGGL(k_pool_sums<true>, dim3(n), dim3(64), 0, st, I, G, w, h, skip, gw, gh, rect_x, rect_y,
-                           step, dim, r);
+                           step, dim, o.dims, r);
     else
         hipLaunchKernelGGL(k_pool_sums<false>, dim3(n), dim3(64), 0, st, I, G, w, h, skip, gw, gh, rect_x, rect_y,
-                           step, dim, r);
+                           step, dim, o.dims, r);
     if (o.chol)
-        hipLaunchKernelGGL(k_pool_tail<true>, dim3((n + 255) / 256), dim3(256), 0, st, r, n, dim / 2, lm ? 1 : 0,
-                           pool, o.chol);
+        hipLaunchKernelGGL(k_pool_tail<true>, dim3((n + 255) / 256), dim3(256), 0, st, r, n, dim, o.dims,
+                           lm ? 1 : 0, pool, o.chol);
     else
-        hipLaunchKernelGGL(k_pool_tail<false>, dim3((n + 255) / 256), dim3(256), 0, st, r, n, dim / 2, lm ? 1 : 0,
-                           pool, nullptr);
+        hipLaunchKernelGGL(k_pool_tail<false>, dim3((n + 255) / 256), dim3(256), 0, st, r, n, dim, o.dims,
+                           lm ? 1 : 0, pool, nullptr);
     return hipGetLastError();
 }
 

@@ -27,6 +27,7 @@ struct PoolOpts {
     const uint16_t* gx16 = nullptr;
     const uint16_t* gy16 = nullptr;
     double* chol = nullptr;
+    const int* dims = nullptr;   // per-node subset sizes (mixed subsets; else `dim`)
 };
 
 // Arguments of the subset solver kernel. Task mode when tasks != null (one
@@ -66,7 +67,7 @@ hipError_t launch_prep_ref_u8(const uint8_t* img, const uint8_t* mask, int w, in
 hipError_t launch_prep_ref_f32(const float* img, int w, int h, float* I, float* gx, float* gy,
                                hipStream_t st);
 hipError_t launch_scan(const uint8_t* mask, int w, int h, int rect_x, int rect_y, int step, int gw, int gh,
-                       int dim, uint8_t* skip, int* count, hipStream_t st);
+                       int dim, uint8_t* skip, int* count, hipStream_t st, const int* dims = nullptr);
 hipError_t launch_pool(const float* I, const float* gx, const float* gy, int w, int h, const uint8_t* skip,
                        int gw, int gh, int rect_x, int rect_y, int step, int dim, bool lm, PoolEntry* pool,
                        void* raw, hipStream_t st, const PoolOpts& opts = PoolOpts());

@@ -113,6 +113,7 @@ def load_dev(path=LIB_DEV):
                                      ctypes.c_int]
     lib.sdk_engine_load_seeder.argtypes = [_VP, ctypes.c_char_p]
     lib.sdk_engine_seed_stats.argtypes = [_VP, _VP]
+    lib.sdk_engine_set_subset_map.argtypes = [_VP, ctypes.c_int, ctypes.c_int, _VP]
     return lib
 
 

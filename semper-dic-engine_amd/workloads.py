@@ -52,6 +52,11 @@ CONFIGS = {
     # Path B flood has real work
     "t_1024_noisy": dict(w=1024, h=1024, seed=777, kind=SIN_GAUSS, p=[3.0, 512.0, 384.0, 2.0, 300.0], step=8,
                          subset=21, window=31, tol=1e-3, noise=9.0),
+    # 1024^2 frame of the batched / incremental sequence at 10% stretch
+    # (exx 0.1, Poisson 0.3), step 8, window 21, subset sizes 15 / 21 / 31 /
+    # 41 assigned to a 4 x 4 region map drawn from region_seed
+    "c3_1024_mixed": dict(w=1024, h=1024, seed=3003, kind=UNIAXIAL, p=[0.1, 0.3], step=8, subset=21, window=21,
+                          tol=1e-3, sizes=(15, 21, 31, 41), region_seed=30),
     # 4096^2 bumps + rotation (the learned-seeding workload)
     "c5_4096_bumps": dict(w=4096, h=4096, seed=5005, kind=BUMPS, p=None, step=8, subset=31, window=31, tol=1e-3),
 }
@@ -85,6 +90,17 @@ def pair(dev, name, on_device=False):
     return (r, d) if on_device else (ra, da)
 
 
+def subset_map(name, gw, gh):
+    """Per-node subset sizes of a mixed-size workload: a 4 x 4 map of regions
+    over the grid, each region's size drawn (seeded) from the workload's sizes;
+    row-major gw x gh int32."""
+    c = config(name)
+    rng = np.random.default_rng(c["region_seed"])
+    regions = rng.choice(np.array(c["sizes"], np.int32), size=(4, 4))
+    gy, gx = np.mgrid[0:gh, 0:gw]
+    return np.ascontiguousarray(regions[gy * 4 // gh, gx * 4 // gw], np.int32)
+
+
 def solve_params(name):
     c = config(name)
     return (0, 0, c["w"], c["h"], c["step"], c["subset"], c["window"], 0)
@@ -111,4 +127,6 @@ def displacement(name, x, y):
             u = u + au * g
             v = v + av * g
         return u, v
+    if c["kind"] == UNIAXIAL:
+        return p[0] * dx, -p[1] * p[0] * dy
     raise ValueError(name)
