@@ -74,6 +74,18 @@ PRECISION_VARIANTS = [("fp64_cholesky_step", (2,)), ("fp16_gradients", (3,)), ("
                       ("all_three", (2, 3, 4))]
 
 
+def settle_tuner(dev, eng, run_once, limit=16):
+    """Untimed runs until the engine's launch-shape tuner has settled (it
+    times each shape on the first runs of a configuration)."""
+    shape = np.zeros(2, np.int32)
+    shape_ms = np.zeros(6, np.float64)
+    for _ in range(limit):
+        if dev.sdk_engine_tuning(eng, shape.ctypes.data_as(ctypes.c_void_p),
+                                 shape_ms.ctypes.data_as(ctypes.c_void_p)) in (0, -3):
+            return
+        run_once()
+
+
 def precision_study(dev, dref, ddef, prm, cap, tol, ref, dfm, subset, use6, steps):
     """The precision variants (FP64 Cholesky Gauss-Newton step, fp16 gradient
     storage, J^T J on MFMA): per variant the first run's Hessian pre-pass (the
@@ -94,6 +106,7 @@ def precision_study(dev, dref, ddef, prm, cap, tol, ref, dfm, subset, use6, step
         n = dev.sdk_engine_run(eng, ddef, 1, P(prm), P(out), cap, P(met), 24, tol)
         dev.sdk_engine_timings(eng, P(tm))
         prepass = float(tm[3])
+        settle_tuner(dev, eng, lambda: dev.sdk_engine_run(eng, ddef, 1, P(prm), P(out), cap, P(met), 24, tol))
         t0 = time.perf_counter()
         for _ in range(steps):
             n = dev.sdk_engine_run(eng, ddef, 1, P(prm), P(out), cap, P(met), 24, tol)
@@ -186,6 +199,7 @@ def learned_seed_study(dev, sa, wl, steps, threads):
         out = np.zeros(cap, np.float32)
         met = np.zeros(24, np.float32)
         n = dev.sdk_engine_run(eng, ddef, 1, P(prm), P(out), cap, P(met), 24, tol)   # warm (pool, pyramids)
+        settle_tuner(dev, eng, lambda: dev.sdk_engine_run(eng, ddef, 1, P(prm), P(out), cap, P(met), 24, tol))
         t0 = time.perf_counter()
         for _ in range(steps):
             n = dev.sdk_engine_run(eng, ddef, 1, P(prm), P(out), cap, P(met), 24, tol)
@@ -223,7 +237,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=14,
+                    help="untimed runs; the launch-shape tuner times its 6 shapes twice each in the first 12")
     ap.add_argument("--workload", default="c4_4096_heterogeneous")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and FP64 check")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -294,6 +309,14 @@ def main():
     n = dev.sdk_engine_run(eng, ddef, 1, P(prm), P(out), cap, P(met), 24, tol)
     solved = float(met[3] + met[4] + met[19])
     n = int(n)
+    shape = np.zeros(2, np.int32)
+    shape_ms = np.zeros(6, np.float64)
+    launch_shape = None
+    tune_rc = dev.sdk_engine_tuning(eng, P(shape), P(shape_ms))
+    if tune_rc >= 0:
+        launch_shape = {"subsets_per_workgroup": int(shape[0]), "tile": int(shape[1]), "settled": tune_rc == 0,
+                        "candidates_ms": {f"{a}x{b}": round(float(t), 4) for (a, b), t in zip(
+                            [(1, 0), (2, 0), (4, 0), (1, 8), (2, 8), (4, 8)], shape_ms) if t >= 0}}
     n_sub = cfg["subset"] ** 2
     # dominant kernel: the Path A solver launch over the grid
     avg_kms = float(np.mean(kern_ms))
@@ -343,7 +366,7 @@ def main():
                      "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": TRAFFIC_BYTES.get(args.workload), "traffic_unit": "bytes per launch",
                      "kernel": "Path A subset solver launch (k_fast + k_slow)", "avg_launch_ms": avg_kms,
-                     "iterations_per_launch": float(np.mean(kern_iters))},
+                     "iterations_per_launch": float(np.mean(kern_iters)), "launch_shape": launch_shape},
         "cpu_baseline": cpu,
         "accuracy_vs_fp64": acc,
         "parity_vs_oracle": parity,

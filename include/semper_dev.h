@@ -59,7 +59,15 @@ SDK_API int sdk_wave_sum_probe(const float* in, int nw, float* out, int32_t* iou
    by FP64 Cholesky of each pool entry, flag 3 = fp16 storage of the reference
    gradient planes, flag 4 = J^T J of the pool on MFMA (fp16 inputs, FP32
    accumulation); flag 5 = learned anchor seeding (needs a seeder loaded by
-   sdk_engine_load_seeder; -3 otherwise). */
+   sdk_engine_load_seeder; -3 otherwise); flag 6 = the launch-shape tuner
+   (default 1): the first runs of each (grid, subset size, interpolator, node
+   count) launch Path A in each shape in turn -- 1 / 2 / 4 subsets per
+   workgroup, row-major / 8 x 8-tile node order -- two runs per shape, timed
+   by device events (the field does not depend on the shape), and later runs
+   use the shape of least mean time; flag 7 = 1..6 forces candidate value - 1
+   of the order given at sdk_engine_tuning on every Path A launch (where only
+   whole grids take tiles, a listed launch takes the same workgroup width
+   row-major; 0: the tuner decides). */
 SDK_API int sdk_engine_set_flag(void* h, int flag, int value);
 
 /* Phase timings of the last run, 14 doubles: total, prep, seeding, prepass,
@@ -74,6 +82,13 @@ SDK_API int sdk_engine_timings(void* h, double* out14);
    size (scan border, Hessian pool, IC-GN); one solver launch per size.
    0 or -3. */
 SDK_API int sdk_engine_set_subset_map(void* h, int gw, int gh, const int32_t* dims);
+
+/* The tuner's latest timing: chosen2 = {subsets per workgroup, tile side (0:
+   row-major)} of least mean time so far, ms6 = the mean Path A launch time
+   per candidate in the order (1, 0) (2, 0) (4, 0) (1, 8) (2, 8) (4, 8), -1
+   where not timed. 0 once the choice is settled, 1 while still timing, -3
+   when nothing has been timed. */
+SDK_API int sdk_engine_tuning(void* h, int32_t* chosen2, double* ms6);
 
 /* Synthetic speckle pair (synth.hpp FieldSpec kinds). */
 SDK_API int sdk_synth_pair(uint64_t seed, int w, int h, int kind, const double* p, int np, double cx, double cy,

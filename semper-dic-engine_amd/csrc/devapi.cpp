@@ -261,6 +261,13 @@ SDK_API int sdk_engine_set_flag(void* h, int flag, int value) {
             if (value && !((DevEngine*)h)->eng.seeder_ready()) return -3;
             ((DevEngine*)h)->eng.learned_seed = value != 0;
             return 0;
+        case 6:
+            ((DevEngine*)h)->eng.autotune = value != 0;
+            return 0;
+        case 7:
+            if (value < 0 || value > 6) return -3;
+            ((DevEngine*)h)->eng.force_shape = value;
+            return 0;
         default:
             return -3;
     }
@@ -437,6 +444,16 @@ SDK_API int sdk_seeder_train(uint64_t seed, int steps, int batch, float lr, cons
     } catch (...) {
         return -3;
     }
+}
+
+SDK_API int sdk_engine_tuning(void* h, int32_t* chosen2, double* ms6) {
+    if (!h || !chosen2 || !ms6) return -3;
+    const auto& r = ((DevEngine*)h)->eng.last_tune;
+    if (!r.have) return -3;
+    chosen2[0] = r.chosen.wpb;
+    chosen2[1] = r.chosen.tile;
+    for (int i = 0; i < 6; ++i) ms6[i] = r.ms[i];
+    return r.settled ? 0 : 1;
 }
 
 SDK_API int sdk_engine_set_subset_map(void* h, int gw, int gh, const int32_t* dims) {

@@ -138,6 +138,7 @@ Engine::~Engine() {
     if (st_) (void)hipStreamSynchronize(st_);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
+    for (hipEvent_t e : class_ev_) (void)hipEventDestroy(e);
     if (st_) (void)hipStreamDestroy(st_);
 }
 
@@ -182,6 +183,45 @@ int Engine::set_reference(const uint8_t* gray, bool on_device, int w, int h, con
 }
 
 // Grid scan and Hessian pool, cached while rect / step / subset / mask hold.
+namespace {
+const Engine::Tune kShapes[6] = {{1, 0}, {2, 0}, {4, 0}, {1, 8}, {2, 8}, {4, 8}};
+int shape_count(bool listed) { return listed ? 3 : 6; }   // tiles order the whole grid only
+}  // namespace
+
+int Engine::shape_for(const std::vector<int>& key, bool listed) {
+    if (force_shape >= 1 && force_shape <= shape_count(listed)) return force_shape - 1;
+    if (force_shape > shape_count(listed)) return force_shape - 4;   // the same shape, row-major
+    if (!autotune) return 0;
+    const TuneState& t = tuned_[key];
+    return t.chosen >= 0 ? t.chosen : t.next % shape_count(listed);
+}
+
+void Engine::shape_timed(const std::vector<int>& key, bool listed, int cand, float ms) {
+    if (!autotune || force_shape != 0) return;
+    TuneState& t = tuned_[key];
+    if (t.chosen >= 0) return;
+    t.samples[cand].push_back(ms);
+    ++t.next;
+    const int nc = shape_count(listed);
+    TuneReport rep;
+    rep.have = true;
+    rep.settled = t.next >= nc * kTuneSamples;
+    double best = 1e30;
+    for (int i = 0; i < 6; ++i) {
+        rep.ms[i] = -1.0;
+        if (i >= nc || t.samples[i].empty()) continue;
+        double sum = 0.0;
+        for (float v : t.samples[i]) sum += v;
+        rep.ms[i] = sum / (double)t.samples[i].size();
+        if (rep.ms[i] < best) {
+            best = rep.ms[i];
+            rep.chosen = kShapes[i];
+            if (rep.settled) t.chosen = i;
+        }
+    }
+    last_tune = rep;
+}
+
 int Engine::set_subset_map(int gw, int gh, const std::vector<int>& dims) {
     if (dims.empty()) {
         subset_map_.clear();
@@ -935,21 +975,41 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             a.h = h_;
             a.lm = 1;
             a.tol = tol;
-            ck(hipEventRecord(ev0_, st_));
-            // one launch per subset size (one in all unless sizes are mixed)
-            for (const DimClass& c : classes_) {
-                a.items = c.off >= 0 ? class_nodes_d_.p + c.off : nullptr;
+            // one launch per subset size (one in all unless sizes are mixed),
+            // each in the shape the tuner gives it and timed on its own
+            const size_t nc = classes_.size();
+            while (class_ev_.size() < nc + 1) {
+                hipEvent_t e;
+                ck(hipEventCreate(&e));
+                class_ev_.push_back(e);
+            }
+            std::vector<std::vector<int>> keys(nc);
+            std::vector<int> shapes(nc);
+            ck(hipEventRecord(class_ev_[0], st_));
+            for (size_t ci = 0; ci < nc; ++ci) {
+                const DimClass& c = classes_[ci];
+                const bool listed = c.off >= 0;
+                keys[ci] = {gw, gh, c.dim, p.use6 ? 1 : 0, listed ? 1 : 0, c.n};
+                shapes[ci] = shape_for(keys[ci], listed);
+                a.items = listed ? class_nodes_d_.p + c.off : nullptr;
                 a.nitems = c.n;
                 a.dim = c.dim;
                 a.grid.dim = c.dim;
+                a.wpb = kShapes[shapes[ci]].wpb;
+                a.tile = listed ? 0 : kShapes[shapes[ci]].tile;
                 ck(launch_solve(a, p.use6, false, st_));
+                ck(hipEventRecord(class_ev_[ci + 1], st_));
             }
-            ck(hipEventRecord(ev1_, st_));
             int sa[5];
             ck(hipMemcpyAsync(sa, counters_.p, sizeof(sa), hipMemcpyDeviceToHost, st_));
             ck(hipStreamSynchronize(st_));
             float kms = 0;
-            ck(hipEventElapsedTime(&kms, ev0_, ev1_));
+            ck(hipEventElapsedTime(&kms, class_ev_[0], class_ev_[nc]));
+            for (size_t ci = 0; ci < nc; ++ci) {
+                float cms = 0;
+                ck(hipEventElapsedTime(&cms, class_ev_[ci], class_ev_[ci + 1]));
+                shape_timed(keys[ci], classes_[ci].off >= 0, shapes[ci], cms);
+            }
             tm_.kernel_path_a = kms;
             tm_.path_a_items = N;
             st_a.points = sa[0];

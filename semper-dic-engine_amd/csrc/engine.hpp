@@ -11,6 +11,7 @@
 #include <atomic>
 #include <cstdint>
 #include <functional>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -165,6 +166,25 @@ public:
     // size odd, 3 .. 63. The reference's semantics apply node by node with
     // the node's own size. 0 or -3.
     int set_subset_map(int gw, int gh, const std::vector<int>& dims);
+    // Launch-shape tuner (no reference counterpart). A shape -- 1, 2 or 4
+    // subsets per fast-path workgroup, row-major or 8 x 8-tile node order --
+    // changes which wave solves which subset, never a subset's arithmetic, so
+    // the first runs of each (grid, subset size, interpolator, item count)
+    // rotate through the shapes as ordinary launches, kTuneSamples each, timed
+    // in place by device events; from then on the shape with the least mean
+    // time is used.
+    static constexpr int kTuneSamples = 2;
+    bool autotune = true;
+    int force_shape = 0;   // 1..6: use that candidate everywhere (tests), 0: tune
+    struct Tune {
+        int wpb = 1, tile = 0;
+    };
+    struct TuneReport {
+        bool have = false, settled = false;
+        Tune chosen;
+        double ms[6] = {0, 0, 0, 0, 0, 0};   // mean per candidate, -1 when not (yet) timed
+    };
+    TuneReport last_tune;
     int load_seeder(const std::string& path);
     bool seeder_ready() const { return seeder_ && seeder_->ready(); }
     struct SeedStats {
@@ -212,6 +232,17 @@ private:
     std::vector<DimClass> classes_;     // Path A launches: nodes of one size each
     DevBuf<int32_t> class_nodes_d_;
     int solve_tasks_dim(const std::vector<Task>& tasks, std::vector<SolveOut>& out, bool use6, int dim, float tol);
+    struct TuneState {
+        int next = 0;                      // runs so far
+        int chosen = -1;                   // settled candidate
+        std::vector<float> samples[6];
+    };
+    std::map<std::vector<int>, TuneState> tuned_;
+    std::vector<hipEvent_t> class_ev_;     // one event before and after each class launch
+    // The candidate for the next Path A launch of a size class (index into the
+    // candidate list), and the report of a finished timing.
+    int shape_for(const std::vector<int>& key, bool listed);
+    void shape_timed(const std::vector<int>& key, bool listed, int cand, float ms);
     DevBuf<uint16_t> gx16_, gy16_;  // fp16 gradient planes (Precision::grad16)
     bool g16_ready_ = false;
     DevBuf<double> chol_;           // FP64 Cholesky factors (Precision::chol64)

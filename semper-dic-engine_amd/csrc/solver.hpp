@@ -51,6 +51,9 @@ struct SolveArgs {
     const void* def;          // uint8 or float plane
     int w, h, dim, lm;
     float tol;
+    // launch shape: subsets (waves) per fast-path workgroup (0 or 1: one),
+    // and Path A's node order (0: row-major; T > 0: T x T tiles)
+    int wpb, tile;
 };
 
 // Pixels-per-lane class of a subset size (-1: larger than kMaxSubset).

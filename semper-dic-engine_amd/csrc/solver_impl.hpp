@@ -1159,13 +1159,37 @@ __device__ __forceinline__ void commit(const SolveArgs& a, int id, const SolveOu
     }
 }
 
-// Fast path, one wave per work item; items it cannot finish are appended to
-// a.bail for k_slow.
+// Grid order of Path A's work items in tiles of T x T nodes (row-major tiles,
+// row-major within a tile; the last tile row and column narrower): a
+// bijection of 0 .. gw gh - 1, so that the consecutive items a workgroup band
+// takes on one XCD are squares of nodes sharing their subsets' rows.
+__device__ __forceinline__ int tile_order(int k, int gw, int gh, int T) {
+    const int band = T * gw;
+    const int ty = k / band, r = k - ty * band;
+    const int th = min(T, gh - ty * T);
+    const int full = (gw / T) * th * T;
+    int tx, tw, local;
+    if (r < full) {
+        tx = r / (th * T);
+        tw = T;
+        local = r - tx * th * T;
+    } else {
+        tx = gw / T;
+        tw = gw - tx * T;
+        local = r - full;
+    }
+    const int iy = local / tw, ix = local - iy * tw;
+    return (ty * T + iy) * gw + tx * T + ix;
+}
+
+// Fast path, one wave per work item (a.wpb items per workgroup); items it
+// cannot finish are appended to a.bail for k_slow.
 template <int PPL, bool K6, typename TD>
-__global__ __launch_bounds__(64) void k_fast(SolveArgs a) {
-    const int k = xcd_remap(blockIdx.x, gridDim.x);
+__global__ __launch_bounds__(256) void k_fast(SolveArgs a) {
+    const int wpb = (int)(blockDim.x >> 6);
+    const int k = xcd_remap(blockIdx.x, gridDim.x) * wpb + (int)(threadIdx.x >> 6);
     if (k >= a.nitems) return;
-    const int id = a.items ? a.items[k] : k;
+    const int id = a.items ? a.items[k] : (a.tile > 0 && !a.tasks) ? tile_order(k, a.grid.gw, a.grid.gh, a.tile) : k;
     Task t;
     if (!item_task(a, id, t)) return;
     SolveOut o = {};

@@ -114,6 +114,7 @@ def load_dev(path=LIB_DEV):
     lib.sdk_engine_load_seeder.argtypes = [_VP, ctypes.c_char_p]
     lib.sdk_engine_seed_stats.argtypes = [_VP, _VP]
     lib.sdk_engine_set_subset_map.argtypes = [_VP, ctypes.c_int, ctypes.c_int, _VP]
+    lib.sdk_engine_tuning.argtypes = [_VP, _VP, _VP]
     return lib
 
 
