@@ -67,7 +67,11 @@ SDK_API int sdk_wave_sum_probe(const float* in, int nw, float* out, int32_t* iou
    use the shape of least mean time; flag 7 = 1..6 forces candidate value - 1
    of the order given at sdk_engine_tuning on every Path A launch (where only
    whole grids take tiles, a listed launch takes the same workgroup width
-   row-major; 0: the tuner decides). */
+   row-major; 0: the tuner decides); flag 8 = layered Path B (default 0, on
+   by itself with learned seeding): every open node next to an accepted one is
+   claimed at once, seeded by its accepted neighbour of least (corr, y, x),
+   and solved in one launch per layer, instead of the reference's flood of 32
+   pops per round. */
 SDK_API int sdk_engine_set_flag(void* h, int flag, int value);
 
 /* Phase timings of the last run, 14 doubles: total, prep, seeding, prepass,

@@ -268,6 +268,9 @@ SDK_API int sdk_engine_set_flag(void* h, int flag, int value) {
             if (value < 0 || value > 6) return -3;
             ((DevEngine*)h)->eng.force_shape = value;
             return 0;
+        case 8:
+            ((DevEngine*)h)->eng.layered_path_b = value != 0;
+            return 0;
         default:
             return -3;
     }

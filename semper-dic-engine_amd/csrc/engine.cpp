@@ -228,12 +228,31 @@ int Engine::set_subset_map(int gw, int gh, const std::vector<int>& dims) {
         map_gw_ = map_gh_ = 0;
         return 0;
     }
-    if (gw <= 0 || gh <= 0 || dims.size() != (size_t)gw * gh) return kErrInit;
-    for (int d : dims)
-        if (d < 3 || d % 2 == 0 || ppl_for(d) < 0) return kErrInit;
+    if (!size_map_ok(gw, gh, dims)) return kErrInit;
     subset_map_ = dims;
     map_gw_ = gw;
     map_gh_ = gh;
+    return 0;
+}
+
+bool Engine::size_map_ok(int cols, int rows, const std::vector<int>& dims) {
+    if (dims.empty()) return true;
+    if (cols <= 0 || rows <= 0 || dims.size() != (size_t)cols * rows) return false;
+    for (int d : dims)
+        if (d < 3 || d % 2 == 0 || ppl_for(d) < 0) return false;
+    return true;
+}
+
+int Engine::set_region_map(int rw, int rh, const std::vector<int>& dims) {
+    if (!size_map_ok(rw, rh, dims)) return kErrInit;
+    if (dims.empty()) {
+        region_map_.clear();
+        region_w_ = region_h_ = 0;
+        return 0;
+    }
+    region_map_ = dims;
+    region_w_ = rw;
+    region_h_ = rh;
     return 0;
 }
 
@@ -505,7 +524,19 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
         const auto t_pre = clk::now();
         bool cached = false;
         const bool mixed = !subset_map_.empty() && map_gw_ == gw && map_gh_ == gh;
-        const int rg = ensure_grid(p, mask, mixed ? &subset_map_ : nullptr, cached);
+        const std::vector<int>* dims = mixed ? &subset_map_ : nullptr;
+        if (!dims && !region_map_.empty()) {
+            region_dims_.resize((size_t)N);
+            for (int gy = 0; gy < gh; ++gy)
+                for (int gx = 0; gx < gw; ++gx) {
+                    const int64_t x = p.rect_x + (int64_t)gx * p.step, y = p.rect_y + (int64_t)gy * p.step;
+                    const int rx = (int)std::min<int64_t>(x * region_w_ / w_, region_w_ - 1);
+                    const int ry = (int)std::min<int64_t>(y * region_h_ / h_, region_h_ - 1);
+                    region_dims_[(size_t)gy * gw + gx] = region_map_[(size_t)ry * region_w_ + rx];
+                }
+            dims = &region_dims_;
+        }
+        const int rg = ensure_grid(p, mask, dims, cached);
         if (rg != 0) return rg;
         tm_.prepass = cached ? 0.0 : ms_since(t_pre);
         const int total_valid = total_valid_;
@@ -1043,7 +1074,96 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             const float* hvx = huy + NN;
             const float* hvy = hvx + NN;
             const float* hcorr = hvy + NN;
-            {
+            const int dx4[4] = {1, -1, 0, 0}, dy4[4] = {0, 0, 1, -1};
+            bool flooded = false;
+            if (layered_path_b || use_cnn) {
+                // Layered Path B. ok: accepted (solved before Path B with a real corr,
+                // or accepted here); w: their warps, seven planes as hp.
+                std::vector<uint8_t> claimed(hsolved.begin(), hsolved.end()), ok(NN, 0);
+                std::vector<float> w(hp);
+                std::vector<int> front, mark(NN, -1);
+                for (size_t i = 0; i < NN; ++i) ok[i] = hsolved[i] && hcorr[i] >= 0.f ? 1 : 0;
+                auto push_open_neighbours = [&](int j, int layer) {
+                    const int x = j % gw, y = j / gw;
+                    for (int k = 0; k < 4; ++k) {
+                        const int nx = x + dx4[k], ny = y + dy4[k];
+                        if (nx < 0 || nx >= gw || ny < 0 || ny >= gh) continue;
+                        const int m = ny * gw + nx;
+                        if (!claimed[m] && mark[m] != layer) {
+                            mark[m] = layer;
+                            front.push_back(m);
+                        }
+                    }
+                };
+                for (size_t i = 0; i < NN; ++i)
+                    if (ok[i]) push_open_neighbours((int)i, 0);
+                flooded = !front.empty();
+                std::vector<int> commit_nodes;
+                std::vector<SolveOut> commit_res;
+                for (int layer = 0; !front.empty(); ++layer) {
+                    if (cancel.requested()) return kErrCancelled;
+                    std::sort(front.begin(), front.end());
+                    std::vector<Task> tasks(front.size());
+                    for (size_t t = 0; t < front.size(); ++t) {
+                        const int i = front[t], x = i % gw, y = i / gw;
+                        claimed[i] = 1;
+                        int best = -1;
+                        for (int k = 0; k < 4; ++k) {
+                            const int nx = x + dx4[k], ny = y + dy4[k];
+                            if (nx < 0 || nx >= gw || ny < 0 || ny >= gh) continue;
+                            const int j = ny * gw + nx;
+                            if (!ok[j]) continue;
+                            const float cj = w[6 * NN + j], cb = best < 0 ? 0.f : w[6 * NN + best];
+                            if (best < 0 || cj < cb || (cj == cb && (ny < best / gw || (ny == best / gw && nx < best % gw))))
+                                best = j;
+                        }
+                        const float ddx = (float)((x - best % gw) * p.step), ddy = (float)((y - best / gw) * p.step);
+                        Task& tk = tasks[t];
+                        tk = Task{};
+                        tk.cx = p.rect_x + x * p.step;
+                        tk.cy = p.rect_y + y * p.step;
+                        tk.pool = i;
+                        tk.node = i;
+                        tk.mode = NO_SEARCH;
+                        tk.g[0] = w[best] + w[2 * NN + best] * ddx + w[3 * NN + best] * ddy;
+                        tk.g[1] = w[NN + best] + w[4 * NN + best] * ddx + w[5 * NN + best] * ddy;
+                        for (int k = 0; k < 4; ++k) tk.g[2 + k] = w[(2 + k) * NN + best];
+                    }
+                    std::vector<SolveOut> res;
+                    if (solve_tasks(tasks, res, p.use6, p.subset, tol) != 0) return kErrInit;
+                    tm_.path_b_launches++;
+                    tm_.path_b_miss += (int)tasks.size();
+                    front.clear();
+                    for (size_t t = 0; t < tasks.size(); ++t) {
+                        const SolveOut& r = res[t];
+                        if (!r.init) continue;
+                        const int i = tasks[t].node;
+                        st_b.record(r);
+                        commit_nodes.push_back(i);
+                        commit_res.push_back(r);
+                        if (r.status == 0 && r.corr <= kCorrAccept) {
+                            st_b.points++;
+                            ok[i] = 1;
+                            for (int k = 0; k < 6; ++k) w[k * NN + i] = r.p[k];
+                            w[6 * NN + i] = r.corr;
+                        }
+                    }
+                    for (size_t t = 0; t < tasks.size(); ++t)
+                        if (ok[tasks[t].node]) push_open_neighbours(tasks[t].node, layer + 1);
+                    ticker.set(anchor_pts + st_a.points + st_b.points);
+                }
+                if (!commit_nodes.empty()) {
+                    const int n = (int)commit_nodes.size();
+                    if (!nodes_d_.reserve(n) || !out_d_.reserve(n)) throw HipFail{hipErrorOutOfMemory};
+                    ck(hipMemcpyAsync(nodes_d_.p, commit_nodes.data(), sizeof(int) * n, hipMemcpyHostToDevice, st_));
+                    ck(hipMemcpyAsync(out_d_.p, commit_res.data(), sizeof(SolveOut) * n, hipMemcpyHostToDevice, st_));
+                    ck(launch_apply(g, nodes_d_.p, out_d_.p, n, -1, true, st_));
+                    ck(hipStreamSynchronize(st_));
+                }
+            }
+            // the reference's flood; the layered one hands over only when no node was
+            // accepted before it (the flood's global seeds start it)
+            if (!flooded) {
                 struct SeedNode {
                     int x, y;
                     float u, v, ux, uy, vx, vy, corr;
@@ -1056,7 +1176,6 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                     }
                 };
                 std::vector<unsigned char> claimed(hsolved.begin(), hsolved.end());
-                const int dx4[4] = {1, -1, 0, 0}, dy4[4] = {0, 0, 1, -1};
                 std::vector<SeedNode> boundary, global;
                 const SeedNode pc{pcx, pcy, gU, gV, 0, 0, 0, 0, 0};
                 bool pc_queued = false;

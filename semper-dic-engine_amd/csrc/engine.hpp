@@ -139,6 +139,15 @@ public:
     // Solve Path B's first claims speculatively in one launch (off: every round
     // is solved when it is reached; the committed field is the same either way).
     bool speculate_path_b = true;
+    // Layered Path B (no reference counterpart; the default in learned
+    // seeding): instead of the reference's priority flood, which claims up to
+    // 32 nodes per round in corr order, every open node next to an accepted
+    // one is claimed at once, seeded by its accepted neighbour of least
+    // (corr, y, x) with the flood's own first-order carry, and all are solved
+    // in one launch; each layer's accepted nodes seed the next. Each node is
+    // still claimed once. Launches: the depth of the unsolved regions instead
+    // of one per round pair.
+    bool layered_path_b = false;
     // `out` of run() is device memory (the packed points stay in HBM)
     bool out_on_device = false;
     // Precision variants (no reference counterpart): the Gauss-Newton step by
@@ -166,6 +175,14 @@ public:
     // size odd, 3 .. 63. The reference's semantics apply node by node with
     // the node's own size. 0 or -3.
     int set_subset_map(int gw, int gh, const std::vector<int>& dims);
+    // The same by image region: an rw x rh map (row-major) over the frame; a
+    // node at pixel (x, y) takes the size of region (x rw / w, y rh / h), for
+    // any grid, so it follows grids whose rect changes (the incremental
+    // session's). A per-node map of the run's grid shape takes precedence. An
+    // empty map clears it. 0 or -3.
+    int set_region_map(int rw, int rh, const std::vector<int>& dims);
+    // A size map's shape and sizes are acceptable (an empty map is).
+    static bool size_map_ok(int cols, int rows, const std::vector<int>& dims);
     // Launch-shape tuner (no reference counterpart). A shape -- 1, 2 or 4
     // subsets per fast-path workgroup, row-major or 8 x 8-tile node order --
     // changes which wave solves which subset, never a subset's arithmetic, so
@@ -224,6 +241,8 @@ private:
     std::unique_ptr<Seeder> seeder_;
     std::vector<int> subset_map_;
     int map_gw_ = 0, map_gh_ = 0;
+    std::vector<int> region_map_, region_dims_;
+    int region_w_ = 0, region_h_ = 0;
     std::vector<int> grid_dims_;        // the grid cache's per-node sizes (empty: uniform)
     DevBuf<int32_t> dims_d_;
     struct DimClass {

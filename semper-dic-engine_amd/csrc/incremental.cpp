@@ -355,6 +355,15 @@ void IncrementalSession::reset() {
     scratch_.clear();
 }
 
+int IncrementalSession::set_region_map(int rw, int rh, const std::vector<int>& dims) {
+    if (!Engine::size_map_ok(rw, rh, dims)) return kErrInit;
+    if (eng_ && eng_->set_region_map(rw, rh, dims) != 0) return kErrInit;
+    region_map_ = dims;
+    region_w_ = dims.empty() ? 0 : rw;
+    region_h_ = dims.empty() ? 0 : rh;
+    return 0;
+}
+
 int IncrementalSession::alive_count() const {
     int n = 0;
     for (uint8_t a : alive_) n += a ? 1 : 0;
@@ -404,6 +413,10 @@ int IncrementalSession::init(const uint8_t* ref, int w, int h, const uint8_t* ma
         for (int y = y0; y < y1; ++y) std::memset(&material_mask0_[(size_t)y * w + x0], 255, (size_t)std::max(0, x1 - x0));
     }
     eng_ = std::make_unique<Engine>();
+    if (!region_map_.empty() && eng_->set_region_map(region_w_, region_h_, region_map_) != 0) {
+        reset();
+        return kErrInit;
+    }
     if (eng_->set_reference(ref, false, w, h, mask) != 0) {
         reset();
         return kErrInit;

@@ -53,8 +53,15 @@ public:
 
     // The solver's stop rule (the reference's fixed 1e-3 through the C ABI).
     float tol = 1e-3f;
+    // Mixed subset sizes by image region (Engine::set_region_map; no
+    // reference counterpart): every solve of the session, on the first
+    // reference and on each adopted one, takes each node's size from the
+    // region of the frame it lies in. Kept across init. 0 or -3.
+    int set_region_map(int rw, int rh, const std::vector<int>& dims);
 
 private:
+    std::vector<int> region_map_;
+    int region_w_ = 0, region_h_ = 0;
     struct Anchor {
         std::vector<uint8_t> patch;   // side x side gray around (ax, ay)
         int side = 0, ax = 0, ay = 0;

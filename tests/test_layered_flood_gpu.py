@@ -1,0 +1,88 @@
+"""Layered Path B (no reference counterpart; the default under learned
+seeding). It claims every open node next to an accepted one at once, seeded by
+the accepted neighbour the reference's flood would pop first, one launch per
+layer, instead of the reference's rounds of 32 pops. A node's converged warp
+does not depend on which accepted neighbour seeded it, up to the stop rule, so
+against the reference flood on the same frames it must report the same point
+set (within a few nodes whose acceptance is marginal), the same field within
+the reference's cross-toolchain bound, no less accuracy against the FP64 IC-GN
+optimum."""
+import ctypes
+import importlib.util
+import os
+
+import numpy as np
+import pytest
+
+from helpers import PKG, P, fp64_polish
+
+pytestmark = pytest.mark.gpu
+
+
+def _load(name, path):
+    spec = importlib.util.spec_from_file_location(name, path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+@pytest.fixture(scope="module")
+def env():
+    sa = _load("semper_amd", os.path.join(PKG, "semper_amd.py"))
+    dev = sa.load_dev()
+    if dev.sdk_device_count() < 1:
+        pytest.skip("no GPU")
+    return dev, _load("workloads", os.path.join(PKG, "workloads.py"))
+
+
+def run(dev, cfg, ref, dfm, prm, layered):
+    h = dev.sdk_engine_create()
+    assert dev.sdk_engine_set_flag(h, 8, 1 if layered else 0) == 0
+    assert dev.sdk_engine_set_reference(h, P(ref), 0, cfg["w"], cfg["h"], None) == 0
+    cap = (cfg["w"] // cfg["step"] + 1) * (cfg["h"] // cfg["step"] + 1) * 8
+    out = np.zeros(cap, np.float32)
+    met = np.zeros(24, np.float32)
+    tm = np.zeros(14, np.float64)
+    n = dev.sdk_engine_run(h, P(dfm), 0, P(prm), P(out), cap, P(met), 24, ctypes.c_float(cfg["tol"]))
+    dev.sdk_engine_timings(h, P(tm))
+    dev.sdk_engine_destroy(h)
+    assert n > 0, n
+    return out[: n * 8].reshape(-1, 8).copy(), met, tm
+
+
+def fp64_error(ref, dfm, pts, subset, k=300):
+    rng = np.random.default_rng(3)
+    sel = pts[rng.choice(len(pts), size=min(k, len(pts)), replace=False)]
+    p0 = np.zeros((len(sel), 6))
+    p0[:, 0], p0[:, 1] = sel[:, 2], sel[:, 3]
+    si, po, _ = fp64_polish(ref.astype(np.float32), dfm.astype(np.float32), subset,
+                            [(int(x), int(y)) for x, y in sel[:, :2]], p0)
+    ok = si[:, 0] == 0
+    return np.maximum(np.abs(po[ok, 0] - sel[ok, 2]), np.abs(po[ok, 1] - sel[ok, 3]))
+
+
+def test_layered_flood_against_reference_flood(env):
+    dev, wl = env
+    name = "t_1024_noisy"
+    cfg = wl.config(name)
+    ref, dfm = wl.pair(dev, name)
+    prm = np.array(wl.solve_params(name), np.int32)
+    p0, m0, t0 = run(dev, cfg, ref, dfm, prm, False)
+    p1, m1, t1 = run(dev, cfg, ref, dfm, prm, True)
+    print(f"reference flood: {len(p0)} points, Path B {int(m0[4])} solved in {int(t0[12])} further launches; "
+          f"layered: {len(p1)} points, Path B {int(m1[4])} solved in {int(t1[12])} launches", flush=True)
+    assert m0[4] > 50   # the frame gives the flood real work
+    assert abs(len(p1) - len(p0)) <= max(3, len(p0) // 100)
+    assert abs(m1[4] - m0[4]) <= max(3, m0[4] // 20)
+    assert m1[3] == m0[3]   # Path A is untouched
+    a = {(x, y): r for x, y, *r in p0.tolist()}
+    b = {(x, y): r for x, y, *r in p1.tolist()}
+    common = sorted(set(a) & set(b))
+    assert len(common) >= len(p0) - max(3, len(p0) // 100)
+    d = np.array([np.abs(np.array(a[k][:2]) - np.array(b[k][:2])) for k in common])
+    print(f"  max |duv| {d.max():.2e}, within 1e-3 {np.mean(d.max(1) <= 1e-3):.4f}", flush=True)
+    assert d.max() <= 5e-3 and np.mean(d.max(1) <= 1e-3) >= 0.99
+    e0, e1 = fp64_error(ref, dfm, p0, int(prm[5])), fp64_error(ref, dfm, p1, int(prm[5]))
+    print(f"  vs FP64 within 1e-3 px: layered {np.mean(e1 <= 1e-3):.4f}, reference flood {np.mean(e0 <= 1e-3):.4f}",
+          flush=True)
+    assert np.mean(e1 <= 1e-3) >= np.mean(e0 <= 1e-3) - 2.0 / len(e0)
