@@ -67,11 +67,13 @@ SDK_API int sdk_wave_sum_probe(const float* in, int nw, float* out, int32_t* iou
    use the shape of least mean time; flag 7 = 1..6 forces candidate value - 1
    of the order given at sdk_engine_tuning on every Path A launch (where only
    whole grids take tiles, a listed launch takes the same workgroup width
-   row-major; 0: the tuner decides); flag 8 = layered Path B (default 0, on
-   by itself with learned seeding): every open node next to an accepted one is
-   claimed at once, seeded by its accepted neighbour of least (corr, y, x),
-   and solved in one launch per layer, instead of the reference's flood of 32
-   pops per round. */
+   row-major; 0: the tuner decides); flag 8 = layered Path B with the given
+   reach, 1 .. 8 (default 0: the reference's flood of 32 pops per round;
+   learned seeding uses reach 8 unless set): each layer claims every open node
+   within `reach` grid steps of an accepted one, seeded from the accepted node
+   a breadth-first search in (corr, y, x) order reaches it from first, and
+   solves them in one launch; a failure beyond one step leaves the node open
+   for a closer claim. */
 SDK_API int sdk_engine_set_flag(void* h, int flag, int value);
 
 /* Phase timings of the last run, 14 doubles: total, prep, seeding, prepass,

@@ -269,7 +269,8 @@ SDK_API int sdk_engine_set_flag(void* h, int flag, int value) {
             ((DevEngine*)h)->eng.force_shape = value;
             return 0;
         case 8:
-            ((DevEngine*)h)->eng.layered_path_b = value != 0;
+            if (value < 0 || value > Engine::kLayeredReachMax) return -3;
+            ((DevEngine*)h)->eng.layered_path_b = value;
             return 0;
         default:
             return -3;

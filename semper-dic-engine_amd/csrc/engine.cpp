@@ -1076,80 +1076,112 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             const float* hcorr = hvy + NN;
             const int dx4[4] = {1, -1, 0, 0}, dy4[4] = {0, 0, 1, -1};
             bool flooded = false;
-            if (layered_path_b || use_cnn) {
-                // Layered Path B. ok: accepted (solved before Path B with a real corr,
-                // or accepted here); w: their warps, seven planes as hp.
-                std::vector<uint8_t> claimed(hsolved.begin(), hsolved.end()), ok(NN, 0);
+            const int reach = layered_path_b > 0 ? layered_path_b : use_cnn ? kLayeredReachLearned : 0;
+            if (reach > 0) {
+                // Layered Path B. ok: accepted (solved before Path B with a real corr, or
+                // accepted here), w: their warps (seven planes as hp); fin: solved,
+                // skipped or finally failed; tried: the least claim distance a node has
+                // failed from (a later claim must come closer).
+                std::vector<uint8_t> ok(NN, 0), fin(hsolved.begin(), hsolved.end());
+                std::vector<int8_t> tried(NN, (int8_t)(reach + 1));
                 std::vector<float> w(hp);
-                std::vector<int> front, mark(NN, -1);
-                for (size_t i = 0; i < NN; ++i) ok[i] = hsolved[i] && hcorr[i] >= 0.f ? 1 : 0;
-                auto push_open_neighbours = [&](int j, int layer) {
-                    const int x = j % gw, y = j / gw;
-                    for (int k = 0; k < 4; ++k) {
-                        const int nx = x + dx4[k], ny = y + dy4[k];
-                        if (nx < 0 || nx >= gw || ny < 0 || ny >= gh) continue;
-                        const int m = ny * gw + nx;
-                        if (!claimed[m] && mark[m] != layer) {
-                            mark[m] = layer;
-                            front.push_back(m);
-                        }
-                    }
-                };
-                for (size_t i = 0; i < NN; ++i)
-                    if (ok[i]) push_open_neighbours((int)i, 0);
-                flooded = !front.empty();
+                std::vector<int> open, stamp(NN, -1), dist(NN, 0), src(NN, -1), bfs, sources;
+                for (size_t i = 0; i < NN; ++i) {
+                    ok[i] = hsolved[i] && hcorr[i] >= 0.f ? 1 : 0;
+                    if (!hsolved[i]) open.push_back((int)i);
+                }
                 std::vector<int> commit_nodes;
                 std::vector<SolveOut> commit_res;
-                for (int layer = 0; !front.empty(); ++layer) {
+                for (int layer = 0; !open.empty(); ++layer) {
                     if (cancel.requested()) return kErrCancelled;
-                    std::sort(front.begin(), front.end());
-                    std::vector<Task> tasks(front.size());
-                    for (size_t t = 0; t < front.size(); ++t) {
-                        const int i = front[t], x = i % gw, y = i / gw;
-                        claimed[i] = 1;
-                        int best = -1;
+                    // the accepted nodes next to open ones, in the order the flood pops them
+                    sources.clear();
+                    for (int i : open) {
+                        const int x = i % gw, y = i / gw;
                         for (int k = 0; k < 4; ++k) {
                             const int nx = x + dx4[k], ny = y + dy4[k];
                             if (nx < 0 || nx >= gw || ny < 0 || ny >= gh) continue;
                             const int j = ny * gw + nx;
-                            if (!ok[j]) continue;
-                            const float cj = w[6 * NN + j], cb = best < 0 ? 0.f : w[6 * NN + best];
-                            if (best < 0 || cj < cb || (cj == cb && (ny < best / gw || (ny == best / gw && nx < best % gw))))
-                                best = j;
+                            if (ok[j] && stamp[j] != layer) {
+                                stamp[j] = layer;
+                                sources.push_back(j);
+                            }
                         }
-                        const float ddx = (float)((x - best % gw) * p.step), ddy = (float)((y - best / gw) * p.step);
-                        Task& tk = tasks[t];
-                        tk = Task{};
+                    }
+                    if (sources.empty()) break;
+                    flooded = true;
+                    std::sort(sources.begin(), sources.end(), [&](int a, int b) {
+                        const float ca = w[6 * NN + a], cb = w[6 * NN + b];
+                        return ca != cb ? ca < cb : a < b;
+                    });
+                    bfs = sources;
+                    for (int j : sources) {
+                        dist[j] = 0;
+                        src[j] = j;
+                    }
+                    for (size_t head = 0; head < bfs.size(); ++head) {
+                        const int j = bfs[head];
+                        if (dist[j] >= reach) continue;
+                        const int x = j % gw, y = j / gw;
+                        for (int k = 0; k < 4; ++k) {
+                            const int nx = x + dx4[k], ny = y + dy4[k];
+                            if (nx < 0 || nx >= gw || ny < 0 || ny >= gh) continue;
+                            const int m = ny * gw + nx;
+                            if (fin[m] || ok[m] || stamp[m] == layer) continue;
+                            stamp[m] = layer;
+                            dist[m] = dist[j] + 1;
+                            src[m] = src[j];
+                            bfs.push_back(m);
+                        }
+                    }
+                    std::vector<Task> tasks;
+                    std::vector<int> tdist;
+                    for (size_t k = sources.size(); k < bfs.size(); ++k) {
+                        const int i = bfs[k];
+                        if (dist[i] >= tried[i]) continue;
+                        const int s0 = src[i], x = i % gw, y = i / gw;
+                        const float ddx = (float)((x - s0 % gw) * p.step), ddy = (float)((y - s0 / gw) * p.step);
+                        Task tk = {};
                         tk.cx = p.rect_x + x * p.step;
                         tk.cy = p.rect_y + y * p.step;
                         tk.pool = i;
                         tk.node = i;
-                        tk.mode = NO_SEARCH;
-                        tk.g[0] = w[best] + w[2 * NN + best] * ddx + w[3 * NN + best] * ddy;
-                        tk.g[1] = w[NN + best] + w[4 * NN + best] * ddx + w[5 * NN + best] * ddy;
-                        for (int k = 0; k < 4; ++k) tk.g[2 + k] = w[(2 + k) * NN + best];
+                        tk.mode = dist[i] == 1 ? NO_SEARCH : NO_SIMPLEX;
+                        tk.g[0] = w[s0] + w[2 * NN + s0] * ddx + w[3 * NN + s0] * ddy;
+                        tk.g[1] = w[NN + s0] + w[4 * NN + s0] * ddx + w[5 * NN + s0] * ddy;
+                        for (int c = 0; c < 4; ++c) tk.g[2 + c] = w[(2 + c) * NN + s0];
+                        tasks.push_back(tk);
+                        tdist.push_back(dist[i]);
                     }
+                    if (tasks.empty()) break;
                     std::vector<SolveOut> res;
                     if (solve_tasks(tasks, res, p.use6, p.subset, tol) != 0) return kErrInit;
                     tm_.path_b_launches++;
                     tm_.path_b_miss += (int)tasks.size();
-                    front.clear();
                     for (size_t t = 0; t < tasks.size(); ++t) {
                         const SolveOut& r = res[t];
-                        if (!r.init) continue;
                         const int i = tasks[t].node;
-                        st_b.record(r);
-                        commit_nodes.push_back(i);
-                        commit_res.push_back(r);
-                        if (r.status == 0 && r.corr <= kCorrAccept) {
+                        if (r.init) {
+                            st_b.record(r);
+                            commit_nodes.push_back(i);
+                            commit_res.push_back(r);
+                        }
+                        if (r.init && r.status == 0 && r.corr <= kCorrAccept) {
                             st_b.points++;
                             ok[i] = 1;
-                            for (int k = 0; k < 6; ++k) w[k * NN + i] = r.p[k];
+                            fin[i] = 1;
+                            for (int c = 0; c < 6; ++c) w[c * NN + i] = r.p[c];
                             w[6 * NN + i] = r.corr;
+                        } else if (tdist[t] == 1 || !r.init) {
+                            fin[i] = 1;
+                        } else {
+                            tried[i] = (int8_t)tdist[t];
                         }
                     }
-                    for (size_t t = 0; t < tasks.size(); ++t)
-                        if (ok[tasks[t].node]) push_open_neighbours(tasks[t].node, layer + 1);
+                    size_t kept = 0;
+                    for (int i : open)
+                        if (!fin[i]) open[kept++] = i;
+                    open.resize(kept);
                     ticker.set(anchor_pts + st_a.points + st_b.points);
                 }
                 if (!commit_nodes.empty()) {

@@ -140,14 +140,18 @@ public:
     // is solved when it is reached; the committed field is the same either way).
     bool speculate_path_b = true;
     // Layered Path B (no reference counterpart; the default in learned
-    // seeding): instead of the reference's priority flood, which claims up to
-    // 32 nodes per round in corr order, every open node next to an accepted
-    // one is claimed at once, seeded by its accepted neighbour of least
-    // (corr, y, x) with the flood's own first-order carry, and all are solved
-    // in one launch; each layer's accepted nodes seed the next. Each node is
-    // still claimed once. Launches: the depth of the unsolved regions instead
-    // of one per round pair.
-    bool layered_path_b = false;
+    // seeding, with reach 8): instead of the reference's priority flood, which
+    // claims up to 32 nodes per round in corr order, each layer claims at once
+    // every open node within `reach` grid steps of an accepted one, from the
+    // accepted node a breadth-first search in (corr, y, x) order reaches it
+    // from first, carrying that node's warp over the offset, and solves them
+    // all in one launch. Adjacent claims solve as the flood's do (Simplex
+    // rescue on) and a failure there is final; farther claims run without
+    // the rescue and a failure leaves the node open for a closer claim. So a
+    // node is claimed at most `reach` times, and the launches number about
+    // the depth of the unsolved regions over `reach`. 0: the reference flood.
+    int layered_path_b = 0;
+    static constexpr int kLayeredReachLearned = 8, kLayeredReachMax = 8;
     // `out` of run() is device memory (the packed points stay in HBM)
     bool out_on_device = false;
     // Precision variants (no reference counterpart): the Gauss-Newton step by

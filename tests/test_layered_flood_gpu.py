@@ -1,8 +1,11 @@
 """Layered Path B (no reference counterpart; the default under learned
-seeding). It claims every open node next to an accepted one at once, seeded by
-the accepted neighbour the reference's flood would pop first, one launch per
-layer, instead of the reference's rounds of 32 pops. A node's converged warp
-does not depend on which accepted neighbour seeded it, up to the stop rule, so
+seeding, reach 8). Each layer claims every open node within `reach` grid
+steps of an accepted one at once, seeded from the accepted node a
+breadth-first search in the reference flood's pop order reaches it from
+first, one launch per layer, instead of the reference's rounds of 32 pops;
+with reach 1 every claim is one the reference's flood could make. A node's
+converged warp does not depend on which accepted node seeded it, up to the
+stop rule, so
 against the reference flood on the same frames it must report the same point
 set (within a few nodes whose acceptance is marginal), the same field within
 the reference's cross-toolchain bound, no less accuracy against the FP64 IC-GN
@@ -35,9 +38,9 @@ def env():
     return dev, _load("workloads", os.path.join(PKG, "workloads.py"))
 
 
-def run(dev, cfg, ref, dfm, prm, layered):
+def run(dev, cfg, ref, dfm, prm, reach):
     h = dev.sdk_engine_create()
-    assert dev.sdk_engine_set_flag(h, 8, 1 if layered else 0) == 0
+    assert dev.sdk_engine_set_flag(h, 8, reach) == 0
     assert dev.sdk_engine_set_reference(h, P(ref), 0, cfg["w"], cfg["h"], None) == 0
     cap = (cfg["w"] // cfg["step"] + 1) * (cfg["h"] // cfg["step"] + 1) * 8
     out = np.zeros(cap, np.float32)
@@ -61,16 +64,18 @@ def fp64_error(ref, dfm, pts, subset, k=300):
     return np.maximum(np.abs(po[ok, 0] - sel[ok, 2]), np.abs(po[ok, 1] - sel[ok, 3]))
 
 
-def test_layered_flood_against_reference_flood(env):
+@pytest.mark.parametrize("reach", [1, 4])
+def test_layered_flood_against_reference_flood(env, reach):
     dev, wl = env
     name = "t_1024_noisy"
     cfg = wl.config(name)
     ref, dfm = wl.pair(dev, name)
     prm = np.array(wl.solve_params(name), np.int32)
-    p0, m0, t0 = run(dev, cfg, ref, dfm, prm, False)
-    p1, m1, t1 = run(dev, cfg, ref, dfm, prm, True)
+    p0, m0, t0 = run(dev, cfg, ref, dfm, prm, 0)
+    p1, m1, t1 = run(dev, cfg, ref, dfm, prm, reach)
     print(f"reference flood: {len(p0)} points, Path B {int(m0[4])} solved in {int(t0[12])} further launches; "
-          f"layered: {len(p1)} points, Path B {int(m1[4])} solved in {int(t1[12])} launches", flush=True)
+          f"layered, reach {reach}: {len(p1)} points, Path B {int(m1[4])} solved in {int(t1[12])} launches",
+          flush=True)
     assert m0[4] > 50   # the frame gives the flood real work
     assert abs(len(p1) - len(p0)) <= max(3, len(p0) // 100)
     assert abs(m1[4] - m0[4]) <= max(3, m0[4] // 20)
