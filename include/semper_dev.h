@@ -73,7 +73,11 @@ SDK_API int sdk_wave_sum_probe(const float* in, int nw, float* out, int32_t* iou
    within `reach` grid steps of an accepted one, seeded from the accepted node
    a breadth-first search in (corr, y, x) order reaches it from first, and
    solves them in one launch; a failure beyond one step leaves the node open
-   for a closer claim. */
+   for a closer claim; flag 9 = graph capture of the fixed per-frame kernel
+   chains (default 1): the end of a run -- valid flags, the five median
+   passes, VSG, the row-major pack, the count download -- is captured the
+   second time it is met with the same grid, parameters and buffers and
+   replayed from then on. */
 SDK_API int sdk_engine_set_flag(void* h, int flag, int value);
 
 /* Phase timings of the last run, 14 doubles: total, prep, seeding, prepass,
@@ -88,6 +92,27 @@ SDK_API int sdk_engine_timings(void* h, double* out14);
    size (scan border, Hessian pool, IC-GN); one solver launch per size.
    0 or -3. */
 SDK_API int sdk_engine_set_subset_map(void* h, int gw, int gh, const int32_t* dims);
+
+/* Mixed subset sizes by image region (no reference counterpart): an rw x rh
+   map (row-major, odd sizes 3 .. 63) over the frame; a node at pixel (x, y)
+   takes the size of region (x rw / w, y rh / h), on any grid. A per-node map
+   of the run's grid shape (sdk_engine_set_subset_map) takes precedence; rw =
+   rh = 0 clears. The session form applies to every solve of an incremental
+   session created by semper_incremental_create, on each adopted reference
+   too, and is kept across semper_incremental_init. 0 or -3. */
+SDK_API int sdk_engine_set_region_map(void* h, int rw, int rh, const int32_t* dims);
+struct semper_incremental;
+SDK_API int sdk_incremental_set_region_map(struct semper_incremental* inc, int rw, int rh, const int32_t* dims);
+
+/* Graph captures and replays of an engine so far: stats2 = {captures,
+   replays}. 0 or -3. */
+SDK_API int sdk_engine_graph_stats(void* h, int32_t* stats2);
+/* The session forms: flag 9 (graph capture, default 1; applies from the next
+   semper_incremental_init, which also runs the session's strain of the total
+   field on the frame-0 grid through a captured chain) and the captures and
+   replays of the session's engine. 0 or -3. */
+SDK_API int sdk_incremental_set_flag(struct semper_incremental* inc, int flag, int value);
+SDK_API int sdk_incremental_graph_stats(struct semper_incremental* inc, int32_t* stats2);
 
 /* The tuner's latest timing: chosen2 = {subsets per workgroup, tile side (0:
    row-major)} of least mean time so far, ms6 = the mean Path A launch time

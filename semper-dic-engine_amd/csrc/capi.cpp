@@ -14,6 +14,7 @@
 
 #include "codec.hpp"
 #include "engine.hpp"
+#include "handles.hpp"
 #include "incremental.hpp"
 #include "semper_c.h"
 
@@ -21,18 +22,6 @@ using sdic::Cancel;
 using sdic::Engine;
 using sdic::Params;
 
-struct semper_engine {
-    Engine eng;
-    Cancel cancel;
-    std::atomic<bool> busy{false};
-};
-
-struct semper_incremental {
-    sdic::IncrementalSession session;
-    Cancel cancel;
-    int width = 0, height = 0;
-    std::atomic<bool> busy{false};
-};
 
 namespace {
 

@@ -9,6 +9,8 @@
 
 #include "codec.hpp"
 #include "engine.hpp"
+#include "handles.hpp"
+#include "semper_c.h"
 #include "semper_dev.h"
 #include "synth.hpp"
 
@@ -272,6 +274,9 @@ SDK_API int sdk_engine_set_flag(void* h, int flag, int value) {
             if (value < 0 || value > Engine::kLayeredReachMax) return -3;
             ((DevEngine*)h)->eng.layered_path_b = value;
             return 0;
+        case 9:
+            ((DevEngine*)h)->eng.use_graphs = value != 0;
+            return 0;
         default:
             return -3;
     }
@@ -448,6 +453,47 @@ SDK_API int sdk_seeder_train(uint64_t seed, int steps, int batch, float lr, cons
     } catch (...) {
         return -3;
     }
+}
+
+SDK_API int sdk_engine_set_region_map(void* h, int rw, int rh, const int32_t* dims) {
+    if (!h || (rw * rh > 0 && !dims)) return -3;
+    std::vector<int> d;
+    if (rw > 0 && rh > 0) d.assign(dims, dims + (size_t)rw * rh);
+    return ((DevEngine*)h)->eng.set_region_map(rw, rh, d) == 0 ? 0 : -3;
+}
+
+SDK_API int sdk_incremental_set_region_map(semper_incremental* inc, int rw, int rh, const int32_t* dims) {
+    if (!inc || (rw * rh > 0 && !dims)) return -3;
+    if (inc->busy.exchange(true, std::memory_order_acquire)) return -3;
+    std::vector<int> d;
+    if (rw > 0 && rh > 0) d.assign(dims, dims + (size_t)rw * rh);
+    const int rc = inc->session.set_region_map(rw, rh, d) == 0 ? 0 : -3;
+    inc->busy.store(false, std::memory_order_release);
+    return rc;
+}
+
+SDK_API int sdk_engine_graph_stats(void* h, int32_t* stats2) {
+    if (!h || !stats2) return -3;
+    const auto g = ((DevEngine*)h)->eng.graph_stats();
+    stats2[0] = g.captures;
+    stats2[1] = g.replays;
+    return 0;
+}
+
+SDK_API int sdk_incremental_set_flag(semper_incremental* inc, int flag, int value) {
+    if (!inc || flag != 9) return -3;
+    if (inc->busy.exchange(true, std::memory_order_acquire)) return -3;
+    inc->session.use_graphs = value != 0;
+    inc->busy.store(false, std::memory_order_release);
+    return 0;
+}
+
+SDK_API int sdk_incremental_graph_stats(semper_incremental* inc, int32_t* stats2) {
+    if (!inc || !stats2) return -3;
+    const auto g = inc->session.graph_stats();
+    stats2[0] = g.captures;
+    stats2[1] = g.replays;
+    return 0;
 }
 
 SDK_API int sdk_engine_tuning(void* h, int32_t* chosen2, double* ms6) {

@@ -136,6 +136,8 @@ Engine::Engine() {
 
 Engine::~Engine() {
     if (st_) (void)hipStreamSynchronize(st_);
+    for (Chain* c : {&tail_chain_, &strain_chain_})
+        if (c->exec) (void)hipGraphExecDestroy(c->exec);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
     for (hipEvent_t e : class_ev_) (void)hipEventDestroy(e);
@@ -187,6 +189,55 @@ namespace {
 const Engine::Tune kShapes[6] = {{1, 0}, {2, 0}, {4, 0}, {1, 8}, {2, 8}, {4, 8}};
 int shape_count(bool listed) { return listed ? 3 : 6; }   // tiles order the whole grid only
 }  // namespace
+
+Engine::GraphStats Engine::graph_stats() const {
+    GraphStats g;
+    for (const Chain* c : {&tail_chain_, &strain_chain_}) {
+        g.captures += c->captures;
+        g.replays += c->replays;
+    }
+    return g;
+}
+
+// Runs a chain of stream work: eagerly, or as a replay of its captured graph.
+// The first call with a key runs eagerly and keeps the key; a second call in
+// a row with the same key captures the chain, and later ones replay it.
+template <typename F>
+void Engine::run_chain(Chain& c, const std::vector<uint64_t>& key, F&& enqueue) {
+    if (!use_graphs) {
+        enqueue();
+        return;
+    }
+    if (c.exec && c.key == key) {
+        ck(hipGraphLaunch(c.exec, st_));
+        ++c.replays;
+        return;
+    }
+    if (c.key != key) {
+        if (c.exec) (void)hipGraphExecDestroy(c.exec);
+        c.exec = nullptr;
+        c.key = key;
+        enqueue();
+        return;
+    }
+    hipGraph_t graph = nullptr;
+    ck(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+    try {
+        enqueue();
+    } catch (...) {
+        (void)hipStreamEndCapture(st_, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    ck(hipStreamEndCapture(st_, &graph));
+    const hipError_t e = hipGraphInstantiate(&c.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) c.exec = nullptr;
+    ck(e);
+    ++c.captures;
+    ck(hipGraphLaunch(c.exec, st_));
+    ++c.replays;
+}
 
 int Engine::shape_for(const std::vector<int>& key, bool listed) {
     if (force_shape >= 1 && force_shape <= shape_count(listed)) return force_shape - 1;
@@ -1438,8 +1489,54 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
 
         // ---- strain and packing (pack_full_field_output) ----------------------
         const auto t_s = clk::now();
-        ck(launch_valid(g, valid_d, N, st_));
+        // median test: five passes, each from the previous pass's flags; a
+        // pass after one that rejected nothing changes nothing, and its kernel
+        // only copies the flags. Then VSG on the surviving nodes and the
+        // row-major pack on the device, then one copy of the packed points.
+        int* pass_fail = counters_.p + 16;   // one rejection count per pass
+        int* pack_counts = counters_.p + 24; // written before the cut, dropped before the cut
+        const int cap_points = out_capacity / 8;
+        if (!pack_out_.reserve(8 * NN) || !pack_blocks_.reserve((size_t)pack_blocks(N)) || !pin_counts_.reserve(32))
+            throw HipFail{hipErrorOutOfMemory};
+        uint8_t* med = valid_d;
+        {
+            uint8_t* dst = ok_a;
+            for (int pass = 0; pass < kOutlierMaxPasses; ++pass) {
+                med = dst;
+                dst = (dst == ok_a) ? ok_b : ok_a;
+            }
+        }
+        float* exx = strain_.p;
+        float* eyy = exx + NN;
+        float* exy = eyy + NN;
+        const std::vector<uint64_t> tail_key = {
+            (uint64_t)gw, (uint64_t)gh, (uint64_t)p.rect_x, (uint64_t)p.rect_y, (uint64_t)p.step,
+            (uint64_t)p.strain_window, (uint64_t)cap_points, (uint64_t)(uintptr_t)gplanes_.p,
+            (uint64_t)(uintptr_t)gflags_.p, (uint64_t)(uintptr_t)strain_.p, (uint64_t)(uintptr_t)counters_.p,
+            (uint64_t)(uintptr_t)pack_out_.p, (uint64_t)(uintptr_t)pack_blocks_.p, (uint64_t)(uintptr_t)pin_counts_.p,
+            (uint64_t)(uintptr_t)giters_.p};
+        run_chain(tail_chain_, tail_key, [&] {
+            ck(launch_valid(g, valid_d, N, st_));
+            ck(launch_zero(pass_fail, 16, st_));
+            uint8_t* src = valid_d;
+            uint8_t* dst = ok_a;
+            for (int pass = 0; pass < kOutlierMaxPasses; ++pass) {
+                ck(launch_outlier_pass(gw, gh, g.u, g.v, src, dst, pass > 0 ? pass_fail + pass - 1 : nullptr,
+                                       pass_fail + pass, st_));
+                src = dst;
+                dst = (dst == ok_a) ? ok_b : ok_a;
+            }
+            ck(launch_vsg(gw, gh, p.step, p.strain_window, vsg_count(p.step, p.strain_window), g.u, g.v, src, exx,
+                          eyy, exy, st_));
+            ck(launch_pack(gw, gh, p.rect_x, p.rect_y, p.step, cap_points, src, g.u, g.v, exx, eyy, exy, g.corr,
+                           pack_blocks_.p, pack_counts, pack_out_.p, st_));
+            ck(hipMemcpyAsync(pin_counts_.p, counters_.p, sizeof(int32_t) * 32, hipMemcpyDeviceToHost, st_));
+        });
+        ck(hipStreamSynchronize(st_));
+        int hc[32];
+        std::memcpy(hc, pin_counts_.p, sizeof(hc));
         if (keep_field) {
+            // the chain leaves the planes and the valid flags as they were
             last_field = Field{};
             last_field.gw = gw;
             last_field.gh = gh;
@@ -1449,8 +1546,10 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             last_field.attempted = total_valid;
             std::vector<float> all(7 * NN);
             last_field.ok.resize(NN);
+            last_field.med.resize(NN);
             ck(hipMemcpyAsync(all.data(), gplanes_.p, sizeof(float) * 7 * NN, hipMemcpyDeviceToHost, st_));
             ck(hipMemcpyAsync(last_field.ok.data(), valid_d, NN, hipMemcpyDeviceToHost, st_));
+            ck(hipMemcpyAsync(last_field.med.data(), med, NN, hipMemcpyDeviceToHost, st_));
             ck(hipStreamSynchronize(st_));
             last_field.u.assign(all.begin(), all.begin() + NN);
             last_field.v.assign(all.begin() + NN, all.begin() + 2 * NN);
@@ -1460,37 +1559,6 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             last_field.vy.assign(all.begin() + 5 * NN, all.begin() + 6 * NN);
             last_field.corr.assign(all.begin() + 6 * NN, all.begin() + 7 * NN);
         }
-        // median test: five passes, each from the previous pass's flags; a
-        // pass after one that rejected nothing changes nothing, and its kernel
-        // only copies the flags.
-        int* pass_fail = counters_.p + 16;   // one rejection count per pass
-        int* pack_counts = counters_.p + 24; // written before the cut, dropped before the cut
-        ck(launch_zero(pass_fail, 16, st_));
-        uint8_t* src = valid_d;
-        uint8_t* dst = ok_a;
-        for (int pass = 0; pass < kOutlierMaxPasses; ++pass) {
-            ck(launch_outlier_pass(gw, gh, g.u, g.v, src, dst, pass > 0 ? pass_fail + pass - 1 : nullptr,
-                                   pass_fail + pass, st_));
-            src = dst;
-            dst = (dst == ok_a) ? ok_b : ok_a;
-        }
-        if (keep_field) {
-            last_field.med.resize(NN);
-            ck(hipMemcpyAsync(last_field.med.data(), src, NN, hipMemcpyDeviceToHost, st_));
-        }
-        float* exx = strain_.p;
-        float* eyy = exx + NN;
-        float* exy = eyy + NN;
-        ck(launch_vsg(gw, gh, p.step, p.strain_window, vsg_count(p.step, p.strain_window), g.u, g.v, src, exx, eyy, exy,
-                      st_));
-        // row-major pack on the device, then one copy of the packed points
-        const int cap_points = out_capacity / 8;
-        if (!pack_out_.reserve(8 * NN) || !pack_blocks_.reserve((size_t)pack_blocks(N))) throw HipFail{hipErrorOutOfMemory};
-        ck(launch_pack(gw, gh, p.rect_x, p.rect_y, p.step, cap_points, src, g.u, g.v, exx, eyy, exy, g.corr,
-                       pack_blocks_.p, pack_counts, pack_out_.p, st_));
-        int hc[32];
-        ck(hipMemcpyAsync(hc, counters_.p, sizeof(hc), hipMemcpyDeviceToHost, st_));
-        ck(hipStreamSynchronize(st_));
         int dropped_outlier = 0;
         for (int pass = 0; pass < kOutlierMaxPasses; ++pass) dropped_outlier += hc[16 + pass];
         const int valid = std::min(hc[24], cap_points);
@@ -1555,31 +1623,44 @@ int Engine::strain_pack(const Field& f, int window, float* out, int out_capacity
     try {
         if (!gplanes_.reserve(7 * NN) || !gflags_.reserve(7 * NN) || !strain_.reserve(3 * NN) || !counters_.reserve(32))
             return kErrInit;
+        if (!pin_in_.reserve(2 * NN) || !pin_out_.reserve(3 * NN) || !pin_ok_in_.reserve(NN) ||
+            !pin_ok_out_.reserve(NN) || !pin_counts_.reserve(32))
+            return kErrInit;
         float* u = gplanes_.p;
         float* v = u + NN;
         uint8_t* ok0 = gflags_.p;
         uint8_t* ok1 = ok0 + NN;
         uint8_t* ok2 = ok1 + NN;
-        ck(hipMemcpyAsync(u, f.u.data(), sizeof(float) * NN, hipMemcpyHostToDevice, st_));
-        ck(hipMemcpyAsync(v, f.v.data(), sizeof(float) * NN, hipMemcpyHostToDevice, st_));
-        ck(hipMemcpyAsync(ok0, f.ok.data(), NN, hipMemcpyHostToDevice, st_));
-        ck(hipMemsetAsync(counters_.p + 12, 0, sizeof(int32_t), st_));
-        uint8_t* src = ok0;
-        uint8_t* dst = ok1;
-        for (int pass = 0; pass < kOutlierMaxPasses; ++pass) {
-            ck(launch_outlier_pass(gw, gh, u, v, src, dst, nullptr, counters_.p + 12, st_));
-            src = dst;
-            dst = (dst == ok1) ? ok2 : ok1;
-        }
         float* exx = strain_.p;
-        ck(launch_vsg(gw, gh, f.step, window, vsg_count(f.step, window), u, v, src, exx, exx + NN, exx + 2 * NN, st_));
-        std::vector<float> hs(3 * NN);
-        std::vector<uint8_t> hok(NN);
-        int rej = 0;
-        ck(hipMemcpyAsync(hs.data(), exx, sizeof(float) * 3 * NN, hipMemcpyDeviceToHost, st_));
-        ck(hipMemcpyAsync(hok.data(), src, NN, hipMemcpyDeviceToHost, st_));
-        ck(hipMemcpyAsync(&rej, counters_.p + 12, sizeof(int), hipMemcpyDeviceToHost, st_));
+        std::memcpy(pin_in_.p, f.u.data(), sizeof(float) * NN);
+        std::memcpy(pin_in_.p + NN, f.v.data(), sizeof(float) * NN);
+        std::memcpy(pin_ok_in_.p, f.ok.data(), NN);
+        const std::vector<uint64_t> key = {
+            (uint64_t)gw, (uint64_t)gh, (uint64_t)f.step, (uint64_t)window, (uint64_t)(uintptr_t)gplanes_.p,
+            (uint64_t)(uintptr_t)gflags_.p, (uint64_t)(uintptr_t)strain_.p, (uint64_t)(uintptr_t)counters_.p,
+            (uint64_t)(uintptr_t)pin_in_.p, (uint64_t)(uintptr_t)pin_out_.p, (uint64_t)(uintptr_t)pin_ok_in_.p,
+            (uint64_t)(uintptr_t)pin_ok_out_.p, (uint64_t)(uintptr_t)pin_counts_.p};
+        run_chain(strain_chain_, key, [&] {
+            ck(hipMemcpyAsync(u, pin_in_.p, sizeof(float) * 2 * NN, hipMemcpyHostToDevice, st_));
+            ck(hipMemcpyAsync(ok0, pin_ok_in_.p, NN, hipMemcpyHostToDevice, st_));
+            ck(launch_zero(counters_.p + 12, 1, st_));
+            uint8_t* src = ok0;
+            uint8_t* dst = ok1;
+            for (int pass = 0; pass < kOutlierMaxPasses; ++pass) {
+                ck(launch_outlier_pass(gw, gh, u, v, src, dst, nullptr, counters_.p + 12, st_));
+                src = dst;
+                dst = (dst == ok1) ? ok2 : ok1;
+            }
+            ck(launch_vsg(gw, gh, f.step, window, vsg_count(f.step, window), u, v, src, exx, exx + NN, exx + 2 * NN,
+                          st_));
+            ck(hipMemcpyAsync(pin_out_.p, exx, sizeof(float) * 3 * NN, hipMemcpyDeviceToHost, st_));
+            ck(hipMemcpyAsync(pin_ok_out_.p, src, NN, hipMemcpyDeviceToHost, st_));
+            ck(hipMemcpyAsync(pin_counts_.p + 12, counters_.p + 12, sizeof(int32_t), hipMemcpyDeviceToHost, st_));
+        });
         ck(hipStreamSynchronize(st_));
+        const float* hs = pin_out_.p;
+        const uint8_t* hok = pin_ok_out_.p;
+        const int rej = pin_counts_.p[12];
         int valid = 0, post = 0;
         bool trunc = false;
         for (int y = 0; y < gh && !trunc; ++y)

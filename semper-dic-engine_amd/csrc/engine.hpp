@@ -47,6 +47,32 @@ struct DevBuf {
     }
 };
 
+// Page-locked host memory (hipHostMalloc), for copies a captured graph
+// replays: their host address is fixed in the graph.
+template <typename T>
+struct PinnedBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    bool reserve(size_t count) {
+        if (count <= n && p) return true;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+        if (hipHostMalloc((void**)&p, sizeof(T) * (count ? count : 1), hipHostMallocDefault) != hipSuccess) {
+            p = nullptr;
+            return false;
+        }
+        n = count;
+        return true;
+    }
+};
+
 struct Params {
     int rect_x = 0, rect_y = 0, rect_w = 0, rect_h = 0;
     int step = 0, subset = 0, strain_window = 0;
@@ -206,6 +232,19 @@ public:
         double ms[6] = {0, 0, 0, 0, 0, 0};   // mean per candidate, -1 when not (yet) timed
     };
     TuneReport last_tune;
+    // Graph capture of a frame's fixed kernel chains (no reference
+    // counterpart): the end of a run (valid flags, the five median passes,
+    // VSG, the row-major pack and the count download) and the session's
+    // strain of the total field (uploads, median passes, VSG, downloads). A
+    // chain seen twice in a row with the same grid, parameters and buffers
+    // is captured on the second call and replayed from then on; host
+    // transfers go through page-locked staging, counters are cleared by a
+    // kernel, never a memset.
+    bool use_graphs = true;
+    struct GraphStats {
+        int captures = 0, replays = 0;
+    };
+    GraphStats graph_stats() const;
     int load_seeder(const std::string& path);
     bool seeder_ready() const { return seeder_ && seeder_->ready(); }
     struct SeedStats {
@@ -277,6 +316,17 @@ private:
     DevBuf<int32_t> giters_;
     DevBuf<float> guess_, tmp_;    // 6 planes each
     DevBuf<float> strain_;         // exx eyy exy
+    struct Chain {
+        std::vector<uint64_t> key;
+        hipGraphExec_t exec = nullptr;
+        int captures = 0, replays = 0;
+    };
+    Chain tail_chain_, strain_chain_;
+    PinnedBuf<int32_t> pin_counts_;   // the counters after a run's chain
+    PinnedBuf<float> pin_in_, pin_out_;
+    PinnedBuf<uint8_t> pin_ok_in_, pin_ok_out_;
+    template <typename F>
+    void run_chain(Chain& c, const std::vector<uint64_t>& key, F&& enqueue);
     DevBuf<float> pack_out_;       // packed output points, 8 floats each
     DevBuf<int32_t> pack_blocks_;  // per-block write counts of the pack
     DevBuf<Task> tasks_d_;

@@ -413,6 +413,7 @@ int IncrementalSession::init(const uint8_t* ref, int w, int h, const uint8_t* ma
         for (int y = y0; y < y1; ++y) std::memset(&material_mask0_[(size_t)y * w + x0], 255, (size_t)std::max(0, x1 - x0));
     }
     eng_ = std::make_unique<Engine>();
+    eng_->use_graphs = use_graphs;
     if (!region_map_.empty() && eng_->set_region_map(region_w_, region_h_, region_map_) != 0) {
         reset();
         return kErrInit;

@@ -58,6 +58,10 @@ public:
     // reference and on each adopted one, takes each node's size from the
     // region of the frame it lies in. Kept across init. 0 or -3.
     int set_region_map(int rw, int rh, const std::vector<int>& dims);
+    // Graph capture of the engine's fixed per-frame chains (Engine::use_graphs;
+    // kept across init), and the captures and replays of the current engine.
+    bool use_graphs = true;
+    Engine::GraphStats graph_stats() const { return eng_ ? eng_->graph_stats() : Engine::GraphStats{}; }
 
 private:
     std::vector<int> region_map_;

@@ -56,7 +56,7 @@ CONFIGS = {
     # (exx 0.1, Poisson 0.3), step 8, window 21, subset sizes 15 / 21 / 31 /
     # 41 assigned to a 4 x 4 region map drawn from region_seed
     "c3_1024_mixed": dict(w=1024, h=1024, seed=3003, kind=UNIAXIAL, p=[0.1, 0.3], step=8, subset=21, window=21,
-                          tol=1e-3, sizes=(15, 21, 31, 41), region_seed=30),
+                          tol=1e-3, sizes=(15, 21, 31, 41), region_seed=30, frames=64, ramp=0.2),
     # 4096^2 bumps + rotation (the learned-seeding workload)
     "c5_4096_bumps": dict(w=4096, h=4096, seed=5005, kind=BUMPS, p=None, step=8, subset=31, window=31, tol=1e-3),
 }
@@ -90,13 +90,43 @@ def pair(dev, name, on_device=False):
     return (r, d) if on_device else (ra, da)
 
 
+def region_map(name):
+    """The 4 x 4 subset-size map of a mixed-size workload over the frame
+    (row-major int32), drawn (seeded) from the workload's sizes."""
+    c = config(name)
+    rng = np.random.default_rng(c["region_seed"])
+    return np.ascontiguousarray(rng.choice(np.array(c["sizes"], np.int32), size=(4, 4)), np.int32)
+
+
+def sequence_strain(name, k):
+    """exx of frame k of a sequence workload: a linear ramp from 0 at frame 0
+    to `ramp` at the last frame."""
+    c = config(name)
+    return c["ramp"] * k / (c["frames"] - 1)
+
+
+def sequence_frame(dev, name, k):
+    """Frame k of a sequence workload (host uint8, h x w): the workload's
+    speckle under uniaxial stretch exx_k with its Poisson ratio, about the
+    frame centre, with noise of its own (frames never share a noise pattern,
+    which would pull each increment toward zero); frame 0 is unstretched."""
+    c = config(name)
+    w, h = c["w"], c["h"]
+    prm = np.array([sequence_strain(name, k), c["p"][1]], np.float64)
+    out = np.zeros((h, w), np.uint8)
+    rc = dev.sdk_synth_frame(c["seed"], c["seed"] * 1000 + 7 * k + 1, w, h, c["kind"],
+                             prm.ctypes.data_as(ctypes.c_void_p), len(prm), w / 2.0, h / 2.0,
+                             ctypes.c_float(c.get("noise", NOISE_SIGMA)), out.ctypes.data_as(ctypes.c_void_p), 0)
+    if rc != 0:
+        raise RuntimeError(f"sdk_synth_frame failed ({rc})")
+    return out
+
+
 def subset_map(name, gw, gh):
     """Per-node subset sizes of a mixed-size workload: a 4 x 4 map of regions
     over the grid, each region's size drawn (seeded) from the workload's sizes;
     row-major gw x gh int32."""
-    c = config(name)
-    rng = np.random.default_rng(c["region_seed"])
-    regions = rng.choice(np.array(c["sizes"], np.int32), size=(4, 4))
+    regions = region_map(name)
     gy, gx = np.mgrid[0:gh, 0:gw]
     return np.ascontiguousarray(regions[gy * 4 // gh, gx * 4 // gw], np.int32)
 
