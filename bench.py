@@ -74,6 +74,54 @@ PRECISION_VARIANTS = [("fp64_cholesky_step", (2,)), ("fp16_gradients", (3,)), ("
                       ("all_three", (2, 3, 4))]
 
 
+def sequence_study(dev, sa, wl):
+    """The batched / incremental sequence: 64 frames of 1024^2 speckle under a
+    uniaxial stretch ramping 0 -> 20% (Poisson 0.3), each frame with its own
+    noise, through the incremental session at update interval 1. The mixed
+    15 / 21 / 31 / 41 region map with graph capture on and off, and plain
+    subset 21 beside them: per-frame time (all 63 frames after the first, the
+    session's own captures included), output points and the last frame's
+    error against the analytic field."""
+    name = "c3_1024_mixed"
+    cfg = wl.config(name)
+    frames = [wl.sequence_frame(dev, name, k) for k in range(cfg["frames"])]
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    res = {"workload": f"{name}: {cfg['frames']} frames {cfg['w']}x{cfg['h']}, exx 0 -> {cfg['ramp']}, Poisson "
+                       f"{cfg['p'][1]}, step {cfg['step']}, strain window {cfg['window']}, update interval 1",
+           "region_map": wl.region_map(name).tolist()}
+    for label, mixed, graphs in (("mixed_graphs", True, True), ("mixed_eager", True, False),
+                                 ("plain_21_graphs", False, True)):
+        s = sa.Incremental(dev)
+        dev.sdk_incremental_set_flag(s.h, 9, 1 if graphs else 0)
+        if mixed:
+            regions = wl.region_map(name)
+            dev.sdk_incremental_set_region_map(s.h, 4, 4, P(regions))
+        assert s.init(frames[0], sa.params((0, 0, cfg["w"], cfg["h"]), cfg["step"], cfg["subset"], cfg["window"]),
+                      1) == 0
+        ms, npts = [], []
+        pts = None
+        for f in frames[1:]:
+            t0 = time.perf_counter()
+            n, pts, _, _ = s.process(f)
+            ms.append((time.perf_counter() - t0) * 1e3)
+            npts.append(int(n))
+        stats = np.zeros(2, np.int32)
+        dev.sdk_incremental_graph_stats(s.h, P(stats))
+        s.close()
+        k = cfg["frames"] - 1
+        exx = wl.sequence_strain(name, k)
+        u = exx * (pts[:, 0] - cfg["w"] / 2.0)
+        v = -cfg["p"][1] * exx * (pts[:, 1] - cfg["h"] / 2.0)
+        err = np.hypot(pts[:, 2] - u, pts[:, 3] - v)
+        res[label] = {"ms_per_frame_mean": float(np.mean(ms)), "ms_per_frame_median": float(np.median(ms)),
+                      "frames_per_s": float(1000.0 / np.mean(ms)), "points_per_s": float(np.sum(npts) / np.sum(ms) * 1e3),
+                      "points_first": npts[0], "points_last": npts[-1],
+                      "last_frame_rms_px": float(np.sqrt(np.mean(err ** 2))),
+                      "last_frame_median_px": float(np.median(err)),
+                      "graph_captures": int(stats[0]), "graph_replays": int(stats[1])}
+    return res
+
+
 def settle_tuner(dev, eng, run_once, limit=16):
     """Untimed runs until the engine's launch-shape tuner has settled (it
     times each shape on the first runs of a configuration)."""
@@ -244,6 +292,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-precision", action="store_true", help="skip the precision-variant study")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-seeding comparison")
+    ap.add_argument("--no-sequence", action="store_true", help="skip the 64-frame incremental sequence")
     ap.add_argument("--tol", type=float, default=None, help="IC-GN stop rule ||dp|| < tol (default: the workload's)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB copy kernel three times (the calibration of the HBM traffic "
@@ -342,6 +391,7 @@ def main():
                                         max(2, args.steps // 2))
         if not args.no_learned:
             learned = learned_seed_study(dev, sa, wl, 3, threads)
+    sequence = None if args.no_sequence else sequence_study(dev, sa, wl)
     frac = acc["within_1e-3px"] if acc else 1.0
     value = float(solved * frac / (ms_per_step / 1000.0))
     line = {
@@ -380,6 +430,7 @@ def main():
         "prepass_ms": prepass_ms,
         "precision_study": precision,
         "learned_seed": learned,
+        "sequence": sequence,
     }
     dev.sdk_engine_destroy(eng)
     dev.sdk_free(dref)

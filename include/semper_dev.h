@@ -109,8 +109,12 @@ SDK_API int sdk_incremental_set_region_map(struct semper_incremental* inc, int r
 SDK_API int sdk_engine_graph_stats(void* h, int32_t* stats2);
 /* The session forms: flag 9 (graph capture, default 1; applies from the next
    semper_incremental_init, which also runs the session's strain of the total
-   field on the frame-0 grid through a captured chain) and the captures and
-   replays of the session's engine. 0 or -3. */
+   field on the frame-0 grid through a captured chain), flag 10 (FP64
+   composition of the total displacement, default 0; from the next init: the
+   totals carried across reference updates are kept in double, each increment
+   added in double and the reported total rounded once, where the reference
+   adds increments to a float total), and the captures and replays of the
+   session's engine. 0 or -3. */
 SDK_API int sdk_incremental_set_flag(struct semper_incremental* inc, int flag, int value);
 SDK_API int sdk_incremental_graph_stats(struct semper_incremental* inc, int32_t* stats2);
 

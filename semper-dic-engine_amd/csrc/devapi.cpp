@@ -481,9 +481,10 @@ SDK_API int sdk_engine_graph_stats(void* h, int32_t* stats2) {
 }
 
 SDK_API int sdk_incremental_set_flag(semper_incremental* inc, int flag, int value) {
-    if (!inc || flag != 9) return -3;
+    if (!inc || (flag != 9 && flag != 10)) return -3;
     if (inc->busy.exchange(true, std::memory_order_acquire)) return -3;
-    inc->session.use_graphs = value != 0;
+    if (flag == 9) inc->session.use_graphs = value != 0;
+    else inc->session.compose64 = value != 0;
     inc->busy.store(false, std::memory_order_release);
     return 0;
 }

@@ -345,6 +345,9 @@ void IncrementalSession::reset() {
     frame0_points_ = -1;
     ub_.clear();
     vb_.clear();
+    ub64_.clear();
+    vb64_.clear();
+    c64_ = false;
     coast_travel_.clear();
     alive_.clear();
     anchors_.clear();
@@ -425,6 +428,9 @@ int IncrementalSession::init(const uint8_t* ref, int w, int h, const uint8_t* ma
     const size_t n = (size_t)gw_ * gh_;
     ub_.assign(n, 0.0f);
     vb_.assign(n, 0.0f);
+    c64_ = compose64;
+    ub64_.assign(c64_ ? n : 0, 0.0);
+    vb64_.assign(c64_ ? n : 0, 0.0);
     alive_.assign(n, 1);
     coast_travel_.assign(n, 0.0f);
     init_ = true;
@@ -443,6 +449,19 @@ int IncrementalSession::process(const uint8_t* def, float* out, int out_capacity
     if (ref_index_ > 0 && alive_count() == 0) return nothing_to_follow(fi, metrics, metrics_len, progress, info);
 
     std::vector<float> U(n0, 0.0f), V(n0, 0.0f), coast_step(n0, 0.0f);
+    // the totals in double when compose64 (U, V then hold them rounded once)
+    std::vector<double> U64(c64_ ? n0 : 0, 0.0), V64(c64_ ? n0 : 0, 0.0);
+    auto add_increment = [&](size_t i, float du, float dv) {
+        if (c64_) {
+            U64[i] = ub64_[i] + (double)du;
+            V64[i] = vb64_[i] + (double)dv;
+            U[i] = (float)U64[i];
+            V[i] = (float)V64[i];
+        } else {
+            U[i] = ub_[i] + du;
+            V[i] = vb_[i] + dv;
+        }
+    };
     std::vector<uint8_t> valid(n0, 0), carried(n0, 0);
     std::vector<AnchorNeed> need(n0, AnchorNeed::kKeep);
     int coasting = 0, written = 0;
@@ -453,8 +472,7 @@ int IncrementalSession::process(const uint8_t* def, float* out, int out_capacity
     auto carry = [&](size_t i, float du, float dv) {
         const float travel = std::hypot(du, dv);
         if (coast_travel_[i] + travel > max_travel) return;
-        U[i] = ub_[i] + du;
-        V[i] = vb_[i] + dv;
+        add_increment(i, du, dv);
         carried[i] = 1;
         coast_step[i] = travel;
         coasting++;
@@ -476,6 +494,10 @@ int IncrementalSession::process(const uint8_t* def, float* out, int out_capacity
         const NodeField nf = node_field(f);
         U = f.u;
         V = f.v;
+        for (size_t i = 0; c64_ && i < n0; ++i) {
+            U64[i] = U[i];
+            V64[i] = V[i];
+        }
         valid = f.med;
         carried = valid;
         if (frame0_points < 0) frame0_points = f.attempted;
@@ -525,8 +547,7 @@ int IncrementalSession::process(const uint8_t* def, float* out, int out_capacity
                     coast(i, inc, px, py);
                     continue;
                 }
-                U[i] = ub_[i] + smp.u;
-                V[i] = vb_[i] + smp.v;
+                add_increment(i, smp.u, smp.v);
                 corr_of[i] = smp.corr;
                 need[i] = anchor_need(inc, gd, px, py, subset);
                 if (coast_travel_[i] > trusted_travel) {
@@ -568,6 +589,10 @@ int IncrementalSession::process(const uint8_t* def, float* out, int out_capacity
                 const float y0 = (float)(p0_.rect_y + (int)(i / gw_) * p0_.step);
                 U[i] = ra[k].x - x0;
                 V[i] = ra[k].y - y0;
+                if (c64_) {
+                    U64[i] = (double)ra[k].x - x0;
+                    V64[i] = (double)ra[k].y - y0;
+                }
                 corr_of[i] = ra[k].corr;
                 valid[i] = 1;
                 carried[i] = 1;
@@ -711,6 +736,10 @@ int IncrementalSession::process(const uint8_t* def, float* out, int out_capacity
         anchors_.swap(anchors);
         ub_.swap(Ub);
         vb_.swap(Vb);
+        if (c64_) {
+            ub64_.swap(U64);
+            vb64_.swap(V64);
+        }
         alive_.swap(alive);
         mask_k_.swap(mask_k);
         pk_ = pk;

@@ -61,6 +61,11 @@ public:
     // Graph capture of the engine's fixed per-frame chains (Engine::use_graphs;
     // kept across init), and the captures and replays of the current engine.
     bool use_graphs = true;
+    // FP64 composition of the total displacement (no reference counterpart;
+    // the reference adds each increment to a float total): the totals carried
+    // across reference updates are kept in double, each increment is added in
+    // double, and the reported total is that sum rounded once. Kept across init.
+    bool compose64 = false;
     Engine::GraphStats graph_stats() const { return eng_ ? eng_->graph_stats() : Engine::GraphStats{}; }
 
 private:
@@ -80,6 +85,8 @@ private:
     std::vector<uint8_t> user_mask0_, material_mask0_;
     int frame0_points_ = -1;
     std::vector<float> ub_, vb_, coast_travel_;
+    std::vector<double> ub64_, vb64_;   // the same totals in double, when c64_
+    bool c64_ = false;                   // compose64 as it was at init
     std::vector<uint8_t> alive_;
     std::map<size_t, Anchor> anchors_;
     int ref_index_ = 0, frames_since_update_ = 0, ref_reported_ = -1, adopted_reported_ = -1, frame_count_ = 0;
