@@ -77,7 +77,9 @@ SDK_API int sdk_wave_sum_probe(const float* in, int nw, float* out, int32_t* iou
    chains (default 1): the end of a run -- valid flags, the five median
    passes, VSG, the row-major pack, the count download -- is captured the
    second time it is met with the same grid, parameters and buffers and
-   replayed from then on. */
+   replayed from then on; flag 11 = the learned seeder's inference on MFMA
+   (default 1: fp16 activations and weights, FP32 accumulation, one
+   workgroup per patch pair; 0: the FP32 forward pass it is trained with). */
 SDK_API int sdk_engine_set_flag(void* h, int flag, int value);
 
 /* Phase timings of the last run, 14 doubles: total, prep, seeding, prepass,
@@ -167,6 +169,10 @@ SDK_API int sdk_seeder_param_count(void);
 SDK_API int sdk_seeder_grad(const float* params, const float* x, const float* t, int batch, float* grads,
                             float* loss);
 SDK_API int sdk_seeder_forward(const float* params, const float* x, int batch, float* y);
+/* The same batch through the inference kernel the engine uses: fp16
+   activations and weights on MFMA (v_mfma_f32_16x16x16f16) with FP32
+   accumulation. */
+SDK_API int sdk_seeder_forward_mfma(const float* params, const float* x, int batch, float* y);
 SDK_API int sdk_seeder_train(uint64_t seed, int steps, int batch, float lr, const char* out_path, float* losses,
                              int nloss);
 SDK_API int sdk_engine_load_seeder(void* h, const char* path);

@@ -277,6 +277,9 @@ SDK_API int sdk_engine_set_flag(void* h, int flag, int value) {
         case 9:
             ((DevEngine*)h)->eng.use_graphs = value != 0;
             return 0;
+        case 11:
+            ((DevEngine*)h)->eng.seeder_mfma = value != 0;
+            return 0;
         default:
             return -3;
     }
@@ -418,6 +421,16 @@ SDK_API int sdk_seeder_forward(const float* params, const float* x, int batch, f
     try {
         Seeder s;
         return s.forward(params, x, batch, y);
+    } catch (...) {
+        return -3;
+    }
+}
+
+SDK_API int sdk_seeder_forward_mfma(const float* params, const float* x, int batch, float* y) {
+    if (!params || !x || !y) return -3;
+    try {
+        Seeder s;
+        return s.forward_mfma(params, x, batch, y);
     } catch (...) {
         return -3;
     }

@@ -737,6 +737,7 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                     off[2 * k] = (int)std::lround(bg[0] / S2);
                     off[2 * k + 1] = (int)std::lround(bg[1] / S2);
                 }
+                seeder_->use_mfma = seeder_mfma;
                 if (seeder_->infer(pyr_r_[SeederShape::kLevel].p, pyr_d_[SeederShape::kLevel].p, w2, h2, c2.data(),
                                    off.data(), n, y.data(), st_) != 0)
                     throw HipFail{hipErrorLaunchFailure};

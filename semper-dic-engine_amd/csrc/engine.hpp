@@ -199,6 +199,9 @@ public:
     // fails), and an anchor whose refined solve fails is solved again the
     // plain way.
     bool learned_seed = false;
+    // The seeder's inference: the fp16 MFMA kernel (true) or the FP32
+    // forward pass it is trained with.
+    bool seeder_mfma = true;
     // Mixed subset sizes (no reference counterpart): the subset size of each
     // node of a gw x gh grid (row-major), used by runs on a grid of that shape
     // (any other shape, or an empty map: the run's subset everywhere). Each

@@ -336,6 +336,201 @@ inline int blocks(size_t n, int bs) { return (int)((n + bs - 1) / bs); }
 
 }  // namespace
 
+namespace {
+
+// ---- inference on MFMA ------------------------------------------------------
+// One workgroup of 8 waves per patch pair. Activations stay in LDS as fp16,
+// channel-last with kPS halves per position (32 channels and 8 of padding);
+// each 3x3 stride-2 convolution is an implicit GEMM on
+// v_mfma_f32_16x16x16f16 (FP32 accumulation): rows are 16 output positions,
+// columns 16 output channels, K runs over (tap, input channel) with the
+// channel fastest, so a lane's four K values are four adjacent channels of
+// one input position, one 8-byte LDS read. Wave w takes channel tile w / 4
+// and position tiles w % 4, w % 4 + 4, ...; its B fragments (the layer's
+// fp16 weights for its 16 channels) are held in registers for the layer.
+// Operand layout of the MFMA: lane l gives A[l % 16][4 (l / 16) + j] and
+// B[4 (l / 16) + j][l % 16]; D[r][c] returns in lane 16 (r / 4) + c, slot
+// r % 4.
+typedef _Float16 sd_half4 __attribute__((ext_vector_type(4)));
+typedef _Float16 sd_half2 __attribute__((ext_vector_type(2)));
+typedef float sd_float4 __attribute__((ext_vector_type(4)));
+
+namespace mf {
+constexpr int kPS = 40;                     // halves per position
+constexpr int kIn = 0;                      // input planes, 64 x 64 x 2
+constexpr int kA1 = 64 * 64 * 2;            // conv 1 out, 32 x 32 x kPS
+constexpr int kA2 = kA1 + 32 * 32 * kPS;    // conv 2 out, 16 x 16 x kPS
+constexpr int kA3 = kA1;                    // conv 3 out, 8 x 8 x kPS (over conv 1's)
+constexpr int kHalves = kA2 + 16 * 16 * kPS;
+constexpr int kW1 = 32 * 32;                // conv 1 [n][k = tap * 2 + ci], 18 of 32 used
+constexpr int kWl = 32 * 288;               // conv 2..4 [n][tap * 32 + ci]
+constexpr int kHalfWeights = kW1 + 3 * kWl;
+}  // namespace mf
+
+// FP32 parameters -> the fp16 weight layouts of the MFMA kernel.
+__global__ void k_half_weights(const float* __restrict__ P, _Float16* __restrict__ wh) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= mf::kHalfWeights) return;
+    float v = 0.0f;
+    if (i < mf::kW1) {
+        const int n = i / 32, k = i % 32;
+        if (k < 18) v = P[S::off_w(0) + (n * S::kInCh + (k & 1)) * 9 + (k >> 1)];
+    } else {
+        const int l = 1 + (i - mf::kW1) / mf::kWl, r = (i - mf::kW1) % mf::kWl;
+        const int n = r / 288, kk = r % 288;
+        v = P[S::off_w(l) + (n * S::kCh + (kk & 31)) * 9 + (kk >> 5)];
+    }
+    wh[i] = (_Float16)v;
+}
+
+__device__ inline float wave_sum64(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// conv 2..4: in (HIN x HIN x kPS) -> out (HIN / 2 square x kPS), or, for the
+// last layer, FP32 features in the network's C x H x W order.
+template <int HIN, bool LAST>
+__device__ inline void mf_conv(const _Float16* __restrict__ in, _Float16* __restrict__ out, float* __restrict__ feat,
+                               const _Float16* __restrict__ wl, const float* __restrict__ bias, int wv, int lane) {
+    constexpr int HO = HIN / 2, M = HO * HO, TILES = M / 16;
+    const int nt = wv >> 2, g = lane >> 4, col = lane & 15, n = nt * 16 + col;
+    if ((wv & 3) >= TILES) return;
+    sd_half4 b[18];
+#pragma unroll
+    for (int c = 0; c < 18; ++c) b[c] = *reinterpret_cast<const sd_half4*>(wl + n * 288 + 16 * c + 4 * g);
+    const float bn = bias[n];
+    for (int pt = wv & 3; pt < TILES; pt += 4) {
+        const int m = pt * 16 + col, oy = m / HO, ox = m % HO;
+        sd_float4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int c = 0; c < 18; ++c) {
+            const int tap = c >> 1, ky = tap / 3, kx = tap % 3;
+            const int iy = 2 * oy + ky - 1, ix = 2 * ox + kx - 1;
+            sd_half4 a = {(_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f};
+            if (iy >= 0 && iy < HIN && ix >= 0 && ix < HIN)
+                a = *reinterpret_cast<const sd_half4*>(in + (iy * HIN + ix) * mf::kPS + 16 * (c & 1) + 4 * g);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16f16(a, b[c], acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int mm = pt * 16 + 4 * g + j;
+            const float v = fmaxf(acc[j] + bn, 0.0f);
+            if (LAST) feat[n * M + mm] = v;
+            else out[mm * mf::kPS + n] = (_Float16)v;
+        }
+    }
+}
+
+// CUT: cut the pair from the level-2 planes (as k_cut) and normalise each
+// plane (as k_normalize); otherwise read pre-normalised pairs from xin.
+// y: B x 6 outputs in the network's scaled units.
+template <bool CUT>
+__global__ __launch_bounds__(512) void k_seed_mfma(const float* __restrict__ r2, const float* __restrict__ d2, int w2,
+                                                   int h2, const int* __restrict__ c, const int* __restrict__ off,
+                                                   const float* __restrict__ xin, const _Float16* __restrict__ wh,
+                                                   const float* __restrict__ P, float* __restrict__ y) {
+    __shared__ __attribute__((aligned(16))) _Float16 lds[mf::kHalves];
+    __shared__ float red[16];
+    const int k = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    for (int ch = 0; ch < 2; ++ch) {
+        float v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int i = t + 512 * q, px = i & 63, py = i >> 6;
+            if (CUT) {
+                const int rx = min(max(c[2 * k] - S::kPatch / 2 + px, 0), w2 - 1);
+                const int ry = min(max(c[2 * k + 1] - S::kPatch / 2 + py, 0), h2 - 1);
+                if (ch == 0) {
+                    v[q] = r2[(size_t)ry * w2 + rx];
+                } else {
+                    const int dx = min(max(rx + off[2 * k], 0), w2 - 1), dy = min(max(ry + off[2 * k + 1], 0), h2 - 1);
+                    v[q] = d2[(size_t)dy * w2 + dx];
+                }
+            } else {
+                v[q] = xin[((size_t)k * 2 + ch) * 4096 + i];
+            }
+        }
+        float mean = 0.0f, inv = 1.0f;
+        if (CUT) {
+            float s = 0.0f;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) s += v[q];
+            s = wave_sum64(s);
+            if (lane == 0) red[wv] = s;
+            __syncthreads();
+            float tot = 0.0f;
+            for (int w = 0; w < 8; ++w) tot += red[w];
+            mean = tot / 4096.0f;
+            __syncthreads();
+            float q2 = 0.0f;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) q2 += (v[q] - mean) * (v[q] - mean);
+            q2 = wave_sum64(q2);
+            if (lane == 0) red[8 + wv] = q2;
+            __syncthreads();
+            float tq = 0.0f;
+            for (int w = 0; w < 8; ++w) tq += red[8 + w];
+            inv = 1.0f / sqrtf(tq / 4096.0f + 1e-6f);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) lds[mf::kIn + (t + 512 * q) * 2 + ch] = (_Float16)((v[q] - mean) * inv);
+    }
+    __syncthreads();
+    // conv 1: 64 x 64 x 2 -> 32 x 32 x 32; K = 9 taps x 2 channels (k = tap * 2 + ci), padded to 32
+    {
+        const int nt = wv >> 2, g = lane >> 4, col = lane & 15, n = nt * 16 + col;
+        const sd_half4 b0 = *reinterpret_cast<const sd_half4*>(wh + n * 32 + 4 * g);
+        const sd_half4 b1 = *reinterpret_cast<const sd_half4*>(wh + n * 32 + 16 + 4 * g);
+        const float bn = P[S::off_b(0) + n];
+        for (int pt = wv & 3; pt < 64; pt += 4) {
+            const int oy = pt >> 1, ox = (pt & 1) * 16 + col;
+            sd_float4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) {
+                sd_half4 a;
+#pragma unroll
+                for (int jj = 0; jj < 2; ++jj) {
+                    const int tap = 8 * cc + 2 * g + jj;
+                    sd_half2 pr = {(_Float16)0.0f, (_Float16)0.0f};
+                    if (tap < 9) {
+                        const int iy = 2 * oy + tap / 3 - 1, ix = 2 * ox + tap % 3 - 1;
+                        if (iy >= 0 && iy < 64 && ix >= 0 && ix < 64)
+                            pr = *reinterpret_cast<const sd_half2*>(lds + mf::kIn + (iy * 64 + ix) * 2);
+                    }
+                    a[2 * jj] = pr[0];
+                    a[2 * jj + 1] = pr[1];
+                }
+                acc = __builtin_amdgcn_mfma_f32_16x16x16f16(a, cc == 0 ? b0 : b1, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int x = (pt & 1) * 16 + 4 * g + j;
+                lds[mf::kA1 + (oy * 32 + x) * mf::kPS + n] = (_Float16)fmaxf(acc[j] + bn, 0.0f);
+            }
+        }
+    }
+    __syncthreads();
+    mf_conv<32, false>(lds + mf::kA1, lds + mf::kA2, nullptr, wh + mf::kW1, P + S::off_b(1), wv, lane);
+    __syncthreads();
+    mf_conv<16, false>(lds + mf::kA2, lds + mf::kA3, nullptr, wh + mf::kW1 + mf::kWl, P + S::off_b(2), wv, lane);
+    __syncthreads();
+    float* feat = reinterpret_cast<float*>(lds + mf::kIn);   // the input planes are spent
+    mf_conv<8, true>(lds + mf::kA3, nullptr, feat, wh + mf::kW1 + 2 * mf::kWl, P + S::off_b(3), wv, lane);
+    __syncthreads();
+    if (wv < S::kOut) {
+        const float* wf = P + S::off_fw() + wv * S::kFeat;
+        float s = 0.0f;
+#pragma unroll
+        for (int q = 0; q < S::kFeat / 64; ++q) s += wf[lane + 64 * q] * feat[lane + 64 * q];
+        s = wave_sum64(s);
+        if (lane == 0) y[(size_t)k * S::kOut + wv] = s + P[S::off_fb() + wv];
+    }
+}
+
+}  // namespace
+
 struct Seeder::Impl {
     hipStream_t st = nullptr;
     float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
@@ -344,6 +539,8 @@ struct Seeder::Impl {
     float *gy = nullptr, *g1 = nullptr, *g2 = nullptr, *g3 = nullptr, *g4 = nullptr, *sq = nullptr;
     int *ci = nullptr, *oi = nullptr;
     int icap = 0;
+    _Float16* wh = nullptr;   // fp16 weights of the MFMA inference
+    int pver = 0, hver = -1;  // parameter writes so far; the one wh was made from
 
     ~Impl() { release(); }
     void release() {
@@ -355,8 +552,11 @@ struct Seeder::Impl {
             }
         if (ci) (void)hipFree(ci);
         if (oi) (void)hipFree(oi);
+        if (wh) (void)hipFree(wh);
         ci = oi = nullptr;
+        wh = nullptr;
         cap = icap = 0;
+        hver = -1;
     }
     void params_alloc() {
         if (p) return;
@@ -391,6 +591,14 @@ struct Seeder::Impl {
         hck(hipMalloc(&gy, sizeof(float) * n * 6));
         hck(hipMalloc(&sq, sizeof(float) * n * 6));
         cap = B;
+    }
+    // the fp16 weight layouts, remade after any parameter write
+    void half_weights() {
+        if (!wh) hck(hipMalloc(&wh, sizeof(_Float16) * mf::kHalfWeights));
+        if (hver == pver) return;
+        hipLaunchKernelGGL(k_half_weights, dim3(blocks(mf::kHalfWeights, 256)), dim3(256), 0, st, p, wh);
+        hck(hipGetLastError());
+        hver = pver;
     }
     void forward(int B) {
         const float* P = p;
@@ -462,6 +670,7 @@ int Seeder::init(uint64_t seed) {
         for (int i = 0; i < S::kOut; ++i) h[S::off_fb() + i] = uni(bf);
         hck(hipMemcpyAsync(d_->p, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice, d_->st));
         hck(hipStreamSynchronize(d_->st));
+        d_->pver++;
         ready_ = true;
         return 0;
     } catch (const HipFailS&) {
@@ -491,6 +700,7 @@ int Seeder::load(const std::string& path) {
         d_->params_alloc();
         hck(hipMemcpyAsync(d_->p, h.data(), sizeof(float) * n, hipMemcpyHostToDevice, d_->st));
         hck(hipStreamSynchronize(d_->st));
+        d_->pver++;
     } catch (const HipFailS&) {
         return -3;
     }
@@ -542,6 +752,7 @@ int Seeder::train(const uint8_t* ref, int w, int h, const SeederTrainConfig& cfg
             // cosine decay from lr to lr_end
             const double c = 0.5 * (1.0 + std::cos(3.14159265358979 * s / (double)cfg.steps));
             const float lr = (float)(cfg.lr_end + (cfg.lr - cfg.lr_end) * c);
+            d.pver++;
             hipLaunchKernelGGL(k_adam, dim3(blocks(S::total(), 256)), dim3(256), 0, d.st, d.p, d.g, d.m, d.v, S::total(),
                                lr, b1, b2, 1e-8f, (float)(1.0 - p1), (float)(1.0 - p2));
             hck(hipGetLastError());
@@ -574,6 +785,7 @@ int Seeder::grad(const float* params, const float* x, const float* t, int batch,
         d.params_alloc();
         d.batch_alloc(batch);
         hck(hipMemcpyAsync(d.p, params, sizeof(float) * S::total(), hipMemcpyHostToDevice, d.st));
+        d.pver++;
         hck(hipMemcpyAsync(d.x, x, sizeof(float) * batch * 2 * 64 * 64, hipMemcpyHostToDevice, d.st));
         hck(hipMemcpyAsync(d.t, t, sizeof(float) * batch * 6, hipMemcpyHostToDevice, d.st));
         d.forward(batch);
@@ -600,8 +812,32 @@ int Seeder::forward(const float* params, const float* x, int batch, float* y) {
         d.params_alloc();
         d.batch_alloc(batch);
         hck(hipMemcpyAsync(d.p, params, sizeof(float) * S::total(), hipMemcpyHostToDevice, d.st));
+        d.pver++;
         hck(hipMemcpyAsync(d.x, x, sizeof(float) * batch * 2 * 64 * 64, hipMemcpyHostToDevice, d.st));
         d.forward(batch);
+        hck(hipMemcpyAsync(y, d.y, sizeof(float) * batch * 6, hipMemcpyDeviceToHost, d.st));
+        hck(hipStreamSynchronize(d.st));
+        ready_ = true;
+        return 0;
+    } catch (const HipFailS&) {
+        return -3;
+    }
+}
+
+int Seeder::forward_mfma(const float* params, const float* x, int batch, float* y) {
+    if (batch < 1 || batch > 4096) return -3;
+    try {
+        if (!ensure_stream(d_->st)) return -3;
+        Impl& d = *d_;
+        d.params_alloc();
+        d.batch_alloc(batch);
+        hck(hipMemcpyAsync(d.p, params, sizeof(float) * S::total(), hipMemcpyHostToDevice, d.st));
+        d.pver++;
+        hck(hipMemcpyAsync(d.x, x, sizeof(float) * batch * 2 * 64 * 64, hipMemcpyHostToDevice, d.st));
+        d.half_weights();
+        hipLaunchKernelGGL(k_seed_mfma<false>, dim3(batch), dim3(512), 0, d.st, nullptr, nullptr, 0, 0, nullptr,
+                           nullptr, d.x, d.wh, d.p, d.y);
+        hck(hipGetLastError());
         hck(hipMemcpyAsync(y, d.y, sizeof(float) * batch * 6, hipMemcpyDeviceToHost, d.st));
         hck(hipStreamSynchronize(d.st));
         ready_ = true;
@@ -634,10 +870,17 @@ int Seeder::infer(const float* r2, const float* d2, int w2, int h2, const int* c
         } swap(d.st, st);
         hck(hipMemcpyAsync(d.ci, c, sizeof(int) * 2 * n, hipMemcpyHostToDevice, st));
         hck(hipMemcpyAsync(d.oi, off, sizeof(int) * 2 * n, hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_cut, dim3(blocks((size_t)n * 4096, 256)), dim3(256), 0, st, r2, d2, w2, h2, d.ci, d.oi, n,
-                           d.x);
-        hipLaunchKernelGGL(k_normalize, dim3(2 * n), dim3(256), 0, st, d.x);
-        d.forward(n);
+        if (use_mfma) {
+            d.half_weights();
+            hipLaunchKernelGGL(k_seed_mfma<true>, dim3(n), dim3(512), 0, st, r2, d2, w2, h2, d.ci, d.oi, nullptr, d.wh,
+                               d.p, d.y);
+            hck(hipGetLastError());
+        } else {
+            hipLaunchKernelGGL(k_cut, dim3(blocks((size_t)n * 4096, 256)), dim3(256), 0, st, r2, d2, w2, h2, d.ci,
+                               d.oi, n, d.x);
+            hipLaunchKernelGGL(k_normalize, dim3(2 * n), dim3(256), 0, st, d.x);
+            d.forward(n);
+        }
         std::vector<float> y((size_t)n * 6);
         hck(hipMemcpyAsync(y.data(), d.y, sizeof(float) * y.size(), hipMemcpyDeviceToHost, st));
         hck(hipStreamSynchronize(st));

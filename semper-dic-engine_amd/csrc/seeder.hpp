@@ -73,6 +73,11 @@ public:
     int grad(const float* params, const float* x, const float* t, int batch, float* grads, float* loss);
     // The forward pass alone on a given batch: y receives B x 6 scaled outputs.
     int forward(const float* params, const float* x, int batch, float* y);
+    // The same through the inference kernel: fp16 activations and weights on
+    // MFMA with FP32 accumulation, one workgroup per pair.
+    int forward_mfma(const float* params, const float* x, int batch, float* y);
+    // infer runs the MFMA kernel (true) or the FP32 forward pass of training.
+    bool use_mfma = true;
     // Warps of n patch pairs cut from level-2 float planes: ref patch centred
     // on level-2 pixel c[2k], c[2k+1] (covering c - 32 .. c + 31), deformed
     // patch on c + off. out: n x 6, full-resolution (u, v, ux, uy, vx, vy) of

@@ -109,6 +109,7 @@ def load_dev(path=LIB_DEV):
     lib.sdk_to_mask8.argtypes = [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP]
     lib.sdk_seeder_grad.argtypes = [_VP, _VP, _VP, ctypes.c_int, _VP, _VP]
     lib.sdk_seeder_forward.argtypes = [_VP, _VP, ctypes.c_int, _VP]
+    lib.sdk_seeder_forward_mfma.argtypes = [_VP, _VP, ctypes.c_int, _VP]
     lib.sdk_seeder_train.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_char_p, _VP,
                                      ctypes.c_int]
     lib.sdk_engine_load_seeder.argtypes = [_VP, ctypes.c_char_p]

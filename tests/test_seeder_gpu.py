@@ -118,3 +118,26 @@ def test_trained_seeder_gains_on_the_learned_seeding_workload(sa):
     print(f"  rms error against the field {rms:.4f} px", flush=True)
     assert rms <= 0.06
     assert ss[0] > 0 and ss[1] >= 0.85 * ss[0]
+
+
+def test_mfma_inference_matches_the_fp32_forward(sa):
+    """The engine's inference kernel -- fp16 activations and weights on
+    v_mfma_f32_16x16x16f16 with FP32 accumulation, one workgroup per pair --
+    against the FP32 forward pass of training, on the committed weights: 96
+    normalised pairs, outputs in the network's scaled units (u, v / 16 px,
+    gradients / 0.1). fp16 keeps 11 significant bits, so through four
+    288-term layers the outputs agree to about 1e-3 of their scale."""
+    dev = sa.load_dev()
+    raw = open(sa.SEEDER_WEIGHTS, "rb").read()
+    n = int(np.frombuffer(raw[8:12], np.int32)[0])
+    params = np.frombuffer(raw[12:12 + 4 * n], np.float32).copy()
+    x, _ = seeder_batch(96, 21)
+    y32 = np.zeros((96, 6), np.float32)
+    y16 = np.zeros((96, 6), np.float32)
+    assert dev.sdk_seeder_forward(P(params), P(x), 96, P(y32)) == 0
+    assert dev.sdk_seeder_forward_mfma(P(params), P(x), 96, P(y16)) == 0
+    d = np.abs(y16 - y32)
+    scale = np.abs(y32).max()
+    print(f"MFMA vs FP32 forward: max |dy| {d.max():.2e}, mean {d.mean():.2e}, output scale {scale:.3f}", flush=True)
+    assert np.all(np.isfinite(y16))
+    assert d.max() <= 0.01 * scale + 1e-3 and d.mean() <= 0.002 * scale + 2e-4
