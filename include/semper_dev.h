@@ -79,7 +79,9 @@ SDK_API int sdk_wave_sum_probe(const float* in, int nw, float* out, int32_t* iou
    second time it is met with the same grid, parameters and buffers and
    replayed from then on; flag 11 = the learned seeder's inference on MFMA
    (default 1: fp16 activations and weights, FP32 accumulation, one
-   workgroup per patch pair; 0: the FP32 forward pass it is trained with). */
+   workgroup per patch pair; 0: the FP32 forward pass it is trained with);
+   flag 12 = keep a per-node report of the next runs (default 0; see
+   sdk_engine_node_report). */
 SDK_API int sdk_engine_set_flag(void* h, int flag, int value);
 
 /* Phase timings of the last run, 14 doubles: total, prep, seeding, prepass,
@@ -105,6 +107,14 @@ SDK_API int sdk_engine_set_subset_map(void* h, int gw, int gh, const int32_t* di
 SDK_API int sdk_engine_set_region_map(void* h, int rw, int rh, const int32_t* dims);
 struct semper_incremental;
 SDK_API int sdk_incremental_set_region_map(struct semper_incremental* inc, int rw, int rh, const int32_t* dims);
+
+/* Per-node provenance of the last run with flag 12 on (the reference's
+   debug export, full_field_debug_export.cpp, in a form of its own), row-major
+   over its grid: tag = the mesh tag of the node's guess (0 none, 1 inside a
+   lattice triangle, 2 extrapolated, 3 anchor, 4 global model), path = 0
+   skipped by the scan, 1 unsolved, 2 anchor, 3 Path A, 4 a later path. The
+   node count, or -3 (no report, or n too small). */
+SDK_API int sdk_engine_node_report(void* h, uint8_t* tag, uint8_t* path, int n);
 
 /* Graph captures and replays of an engine so far: stats2 = {captures,
    replays}. 0 or -3. */

@@ -280,6 +280,9 @@ SDK_API int sdk_engine_set_flag(void* h, int flag, int value) {
         case 11:
             ((DevEngine*)h)->eng.seeder_mfma = value != 0;
             return 0;
+        case 12:
+            ((DevEngine*)h)->eng.node_report = value != 0;
+            return 0;
         default:
             return -3;
     }
@@ -483,6 +486,15 @@ SDK_API int sdk_incremental_set_region_map(semper_incremental* inc, int rw, int 
     const int rc = inc->session.set_region_map(rw, rh, d) == 0 ? 0 : -3;
     inc->busy.store(false, std::memory_order_release);
     return rc;
+}
+
+SDK_API int sdk_engine_node_report(void* h, uint8_t* tag, uint8_t* path, int n) {
+    if (!h || !tag || !path) return -3;
+    const Engine& e = ((DevEngine*)h)->eng;
+    if (e.report_path.empty() || n < (int)e.report_path.size()) return -3;
+    std::memcpy(tag, e.report_tag.data(), e.report_tag.size());
+    std::memcpy(path, e.report_path.data(), e.report_path.size());
+    return (int)e.report_path.size();
 }
 
 SDK_API int sdk_engine_graph_stats(void* h, int32_t* stats2) {

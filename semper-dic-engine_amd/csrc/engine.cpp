@@ -804,6 +804,14 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                 rv[i] = a.p[1] - gs[1];
             }
             mark_kept(r.an, LW, LH, ru, rv);
+            // learned seeding (no reference counterpart): its lattice is dense
+            // enough that a smooth field's variation between neighbours exceeds
+            // the median test's 1 px, so an anchor that passed the output gate
+            // -- a converged ZNSSD match -- is kept without it; looser vertices
+            // still take the test, and the finished field takes its own
+            if (use_cnn)
+                for (AnchorRec& a : r.an)
+                    if (a.output) a.kept = true;
             std::vector<float> us, vs;
             std::vector<std::pair<double, double>> pts;
             for (const AnchorRec& a : r.an) {
@@ -1048,6 +1056,10 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             a.grid.step = p.step;
             a.grid.w = w_;
             a.grid.h = h_;
+            // learned seeding: a node the lattice does not cover has only the
+            // global model's guess, which the field's bumps can leave far
+            // behind; the layered flood claims it from its solved neighbours
+            a.grid.skip_model = use_cnn ? 1 : 0;
             a.pool = pool_.p;
             a.refI = I_.p;
             a.refGx = gx_.p;
@@ -1095,6 +1107,10 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             }
             tm_.kernel_path_a = kms;
             tm_.path_a_items = N;
+            if (node_report) {
+                after_a_.resize(NN);
+                ck(hipMemcpy(after_a_.data(), g.solved, NN, hipMemcpyDeviceToHost));
+            }
             st_a.points = sa[0];
             st_a.iters = sa[1];
             st_a.calls = sa[2];
@@ -1132,8 +1148,12 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             if (reach > 0) {
                 // Layered Path B. ok: accepted (solved before Path B with a real corr, or
                 // accepted here), w: their warps (seven planes as hp); fin: solved,
-                // skipped or finally failed; tried: the least claim distance a node has
-                // failed from (a later claim must come closer).
+                // skipped or finally failed; tried: claims must come closer than this
+                // (after a failed far claim only an adjacent one is made, so a node is
+                // claimed at most twice). Under learned seeding no claim takes the
+                // Simplex rescue: an adjacent claim's carried guess is off by the
+                // field's curvature over one step, and the nodes that fail it are
+                // mostly those whose subsets leave the frame.
                 std::vector<uint8_t> ok(NN, 0), fin(hsolved.begin(), hsolved.end());
                 std::vector<int8_t> tried(NN, (int8_t)(reach + 1));
                 std::vector<float> w(hp);
@@ -1198,7 +1218,7 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                         tk.cy = p.rect_y + y * p.step;
                         tk.pool = i;
                         tk.node = i;
-                        tk.mode = dist[i] == 1 ? NO_SEARCH : NO_SIMPLEX;
+                        tk.mode = dist[i] == 1 && !use_cnn ? NO_SEARCH : NO_SIMPLEX;
                         tk.g[0] = w[s0] + w[2 * NN + s0] * ddx + w[3 * NN + s0] * ddy;
                         tk.g[1] = w[NN + s0] + w[4 * NN + s0] * ddx + w[5 * NN + s0] * ddy;
                         for (int c = 0; c < 4; ++c) tk.g[2 + c] = w[(2 + c) * NN + s0];
@@ -1227,7 +1247,7 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                         } else if (tdist[t] == 1 || !r.init) {
                             fin[i] = 1;
                         } else {
-                            tried[i] = (int8_t)tdist[t];
+                            tried[i] = 2;   // a failed far claim waits for an adjacent one
                         }
                     }
                     size_t kept = 0;
@@ -1559,6 +1579,23 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             last_field.vx.assign(all.begin() + 4 * NN, all.begin() + 5 * NN);
             last_field.vy.assign(all.begin() + 5 * NN, all.begin() + 6 * NN);
             last_field.corr.assign(all.begin() + 6 * NN, all.begin() + 7 * NN);
+        }
+        if (node_report) {
+            std::vector<uint8_t> sol(NN), tg(NN);
+            std::vector<float> corr(NN);
+            ck(hipMemcpyAsync(sol.data(), g.solved, NN, hipMemcpyDeviceToHost, st_));
+            ck(hipMemcpyAsync(tg.data(), g.tag, NN, hipMemcpyDeviceToHost, st_));
+            ck(hipMemcpyAsync(corr.data(), g.corr, sizeof(float) * NN, hipMemcpyDeviceToHost, st_));
+            ck(hipStreamSynchronize(st_));
+            report_tag = tg;
+            report_path.assign(NN, kPathUnsolved);
+            for (size_t i = 0; i < NN; ++i) {
+                if (sol[i] && corr[i] < 0.0f) report_path[i] = kPathSkipped;
+                else if (!sol[i]) report_path[i] = kPathUnsolved;
+                else if (tg[i] == kMeshAnchor) report_path[i] = kPathAnchor;
+                else if (i < after_a_.size() && after_a_[i]) report_path[i] = kPathA;
+                else report_path[i] = kPathLater;
+            }
         }
         int dropped_outlier = 0;
         for (int pass = 0; pass < kOutlierMaxPasses; ++pass) dropped_outlier += hc[16 + pass];

@@ -172,10 +172,11 @@ public:
     // accepted node a breadth-first search in (corr, y, x) order reaches it
     // from first, carrying that node's warp over the offset, and solves them
     // all in one launch. Adjacent claims solve as the flood's do (Simplex
-    // rescue on) and a failure there is final; farther claims run without
-    // the rescue and a failure leaves the node open for a closer claim. So a
-    // node is claimed at most `reach` times, and the launches number about
-    // the depth of the unsolved regions over `reach`. 0: the reference flood.
+    // rescue on, except under learned seeding) and a failure there is final;
+    // farther claims run without
+    // the rescue and a failure leaves the node open for an adjacent claim. So
+    // a node is claimed at most twice, and the launches number about the
+    // depth of the unsolved regions over `reach`. 0: the reference flood.
     int layered_path_b = 0;
     static constexpr int kLayeredReachLearned = 8, kLayeredReachMax = 8;
     // `out` of run() is device memory (the packed points stay in HBM)
@@ -244,6 +245,13 @@ public:
     // transfers go through page-locked staging, counters are cleared by a
     // kernel, never a memset.
     bool use_graphs = true;
+    // Per-node provenance of the last run (the reference's debug export,
+    // full_field_debug_export.cpp, in a form of its own): for each grid node
+    // the mesh tag of its guess (Tag) and the path that solved it (NodePath).
+    // Costs two grid-sized copies per run while on.
+    bool node_report = false;
+    enum NodePath : uint8_t { kPathSkipped = 0, kPathUnsolved = 1, kPathAnchor = 2, kPathA = 3, kPathLater = 4 };
+    std::vector<uint8_t> report_tag, report_path;
     struct GraphStats {
         int captures = 0, replays = 0;
     };
@@ -325,6 +333,7 @@ private:
         int captures = 0, replays = 0;
     };
     Chain tail_chain_, strain_chain_;
+    std::vector<uint8_t> after_a_;   // solved flags after Path A, when node_report
     PinnedBuf<int32_t> pin_counts_;   // the counters after a run's chain
     PinnedBuf<float> pin_in_, pin_out_;
     PinnedBuf<uint8_t> pin_ok_in_, pin_ok_out_;

@@ -1111,10 +1111,11 @@ __device__ __forceinline__ bool item_task(const SolveArgs& a, int id, Task& t) {
     }
     const GridArgs& g = a.grid;
     if (!g.in_mesh[id] || g.grid.solved[id]) return false;
+    const int tag = g.grid.tag[id];
+    if (g.skip_model && tag == kMeshHomography) return false;
     t.cx = g.rect_x + (id % g.gw) * g.step;
     t.cy = g.rect_y + (id / g.gw) * g.step;
     t.pool = id;
-    const int tag = g.grid.tag[id];
     t.mode = (tag == kMeshExtrapolated || tag == kMeshHomography) ? NO_SIMPLEX : NO_SEARCH;
     t.g[0] = g.gu[id];
     t.g[1] = g.gv[id];

@@ -91,3 +91,33 @@ def test_layered_flood_against_reference_flood(env, reach):
     print(f"  vs FP64 within 1e-3 px: layered {np.mean(e1 <= 1e-3):.4f}, reference flood {np.mean(e0 <= 1e-3):.4f}",
           flush=True)
     assert np.mean(e1 <= 1e-3) >= np.mean(e0 <= 1e-3) - 2.0 / len(e0)
+
+
+def test_node_report_agrees_with_the_metrics(env):
+    """The per-node provenance report (flag 12) against the run's own metrics
+    slots: anchors published (slot 19), Path A points (slot 3) and the later
+    paths' points (slot 4) are the nodes the report gives to each path."""
+    dev, wl = env
+    name = "t_1024_noisy"
+    cfg = wl.config(name)
+    ref, dfm = wl.pair(dev, name)
+    prm = np.array(wl.solve_params(name), np.int32)
+    gw, gh = cfg["w"] // cfg["step"], cfg["h"] // cfg["step"]
+    h = dev.sdk_engine_create()
+    assert dev.sdk_engine_set_flag(h, 12, 1) == 0
+    assert dev.sdk_engine_set_reference(h, P(ref), 0, cfg["w"], cfg["h"], None) == 0
+    cap = (gw + 1) * (gh + 1) * 8
+    out = np.zeros(cap, np.float32)
+    met = np.zeros(24, np.float32)
+    assert dev.sdk_engine_run(h, P(dfm), 0, P(prm), P(out), cap, P(met), 24, ctypes.c_float(cfg["tol"])) > 0
+    tag = np.zeros(gw * gh, np.uint8)
+    path = np.zeros(gw * gh, np.uint8)
+    assert dev.sdk_engine_node_report(h, P(tag), P(path), gw * gh - 1) == -3
+    assert dev.sdk_engine_node_report(h, P(tag), P(path), gw * gh) == gw * gh
+    dev.sdk_engine_destroy(h)
+    counts = np.bincount(path, minlength=5)
+    print(f"report: {counts.tolist()}, metrics: anchors {met[19]:.0f}, A {met[3]:.0f}, B {met[4]:.0f}, "
+          f"attempted {met[0]:.0f}", flush=True)
+    assert counts[2] == met[19] and counts[3] == met[3] and counts[4] == met[4]
+    assert counts[1:].sum() == met[0]
+    assert np.all(tag[path == 2] == 3)
