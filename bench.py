@@ -293,6 +293,8 @@ def main():
     ap.add_argument("--no-precision", action="store_true", help="skip the precision-variant study")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-seeding comparison")
     ap.add_argument("--no-sequence", action="store_true", help="skip the 64-frame incremental sequence")
+    ap.add_argument("--u8-deformed", action="store_true",
+                    help="the solver samples the u8 deformed frame instead of its FP32 copy (same field)")
     ap.add_argument("--tol", type=float, default=None, help="IC-GN stop rule ||dp|| < tol (default: the workload's)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB copy kernel three times (the calibration of the HBM traffic "
@@ -318,6 +320,8 @@ def main():
     dref, ddef = wl.pair(dev, args.workload, on_device=True)
     t_gen = time.perf_counter() - t0
     eng = dev.sdk_engine_create()
+    if args.u8_deformed:
+        assert dev.sdk_engine_set_flag(eng, 13, 0) == 0
     assert dev.sdk_engine_set_reference(eng, dref, 1, w, h, None) == 0
     gw, gh = w // cfg["step"], h // cfg["step"]
     cap = gw * gh * 8

@@ -283,6 +283,9 @@ SDK_API int sdk_engine_set_flag(void* h, int flag, int value) {
         case 12:
             ((DevEngine*)h)->eng.node_report = value != 0;
             return 0;
+        case 13:
+            ((DevEngine*)h)->eng.def_f32 = value != 0;
+            return 0;
         default:
             return -3;
     }

@@ -441,13 +441,13 @@ int Engine::solve_tasks_dim(const std::vector<Task>& tasks, std::vector<SolveOut
     a.refGx = gx_.p;
     a.refGy = gy_.p;
     set_precision_args(a);
-    a.def = def_u8_.p;
+    a.def = def_plane();
     a.w = w_;
     a.h = h_;
     a.dim = dim;
     a.lm = 1;
     a.tol = tol;
-    ck(launch_solve(a, use6, false, st_));
+    ck(launch_solve(a, use6, def_f32_ready_, st_));
     ck(hipMemcpyAsync(out.data(), out_d_.p, sizeof(SolveOut) * n, hipMemcpyDeviceToHost, st_));
     ck(hipStreamSynchronize(st_));
     return 0;
@@ -520,13 +520,13 @@ int Engine::match_anchors(std::vector<AnchorMatch>& reqs, int subset, bool use6,
             a.refI = cI;
             a.refGx = cgx;
             a.refGy = cgy;
-            a.def = def_u8_.p;
+            a.def = def_plane();
             a.w = w_;
             a.h = h_;
             a.dim = subset;
             a.lm = 1;
             a.tol = tol;
-            ck(launch_solve(a, use6, false, st_));
+            ck(launch_solve(a, use6, def_f32_ready_, st_));
             std::vector<SolveOut> res(n);
             ck(hipMemcpyAsync(res.data(), out_d_.p, sizeof(SolveOut) * n, hipMemcpyDeviceToHost, st_));
             ck(hipStreamSynchronize(st_));
@@ -567,6 +567,12 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
         const auto t_prep = clk::now();
         if (!def_u8_.reserve(npx)) return kErrInit;
         ck(hipMemcpyAsync(def_u8_.p, def, npx, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st_));
+        def_f32_ready_ = false;
+        if (def_f32) {
+            if (!def_f32_.reserve(npx)) return kErrInit;
+            ck(launch_u8_to_f32(def_u8_.p, npx, def_f32_.p, st_));
+            def_f32_ready_ = true;
+        }
         tm_.prep = ms_since(t_prep);
         const int gw = p.rect_w / p.step, gh = p.rect_h / p.step, N = gw * gh;
         if (vsg_count(p.step, p.strain_window) < 3) return kErrStrainWindow;
@@ -1065,7 +1071,7 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             a.refGx = gx_.p;
             a.refGy = gy_.p;
             set_precision_args(a);
-            a.def = def_u8_.p;
+            a.def = def_plane();
             a.w = w_;
             a.h = h_;
             a.lm = 1;
@@ -1084,7 +1090,7 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             for (size_t ci = 0; ci < nc; ++ci) {
                 const DimClass& c = classes_[ci];
                 const bool listed = c.off >= 0;
-                keys[ci] = {gw, gh, c.dim, p.use6 ? 1 : 0, listed ? 1 : 0, c.n};
+                keys[ci] = {gw, gh, c.dim, p.use6 ? 1 : 0, listed ? 1 : 0, c.n, def_f32_ready_ ? 1 : 0};
                 shapes[ci] = shape_for(keys[ci], listed);
                 a.items = listed ? class_nodes_d_.p + c.off : nullptr;
                 a.nitems = c.n;
@@ -1092,7 +1098,7 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                 a.grid.dim = c.dim;
                 a.wpb = kShapes[shapes[ci]].wpb;
                 a.tile = listed ? 0 : kShapes[shapes[ci]].tile;
-                ck(launch_solve(a, p.use6, false, st_));
+                ck(launch_solve(a, p.use6, def_f32_ready_, st_));
                 ck(hipEventRecord(class_ev_[ci + 1], st_));
             }
             int sa[5];

@@ -203,6 +203,10 @@ public:
     // The seeder's inference: the fp16 MFMA kernel (true) or the FP32
     // forward pass it is trained with.
     bool seeder_mfma = true;
+    // The solver samples an FP32 copy of the deformed frame (made once per
+    // run; the values are the u8 ones, so the field is the same bit for bit)
+    // instead of unpacking four bytes per row of every Keys gather.
+    bool def_f32 = true;
     // Mixed subset sizes (no reference counterpart): the subset size of each
     // node of a gw x gh grid (row-major), used by runs on a grid of that shape
     // (any other shape, or an empty map: the run's subset everywhere). Each
@@ -279,6 +283,9 @@ private:
     bool have_ref_ = false;
     int w_ = 0, h_ = 0;
     DevBuf<uint8_t> ref_u8_, def_u8_;
+    DevBuf<float> def_f32_;      // the deformed frame as FP32, when def_f32
+    bool def_f32_ready_ = false;
+    const void* def_plane() const { return def_f32_ready_ ? (const void*)def_f32_.p : (const void*)def_u8_.p; }
     DevBuf<float> I_, gx_, gy_;
     DevBuf<float> pyr_r_[4], pyr_d_[4];
     int pyr_ref_levels_ = 0;

@@ -571,6 +571,26 @@ hipError_t launch_to_f16(const float* src, size_t n, uint16_t* dst, hipStream_t 
     return hipGetLastError();
 }
 
+// u8 -> FP32, four pixels per thread (exact: every u8 value is a float)
+__global__ void k_u8_to_f32(const uint8_t* __restrict__ src, size_t n, float* __restrict__ dst) {
+    const size_t i = 4 * ((size_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (i + 3 < n) {
+        const uint32_t b = *reinterpret_cast<const uint32_t*>(src + i);
+        const float4 v = make_float4((float)(b & 0xffu), (float)((b >> 8) & 0xffu), (float)((b >> 16) & 0xffu),
+                                     (float)(b >> 24));
+        *reinterpret_cast<float4*>(dst + i) = v;
+    } else {
+        for (size_t k = i; k < n; ++k) dst[k] = (float)src[k];
+    }
+}
+
+hipError_t launch_u8_to_f32(const uint8_t* src, size_t n, float* dst, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const size_t threads = (n + 3) / 4;
+    hipLaunchKernelGGL(k_u8_to_f32, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, src, n, dst);
+    return hipGetLastError();
+}
+
 hipError_t launch_pool_list(const float* I, const float* gx, const float* gy, int w, int h, const int* cxy,
                             int n, int dim, bool lm, PoolEntry* pool, hipStream_t st) {
     if (n <= 0) return hipSuccess;

@@ -82,6 +82,8 @@ hipError_t launch_pool_list(const float* I, const float* gx, const float* gy, in
 hipError_t launch_downsample_u8(const uint8_t* src, int w, int h, float* dst, hipStream_t st);
 // fp16 copy of a float plane (round to nearest), as half bits
 hipError_t launch_to_f16(const float* src, size_t n, uint16_t* dst, hipStream_t st);
+// u8 -> FP32 copy of a plane (src 4-byte aligned)
+hipError_t launch_u8_to_f32(const uint8_t* src, size_t n, float* dst, hipStream_t st);
 hipError_t launch_downsample_f32(const float* src, int w, int h, float* dst, hipStream_t st);
 
 // Global-motion patch search: per patch the best integer shift after the

@@ -93,3 +93,21 @@ def test_every_shape_gives_the_same_field(env, name):
     assert tried == list(range(nc))
     k = CANDIDATES.index(tuple(int(c) for c in chosen))
     assert ms[k] == min(ms[i] for i in tried)
+
+
+@pytest.mark.parametrize("use6", [0, 1], ids=["keys4", "keys6"])
+def test_fp32_deformed_plane_gives_the_same_field(env, use6):
+    """The solver sampling an FP32 copy of the deformed frame (flag 13, the
+    default) against sampling the u8 frame: the copy holds the same values,
+    and both kernels gather and sum in the same order, so the field, the
+    point count and the metrics are the same bit for bit."""
+    dev, wl = env
+    name = "t_1024_noisy"
+    cfg = wl.config(name)
+    ref, dfm = wl.pair(dev, name)
+    prm = np.array(wl.solve_params(name), np.int32)
+    prm[7] = use6
+    [(a, ma)], _ = run(dev, cfg, ref, dfm, prm, [(13, 0)])
+    [(b, mb)], _ = run(dev, cfg, ref, dfm, prm, [(13, 1)])
+    print(f"use6 {use6}: {len(a) // 8} points", flush=True)
+    assert np.array_equal(a, b) and np.array_equal(ma[COUNTS], mb[COUNTS])
