@@ -43,7 +43,7 @@ PEAK_FP32_TFLOPS = 157.3
 # HBM bytes per Path A launch from two rocprofv3 --pmc runs (FETCH_SIZE x 2.0,
 # WRITE_SIZE x 1.0, the factors a 256 MiB copy kernel reads under the same
 # counters); measured per workload, recorded with the kernel it was measured on
-TRAFFIC_BYTES = {"c4_4096_heterogeneous": 155575.8 * 1024 * 2.0 + 24357.5 * 1024 * 1.0}
+TRAFFIC_BYTES = {"c4_4096_heterogeneous": 178023.6 * 1024 * 2.0 + 24360.2 * 1024 * 1.0}
 
 
 def fp64_accuracy(ref, dfm, subset, pts, use6, k=2000, seed=0):
