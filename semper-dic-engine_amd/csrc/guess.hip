@@ -94,16 +94,30 @@ __global__ void k_guess(GuessArgs a, GridDev g, Planes p) {
         // 2. extrapolation within 3 steps of a triangle (FULL meshes only)
         if (a.quality == 2) {
             const float lim = -3.0f * a.step;
-            const int ntri = a.cell_tri[(a.LW - 1) * (a.LH - 1)];
             float best = lim - 1.0f;
             int bt = -1;
-            for (int k = 0; k < ntri; ++k) {
-                const TriDev& t = a.tris[k];
-                if (X < t.bx0 + lim || X > t.bx1 - lim || Y < t.by0 + lim || Y > t.by1 - lim) continue;
-                const double d = tri_test(t, X, Y, true);
-                if (d >= lim && (float)d > best) {
-                    best = (float)d;
-                    bt = k;
+            // a triangle lies in its cell, its box padded by 15 px: only cells
+            // within 15 px + 3 steps of the node can hold one that qualifies;
+            // visiting them in cell order keeps the triangles' order
+            const float reach = 15.0f - lim;
+            for (int lj = 0; lj + 1 < a.LH; ++lj) {
+                if ((float)(a.rect_y + a.lat_y[lj + 1] * a.step) + reach < Y ||
+                    (float)(a.rect_y + a.lat_y[lj] * a.step) - reach > Y)
+                    continue;
+                for (int li = 0; li + 1 < a.LW; ++li) {
+                    if ((float)(a.rect_x + a.lat_x[li + 1] * a.step) + reach < X ||
+                        (float)(a.rect_x + a.lat_x[li] * a.step) - reach > X)
+                        continue;
+                    const int c = lj * (a.LW - 1) + li;
+                    for (int k = a.cell_tri[c]; k < a.cell_tri[c + 1]; ++k) {
+                        const TriDev& t = a.tris[k];
+                        if (X < t.bx0 + lim || X > t.bx1 - lim || Y < t.by0 + lim || Y > t.by1 - lim) continue;
+                        const double d = tri_test(t, X, Y, true);
+                        if (d >= lim && (float)d > best) {
+                            best = (float)d;
+                            bt = k;
+                        }
+                    }
                 }
             }
             if (bt >= 0) {
