@@ -22,9 +22,13 @@
 //   PNM   OpenCV's PxM decoder: P1-P6, 8-bit samples rescaled v * 255 /
 //         maxval when maxval < 255, 16-bit samples (maxval > 255) kept,
 //         PBM ones black.
+//   JPEG  libjpeg-turbo's default decompression (jpeg.cpp): sequential and
+//         progressive Huffman, 8-bit, grey or YCbCr / RGB, ISLOW IDCT, fancy
+//         upsampling, EXIF orientation; masks take the Y plane.
 // Not decoded (the call fails with the reference's decode-failure result):
-// JPEG, WebP, BigTIFF, BMP run-length encoding, TIFF JPEG / CCITT / YCbCr
-// / CMYK / 12-bit samples, 16-bit palette or min-is-white TIFF.
+// CMYK / arithmetic / lossless / 12-bit JPEG, WebP, BigTIFF, BMP run-length
+// encoding, TIFF JPEG / CCITT / YCbCr / CMYK / 12-bit samples, 16-bit palette
+// or min-is-white TIFF.
 #include "codec.hpp"
 
 #include <zlib.h>
@@ -999,12 +1003,13 @@ bool header_dimensions(const uint8_t* d, size_t len, uint64_t& w, uint64_t& h) {
     return false;
 }
 
-Raster imdecode(const uint8_t* d, size_t len) {
+Raster imdecode(const uint8_t* d, size_t len, bool grey) {
     if (d == nullptr || len == 0 || len > (size_t)INT32_MAX) return {};
     uint64_t hw = 0, hh = 0;
     if (header_dimensions(d, len, hw, hh) && !dims_ok(hw, hh)) return {};
     Raster r;
     if (len >= 8 && std::memcmp(d, "\x89PNG\r\n\x1a\n", 8) == 0) r = decode_png(d, len);
+    else if (len >= 3 && d[0] == 0xFF && d[1] == 0xD8 && d[2] == 0xFF) r = decode_jpeg(d, len, grey);
     else if (len >= 4 && (std::memcmp(d, "II*\0", 4) == 0 || std::memcmp(d, "MM\0*", 4) == 0)) r = decode_tiff(d, len);
     else if (len >= 2 && d[0] == 'B' && d[1] == 'M') r = decode_bmp(d, len);
     else if (len >= 3 && d[0] == 'P') r = decode_pnm(d, len);
@@ -1198,11 +1203,11 @@ namespace {
 
 template <typename Convert>
 bool decode_any(const uint8_t* d, size_t len, int ew, int eh, std::vector<uint8_t>& out, int& w, int& h,
-                Convert convert) {
+                bool grey, Convert convert) {
     w = h = 0;
     if (d == nullptr || len == 0) return false;
     auto encoded = [&] {
-        const Raster r = imdecode(d, len);
+        const Raster r = imdecode(d, len, grey);
         if (r.empty() || !convert(r, out)) return false;
         w = r.w;
         h = r.h;
@@ -1234,11 +1239,14 @@ bool decode_any(const uint8_t* d, size_t len, int ew, int eh, std::vector<uint8_
 }  // namespace
 
 bool decode_gray(const uint8_t* d, size_t len, int ew, int eh, std::vector<uint8_t>& out, int& w, int& h) {
-    return decode_any(d, len, ew, eh, out, w, h, [](const Raster& r, std::vector<uint8_t>& o) { return to_gray8(r, o); });
+    return decode_any(d, len, ew, eh, out, w, h, false,
+                      [](const Raster& r, std::vector<uint8_t>& o) { return to_gray8(r, o); });
 }
 
 bool decode_mask(const uint8_t* d, size_t len, int ew, int eh, std::vector<uint8_t>& out, int& w, int& h) {
-    return decode_any(d, len, ew, eh, out, w, h, [](const Raster& r, std::vector<uint8_t>& o) { return to_mask8(r, o); });
+    // the reference reads masks with IMREAD_GRAYSCALE: a colour JPEG gives its Y plane
+    return decode_any(d, len, ew, eh, out, w, h, true,
+                      [](const Raster& r, std::vector<uint8_t>& o) { return to_mask8(r, o); });
 }
 
 void resize_nearest(const std::vector<uint8_t>& in, int sw, int sh, int dw, int dh, std::vector<uint8_t>& out) {

@@ -41,7 +41,11 @@ void image_dimensions(const uint8_t* d, size_t len, int& w, int& h);
 
 // An encoded file to a raster (empty when the bytes are not a file this
 // decoder reads, or declare more than kMaxPixels).
-Raster imdecode(const uint8_t* d, size_t len);
+// grey: the caller wants one channel (the reference's IMREAD_GRAYSCALE): a
+// colour JPEG then yields its Y plane; other formats ignore it.
+Raster imdecode(const uint8_t* d, size_t len, bool grey = false);
+// JPEG (jpeg.cpp): an empty raster when the file does not decode.
+Raster decode_jpeg(const uint8_t* d, size_t len, bool grey);
 
 // to_gray8 / to_mask8 (image_codec.cpp:221-252). False on an empty raster.
 bool to_gray8(const Raster& r, std::vector<uint8_t>& out);

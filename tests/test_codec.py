@@ -525,10 +525,95 @@ def test_truncated_tiff_is_refused(lib):
 
 
 def test_unsupported_formats_fail(lib):
-    """JPEG and WebP are not decoded here: the call fails, as a file that does
-    not decode fails in the reference."""
-    assert dec(lib, enc(Image.fromarray(G8), "JPEG")) is None
+    """WebP and CMYK JPEG are not decoded here: the call fails, as a file
+    that does not decode fails in the reference."""
     assert dec(lib, enc(Image.fromarray(RGB), "WEBP")) is None
+    assert dec(lib, enc(Image.fromarray(RGB).convert("CMYK"), "JPEG")) is None
+
+
+def speckle(h, w, seed, rgb):
+    """A smooth random texture (JPEG's natural case) with sharp edges mixed in."""
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[:h, :w]
+    chans = []
+    for c in range(3 if rgb else 1):
+        f = sum(np.sin(x * rng.uniform(0.05, 0.9) + y * rng.uniform(0.05, 0.9) + rng.uniform(0, 6)) for _ in range(5))
+        f = 128 + 25 * f + rng.normal(0, 12, (h, w))
+        f[(x // 7 + y // 5 + c) % 4 == 0] = 255 * (c % 2)
+        chans.append(np.clip(f, 0, 255))
+    a = np.stack(chans, -1).astype(np.uint8)
+    return a if rgb else a[..., 0]
+
+
+JPEG_CASES = [
+    ("grey", dict(quality=75)), ("grey", dict(quality=100)), ("grey", dict(quality=20, optimize=True)),
+    ("grey", dict(quality=85, progressive=True)),
+    ("444", dict(quality=90, subsampling=0)), ("422", dict(quality=80, subsampling=1)),
+    ("420", dict(quality=75, subsampling=2)), ("420", dict(quality=95, subsampling=2, optimize=True)),
+    ("420", dict(quality=70, subsampling=2, progressive=True)), ("444", dict(quality=60, subsampling=0, progressive=True)),
+    ("422", dict(quality=90, subsampling=1, restart_marker_blocks=3)),
+    ("420", dict(quality=85, subsampling=2, restart_marker_rows=1, progressive=True)),
+]
+
+
+@pytest.mark.parametrize("size", [(23, 37), (48, 64), (9, 17), (1, 1), (8, 8), (2, 3), (61, 3)])
+@pytest.mark.parametrize("kind,kw", JPEG_CASES, ids=[f"{k}-{i}" for i, (k, _) in enumerate(JPEG_CASES)])
+def test_jpeg_matches_libjpeg_turbo(lib, kind, kw, size):
+    """JPEG as libjpeg-turbo decodes it with its defaults (ISLOW IDCT, fancy
+    upsampling, JFIF YCbCr -> RGB), which is what cv::imdecode hands the
+    reference: PIL's decode of the same file, through the same luma, bit for
+    bit -- sequential and progressive, optimised Huffman tables, restart
+    intervals, 4:4:4 / 4:2:2 / 4:2:0, odd sizes down to 1 x 1."""
+    h, w = size
+    a = speckle(h, w, h * 131 + w, kind != "grey")
+    data = enc(Image.fromarray(a), "JPEG", **kw)
+    got = dec(lib, data)
+    assert got is not None
+    want = pil_gray(data)
+    assert got.shape == want.shape and np.array_equal(got, want), np.abs(got.astype(int) - want).max()
+
+
+@pytest.mark.parametrize("sub", [0, 1, 2])
+def test_jpeg_mask_is_the_y_plane(lib, sub):
+    """Masks are read with IMREAD_GRAYSCALE: libjpeg then delivers the Y
+    component itself, not a luma of the decoded RGB (PIL's YCbCr draft gives
+    the decoder's Y plane)."""
+    a = speckle(40, 56, 7 + sub, True)
+    data = enc(Image.fromarray(a), "JPEG", quality=80, subsampling=sub)
+    got = dec(lib, data, mask=True)
+    im = Image.open(io.BytesIO(data))
+    im.draft("YCbCr", im.size)
+    y = np.array(im)[..., 0]
+    assert im.mode == "YCbCr" and np.array_equal(got, y)
+    assert not np.array_equal(got, pil_gray(data))   # the luma of RGB differs
+
+
+@pytest.mark.parametrize("orientation", range(1, 9))
+def test_jpeg_exif_orientation(lib, orientation):
+    """OpenCV applies the EXIF orientation after decode (these flags do not
+    ask it to ignore it): the eight orientations against PIL's transpose."""
+    from PIL import ImageOps
+    a = speckle(30, 44, orientation, True)
+    exif = Image.Exif()
+    exif[0x0112] = orientation
+    data = enc(Image.fromarray(a), "JPEG", quality=90, exif=exif.tobytes())
+    got = dec(lib, data)
+    im = ImageOps.exif_transpose(Image.open(io.BytesIO(data)))
+    want = luma(np.array(im.convert("RGB")))
+    assert got.shape == want.shape and np.array_equal(got, want)
+
+
+def test_jpeg_truncated_and_garbage(lib):
+    """A JPEG cut short decodes with the missing data read as zero bits, as
+    libjpeg does (it warns and goes on); bytes that only start like a JPEG
+    fail."""
+    data = enc(Image.fromarray(speckle(64, 64, 3, True)), "JPEG", quality=90)
+    cut = data[: len(data) * 2 // 3]
+    got = dec(lib, cut)
+    full = dec(lib, data)
+    assert got is not None and got.shape == full.shape
+    assert np.array_equal(got[:8], full[:8])
+    assert dec(lib, b"\xff\xd8\xff\xe0" + b"\x00" * 40) is None
 
 
 def test_pnm_variants(lib):
