@@ -131,6 +131,12 @@ SDK_API int sdk_engine_graph_stats(void* h, int32_t* stats2);
    session's engine. 0 or -3. */
 SDK_API int sdk_incremental_set_flag(struct semper_incremental* inc, int flag, int value);
 SDK_API int sdk_incremental_graph_stats(struct semper_incremental* inc, int32_t* stats2);
+/* Host milliseconds of the session's last frame, 8 doubles: total, the
+   engine's run, the composition of the totals (sampling, re-anchoring,
+   strain of the total field), the reference update, and within the update
+   the field fill, the mask warp, the warped mask's bounds and the adoption
+   of the frame as the new reference. 0 or -3. */
+SDK_API int sdk_incremental_timings(struct semper_incremental* inc, double* t8);
 
 /* The tuner's latest timing: chosen2 = {subsets per workgroup, tile side (0:
    row-major)} of least mean time so far, ms6 = the mean Path A launch time

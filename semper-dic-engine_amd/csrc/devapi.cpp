@@ -517,6 +517,12 @@ SDK_API int sdk_incremental_set_flag(semper_incremental* inc, int flag, int valu
     return 0;
 }
 
+SDK_API int sdk_incremental_timings(semper_incremental* inc, double* t8) {
+    if (!inc || !t8) return -3;
+    for (int i = 0; i < 8; ++i) t8[i] = inc->session.timing[i];
+    return 0;
+}
+
 SDK_API int sdk_incremental_graph_stats(semper_incremental* inc, int32_t* stats2) {
     if (!inc || !stats2) return -3;
     const auto g = inc->session.graph_stats();

@@ -5,6 +5,7 @@
 #include "incremental.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -438,8 +439,15 @@ int IncrementalSession::init(const uint8_t* ref, int w, int h, const uint8_t* ma
 }
 
 // IncrementalSession::Impl::process (incremental_session.cpp:289-549)
+namespace {
+using sclk = std::chrono::steady_clock;
+double ms_from(sclk::time_point t) { return std::chrono::duration<double, std::milli>(sclk::now() - t).count(); }
+}  // namespace
+
 int IncrementalSession::process(const uint8_t* def, float* out, int out_capacity, float* metrics, int metrics_len,
                                 Cancel& cancel, const std::function<void(int)>& progress, FrameInfo* info) {
+    const auto t_all = sclk::now();
+    for (double& t : timing) t = 0.0;
     if (!init_ || !eng_ || !out || !def) return kErrInit;
     FrameInfo fi;
     fi.reference_frame = ref_index_;
@@ -486,8 +494,10 @@ int IncrementalSession::process(const uint8_t* def, float* out, int out_capacity
     eng_->keep_field = true;
     if (ref_index_ == 0) {
         // the engine's own run on frame 0, straight into the caller's buffer
+        const auto t_run = sclk::now();
         const int rc = eng_->run(def, false, user_mask0_.empty() ? nullptr : user_mask0_.data(), p0_, out,
                                  out_capacity, metrics, metrics_len, cancel, progress, tol);
+        timing[1] = ms_from(t_run);
         if (rc < 0) return rc;
         written = rc;
         const Engine::Field& f = eng_->last_field;
@@ -508,6 +518,7 @@ int IncrementalSession::process(const uint8_t* def, float* out, int out_capacity
     } else {
         const int gw = pk_.rect_w / pk_.step, gh = pk_.rect_h / pk_.step;
         scratch_.resize((size_t)std::max(gw * gh, 1) * 8);
+        const auto t_run = sclk::now();
         const int rc = eng_->run(def, false, mask_k_.data(), pk_, scratch_.data(), (int)scratch_.size(), metrics,
                                  metrics_len, cancel, progress, tol);
         if (rc == kErrRoi) {
@@ -517,6 +528,8 @@ int IncrementalSession::process(const uint8_t* def, float* out, int out_capacity
             anchors_.clear();
             return nothing_to_follow(fi, metrics, metrics_len, progress, info);
         }
+        timing[1] = ms_from(t_run);
+        const auto t_comp = sclk::now();
         if (rc < 0) return rc;
         const Engine::Field inc_field = eng_->last_field;
         const NodeField inc = node_field(inc_field);
@@ -611,6 +624,7 @@ int IncrementalSession::process(const uint8_t* def, float* out, int out_capacity
         const int rcp = eng_->strain_pack(tot, p0_.strain_window, out, out_capacity, nullptr, nullptr);
         if (rcp < 0) return rcp;
         written = rcp;
+        timing[2] = ms_from(t_comp);
     }
 
     for (uint8_t c : carried) fi.tracked_points += c ? 1 : 0;
@@ -627,34 +641,11 @@ int IncrementalSession::process(const uint8_t* def, float* out, int out_capacity
     }
 
     if (will_update) {
+        const auto t_up = sclk::now();
         const int before = (ref_index_ == 0) ? fi.frame0_points : alive_count();
         std::vector<float> coast_travel = coast_travel_;
-        auto anchors = anchors_;
-        for (size_t i = 0; i < n0; ++i) {
+        for (size_t i = 0; i < n0; ++i)
             coast_travel[i] = (carried[i] && !valid[i]) ? coast_travel[i] + coast_step[i] : 0.0f;
-            if (!carried[i] || (valid[i] && need[i] == AnchorNeed::kDrop)) {
-                anchors.erase(i);
-            } else if (valid[i] && need[i] == AnchorNeed::kRefresh) {
-                // make_anchor (incremental_math.cpp:224-235) on the frame becoming the reference
-                const float x = (float)(p0_.rect_x + (int)(i % gw_) * p0_.step) + U[i];
-                const float y = (float)(p0_.rect_y + (int)(i / gw_) * p0_.step) + V[i];
-                const int ax = (int)std::lround(x), ay = (int)std::lround(y);
-                const int r = subset / 2 + kAnchorMarginPx;
-                if (ax - r >= 0 && ay - r >= 0 && ax + r < w_ && ay + r < h_) {
-                    Anchor an;
-                    an.side = 2 * r + 1;
-                    an.ax = ax;
-                    an.ay = ay;
-                    an.dx = x - (float)ax;
-                    an.dy = y - (float)ay;
-                    an.patch.resize((size_t)an.side * an.side);
-                    for (int yy = 0; yy < an.side; ++yy)
-                        std::memcpy(&an.patch[(size_t)yy * an.side], def + (size_t)(ay - r + yy) * w_ + (ax - r),
-                                    (size_t)an.side);
-                    anchors[i] = std::move(an);
-                }
-            }
-        }
         // the next reference's mask (warp_material_mask) and grid (prepare_reference)
         std::vector<float> Ub = U, Vb = V;
         std::vector<uint8_t> alive = carried;
@@ -663,7 +654,10 @@ int IncrementalSession::process(const uint8_t* def, float* out, int out_capacity
         std::vector<uint8_t> mask_k((size_t)w_ * h_, 0);
         {
             std::vector<float> Uf = Ub, Vf = Vb;
-            if (fill_field(gw_, gh_, Uf, Vf, alive)) {
+            const auto t_fill = sclk::now();
+            const bool filled = fill_field(gw_, gh_, Uf, Vf, alive);
+            timing[4] = ms_from(t_fill);
+            if (filled) {
                 double stretch = 1.0;
                 for (int gy = 0; gy < gh_; ++gy)
                     for (int gx = 0; gx < gw_; ++gx) {
@@ -680,19 +674,38 @@ int IncrementalSession::process(const uint8_t* def, float* out, int out_capacity
                 const int Y0 = std::max(0, p0_.rect_y - margin);
                 const int Y1 = std::min(h_ - 1, p0_.rect_y + (gh_ - 1) * p0_.step + margin);
                 const MaskGrid mg{gw_, gh_, p0_.rect_x, p0_.rect_y, p0_.step};
+                const auto t_warp = sclk::now();
                 if (mask_warp(material_mask0_.data(), w_, h_, mg, Uf, Vf, X0, Y0, X1, Y1, per_px, mask_k, nullptr) != 0)
                     return kErrInit;
+                timing[5] = ms_from(t_warp);
             }
         }
+        const auto t_box = sclk::now();
         int bx0 = w_, by0 = h_, bx1 = -1, by1 = -1;
-        for (int y = 0; y < h_; ++y)
-            for (int x = 0; x < w_; ++x)
-                if (mask_k[(size_t)y * w_ + x]) {
-                    bx0 = std::min(bx0, x);
-                    bx1 = std::max(bx1, x);
-                    by0 = std::min(by0, y);
-                    by1 = std::max(by1, y);
-                }
+        for (int y = 0; y < h_; ++y) {
+            // first and last set pixel of the row, eight bytes at a time
+            const uint8_t* row = &mask_k[(size_t)y * w_];
+            int first = -1, last = -1, x = 0;
+            for (; x + 8 <= w_ && first < 0; x += 8) {
+                uint64_t v;
+                std::memcpy(&v, row + x, 8);
+                if (v) first = x + __builtin_ctzll(v) / 8;
+            }
+            for (; x < w_ && first < 0; ++x)
+                if (row[x]) first = x;
+            if (first < 0) continue;
+            for (x = w_; x - 8 >= first && last < 0; x -= 8) {
+                uint64_t v;
+                std::memcpy(&v, row + x - 8, 8);
+                if (v) last = x - 8 + (63 - __builtin_clzll(v)) / 8;
+            }
+            for (--x; x >= first && last < 0; --x)
+                if (row[x]) last = x;
+            bx0 = std::min(bx0, first);
+            bx1 = std::max(bx1, last);
+            by0 = std::min(by0, y);
+            by1 = std::max(by1, y);
+        }
         Params pk = p0_;
         float mnx = 1e30f, mny = 1e30f, mxx = -1e30f, mxy = -1e30f;
         for (int gy = 0; gy < gh_; ++gy)
@@ -728,12 +741,38 @@ int IncrementalSession::process(const uint8_t* def, float* out, int out_capacity
         }
         if (bx1 < 0) std::fill(alive.begin(), alive.end(), 0);
         // adopt the frame just solved, with the warped mask as its Ghost Wall
+        timing[6] = ms_from(t_box);
+        // the anchors (make_anchor, incremental_math.cpp:224-235), updated in
+        // place: nothing after this point can fail and leave the session whole
+        for (size_t i = 0; i < n0; ++i) {
+            if (!carried[i] || (valid[i] && need[i] == AnchorNeed::kDrop)) {
+                anchors_.erase(i);
+            } else if (valid[i] && need[i] == AnchorNeed::kRefresh) {
+                // on the frame becoming the reference
+                const float x = (float)(p0_.rect_x + (int)(i % gw_) * p0_.step) + U[i];
+                const float y = (float)(p0_.rect_y + (int)(i / gw_) * p0_.step) + V[i];
+                const int ax = (int)std::lround(x), ay = (int)std::lround(y);
+                const int r = subset / 2 + kAnchorMarginPx;
+                if (ax - r >= 0 && ay - r >= 0 && ax + r < w_ && ay + r < h_) {
+                    Anchor& an = anchors_[i];
+                    an.side = 2 * r + 1;
+                    an.ax = ax;
+                    an.ay = ay;
+                    an.dx = x - (float)ax;
+                    an.dy = y - (float)ay;
+                    an.patch.resize((size_t)an.side * an.side);
+                    for (int yy = 0; yy < an.side; ++yy)
+                        std::memcpy(&an.patch[(size_t)yy * an.side], def + (size_t)(ay - r + yy) * w_ + (ax - r),
+                                    (size_t)an.side);
+                }
+            }
+        }
+        const auto t_adopt = sclk::now();
         if (eng_->set_reference(def, false, w_, h_, mask_k.data()) != 0) {
             init_ = false;   // the engine lost its reference: the session cannot go on
             return kErrInit;
         }
         coast_travel_.swap(coast_travel);
-        anchors_.swap(anchors);
         ub_.swap(Ub);
         vb_.swap(Vb);
         if (c64_) {
@@ -751,6 +790,8 @@ int IncrementalSession::process(const uint8_t* def, float* out, int out_capacity
         frame0_points_ = frame0_points;
         fi.reference_updated = true;
         fi.lost_this_frame = std::max(0, before - alive_count());
+        timing[7] = ms_from(t_adopt);
+        timing[3] = ms_from(t_up);
     } else {
         frame_count_++;
         frames_since_update_++;
@@ -758,6 +799,7 @@ int IncrementalSession::process(const uint8_t* def, float* out, int out_capacity
         frame0_points_ = frame0_points;
     }
     if (info) *info = fi;
+    timing[0] = ms_from(t_all);
     return written;
 }
 

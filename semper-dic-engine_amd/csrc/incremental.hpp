@@ -66,6 +66,11 @@ public:
     // across reference updates are kept in double, each increment is added in
     // double, and the reported total is that sum rounded once. Kept across init.
     bool compose64 = false;
+    // Host milliseconds of the last process call: total, the engine's run,
+    // the composition of the totals (sampling, re-anchoring, strain of the
+    // total field), the reference update, and within it the field fill, the
+    // mask warp, the warped mask's bounds and the adoption of the frame.
+    double timing[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     Engine::GraphStats graph_stats() const { return eng_ ? eng_->graph_stats() : Engine::GraphStats{}; }
 
 private:
