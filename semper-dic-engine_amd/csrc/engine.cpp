@@ -1066,6 +1066,9 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             // global model's guess, which the field's bumps can leave far
             // behind; the layered flood claims it from its solved neighbours
             a.grid.skip_model = use_cnn ? 1 : 0;
+            // and a node IC-GN cannot finish from the lattice's guess is claimed
+            // again by the flood from a solved neighbour, so no rescue here
+            a.grid.no_rescue = use_cnn ? 1 : 0;
             a.pool = pool_.p;
             a.refI = I_.p;
             a.refGx = gx_.p;

@@ -17,6 +17,7 @@ struct GridArgs {
     const float *gu, *gv, *gux, *guy, *gvx, *gvy;
     int gw, gh, rect_x, rect_y, step, dim, w, h;
     int skip_model;   // leave nodes guessed from the global model to a later path
+    int no_rescue;    // no Simplex rescue: a failure is left to a later path
 };
 
 // Optional precision variants of the Hessian pool and the solver: J^T J on

@@ -1116,7 +1116,7 @@ __device__ __forceinline__ bool item_task(const SolveArgs& a, int id, Task& t) {
     t.cx = g.rect_x + (id % g.gw) * g.step;
     t.cy = g.rect_y + (id / g.gw) * g.step;
     t.pool = id;
-    t.mode = (tag == kMeshExtrapolated || tag == kMeshHomography) ? NO_SIMPLEX : NO_SEARCH;
+    t.mode = (tag == kMeshExtrapolated || tag == kMeshHomography || g.no_rescue) ? NO_SIMPLEX : NO_SEARCH;
     t.g[0] = g.gu[id];
     t.g[1] = g.gv[id];
     t.g[2] = g.gux[id];
