@@ -56,6 +56,7 @@ struct Huffman {
             valptr[len] = k;
             mincode[len] = code;
             for (int i = 0; i < counts[len - 1]; ++i) {
+                if (code >= (1 << len)) return false;   // over-subscribed
                 if (len <= 9) {
                     const int shift = 9 - len;
                     for (int f = 0; f < (1 << shift); ++f) look[(code << shift) | f] = (uint16_t)(len << 8 | v[k]);
@@ -64,7 +65,6 @@ struct Huffman {
                 ++k;
             }
             maxcode[len] = counts[len - 1] ? code - 1 : -1;
-            if (code > (1 << len)) return false;   // over-subscribed
             code <<= 1;
         }
         maxcode[17] = 0x7fffffff;
@@ -127,8 +127,9 @@ public:
         --nbits_;
         return r;
     }
+    // n <= 16 (callers clamp sizes read from a table)
     int get(int n) {
-        if (n == 0) return 0;
+        if (n <= 0) return 0;
         if (nbits_ < n) fill();
         const int r = (int)(bits_ >> (32 - n));
         bits_ <<= n;
@@ -171,6 +172,9 @@ private:
 };
 
 inline int extend(int v, int s) { return s == 0 ? 0 : (v < (1 << (s - 1)) ? v - (1 << s) + 1 : v); }
+
+// a DC size category from a (possibly corrupt) table: at most 16 bits follow
+inline int dc_size(int t) { return t > 16 ? 16 : t; }
 
 // ---- inverse DCT (ISLOW) ------------------------------------------------------
 constexpr int kConstBits = 13, kPass1Bits = 2;
@@ -543,7 +547,7 @@ Raster decode_jpeg(const uint8_t* d, size_t len, bool grey) {
             auto block = [&](Component& c, int16_t* blk) {
                 if (!progressive) {
                     int16_t z[64] = {};
-                    const int t = rd.decode(dc[c.td]);
+                    const int t = dc_size(rd.decode(dc[c.td]));
                     c.pred += extend(rd.get(t), t);
                     z[0] = (int16_t)c.pred;
                     for (int k = 1; k < 64;) {
@@ -562,7 +566,7 @@ Raster decode_jpeg(const uint8_t* d, size_t len, bool grey) {
                 }
                 if (Ss == 0) {                       // DC scans
                     if (Ah == 0) {
-                        const int t = rd.decode(dc[c.td]);
+                        const int t = dc_size(rd.decode(dc[c.td]));
                         c.pred += extend(rd.get(t), t);
                         blk[0] = (int16_t)(c.pred * (1 << Al));
                     } else if (rd.bit()) {

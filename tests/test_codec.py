@@ -650,3 +650,23 @@ def test_bmp_16bit_and_top_down(lib):
     info = struct.pack("<IiiHHIIiiII", 40, w, -h, 1, 24, 0, len(body), 0, 0, 0, 0)
     bmp = b"BM" + struct.pack("<IHHI", 14 + 40 + len(body), 0, 0, 54) + info + body
     assert np.array_equal(dec(lib, bmp), luma(rgb))
+
+
+def test_jpeg_damage_never_breaks_the_decoder(lib):
+    """Byte flips, truncations and spliced markers in valid JPEG files
+    (sequential, progressive, restart intervals): each either decodes to an
+    image of a plausible size or fails, never faults or hangs."""
+    rng = np.random.default_rng(5)
+    bases = [enc(Image.fromarray(speckle(37, 53, 1, True)), "JPEG", quality=80, subsampling=2),
+             enc(Image.fromarray(speckle(40, 33, 2, True)), "JPEG", quality=70, progressive=True),
+             enc(Image.fromarray(speckle(24, 64, 3, False)), "JPEG", quality=90, restart_marker_blocks=2)]
+    for base in bases:
+        for trial in range(400):
+            b = bytearray(base)
+            for _ in range(int(rng.integers(1, 6))):
+                i = int(rng.integers(2, len(b)))
+                b[i] = int(rng.integers(0, 256)) if rng.random() < 0.8 else 0xFF
+            if trial % 5 == 0:
+                b = b[: int(rng.integers(4, len(b)))]
+            got = dec(lib, bytes(b))
+            assert got is None or (got.ndim == 2 and got.size <= 1 << 28)
