@@ -1139,12 +1139,14 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
         ck(hipStreamSynchronize(st_));
         // with every node solved or skipped there is nothing to claim, and no global seed to try
         if (open_nodes > 0) {
-            std::vector<float> hp(7 * NN);
-            std::vector<uint8_t> hsolved(NN);
-            ck(hipMemcpyAsync(hp.data(), gplanes_.p, sizeof(float) * 7 * NN, hipMemcpyDeviceToHost, st_));
-            ck(hipMemcpyAsync(hsolved.data(), g.solved, NN, hipMemcpyDeviceToHost, st_));
+            // the node planes and flags, through page-locked staging
+            if (!pin_planes_.reserve(7 * NN) || !pin_flags_.reserve(NN)) throw HipFail{hipErrorOutOfMemory};
+            float* const hp = pin_planes_.p;
+            const uint8_t* const hsolved = pin_flags_.p;
+            ck(hipMemcpyAsync(hp, gplanes_.p, sizeof(float) * 7 * NN, hipMemcpyDeviceToHost, st_));
+            ck(hipMemcpyAsync(pin_flags_.p, g.solved, NN, hipMemcpyDeviceToHost, st_));
             ck(hipStreamSynchronize(st_));
-            const float* hu = hp.data();
+            const float* hu = hp;
             const float* hv = hu + NN;
             const float* hux = hv + NN;
             const float* huy = hux + NN;
@@ -1156,16 +1158,17 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             const int reach = layered_path_b > 0 ? layered_path_b : use_cnn ? kLayeredReachLearned : 0;
             if (reach > 0) {
                 // Layered Path B. ok: accepted (solved before Path B with a real corr, or
-                // accepted here), w: their warps (seven planes as hp); fin: solved,
+                // accepted here), w: their warps (the seven planes of hp, updated in
+                // place: the reference flood below runs only when no layer ran); fin: solved,
                 // skipped or finally failed; tried: claims must come closer than this
                 // (after a failed far claim only an adjacent one is made, so a node is
                 // claimed at most twice). Under learned seeding no claim takes the
                 // Simplex rescue: an adjacent claim's carried guess is off by the
                 // field's curvature over one step, and the nodes that fail it are
                 // mostly those whose subsets leave the frame.
-                std::vector<uint8_t> ok(NN, 0), fin(hsolved.begin(), hsolved.end());
+                std::vector<uint8_t> ok(NN, 0), fin(hsolved, hsolved + NN);
                 std::vector<int8_t> tried(NN, (int8_t)(reach + 1));
-                std::vector<float> w(hp);
+                float* const w = hp;
                 std::vector<int> open, stamp(NN, -1), dist(NN, 0), src(NN, -1), bfs, sources;
                 for (size_t i = 0; i < NN; ++i) {
                     ok[i] = hsolved[i] && hcorr[i] >= 0.f ? 1 : 0;
@@ -1288,7 +1291,7 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                         return a.x > b.x;
                     }
                 };
-                std::vector<unsigned char> claimed(hsolved.begin(), hsolved.end());
+                std::vector<unsigned char> claimed(hsolved, hsolved + NN);
                 std::vector<SeedNode> boundary, global;
                 const SeedNode pc{pcx, pcy, gU, gV, 0, 0, 0, 0, 0};
                 bool pc_queued = false;

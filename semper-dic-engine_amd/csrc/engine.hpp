@@ -344,6 +344,8 @@ private:
     PinnedBuf<int32_t> pin_counts_;   // the counters after a run's chain
     PinnedBuf<float> pin_in_, pin_out_;
     PinnedBuf<uint8_t> pin_ok_in_, pin_ok_out_;
+    PinnedBuf<float> pin_planes_;    // Path B's copy of the node planes
+    PinnedBuf<uint8_t> pin_flags_;
     template <typename F>
     void run_chain(Chain& c, const std::vector<uint64_t>& key, F&& enqueue);
     DevBuf<float> pack_out_;       // packed output points, 8 floats each
