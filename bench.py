@@ -254,6 +254,8 @@ def learned_seed_study(dev, sa, wl, steps, threads):
         ms = (time.perf_counter() - t0) * 1000.0 / steps
         ss = np.zeros(3, np.int32)
         dev.sdk_engine_seed_stats(eng, P(ss))
+        tm = np.zeros(14, np.float64)
+        dev.sdk_engine_timings(eng, P(tm))
         dev.sdk_engine_destroy(eng)
         n = int(n)
         pts = out[: max(n, 0) * 8].reshape(-1, 8).copy()
@@ -262,7 +264,11 @@ def learned_seed_study(dev, sa, wl, steps, threads):
         frac = acc["within_1e-3px"] if acc else 0.0
         res[mode] = {"ms_per_step": ms, "output_points": n, "value": solved * frac / (ms / 1000.0),
                      "accuracy_vs_fp64": acc, "anchors_cnn_seeded": int(ss[0]), "anchors_cnn_held": int(ss[1]),
-                     "anchors_plain_fallback": int(ss[2])}
+                     "anchors_plain_fallback": int(ss[2]),
+                     "phases_ms": {k: float(v) for k, v in zip(
+                         ["total", "prep", "seeding", "prepass", "mesh", "path_a", "path_b", "strain"], tm[:8])},
+                     "path_a_kernel_ms": float(tm[8]), "path_a_iterations": int(tm[10]),
+                     "path_b_launches": int(tm[12]), "path_b_solves": int(tm[13])}
     dev.sdk_free(dref)
     dev.sdk_free(ddef)
     if threads:

@@ -92,6 +92,12 @@ SDK_API int sdk_engine_set_flag(void* h, int flag, int value);
    Path B launches, subsets those launches solved. */
 SDK_API int sdk_engine_timings(void* h, double* out14);
 
+/* The subset-solver launches of the last run, in order: for each, the work
+   items and how many of them the fast path handed to the full solver
+   (masked iterations, coarse searches, Simplex rescues). Writes up to cap
+   (items, handed) pairs to out; returns the launch count, or -3. */
+SDK_API int sdk_engine_launch_log(void* h, int32_t* out, int cap);
+
 /* Mixed subset sizes (no reference counterpart): the subset size of each node
    of a gw x gh grid (row-major, odd, 3 .. 63) for the engine's runs on a grid
    of that shape; dims NULL clears the map. Each node is solved with its own
@@ -117,6 +123,10 @@ SDK_API int sdk_incremental_set_region_map(struct semper_incremental* inc, int r
    skipped by the scan, 1 unsolved, 2 anchor, 3 Path A, 4 a later path. The
    node count, or -3 (no report, or n too small). */
 SDK_API int sdk_engine_node_report(void* h, uint8_t* tag, uint8_t* path, int n);
+/* The node planes of the same report: u, v, ux, uy, vx, vy, corr, plane after
+   plane (7 x node count floats), at the end of the run (which 0) or after
+   Path A (which 1). The float count, or -3. */
+SDK_API int sdk_engine_node_planes(void* h, int which, float* out, int n);
 
 /* Graph captures and replays of an engine so far: stats2 = {captures,
    replays}. 0 or -3. */

@@ -500,6 +500,15 @@ SDK_API int sdk_engine_node_report(void* h, uint8_t* tag, uint8_t* path, int n) 
     return (int)e.report_path.size();
 }
 
+SDK_API int sdk_engine_node_planes(void* h, int which, float* out, int n) {
+    if (!h || !out || (which != 0 && which != 1)) return -3;
+    const Engine& e = ((DevEngine*)h)->eng;
+    const std::vector<float>& v = which == 0 ? e.report_planes : e.report_planes_a;
+    if (v.empty() || n < (int)v.size()) return -3;
+    std::memcpy(out, v.data(), sizeof(float) * v.size());
+    return (int)v.size();
+}
+
 SDK_API int sdk_engine_graph_stats(void* h, int32_t* stats2) {
     if (!h || !stats2) return -3;
     const auto g = ((DevEngine*)h)->eng.graph_stats();
@@ -559,6 +568,17 @@ SDK_API int sdk_engine_load_seeder(void* h, const char* path) {
     } catch (...) {
         return -3;
     }
+}
+
+SDK_API int sdk_engine_launch_log(void* h, int32_t* out, int cap) {
+    if (!h || (cap > 0 && !out) || cap < 0) return -3;
+    const auto& log = ((DevEngine*)h)->eng.launch_log;
+    const int n = (int)log.size();
+    for (int i = 0; i < n && i < cap; ++i) {
+        out[2 * i] = log[i].first;
+        out[2 * i + 1] = log[i].second;
+    }
+    return n;
 }
 
 SDK_API int sdk_engine_seed_stats(void* h, int32_t* out3) {

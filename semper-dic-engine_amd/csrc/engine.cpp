@@ -449,7 +449,10 @@ int Engine::solve_tasks_dim(const std::vector<Task>& tasks, std::vector<SolveOut
     a.tol = tol;
     ck(launch_solve(a, use6, def_f32_ready_, st_));
     ck(hipMemcpyAsync(out.data(), out_d_.p, sizeof(SolveOut) * n, hipMemcpyDeviceToHost, st_));
+    int handed = 0;
+    ck(hipMemcpyAsync(&handed, a.nbail, sizeof(int), hipMemcpyDeviceToHost, st_));
     ck(hipStreamSynchronize(st_));
+    launch_log.emplace_back(n, handed);
     return 0;
 }
 
@@ -556,6 +559,7 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                 const std::function<void(int)>& progress, float tol) {
     const auto t_total = clk::now();
     tm_ = Timings{};
+    launch_log.clear();
     if (!out || !have_ref_ || !def) return kErrInit;
     Params p = pin;
     const int rc0 = validate(p, w_, h_);
@@ -1104,7 +1108,7 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                 ck(launch_solve(a, p.use6, def_f32_ready_, st_));
                 ck(hipEventRecord(class_ev_[ci + 1], st_));
             }
-            int sa[5];
+            int sa[9];
             ck(hipMemcpyAsync(sa, counters_.p, sizeof(sa), hipMemcpyDeviceToHost, st_));
             ck(hipStreamSynchronize(st_));
             float kms = 0;
@@ -1116,9 +1120,12 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             }
             tm_.kernel_path_a = kms;
             tm_.path_a_items = N;
+            launch_log.emplace_back((int)N, sa[8]);
             if (node_report) {
                 after_a_.resize(NN);
                 ck(hipMemcpy(after_a_.data(), g.solved, NN, hipMemcpyDeviceToHost));
+                report_planes_a.resize(7 * NN);
+                ck(hipMemcpy(report_planes_a.data(), gplanes_.p, sizeof(float) * 7 * NN, hipMemcpyDeviceToHost));
             }
             st_a.points = sa[0];
             st_a.iters = sa[1];
@@ -1174,7 +1181,10 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                     ok[i] = hsolved[i] && hcorr[i] >= 0.f ? 1 : 0;
                     if (!hsolved[i]) open.push_back((int)i);
                 }
-                std::vector<int> commit_nodes;
+                // one entry per node, holding its last claim: a node claimed twice
+                // (a failed far claim, then an adjacent one) must not reach the
+                // apply kernel twice, where the two writes would race
+                std::vector<int> commit_nodes, commit_at(NN, -1);
                 std::vector<SolveOut> commit_res;
                 for (int layer = 0; !open.empty(); ++layer) {
                     if (cancel.requested()) return kErrCancelled;
@@ -1247,8 +1257,13 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                         const int i = tasks[t].node;
                         if (r.init) {
                             st_b.record(r);
-                            commit_nodes.push_back(i);
-                            commit_res.push_back(r);
+                            if (commit_at[i] < 0) {
+                                commit_at[i] = (int)commit_nodes.size();
+                                commit_nodes.push_back(i);
+                                commit_res.push_back(r);
+                            } else {
+                                commit_res[commit_at[i]] = r;
+                            }
                         }
                         if (r.init && r.status == 0 && r.corr <= kCorrAccept) {
                             st_b.points++;
@@ -1598,6 +1613,8 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             ck(hipMemcpyAsync(sol.data(), g.solved, NN, hipMemcpyDeviceToHost, st_));
             ck(hipMemcpyAsync(tg.data(), g.tag, NN, hipMemcpyDeviceToHost, st_));
             ck(hipMemcpyAsync(corr.data(), g.corr, sizeof(float) * NN, hipMemcpyDeviceToHost, st_));
+            report_planes.resize(7 * NN);
+            ck(hipMemcpyAsync(report_planes.data(), gplanes_.p, sizeof(float) * 7 * NN, hipMemcpyDeviceToHost, st_));
             ck(hipStreamSynchronize(st_));
             report_tag = tg;
             report_path.assign(NN, kPathUnsolved);

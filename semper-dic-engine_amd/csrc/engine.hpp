@@ -256,6 +256,9 @@ public:
     bool node_report = false;
     enum NodePath : uint8_t { kPathSkipped = 0, kPathUnsolved = 1, kPathAnchor = 2, kPathA = 3, kPathLater = 4 };
     std::vector<uint8_t> report_tag, report_path;
+    // with node_report: the seven node planes (u, v, ux, uy, vx, vy, corr,
+    // plane after plane) after Path A and at the end of the run
+    std::vector<float> report_planes_a, report_planes;
     struct GraphStats {
         int captures = 0, replays = 0;
     };
@@ -266,6 +269,10 @@ public:
         int cnn = 0, cnn_ok = 0, fallback = 0;
     };
     SeedStats seed_stats;
+    // (work items, items the fast path handed to the full solver) of each
+    // subset-solver launch of the last run, in launch order; a Path A launch
+    // over several subset sizes reports its last size's hand-offs
+    std::vector<std::pair<int, int>> launch_log;
     Field last_field;
 
     // Median test, VSG strain and row-major packing of a node field given on

@@ -253,13 +253,9 @@ struct Res {
 
 struct WaveLds {
     float H0[36];     // the pool's step matrix
-    float H[36];      // the step matrix of a masked iteration
-    float Hd[36];     // masked Hessian
-    float work[144];  // well_conditioned / Gauss-Jordan scratch
     float P[7 * 6];   // Simplex vertices
     float Y[7];
     int idx[7];
-    int flag;
 };
 
 enum Phase : int { PH_ICGN = 0, PH_SIMPLEX = 1, PH_DONE = 2 };
@@ -738,58 +734,63 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
         err = wave_sum(err);
 #pragma unroll
         for (int k = 0; k < 6; ++k) dp[k] = wave_sum(dp[k]);
-        const float* Hs = L.H0;
+        float d[6];
         if (masked) {
 #pragma unroll
             for (int k = 0; k < 21; ++k) hh[k] = wave_sum(hh[k]);
-            // masked Hessian (:373-413): 2x2 test, LM, rcond, inverse -- lane 0, in LDS
-            if (lane == 0) {
+            // masked Hessian (:373-413): 2x2 test, LM, rcond, inverse. The sums
+            // are wave-uniform, so every lane runs the same sequence in its own
+            // registers (the pool precompute's, hence the same bits).
+            float Hd[36], Hm[36];
+            {
                 int k = 0;
+#pragma unroll
                 for (int a = 0; a < 6; ++a)
+#pragma unroll
                     for (int b = a; b < 6; ++b) {
-                        L.Hd[a * 6 + b] = hh[k];
-                        L.Hd[b * 6 + a] = hh[k];
+                        Hd[a * 6 + b] = hh[k];
+                        Hd[b * 6 + a] = hh[k];
                         ++k;
                     }
-                const float* Hd = L.Hd;
-                const float det2 = Hd[0] * Hd[7] - Hd[6] * Hd[1];
-                const float nrm2 = Hd[0] * Hd[0] + Hd[1] * Hd[1] + Hd[6] * Hd[6] + Hd[7] * Hd[7];
-                const float cond2 = (fabsf(det2) > 1e-12f) ? (nrm2 / fabsf(det2)) : 1.0e13f;
-                int flag = 0;
-                if (cond2 > 1.0e12f) {
-                    flag = 1;
-                } else {
-                    if (lm) {
-                        L.Hd[0] += kLmAlpha;
-                        L.Hd[7] += kLmAlpha;
-                    }
-                    if (!well_conditioned_lds(L.Hd, half, L.work)) flag = 1;
-                    else gj_inverse_lds(L.Hd, L.H, L.work);
-                }
-                L.flag = flag;
             }
-            __syncthreads();
-            if (L.flag) {
+            const float det2 = Hd[0] * Hd[7] - Hd[6] * Hd[1];
+            const float nrm2 = Hd[0] * Hd[0] + Hd[1] * Hd[1] + Hd[6] * Hd[6] + Hd[7] * Hd[7];
+            const float cond2 = (fabsf(det2) > 1e-12f) ? (nrm2 / fabsf(det2)) : 1.0e13f;
+            bool flag = cond2 > 1.0e12f;
+            if (!flag) {
+                if (lm) {
+                    Hd[0] += kLmAlpha;
+                    Hd[7] += kLmAlpha;
+                }
+                flag = !well_conditioned_reg(Hd, half);
+            }
+            if (flag) {
                 finish_icgn(icgn_result(1, 2.0f, it));
-                __syncthreads();
                 continue;
             }
-            Hs = L.H;
-        }
-        final_score = err / (float)nv;
-        float d[6];
-        if (!masked && use_chol) {
-            chol64_step(ch, dp, d);
-        } else {
+            gj_inverse_reg(Hd, Hm);
+            final_score = err / (float)nv;
 #pragma unroll
             for (int r = 0; r < 6; ++r) {
                 float acc = 0.0f;
 #pragma unroll
-                for (int c = 0; c < 6; ++c) acc += Hs[r * 6 + c] * dp[c];
+                for (int c = 0; c < 6; ++c) acc += Hm[r * 6 + c] * dp[c];
                 d[r] = -acc;
             }
+        } else {
+            final_score = err / (float)nv;
+            if (use_chol) {
+                chol64_step(ch, dp, d);
+            } else {
+#pragma unroll
+                for (int r = 0; r < 6; ++r) {
+                    float acc = 0.0f;
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) acc += L.H0[r * 6 + c] * dp[c];
+                    d[r] = -acc;
+                }
+            }
         }
-        if (masked) __syncthreads();
         const float dW[9] = {1.0f + d[2], d[3], d[0], d[4], 1.0f + d[5], d[1], 0.0f, 0.0f, 1.0f};
         const float det = dW[0] * dW[4] - dW[1] * dW[3];
         bool fin = true;

@@ -57,75 +57,6 @@ __device__ __forceinline__ bool finite_f(float f) {
     return (__float_as_uint(f) & 0x7f800000u) != 0x7f800000u;
 }
 
-// Serial 6x6 algebra for one lane on (LDS) memory: Gauss-Jordan inverse with
-// partial pivoting -- the CPU oracle's sequence -- and hessian::well_conditioned.
-__device__ inline void gj_inverse_lds(const float* A, float* out, float* m /* 72 */) {
-    for (int r = 0; r < 6; ++r)
-        for (int c = 0; c < 6; ++c) {
-            m[r * 12 + c] = A[r * 6 + c];
-            m[r * 12 + 6 + c] = (r == c) ? 1.0f : 0.0f;
-        }
-    for (int k = 0; k < 6; ++k) {
-        int piv = k;
-        float best = fabsf(m[k * 12 + k]);
-        for (int r = k + 1; r < 6; ++r)
-            if (fabsf(m[r * 12 + k]) > best) {
-                best = fabsf(m[r * 12 + k]);
-                piv = r;
-            }
-        if (piv != k)
-            for (int c = 0; c < 12; ++c) {
-                const float t = m[k * 12 + c];
-                m[k * 12 + c] = m[piv * 12 + c];
-                m[piv * 12 + c] = t;
-            }
-        const float ip = 1.0f / m[k * 12 + k];
-        for (int c = 0; c < 12; ++c) m[k * 12 + c] *= ip;
-        for (int r = 0; r < 6; ++r) {
-            if (r == k) continue;
-            const float f = m[r * 12 + k];
-            if (f == 0.0f) continue;
-            for (int c = 0; c < 12; ++c) m[r * 12 + c] -= f * m[k * 12 + c];
-        }
-    }
-    for (int r = 0; r < 6; ++r)
-        for (int c = 0; c < 6; ++c) out[r * 6 + c] = m[r * 12 + 6 + c];
-}
-
-__device__ inline float norm1_lds(const float* m) {
-    float best = 0.0f;
-    for (int c = 0; c < 6; ++c) {
-        float s = 0.0f;
-        for (int r = 0; r < 6; ++r) s += fabsf(m[r * 6 + c]);
-        best = s > best ? s : best;
-    }
-    return best;
-}
-
-__device__ inline bool finite_lds(const float* m, int n) {
-    for (int i = 0; i < n; ++i)
-        if (!finite_f(m[i])) return false;
-    return true;
-}
-
-// hessian.hpp:40-52 on LDS scratch (work: 36 + 36 + 72 floats)
-__device__ inline bool well_conditioned_lds(const float* H, int half, float* work) {
-    const float s = half > 1 ? 1.0f / (float)half : 1.0f;
-    float* Hn = work;
-    float* Hi = work + 36;
-    float* gj = work + 72;
-    for (int r = 0; r < 6; ++r)
-        for (int c = 0; c < 6; ++c) {
-            const float dr = r < 2 ? 1.0f : s, dc = c < 2 ? 1.0f : s;
-            Hn[r * 6 + c] = dr * H[r * 6 + c] * dc;
-        }
-    if (!finite_lds(Hn, 36)) return false;
-    gj_inverse_lds(Hn, Hi, gj);
-    if (!finite_lds(Hi, 36)) return false;
-    const float a = norm1_lds(Hn), b = norm1_lds(Hi);
-    return 1.0f / (a * b) > 1e-6f;
-}
-
 // 3x3 inverse by cofactors, determinant from the first column.
 __device__ inline void inv3(const float* a, float* r) {
     auto cof = [&](int i, int j) {
@@ -152,10 +83,10 @@ __device__ inline void mul3(const float* a, const float* b, float* r) {
         }
 }
 
-// The same Gauss-Jordan sequence as gj_inverse_lds on a matrix held in
-// registers (every index is a compile-time constant once unrolled; the pivot
-// row moves by selects). Each element sees the same operations in the same
-// order, so the result is the same bits.
+// Gauss-Jordan inverse of a 6x6 with partial pivoting -- the CPU oracle's
+// sequence -- on a matrix held in registers (every index is a compile-time
+// constant once unrolled; the pivot row moves by selects): each element sees
+// the oracle's operations in its order.
 __device__ __forceinline__ void gj_inverse_reg(const float* A, float* out) {
     float m[6][12];
 #pragma unroll
@@ -214,7 +145,7 @@ __device__ __forceinline__ float norm1_reg(const float* m) {
     return best;
 }
 
-// hessian.hpp:40-52 in registers (the sequence of well_conditioned_lds).
+// hessian::well_conditioned (hessian.hpp:40-52) in registers.
 __device__ __forceinline__ bool well_conditioned_reg(const float* H, int half) {
     const float s = half > 1 ? 1.0f / (float)half : 1.0f;
     float Hn[36], Hi[36];

@@ -95,6 +95,7 @@ def load_dev(path=LIB_DEV):
     lib.sdk_engine_run.argtypes = [_VP, _VP, ctypes.c_int, _VP, _VP, ctypes.c_int, _VP, ctypes.c_int,
                                    ctypes.c_float]
     lib.sdk_engine_timings.argtypes = [_VP, _VP]
+    lib.sdk_engine_launch_log.argtypes = [_VP, _VP, ctypes.c_int]
     lib.sdk_engine_set_flag.argtypes = [_VP, ctypes.c_int, ctypes.c_int]
     lib.sdk_wave_sum_probe.argtypes = [_VP, ctypes.c_int, _VP, _VP]
     lib.sdk_synth_frame.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP,
@@ -120,6 +121,7 @@ def load_dev(path=LIB_DEV):
     lib.sdk_incremental_set_region_map.argtypes = [_VP, ctypes.c_int, ctypes.c_int, _VP]
     lib.sdk_engine_graph_stats.argtypes = [_VP, _VP]
     lib.sdk_engine_node_report.argtypes = [_VP, _VP, _VP, ctypes.c_int]
+    lib.sdk_engine_node_planes.argtypes = [_VP, ctypes.c_int, _VP, ctypes.c_int]
     lib.sdk_incremental_set_flag.argtypes = [_VP, ctypes.c_int, ctypes.c_int]
     lib.sdk_incremental_graph_stats.argtypes = [_VP, _VP]
     lib.sdk_incremental_timings.argtypes = [_VP, _VP]

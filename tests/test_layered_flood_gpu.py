@@ -121,3 +121,49 @@ def test_node_report_agrees_with_the_metrics(env):
     assert counts[2] == met[19] and counts[3] == met[3] and counts[4] == met[4]
     assert counts[1:].sum() == met[0]
     assert np.all(tag[path == 2] == 3)
+
+
+def test_learned_frame_repeats_bit_for_bit(env):
+    """The learned-seeding workload at full size (4096^2, eight Gaussian bumps
+    of up to +-30 px and a rotation of up to 5 degrees, step 8, subset 31),
+    run six times on one engine: every run reports the same points bit for
+    bit, and every node the report calls skipped is one the scan skipped
+    (the attempted count of slot 0). Under the layered flood a node can be
+    claimed twice, a failed far claim and then an adjacent one; both results
+    reaching the apply kernel raced, and a node accepted by the second claim
+    was now and then left with the first claim's invalid corr."""
+    dev, wl = env
+    sa = _load("semper_amd", os.path.join(PKG, "semper_amd.py"))
+    name = "c5_4096_bumps"
+    cfg = wl.config(name)
+    ref, dfm = wl.pair(dev, name)
+    prm = np.array(wl.solve_params(name), np.int32)
+    gw, gh = cfg["w"] // cfg["step"], cfg["h"] // cfg["step"]
+    cap = (gw + 1) * (gh + 1) * 8
+    h = dev.sdk_engine_create()
+    assert dev.sdk_engine_load_seeder(h, sa.SEEDER_WEIGHTS.encode()) == 0
+    assert dev.sdk_engine_set_flag(h, 5, 1) == 0
+    assert dev.sdk_engine_set_flag(h, 12, 1) == 0
+    assert dev.sdk_engine_set_reference(h, P(ref), 0, cfg["w"], cfg["h"], None) == 0
+    first = None
+    tag = np.zeros(gw * gh, np.uint8)
+    path = np.zeros(gw * gh, np.uint8)
+    log = np.zeros(2 * 64, np.int32)
+    for k in range(6):
+        out = np.zeros(cap, np.float32)
+        met = np.zeros(24, np.float32)
+        n = dev.sdk_engine_run(h, P(dfm), 0, P(prm), P(out), cap, P(met), 24, ctypes.c_float(cfg["tol"]))
+        assert n > 0, n
+        assert dev.sdk_engine_node_report(h, P(tag), P(path), gw * gh) == gw * gh
+        counts = np.bincount(path, minlength=5)
+        nl = dev.sdk_engine_launch_log(h, P(log), 64)
+        print(f"run {k}: {n} points, report {counts.tolist()}, attempted {met[0]:.0f}, {nl} solver launches",
+              flush=True)
+        assert counts[1:].sum() == met[0]
+        assert nl >= 3 and log[0] > 0 and np.all(log[1:2 * min(nl, 64):2] <= log[0:2 * min(nl, 64):2])
+        pts = out[: n * 8].copy()
+        if first is None:
+            first = pts
+        else:
+            assert pts.shape == first.shape and np.array_equal(pts.view(np.uint32), first.view(np.uint32))
+    dev.sdk_engine_destroy(h)
