@@ -83,7 +83,9 @@ SDK_API int sdk_wave_sum_probe(const float* in, int nw, float* out, int32_t* iou
    flag 12 = keep a per-node report of the next runs (default 0; see
    sdk_engine_node_report); flag 13 = the solver samples an FP32 copy of the
    deformed frame made once per run (default 1; the same field bit for bit)
-   rather than unpacking the u8 frame in every gather. */
+   rather than unpacking the u8 frame in every gather; flag 14 = a CNN-seeded
+   anchor whose IC-GN fails takes the Simplex rescue before the plain anchor
+   solve (default 1; 0: the plain solve at once). */
 SDK_API int sdk_engine_set_flag(void* h, int flag, int value);
 
 /* Phase timings of the last run, 14 doubles: total, prep, seeding, prepass,

@@ -286,6 +286,9 @@ SDK_API int sdk_engine_set_flag(void* h, int flag, int value) {
         case 13:
             ((DevEngine*)h)->eng.def_f32 = value != 0;
             return 0;
+        case 14:
+            ((DevEngine*)h)->eng.cnn_rescue = value != 0;
+            return 0;
         default:
             return -3;
     }

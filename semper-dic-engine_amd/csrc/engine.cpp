@@ -758,8 +758,9 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                     tasks[k].g[0] = (float)(U + q[2] * dx + q[3] * dy);
                     tasks[k].g[1] = (float)(V + q[4] * dx + q[5] * dy);
                     for (int j = 2; j < 6; ++j) tasks[k].g[j] = q[j];
-                    // IC-GN from the CNN's warp, with the Simplex rescue
-                    tasks[k].mode = NO_SEARCH;
+                    // IC-GN from the CNN's warp; where it fails, the plain
+                    // anchor solve below (the Simplex rescue first, if asked)
+                    tasks[k].mode = cnn_rescue ? NO_SEARCH : NO_SIMPLEX;
                 }
             }
             if (solve_tasks(tasks, res, p.use6, p.subset, tol) != 0) throw HipFail{hipErrorOutOfMemory};

@@ -203,6 +203,11 @@ public:
     // The seeder's inference: the fp16 MFMA kernel (true) or the FP32
     // forward pass it is trained with.
     bool seeder_mfma = true;
+    // A CNN-seeded anchor whose IC-GN fails takes the Simplex rescue before
+    // the plain solve (true), or goes to the plain solve at once (false:
+    // measured slower, as the anchors the rescue would hold are lost and the
+    // lattice misses its quality gate and is solved a second time).
+    bool cnn_rescue = true;
     // The solver samples an FP32 copy of the deformed frame (made once per
     // run; the values are the u8 ones, so the field is the same bit for bit)
     // instead of unpacking four bytes per row of every Keys gather.
