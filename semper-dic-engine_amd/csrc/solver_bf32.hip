@@ -6,13 +6,13 @@ namespace sdic {
 hipError_t launch_solve_bf32(const SolveArgs& a, int ppl, hipStream_t st) {
     if (a.nitems <= 0) return hipSuccess;
     const int wpb = a.wpb == 2 || a.wpb == 4 ? a.wpb : 1;
-    const dim3 block(64), fast_block(64 * wpb), fast_grid((a.nitems + wpb - 1) / wpb),
+    const dim3 block(64 * kSlowWaves), fast_block(64 * wpb), fast_grid((a.nitems + wpb - 1) / wpb),
         slow_grid(a.nitems < 2048 ? a.nitems : 2048);
     switch (ppl) {
 #define SDIC_CASE(P)                                                          \
     case P:                                                                   \
         hipLaunchKernelGGL((k_fast<P, false, float>), fast_grid, fast_block, 0, st, a); \
-        hipLaunchKernelGGL((k_slow<P, false, float>), slow_grid, block, 0, st, a); \
+        hipLaunchKernelGGL((k_slow<P, false, float, kSlowWaves>), slow_grid, block, 0, st, a); \
         break;
         SDIC_CASE(4)
         SDIC_CASE(8)

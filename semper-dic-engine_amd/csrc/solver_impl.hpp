@@ -258,10 +258,67 @@ struct WaveLds {
     int idx[7];
 };
 
+// Sums over the NW waves that solve one subset together (the full solver):
+// each wave's butterfly sum, then the waves' partials in wave order, so that
+// every wave holds the same bits and takes the same branches. red is
+// 2 x NW x kGroupSlots floats of LDS; par alternates its halves, so that one
+// sum's writes cannot overtake a slower wave still reading the previous sum
+// (the barrier between them orders every wave's reads before the next writes).
+constexpr int kGroupSlots = 24;
+constexpr int kSlowWaves = 4;   // waves per subset in the full solver
+
+template <int NW, int K>
+__device__ __forceinline__ void group_sum_n(float (&v)[K], float* red, int& par) {
+    static_assert(K <= kGroupSlots, "too many sums at once");
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+    if constexpr (NW > 1) {
+        const int wid = (int)(threadIdx.x >> 6);
+        float* b = red + par * NW * kGroupSlots;
+        if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) b[wid * kGroupSlots + k] = v[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            float acc = b[k];
+#pragma unroll
+            for (int w = 1; w < NW; ++w) acc += b[w * kGroupSlots + k];
+            v[k] = acc;
+        }
+        par ^= 1;
+    }
+}
+
+template <int NW>
+__device__ __forceinline__ float group_sum(float v, float* red, int& par) {
+    float a[1] = {v};
+    group_sum_n<NW, 1>(a, red, par);
+    return a[0];
+}
+
+template <int NW>
+__device__ __forceinline__ int group_sum_i(int v, float* red, int& par) {
+    v = wave_sum_i(v);
+    if constexpr (NW > 1) {
+        const int wid = (int)(threadIdx.x >> 6);
+        float* b = red + par * NW * kGroupSlots;
+        if ((threadIdx.x & 63) == 0) b[wid * kGroupSlots] = __int_as_float(v);
+        __syncthreads();
+        int acc = __float_as_int(b[0]);
+#pragma unroll
+        for (int w = 1; w < NW; ++w) acc += __float_as_int(b[w * kGroupSlots]);
+        v = acc;
+        par ^= 1;
+    }
+    return v;
+}
+
 enum Phase : int { PH_ICGN = 0, PH_SIMPLEX = 1, PH_DONE = 2 };
 enum SStage : int { S_INIT = 0, S_REFLECT = 1, S_EXPAND = 2, S_CONTRACT = 3, S_SHRINK = 4 };
 
-template <int PPL, bool K6, typename TD>
+template <int PPL, bool K6, typename TD, int NW>
 __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __restrict__ pool,
                                           const float* __restrict__ refI,
                                           const float* __restrict__ refGx,
@@ -269,8 +326,14 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
                                           const uint16_t* __restrict__ gx16, const uint16_t* __restrict__ gy16,
                                           const double* __restrict__ chol, const TD* __restrict__ def,
                                           int w, int h, int dim, bool lm, float conv_tol,
-                                          WaveLds& L, SolveOut& out) {
+                                          WaveLds& L, float* red, SolveOut& out) {
     const int lane = threadIdx.x & 63;
+    // NW waves solve the subset together: thread tid holds pixels tid + T j
+    constexpr int T = 64 * NW;
+    constexpr int SL = PPL / NW;
+    static_assert(PPL % NW == 0, "the pixel slots must split over the waves");
+    const int tid = (int)threadIdx.x;
+    int par = 0;   // the half of red the next group sum writes
     const PoolEntry* pe = pool + t.pool;
     out.init = 0;
     out.simplex = 0;
@@ -280,22 +343,22 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
     const int n = dim * dim;
 
     // ---- prologue: the subset in registers --------------------------------
-    float nref[PPL], gxs[PPL], gys[PPL], val[PPL];
+    float nref[SL], gxs[SL], gys[SL], val[SL];
     uint64_t refok = 0, inmask = 0, ok = 0;
     const float mean_r = pe->mean, inv_r = pe->inv_std;
-    // (q, r) = divmod(lane + 64 j, dim), advanced by divmod(64, dim) per slot
-    const int q64 = 64 / dim, r64 = 64 % dim;
-    const int q0 = lane / dim, r0 = lane % dim;
+    // (q, r) = divmod(tid + T j, dim), advanced by divmod(T, dim) per slot
+    const int q64 = T / dim, r64 = T % dim;
+    const int q0 = tid / dim, r0 = tid % dim;
     int bad = 0;
     {
         int q = q0, r = r0;
 #pragma unroll
-        for (int j = 0; j < PPL; ++j) {
+        for (int j = 0; j < SL; ++j) {
             nref[j] = 0.0f;
             gxs[j] = 0.0f;
             gys[j] = 0.0f;
             val[j] = 0.0f;
-            if (lane + 64 * j < n) {
+            if (tid + T * j < n) {
                 const int a = (t.cy + q - half) * w + t.cx + r - half;
                 const float ri = refI[a];
                 nref[j] = (ri - mean_r) * inv_r;
@@ -313,7 +376,7 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
             }
         }
     }
-    const int invalid_ref = wave_sum_i(bad);
+    const int invalid_ref = group_sum_i<NW>(bad, red, par);
     if (lane < 36) L.H0[lane] = pe->hstep[lane];
     // the FP64 Cholesky step on fully valid iterations, when the pool has one
     const double* ch = chol ? chol + (size_t)t.pool * kCholStride : nullptr;
@@ -326,14 +389,14 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
 #pragma unroll
     for (int k = 0; k < 6; ++k) start[k] = t.g[k];
     if (t.mode == AUTO_SEARCH && t.g[0] == 0.0f && t.g[1] == 0.0f) {
-        constexpr int SPL = 7;   // ceil(ceil(3969 / 9) / 64)
+        constexpr int SPL = (7 * 64 + T - 1) / T;   // ceil(ceil(3969 / 9) / T)
         float rr[SPL];
         int sxo[SPL], syo[SPL];
         uint32_t has = 0;
         const int sampled = (n + 8) / 9;
 #pragma unroll
         for (int k = 0; k < SPL; ++k) {
-            const int i = 9 * (lane + 64 * k);
+            const int i = 9 * (tid + T * k);
             rr[k] = 0.0f;
             sxo[k] = syo[k] = 0;
             if (i < n) {
@@ -363,10 +426,10 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
                 smd += dd[k];
                 m++;
             }
-            m = wave_sum_i(m);
+            m = group_sum_i<NW>(m, red, par);
             if (m < 3 || 2 * m < sampled) continue;
             const float fm = (float)m;
-            const float mr = wave_sum(smr) / fm, md = wave_sum(smd) / fm;
+            const float mr = group_sum<NW>(smr, red, par) / fm, md = group_sum<NW>(smd, red, par) / fm;
             float vr = 0.0f, vd = 0.0f;
 #pragma unroll
             for (int k = 0; k < SPL; ++k)
@@ -374,8 +437,8 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
                     vr += (rr[k] - mr) * (rr[k] - mr);
                     vd += (dd[k] - md) * (dd[k] - md);
                 }
-            vr = wave_sum(vr);
-            vd = wave_sum(vd);
+            vr = group_sum<NW>(vr, red, par);
+            vd = group_sum<NW>(vd, red, par);
             const float sr = sqrtf(vr / fm), sdv = sqrtf(vd / fm);
             if (!(sr > 1e-5f) || !(sdv > 1e-5f)) continue;
             float sc = 0.0f;
@@ -385,7 +448,7 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
                     const float e = (rr[k] - mr) / sr - (dd[k] - md) / sdv;
                     sc += e * e;
                 }
-            sc = wave_sum(sc) / fm;
+            sc = group_sum<NW>(sc, red, par) / fm;
             if (sc < best) {
                 best = sc;
                 bu = (float)u;
@@ -548,7 +611,7 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
             int qq = q0, rq = r0;
             asm volatile("" : "+v"(qq), "+v"(rq));
 #pragma unroll
-            for (int j = 0; j < PPL; ++j) {
+            for (int j = 0; j < SL; ++j) {
                 val[j] = 0.0f;
                 if ((inmask >> j) & 1) {
                     const float x = (float)(rq - half), y = (float)(qq - half);
@@ -572,8 +635,8 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
                 }
             }
         }
-        sum = wave_sum(sum);
-        const int nv = wave_sum_i(cnt);
+        sum = group_sum<NW>(sum, red, par);
+        const int nv = group_sum_i<NW>(cnt, red, par);
 
         if (!icgn) {
             // ---- evaluate_znssd (:467-528) ---------------------------------
@@ -582,24 +645,24 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
                 const float mean = sum / (float)nv;
                 float s2 = 0.0f;
 #pragma unroll
-                for (int j = 0; j < PPL; ++j)
+                for (int j = 0; j < SL; ++j)
                     if ((ok >> j) & 1) {
                         const float d = val[j] - mean;
                         s2 += d * d;
                     }
-                float dstd = sqrtf(wave_sum(s2) / (float)nv);
+                float dstd = sqrtf(group_sum<NW>(s2, red, par) / (float)nv);
                 if (dstd < 1e-5f) dstd = 1.0f;
                 const float inv = 1.0f / dstd;
                 const bool full = nv == n;
                 float z = 0.0f;
 #pragma unroll
-                for (int j = 0; j < PPL; ++j)
+                for (int j = 0; j < SL; ++j)
                     if ((ok >> j) & 1) {
                         const float nd = full ? (val[j] - mean) * inv : (val[j] - mean) / dstd;
                         const float d = nref[j] - nd;
                         z += d * d;
                     }
-                score = wave_sum(z) / (float)nv;
+                score = group_sum<NW>(z, red, par) / (float)nv;
                 if (!finite_f(score)) score = 2.0f;
             }
             // ---- Nelder-Mead step (:531-651) --------------------------------
@@ -689,12 +752,12 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
         const float mean = sum / (float)nv;
         float s2 = 0.0f;
 #pragma unroll
-        for (int j = 0; j < PPL; ++j)
+        for (int j = 0; j < SL; ++j)
             if ((ok >> j) & 1) {
                 const float d = val[j] - mean;
                 s2 += d * d;
             }
-        float dstd = sqrtf(wave_sum(s2) / (float)nv);
+        float dstd = sqrtf(group_sum<NW>(s2, red, par) / (float)nv);
         if (dstd < 1e-5f) dstd = 1.0f;
         const float inv = 1.0f / dstd;
         float err = 0.0f, dp[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
@@ -706,7 +769,7 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
             int qq = q0, rq = r0;
             asm volatile("" : "+v"(qq), "+v"(rq));
 #pragma unroll
-            for (int j = 0; j < PPL; ++j) {
+            for (int j = 0; j < SL; ++j) {
                 if ((ok >> j) & 1) {
                     const float x = (float)(rq - half), y = (float)(qq - half);
                     const float nd = (val[j] - mean) * inv;
@@ -731,13 +794,11 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
                 }
             }
         }
-        err = wave_sum(err);
-#pragma unroll
-        for (int k = 0; k < 6; ++k) dp[k] = wave_sum(dp[k]);
+        err = group_sum<NW>(err, red, par);
+        group_sum_n<NW, 6>(dp, red, par);
         float d[6];
         if (masked) {
-#pragma unroll
-            for (int k = 0; k < 21; ++k) hh[k] = wave_sum(hh[k]);
+            group_sum_n<NW, 21>(hh, red, par);
             // masked Hessian (:373-413): 2x2 test, LM, rcond, inverse. The sums
             // are wave-uniform, so every lane runs the same sequence in its own
             // registers (the pool precompute's, hence the same bits).
@@ -818,15 +879,15 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
             // texture check over the surviving pixels (:436-459)
             float sx = 0.0f, sy = 0.0f;
 #pragma unroll
-            for (int j = 0; j < PPL; ++j)
+            for (int j = 0; j < SL; ++j)
                 if ((ok >> j) & 1) {
                     float a = gxs[j], b = gys[j];
                     asm volatile("" : "+v"(a), "+v"(b));
                     sx += a * a;
                     sy += b * b;
                 }
-            sx = wave_sum(sx);
-            sy = wave_sum(sy);
+            sx = group_sum<NW>(sx, red, par);
+            sy = group_sum<NW>(sy, red, par);
             if (fminf(sx, sy) <= 0.0f || !finite_f(final_score))
                 finish_icgn(icgn_result(1, 2.0f, it + 1));
             else
@@ -1205,20 +1266,25 @@ __global__ __launch_bounds__(256) void k_fast(SolveArgs a) {
     commit(a, id, o);
 }
 
-// Full solver over the bail list: a fixed grid of waves, each striding over
-// the list (its length is read on the device, so no host round trip).
-template <int PPL, bool K6, typename TD>
-__global__ __launch_bounds__(64) void k_slow(SolveArgs a) {
-    __shared__ WaveLds L;
+// Full solver over the bail list: a fixed grid of NW-wave workgroups, each
+// striding over the list (its length is read on the device, so no host round
+// trip).
+template <int PPL, bool K6, typename TD, int NW>
+__global__ __launch_bounds__(64 * NW) void k_slow(SolveArgs a) {
+    // each wave keeps the solver's uniform state in its own copy; the waves
+    // meet only in the group sums
+    __shared__ WaveLds Ls[NW];
+    __shared__ float red[2 * NW * kGroupSlots];
+    WaveLds& L = Ls[threadIdx.x >> 6];
     const int nb = *a.nbail;
     for (int k = blockIdx.x; k < nb; k += gridDim.x) {
         const int id = a.bail[k];
         Task t;
         if (!item_task(a, id, t)) continue;
         SolveOut o = {};
-        full_solve<PPL, K6, TD>(t, a.pool, a.refI, a.refGx, a.refGy, a.refGx16, a.refGy16, a.chol,
-                                (const TD*)a.def, a.w, a.h, a.dim, a.lm != 0, a.tol, L, o);
-        if ((threadIdx.x & 63) == 0) commit(a, id, o);
+        full_solve<PPL, K6, TD, NW>(t, a.pool, a.refI, a.refGx, a.refGy, a.refGx16, a.refGy16, a.chol,
+                                    (const TD*)a.def, a.w, a.h, a.dim, a.lm != 0, a.tol, L, red, o);
+        if (threadIdx.x == 0) commit(a, id, o);
         __syncthreads();
     }
 }
