@@ -429,7 +429,11 @@ int Engine::solve_tasks_dim(const std::vector<Task>& tasks, std::vector<SolveOut
     out.assign(n, SolveOut{});
     if (n == 0) return 0;
     if (!tasks_d_.reserve(n) || !out_d_.reserve(n) || !bail_.reserve(n) || !counters_.reserve(32)) return kErrInit;
-    ck(hipMemcpyAsync(tasks_d_.p, tasks.data(), sizeof(Task) * n, hipMemcpyHostToDevice, st_));
+    // through page-locked staging: the copies are DMA transfers the stream
+    // orders, with no staging by the runtime
+    if (!pin_tasks_.reserve(n) || !pin_res_.reserve(n) || !pin_handed_.reserve(1)) return kErrInit;
+    std::memcpy(pin_tasks_.p, tasks.data(), sizeof(Task) * n);
+    ck(hipMemcpyAsync(tasks_d_.p, pin_tasks_.p, sizeof(Task) * n, hipMemcpyHostToDevice, st_));
     SolveArgs a = {};
     a.tasks = tasks_d_.p;
     a.out = out_d_.p;
@@ -448,11 +452,11 @@ int Engine::solve_tasks_dim(const std::vector<Task>& tasks, std::vector<SolveOut
     a.lm = 1;
     a.tol = tol;
     ck(launch_solve(a, use6, def_f32_ready_, st_));
-    ck(hipMemcpyAsync(out.data(), out_d_.p, sizeof(SolveOut) * n, hipMemcpyDeviceToHost, st_));
-    int handed = 0;
-    ck(hipMemcpyAsync(&handed, a.nbail, sizeof(int), hipMemcpyDeviceToHost, st_));
+    ck(hipMemcpyAsync(pin_res_.p, out_d_.p, sizeof(SolveOut) * n, hipMemcpyDeviceToHost, st_));
+    ck(hipMemcpyAsync(pin_handed_.p, a.nbail, sizeof(int32_t), hipMemcpyDeviceToHost, st_));
     ck(hipStreamSynchronize(st_));
-    launch_log.emplace_back(n, handed);
+    std::memcpy(out.data(), pin_res_.p, sizeof(SolveOut) * n);
+    launch_log.emplace_back(n, pin_handed_.p[0]);
     return 0;
 }
 
@@ -1174,10 +1178,24 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                 // Simplex rescue: an adjacent claim's carried guess is off by the
                 // field's curvature over one step, and the nodes that fail it are
                 // mostly those whose subsets leave the frame.
-                std::vector<uint8_t> ok(NN, 0), fin(hsolved, hsolved + NN);
-                std::vector<int8_t> tried(NN, (int8_t)(reach + 1));
+                // (node-sized scratch kept on the engine: refilled, not reallocated)
+                FloodScratch& fs = flood_;
+                fs.ok.assign(NN, 0);
+                fs.fin.assign(hsolved, hsolved + NN);
+                fs.tried.assign(NN, (int8_t)(reach + 1));
+                fs.stamp.assign(NN, -1);
+                fs.dist.assign(NN, 0);
+                fs.src.assign(NN, -1);
+                fs.commit_at.assign(NN, -1);
+                std::vector<uint8_t>& ok = fs.ok;
+                std::vector<uint8_t>& fin = fs.fin;
+                std::vector<int8_t>& tried = fs.tried;
+                std::vector<int>& stamp = fs.stamp;
+                std::vector<int>& dist = fs.dist;
+                std::vector<int>& src = fs.src;
+                std::vector<int>& commit_at = fs.commit_at;
                 float* const w = hp;
-                std::vector<int> open, stamp(NN, -1), dist(NN, 0), src(NN, -1), bfs, sources;
+                std::vector<int> open, bfs, sources;
                 for (size_t i = 0; i < NN; ++i) {
                     ok[i] = hsolved[i] && hcorr[i] >= 0.f ? 1 : 0;
                     if (!hsolved[i]) open.push_back((int)i);
@@ -1185,7 +1203,7 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                 // one entry per node, holding its last claim: a node claimed twice
                 // (a failed far claim, then an adjacent one) must not reach the
                 // apply kernel twice, where the two writes would race
-                std::vector<int> commit_nodes, commit_at(NN, -1);
+                std::vector<int> commit_nodes;
                 std::vector<SolveOut> commit_res;
                 for (int layer = 0; !open.empty(); ++layer) {
                     if (cancel.requested()) return kErrCancelled;

@@ -358,6 +358,15 @@ private:
     PinnedBuf<uint8_t> pin_ok_in_, pin_ok_out_;
     PinnedBuf<float> pin_planes_;    // Path B's copy of the node planes
     PinnedBuf<uint8_t> pin_flags_;
+    struct FloodScratch {   // the layered Path B's per-node state
+        std::vector<uint8_t> ok, fin;
+        std::vector<int8_t> tried;
+        std::vector<int> stamp, dist, src, commit_at;
+    };
+    FloodScratch flood_;
+    PinnedBuf<Task> pin_tasks_;        // a task launch's upload
+    PinnedBuf<SolveOut> pin_res_;      // and its results
+    PinnedBuf<int32_t> pin_handed_;    // and its hand-off count
     template <typename F>
     void run_chain(Chain& c, const std::vector<uint64_t>& key, F&& enqueue);
     DevBuf<float> pack_out_;       // packed output points, 8 floats each
