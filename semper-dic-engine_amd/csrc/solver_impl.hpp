@@ -1270,7 +1270,7 @@ __global__ __launch_bounds__(256) void k_fast(SolveArgs a) {
 // striding over the list (its length is read on the device, so no host round
 // trip).
 template <int PPL, bool K6, typename TD, int NW>
-__global__ __launch_bounds__(64 * NW) void k_slow(SolveArgs a) {
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2))) void k_slow(SolveArgs a) {
     // each wave keeps the solver's uniform state in its own copy; the waves
     // meet only in the group sums
     __shared__ WaveLds Ls[NW];
