@@ -1152,11 +1152,16 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
         // with every node solved or skipped there is nothing to claim, and no global seed to try
         if (open_nodes > 0) {
             // the node planes and flags, through page-locked staging
-            if (!pin_planes_.reserve(7 * NN) || !pin_flags_.reserve(NN)) throw HipFail{hipErrorOutOfMemory};
+            // (and the accepted flags, solved with a real corr, made on the device
+            // into the valid-flag buffer, which the end of the run rewrites)
+            if (!pin_planes_.reserve(7 * NN) || !pin_flags_.reserve(2 * NN)) throw HipFail{hipErrorOutOfMemory};
             float* const hp = pin_planes_.p;
             const uint8_t* const hsolved = pin_flags_.p;
+            const uint8_t* const haccepted = pin_flags_.p + NN;
+            ck(launch_valid(g, valid_d, N, st_));
             ck(hipMemcpyAsync(hp, gplanes_.p, sizeof(float) * 7 * NN, hipMemcpyDeviceToHost, st_));
             ck(hipMemcpyAsync(pin_flags_.p, g.solved, NN, hipMemcpyDeviceToHost, st_));
+            ck(hipMemcpyAsync(pin_flags_.p + NN, valid_d, NN, hipMemcpyDeviceToHost, st_));
             ck(hipStreamSynchronize(st_));
             const float* hu = hp;
             const float* hv = hu + NN;
@@ -1180,13 +1185,20 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                 // mostly those whose subsets leave the frame.
                 // (node-sized scratch kept on the engine: refilled, not reallocated)
                 FloodScratch& fs = flood_;
-                fs.ok.assign(NN, 0);
+                fs.ok.assign(haccepted, haccepted + NN);
                 fs.fin.assign(hsolved, hsolved + NN);
                 fs.tried.assign(NN, (int8_t)(reach + 1));
-                fs.stamp.assign(NN, -1);
-                fs.dist.assign(NN, 0);
-                fs.src.assign(NN, -1);
+                // dist and src are written before they are read (for each node a
+                // layer's search reaches); stamps are compared against this run's
+                // epoch plus the layer, so neither needs clearing between runs
+                if (fs.stamp.size() != NN || fs.epoch > (1 << 30)) {
+                    fs.stamp.assign(NN, -1);
+                    fs.epoch = 0;
+                }
+                fs.dist.resize(NN);
+                fs.src.resize(NN);
                 fs.commit_at.assign(NN, -1);
+                const int epoch = fs.epoch;
                 std::vector<uint8_t>& ok = fs.ok;
                 std::vector<uint8_t>& fin = fs.fin;
                 std::vector<int8_t>& tried = fs.tried;
@@ -1196,9 +1208,18 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                 std::vector<int>& commit_at = fs.commit_at;
                 float* const w = hp;
                 std::vector<int> open, bfs, sources;
-                for (size_t i = 0; i < NN; ++i) {
-                    ok[i] = hsolved[i] && hcorr[i] >= 0.f ? 1 : 0;
-                    if (!hsolved[i]) open.push_back((int)i);
+                // the open nodes, eight flags at a time (most are solved)
+                {
+                    size_t i = 0;
+                    for (; i + 8 <= NN; i += 8) {
+                        uint64_t f8;
+                        std::memcpy(&f8, hsolved + i, 8);
+                        if (f8 == 0x0101010101010101ull) continue;
+                        for (size_t k = i; k < i + 8; ++k)
+                            if (!hsolved[k]) open.push_back((int)k);
+                    }
+                    for (; i < NN; ++i)
+                        if (!hsolved[i]) open.push_back((int)i);
                 }
                 // one entry per node, holding its last claim: a node claimed twice
                 // (a failed far claim, then an adjacent one) must not reach the
@@ -1206,6 +1227,7 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                 std::vector<int> commit_nodes;
                 std::vector<SolveOut> commit_res;
                 for (int layer = 0; !open.empty(); ++layer) {
+                    fs.epoch = epoch + layer + 1;
                     if (cancel.requested()) return kErrCancelled;
                     // the accepted nodes next to open ones, in the order the flood pops them
                     sources.clear();
@@ -1215,8 +1237,8 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                             const int nx = x + dx4[k], ny = y + dy4[k];
                             if (nx < 0 || nx >= gw || ny < 0 || ny >= gh) continue;
                             const int j = ny * gw + nx;
-                            if (ok[j] && stamp[j] != layer) {
-                                stamp[j] = layer;
+                            if (ok[j] && stamp[j] != epoch + layer) {
+                                stamp[j] = epoch + layer;
                                 sources.push_back(j);
                             }
                         }
@@ -1240,8 +1262,8 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                             const int nx = x + dx4[k], ny = y + dy4[k];
                             if (nx < 0 || nx >= gw || ny < 0 || ny >= gh) continue;
                             const int m = ny * gw + nx;
-                            if (fin[m] || ok[m] || stamp[m] == layer) continue;
-                            stamp[m] = layer;
+                            if (fin[m] || ok[m] || stamp[m] == epoch + layer) continue;
+                            stamp[m] = epoch + layer;
                             dist[m] = dist[j] + 1;
                             src[m] = src[j];
                             bfs.push_back(m);

@@ -362,6 +362,7 @@ private:
         std::vector<uint8_t> ok, fin;
         std::vector<int8_t> tried;
         std::vector<int> stamp, dist, src, commit_at;
+        int epoch = 0;   // stamps below it belong to earlier layers or runs
     };
     FloodScratch flood_;
     PinnedBuf<Task> pin_tasks_;        // a task launch's upload
