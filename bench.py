@@ -217,6 +217,55 @@ def copy_probe():
     print("copy probe: 256 MiB read, 256 MiB written per dispatch", flush=True)
 
 
+def frames_in_flight(dev, sa, dref, ddef, prm, cap, tol, w, h, steps, k, expect, learned=True):
+    """One frame pair solved with k frames in flight: k engines, each with its
+    own stream and the same cached reference, driven by k host threads
+    (ctypes releases the GIL for the call), so that one frame's host phases
+    overlap the others' kernels. Every engine must report the single
+    engine's points bit for bit."""
+    import threading
+
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    engs, outs, mets = [], [], []
+    for _ in range(k):
+        e = dev.sdk_engine_create()
+        if learned:
+            if dev.sdk_engine_load_seeder(e, sa.SEEDER_WEIGHTS.encode()) != 0:
+                raise RuntimeError("seeder weights missing: " + sa.SEEDER_WEIGHTS)
+            assert dev.sdk_engine_set_flag(e, 5, 1) == 0
+        assert dev.sdk_engine_set_reference(e, dref, 1, w, h, None) == 0
+        out = np.zeros(cap, np.float32)
+        met = np.zeros(24, np.float32)
+        run = lambda e=e, out=out, met=met: dev.sdk_engine_run(e, ddef, 1, P(prm), P(out), cap, P(met), 24, tol)
+        run()
+        settle_tuner(dev, e, run)
+        run()
+        engs.append(e)
+        outs.append(out)
+        mets.append(met)
+    counts = [[] for _ in range(k)]
+
+    def work(i):
+        for _ in range(steps):
+            counts[i].append(int(dev.sdk_engine_run(engs[i], ddef, 1, P(prm), P(outs[i]), cap, P(mets[i]), 24,
+                                                    tol)))
+
+    ths = [threading.Thread(target=work, args=(i,)) for i in range(k)]
+    t0 = time.perf_counter()
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    dt = time.perf_counter() - t0
+    same = all(c == [expect[0]] * steps for c in counts) and all(
+        np.array_equal(o[: expect[0] * 8].view(np.uint32), expect[1].view(np.uint32)) for o in outs)
+    solved = float(mets[0][3] + mets[0][4] + mets[0][19])
+    for e in engs:
+        dev.sdk_engine_destroy(e)
+    return {"frames_in_flight": k, "frames": k * steps, "ms_per_frame": dt * 1000.0 / (k * steps),
+            "solved_per_frame": solved, "same_points_as_one_engine": bool(same)}
+
+
 def learned_seed_study(dev, sa, wl, steps, threads):
     """The learned-seeding workload (4096^2, eight seeded Gaussian bumps of up
     to +-30 px plus a global rotation of up to 5 degrees, step 8, subset 31):
@@ -237,6 +286,7 @@ def learned_seed_study(dev, sa, wl, steps, threads):
     tol = ctypes.c_float(cfg["tol"])
     res = {"workload": f"{name}: {w}x{h} speckle, eight Gaussian bumps (+-30 px, width 200-800 px) and a rotation "
                        f"of up to 5 deg, step {cfg['step']}, subset {cfg['subset']}", "weights": "seeder_weights.bin"}
+    single = None
     for mode in ("plain_gpu", "learned_seed_gpu"):
         eng = dev.sdk_engine_create()
         if mode == "learned_seed_gpu":
@@ -262,6 +312,8 @@ def learned_seed_study(dev, sa, wl, steps, threads):
         acc = fp64_accuracy(ref, dfm, cfg["subset"], pts, False) if n > 0 else None
         solved = float(met[3] + met[4] + met[19]) if n > 0 else 0.0
         frac = acc["within_1e-3px"] if acc else 0.0
+        if mode == "learned_seed_gpu" and n > 0:
+            single = (n, out[: n * 8].copy())
         res[mode] = {"ms_per_step": ms, "output_points": n, "value": solved * frac / (ms / 1000.0),
                      "accuracy_vs_fp64": acc, "anchors_cnn_seeded": int(ss[0]), "anchors_cnn_held": int(ss[1]),
                      "anchors_plain_fallback": int(ss[2]),
@@ -269,6 +321,12 @@ def learned_seed_study(dev, sa, wl, steps, threads):
                          ["total", "prep", "seeding", "prepass", "mesh", "path_a", "path_b", "strain"], tm[:8])},
                      "path_a_kernel_ms": float(tm[8]), "path_a_iterations": int(tm[10]),
                      "path_b_launches": int(tm[12]), "path_b_solves": int(tm[13])}
+    if single is not None:
+        frac = (res["learned_seed_gpu"]["accuracy_vs_fp64"] or {}).get("within_1e-3px", 0.0)
+        for k in (2, 3):
+            fl = frames_in_flight(dev, sa, dref, ddef, prm, cap, tol, w, h, max(4, 2 * steps), k, single)
+            fl["value"] = fl["solved_per_frame"] * frac / (fl["ms_per_frame"] / 1000.0)
+            res[f"learned_seed_gpu_{k}_in_flight"] = fl
     dev.sdk_free(dref)
     dev.sdk_free(ddef)
     if threads:
@@ -375,6 +433,13 @@ def main():
     host_ms = (time.perf_counter() - t1) * 1000.0 / nh
     # restore the device-input run's output for the checks below
     n = dev.sdk_engine_run(eng, ddef, 1, P(prm), P(out), cap, P(met), 24, tol)
+    # the same frame with 2 and 3 frames in flight (engines on their own
+    # streams and host threads); reported beside value, never as it
+    in_flight = {}
+    if n > 0:
+        for k in (2, 3):
+            in_flight[str(k)] = frames_in_flight(dev, sa, dref, ddef, prm, cap, tol, w, h, max(4, args.steps), k,
+                                                 (int(n), out[: int(n) * 8].copy()), learned=args.learned_seed)
     solved = float(met[3] + met[4] + met[19])
     n = int(n)
     shape = np.zeros(2, np.int32)
@@ -413,6 +478,8 @@ def main():
     sequence = None if args.no_sequence else sequence_study(dev, sa, wl)
     frac = acc["within_1e-3px"] if acc else 1.0
     value = float(solved * frac / (ms_per_step / 1000.0))
+    for fl in in_flight.values():
+        fl["value"] = fl["solved_per_frame"] * frac / (fl["ms_per_frame"] / 1000.0)
     line = {
         "metric": "converged subsets/sec at \u22641e-3 px: learned-seed vs plain GPU vs CPU, 4096\u00b2",
         "value": value,
@@ -447,6 +514,7 @@ def main():
         "phases_ms": {k: float(v) for k, v in zip(
             ["total", "prep", "seeding", "prepass", "mesh", "path_a", "path_b", "strain"], tm[:8])},
         "path_b": {"speculative_solves": int(tm[11]), "further_launches": int(tm[12]), "further_solves": int(tm[13])},
+        "frames_in_flight": in_flight,
         "prepass_ms": prepass_ms,
         "precision_study": precision,
         "learned_seed": learned,
