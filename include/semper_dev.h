@@ -68,8 +68,8 @@ SDK_API int sdk_wave_sum_probe(const float* in, int nw, float* out, int32_t* iou
    of the order given at sdk_engine_tuning on every Path A launch (where only
    whole grids take tiles, a listed launch takes the same workgroup width
    row-major; 0: the tuner decides); flag 8 = layered Path B with the given
-   reach, 1 .. 8 (default 0: the reference's flood of 32 pops per round;
-   learned seeding uses reach 8 unless set): each layer claims every open node
+   reach, 1 .. 32 (default 0: the reference's flood of 32 pops per round;
+   learned seeding uses reach 16 unless set): each layer claims every open node
    within `reach` grid steps of an accepted one, seeded from the accepted node
    a breadth-first search in (corr, y, x) order reaches it from first, and
    solves them in one launch; a failure beyond one step leaves the node open
