@@ -209,7 +209,11 @@ bool consensus_affine(const std::vector<double>& x, const std::vector<double>& y
         double res = 0.0;
         for (int q = 0; q < n; ++q) {
             const double dx = x[q] - cx, dy = y[q] - cy;
-            const double e = std::hypot(su[0] + su[1] * dx + su[2] * dy - u[q], sv[0] + sv[1] * dx + sv[2] * dy - v[q]);
+            const double eu = su[0] + su[1] * dx + su[2] * dy - u[q], ev = sv[0] + sv[1] * dx + sv[2] * dy - v[q];
+            // a patch clearly beyond the tolerance is out without the hypot
+            // (which is within an ulp of the root, far inside the margin)
+            if (eu * eu + ev * ev > kConsensusTol * kConsensusTol * (1.0 + 1e-9)) continue;
+            const double e = std::hypot(eu, ev);
             if (e <= kConsensusTol) {
                 ++in;
                 res += e;
