@@ -426,10 +426,13 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
                 smd += dd[k];
                 m++;
             }
-            m = group_sum_i<NW>(m, red, par);
+            // the count with the two sums in one group sum (counts add exactly in float)
+            float msd[3] = {smr, smd, (float)m};
+            group_sum_n<NW, 3>(msd, red, par);
+            m = (int)msd[2];
             if (m < 3 || 2 * m < sampled) continue;
             const float fm = (float)m;
-            const float mr = group_sum<NW>(smr, red, par) / fm, md = group_sum<NW>(smd, red, par) / fm;
+            const float mr = msd[0] / fm, md = msd[1] / fm;
             float vr = 0.0f, vd = 0.0f;
 #pragma unroll
             for (int k = 0; k < SPL; ++k)
@@ -437,8 +440,10 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
                     vr += (rr[k] - mr) * (rr[k] - mr);
                     vd += (dd[k] - md) * (dd[k] - md);
                 }
-            vr = group_sum<NW>(vr, red, par);
-            vd = group_sum<NW>(vd, red, par);
+            float vv[2] = {vr, vd};
+            group_sum_n<NW, 2>(vv, red, par);
+            vr = vv[0];
+            vd = vv[1];
             const float sr = sqrtf(vr / fm), sdv = sqrtf(vd / fm);
             if (!(sr > 1e-5f) || !(sdv > 1e-5f)) continue;
             float sc = 0.0f;
@@ -635,8 +640,11 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
                 }
             }
         }
-        sum = group_sum<NW>(sum, red, par);
-        const int nv = group_sum_i<NW>(cnt, red, par);
+        // the sum and the count in one group sum (counts add exactly in float)
+        float sc[2] = {sum, (float)cnt};
+        group_sum_n<NW, 2>(sc, red, par);
+        sum = sc[0];
+        const int nv = (int)sc[1];
 
         if (!icgn) {
             // ---- evaluate_znssd (:467-528) ---------------------------------
@@ -794,8 +802,13 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
                 }
             }
         }
-        err = group_sum<NW>(err, red, par);
-        group_sum_n<NW, 6>(dp, red, par);
+        {
+            float ed[7] = {err, dp[0], dp[1], dp[2], dp[3], dp[4], dp[5]};
+            group_sum_n<NW, 7>(ed, red, par);
+            err = ed[0];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) dp[k] = ed[1 + k];
+        }
         float d[6];
         if (masked) {
             group_sum_n<NW, 21>(hh, red, par);
@@ -886,8 +899,10 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
                     sx += a * a;
                     sy += b * b;
                 }
-            sx = group_sum<NW>(sx, red, par);
-            sy = group_sum<NW>(sy, red, par);
+            float sxy[2] = {sx, sy};
+            group_sum_n<NW, 2>(sxy, red, par);
+            sx = sxy[0];
+            sy = sxy[1];
             if (fminf(sx, sy) <= 0.0f || !finite_f(final_score))
                 finish_icgn(icgn_result(1, 2.0f, it + 1));
             else
