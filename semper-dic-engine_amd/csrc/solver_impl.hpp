@@ -836,13 +836,36 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
                     Hd[0] += kLmAlpha;
                     Hd[7] += kLmAlpha;
                 }
-                flag = !well_conditioned_reg(Hd, half);
+                if constexpr (NW > 1) {
+                    // the rcond test on wave 0 and the inverse on wave 1 at once
+                    // (the same sequences, so the same bits); every wave takes
+                    // both from the exchange slots after the group-sum halves,
+                    // which the next write reaches only past later group sums
+                    float* xch = red + 2 * NW * kGroupSlots;
+                    const int wid = (int)(threadIdx.x >> 6);
+                    if (wid == 0) {
+                        const bool wc = well_conditioned_reg(Hd, half);
+                        if (lane == 0) xch[36] = wc ? 1.0f : 0.0f;
+                    } else if (wid == 1) {
+                        gj_inverse_reg(Hd, Hm);
+                        if (lane == 0) {
+#pragma unroll
+                            for (int q = 0; q < 36; ++q) xch[q] = Hm[q];
+                        }
+                    }
+                    __syncthreads();
+                    flag = xch[36] == 0.0f;
+#pragma unroll
+                    for (int q = 0; q < 36; ++q) Hm[q] = xch[q];
+                } else {
+                    flag = !well_conditioned_reg(Hd, half);
+                    if (!flag) gj_inverse_reg(Hd, Hm);
+                }
             }
             if (flag) {
                 finish_icgn(icgn_result(1, 2.0f, it));
                 continue;
             }
-            gj_inverse_reg(Hd, Hm);
             final_score = err / (float)nv;
 #pragma unroll
             for (int r = 0; r < 6; ++r) {
@@ -1289,7 +1312,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2))) vo
     // each wave keeps the solver's uniform state in its own copy; the waves
     // meet only in the group sums
     __shared__ WaveLds Ls[NW];
-    __shared__ float red[2 * NW * kGroupSlots];
+    __shared__ float red[2 * NW * kGroupSlots + 40];   // group-sum halves, then the 6x6 exchange
     WaveLds& L = Ls[threadIdx.x >> 6];
     const int nb = *a.nbail;
     for (int k = blockIdx.x; k < nb; k += gridDim.x) {
