@@ -127,7 +127,9 @@ SDK_API int sdk_incremental_set_region_map(struct semper_incremental* inc, int r
 SDK_API int sdk_engine_node_report(void* h, uint8_t* tag, uint8_t* path, int n);
 /* The node planes of the same report: u, v, ux, uy, vx, vy, corr, plane after
    plane (7 x node count floats), at the end of the run (which 0) or after
-   Path A (which 1). The float count, or -3. */
+   Path A (which 1); which 2: the six guess planes (u, v, ux, uy, vx, vy) of
+   the mesh guess field Path A started from (6 x node count floats). The float
+   count, or -3. */
 SDK_API int sdk_engine_node_planes(void* h, int which, float* out, int n);
 
 /* Graph captures and replays of an engine so far: stats2 = {captures,

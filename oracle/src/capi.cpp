@@ -252,6 +252,44 @@ int orc_engine_run(orc_engine* e, const uint8_t* def, const uint8_t* mask, const
     }
 }
 
+// orc_engine_run that also writes the run's node planes: 7 planes of gw x gh
+// floats (u, v, ux, uy, vx, vy, corr), corr -1 where a node holds no accepted
+// result; gw = rect_w / step, gh = rect_h / step of the validated params
+// (written to gwh[0..1]). planes must hold 7 x gw x gh floats (plane_cap).
+int orc_engine_run_planes(orc_engine* e, const uint8_t* def, const uint8_t* mask, const int32_t* params,
+                          float* out, int out_capacity, float* metrics, int metrics_len, int threads,
+                          float conv_tol, float* planes, int plane_cap, int32_t* gwh) {
+    if (!e || !params || !planes || !gwh) return kErrInit;
+    Params p;
+    p.rect_x = params[0];
+    p.rect_y = params[1];
+    p.rect_w = params[2];
+    p.rect_h = params[3];
+    p.step = params[4];
+    p.subset = params[5];
+    p.strain_window = params[6];
+    p.use6 = params[7] != 0;
+    RunOptions o;
+    o.threads = threads;
+    o.conv_tol = conv_tol;
+    Capture cap;
+    o.capture = &cap;
+    int rc;
+    try {
+        rc = run_full_field(e->cache, def, mask, p, out, out_capacity, metrics, metrics_len, o);
+    } catch (...) {
+        return kErrInit;
+    }
+    if (rc < 0) return rc;
+    const size_t n = (size_t)cap.gw * cap.gh;
+    gwh[0] = cap.gw;
+    gwh[1] = cap.gh;
+    if ((size_t)plane_cap < 7 * n) return kErrInit;
+    const std::vector<float>* pl[7] = {&cap.u, &cap.v, &cap.ux, &cap.uy, &cap.vx, &cap.vy, &cap.corr};
+    for (int k = 0; k < 7; ++k) std::copy(pl[k]->begin(), pl[k]->end(), planes + k * n);
+    return rc;
+}
+
 // IncrementalSession: params as orc_engine_run; info receives 6 ints
 // (reference_frame, reference_updated, tracked_points, frame0_points,
 // taylor_fallbacks, lost_this_frame).

@@ -289,6 +289,7 @@ SDK_API int sdk_engine_set_flag(void* h, int flag, int value) {
         case 14:
             ((DevEngine*)h)->eng.cnn_rescue = value != 0;
             return 0;
+
         default:
             return -3;
     }
@@ -504,9 +505,9 @@ SDK_API int sdk_engine_node_report(void* h, uint8_t* tag, uint8_t* path, int n) 
 }
 
 SDK_API int sdk_engine_node_planes(void* h, int which, float* out, int n) {
-    if (!h || !out || (which != 0 && which != 1)) return -3;
+    if (!h || !out || which < 0 || which > 2) return -3;
     const Engine& e = ((DevEngine*)h)->eng;
-    const std::vector<float>& v = which == 0 ? e.report_planes : e.report_planes_a;
+    const std::vector<float>& v = which == 0 ? e.report_planes : which == 1 ? e.report_planes_a : e.report_guess;
     if (v.empty() || n < (int)v.size()) return -3;
     std::memcpy(out, v.data(), sizeof(float) * v.size());
     return (int)v.size();

@@ -1131,6 +1131,8 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                 ck(hipMemcpy(after_a_.data(), g.solved, NN, hipMemcpyDeviceToHost));
                 report_planes_a.resize(7 * NN);
                 ck(hipMemcpy(report_planes_a.data(), gplanes_.p, sizeof(float) * 7 * NN, hipMemcpyDeviceToHost));
+                report_guess.resize(6 * NN);
+                ck(hipMemcpy(report_guess.data(), guess_.p, sizeof(float) * 6 * NN, hipMemcpyDeviceToHost));
             }
             st_a.points = sa[0];
             st_a.iters = sa[1];

@@ -267,6 +267,8 @@ public:
     // with node_report: the seven node planes (u, v, ux, uy, vx, vy, corr,
     // plane after plane) after Path A and at the end of the run
     std::vector<float> report_planes_a, report_planes;
+    // and the six guess planes (u, v, ux, uy, vx, vy) Path A started from
+    std::vector<float> report_guess;
     struct GraphStats {
         int captures = 0, replays = 0;
     };

@@ -45,6 +45,8 @@ def oracle():
         lib.orc_engine_set_reference.argtypes = [VP, VP, ctypes.c_int, ctypes.c_int, VP]
         lib.orc_engine_run.argtypes = [VP, VP, VP, VP, VP, ctypes.c_int, VP, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_float]
+        lib.orc_engine_run_planes.argtypes = [VP, VP, VP, VP, VP, ctypes.c_int, VP, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_float, VP, ctypes.c_int, VP]
         lib.orc_solve_nodes.argtypes = [VP, VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_float, ctypes.c_int, VP, VP, VP, VP, VP]
         lib.orc_icgn_fp64.argtypes = [VP, VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -169,6 +171,128 @@ class OracleEngine:
         n = self.lib.orc_engine_run(self.h, P(img), P(m), P(pr), P(out), cap, P(met), 24, threads,
                                     ctypes.c_float(tol))
         return n, out[: max(n, 0) * 8].reshape(-1, 8).copy(), met
+
+
+def _oracle_run_planes(self, img, prm, threads=0, tol=1e-3):
+    """run() that also returns the node planes (7, gh, gw): u, v, ux, uy, vx,
+    vy, corr (-1 where the node holds no accepted result)."""
+    img = np.ascontiguousarray(img, np.uint8)
+    pr = np.array(prm, np.int32)
+    gw, gh = pr[2] // max(pr[4], 1) + 1, pr[3] // max(pr[4], 1) + 1
+    cap = gw * gh * 8
+    out = np.zeros(max(cap, 1), np.float32)
+    met = np.zeros(24, np.float32)
+    planes = np.zeros(7 * gw * gh, np.float32)
+    gwh = np.zeros(2, np.int32)
+    n = self.lib.orc_engine_run_planes(self.h, P(img), None, P(pr), P(out), cap, P(met), 24, threads,
+                                       ctypes.c_float(tol), P(planes), planes.size, P(gwh))
+    gw, gh = int(gwh[0]), int(gwh[1])
+    return n, out[: max(n, 0) * 8].reshape(-1, 8).copy(), met, planes[: 7 * gw * gh].reshape(7, gh, gw).copy()
+
+
+OracleEngine.run_planes = _oracle_run_planes
+
+
+def split_analysis(ref, dfm, prm, planes_g, planes_o, dim, tol, lm=True, agree=1e-5, guess_g=None, path_g=None,
+                   tag_g=None, agree_guess=1e-4):
+    """Node-by-node comparison of two full-field solves of the same frame
+    (planes (7, gh, gw): u, v, ux, uy, vx, vy, corr; corr < 0 unsolved), the
+    first side's against the oracle's.
+
+    Both sides iterate IC-GN until ||dp|| < tol, with FP32 arithmetic in
+    different orders, so where the two differ by more than `agree` px the
+    difference has one of these causes, and this names which:
+
+    "gpu" / "oracle": a stop split. A node whose ||dp|| lands next to tol
+    stops one iteration earlier on one side than on the other. IC-GN with a
+    fixed Hessian is memoryless -- the next step depends on the current warp
+    alone -- so one IC-GN step of the oracle from the earlier side's result
+    lands on the later side's result ("gpu": the first side stopped first).
+
+    "guess": the node started from another guess. The anchors' last-bit
+    differences (or a stop split at an anchor) move the mesh guesses of the
+    Path A nodes around them. With the first side's guess planes (6, gh, gw),
+    node paths and mesh tags (sdk_engine_node_report: path 3 = Path A; tags 2
+    extrapolated and 4 global model are solved NO_SIMPLEX, others NO_SEARCH),
+    the oracle's own solve from that guess lands within `agree_guess` px of
+    the first side's result, or one IC-GN step from it. (Two FP32 IC-GN
+    trajectories from one guess drift apart by the rounding of their sums
+    over their iterations: up to ~6e-5 px over 20 iterations on the
+    reference's noiseless affine fixture.)
+
+    "converged": the first side's result is a converged solution in the
+    oracle's own arithmetic: the oracle's IC-GN started from it stops at its
+    first step (||dp|| < tol) with status 0.
+
+    "-": none of these (a real discrepancy).
+
+    Returns the solved-node masks, the per-node u, v difference, the nodes
+    over `agree` px and for each the cause (how) and whether it is a split."""
+    vg = planes_g[6] >= 0
+    vo = planes_o[6] >= 0
+    d = np.maximum(np.abs(planes_g[0] - planes_o[0]), np.abs(planes_g[1] - planes_o[1]))
+    d = np.where(vg & vo, d, 0.0)
+    iy, ix = np.nonzero(d > agree)
+    res = dict(valid_g=vg, valid_o=vo, duv=d, nodes=list(zip(ix.tolist(), iy.tolist())), split=[], first=[],
+               how=[])
+    if len(ix) == 0:
+        return res
+    step = prm[4]
+    cxy = [(prm[0] + x * step, prm[1] + y * step) for x, y in zip(ix, iy)]
+    pg = planes_g[:6, iy, ix].T.astype(np.float32)
+    po = planes_o[:6, iy, ix].T.astype(np.float32)
+    ref_f = np.ascontiguousarray(ref, np.float32)
+    dfm_f = np.ascontiguousarray(dfm, np.float32)
+
+    def one_step(c, p):   # NO_SIMPLEX, the huge stop rule takes the first step as converged
+        io, fo = oracle_solve_nodes(ref_f, dfm_f, dim, c, p, [2] * len(c), lm=lm, tol=1e30)
+        return (io[:, 2] == 1), fo[:, :6]
+
+    ok_g, st_g = one_step(cxy, pg)
+    ok_o, st_o = one_step(cxy, po)
+    g_first = ok_g & (np.abs(st_g[:, :2] - po[:, :2]).max(1) <= agree)
+    o_first = ok_o & (np.abs(st_o[:, :2] - pg[:, :2]).max(1) <= agree)
+    how = np.where(g_first, "gpu", np.where(o_first, "oracle", "-")).astype(object)
+    if guess_g is not None:
+        todo = [k for k in range(len(ix)) if how[k] == "-" and path_g[iy[k], ix[k]] == 3]
+        if todo:
+            c2 = [cxy[k] for k in todo]
+            gs = guess_g[:, iy[todo], ix[todo]].T.astype(np.float32)
+            md = [2 if tag_g[iy[k], ix[k]] in (2, 4) else 1 for k in todo]
+            io_r, fo_r = oracle_solve_nodes(ref_f, dfm_f, dim, c2, gs, md, lm=lm, tol=tol)
+            rr = fo_r[:, :6].astype(np.float32)
+            near = np.abs(rr[:, :2] - pg[todo, :2]).max(1) <= agree_guess
+            ok_a, st_a = one_step(c2, rr)
+            ok_b, st_b = one_step(c2, pg[todo])
+            a_b = ok_a & (np.abs(st_a[:, :2] - pg[todo, :2]).max(1) <= agree_guess)
+            b_a = ok_b & (np.abs(st_b[:, :2] - rr[:, :2]).max(1) <= agree_guess)
+            for j, k in enumerate(todo):
+                if io_r[j, 1] == 0 and (near[j] or a_b[j] or b_a[j]):
+                    how[k] = "guess"
+    rest = [k for k in range(len(ix)) if how[k] == "-"]
+    if rest:
+        io_c, _ = oracle_solve_nodes(ref_f, dfm_f, dim, [cxy[k] for k in rest], pg[rest], [2] * len(rest), lm=lm,
+                                     tol=tol)
+        for j, k in enumerate(rest):
+            if io_c[j, 1] == 0 and io_c[j, 2] == 1:
+                how[k] = "converged"
+    res["how"] = [str(h) for h in how]
+    res["first"] = [h if h in ("gpu", "oracle") else "-" for h in res["how"]]
+    res["split"] = [h in ("gpu", "oracle") for h in res["how"]]
+    return res
+
+
+def oracle_vsg(planes, step, window):
+    """The oracle's median test and VSG strain (pack_full_field_output's
+    order) over a node field (7, gh, gw) of planes: (valid after the median
+    test, exx, eyy, exy) on the grid."""
+    gh, gw = planes.shape[1:]
+    u = np.ascontiguousarray(planes[0], np.float32)
+    v = np.ascontiguousarray(planes[1], np.float32)
+    ok = np.ascontiguousarray(planes[6] >= 0, np.uint8)
+    exx, eyy, exy = (np.zeros((gh, gw), np.float32) for _ in range(3))
+    oracle().orc_outliers_and_vsg(gw, gh, step, window, P(u), P(v), P(ok), P(exx), P(eyy), P(exy))
+    return ok.astype(bool), exx, eyy, exy
 
 
 def oracle_solve_nodes(ref, dfm, dim, nodes, guesses, modes, use6=False, lm=False, tol=1e-3):

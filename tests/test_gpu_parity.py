@@ -7,7 +7,8 @@ import numpy as np
 import pytest
 
 from helpers import (FF_SLOTS, P, OracleEngine, corpus_images, corpus_nodes, engine_module, ff_golden_images,
-                     fp64_polish, full_field_golden, golden_corpus, oracle_solve_nodes, ref_speckle)
+                     fp64_polish, full_field_golden, golden_corpus, oracle_solve_nodes, oracle_vsg, ref_speckle,
+                     split_analysis)
 
 pytestmark = pytest.mark.gpu
 
@@ -160,7 +161,7 @@ def test_corpus_gpu_vs_oracle(dev, use6):
 # ---- FullFieldGolden through semper_run (test_full_field_golden.cpp:280-395) ---
 
 @pytest.mark.parametrize("scenario", [0, 1])
-def test_full_field_golden_gpu(scenario):
+def test_full_field_golden_gpu(dev, scenario):
     code, counts, gpts = full_field_golden()[scenario]
     ref, dfm = ff_golden_images(scenario)
     eng = sa.Engine()
@@ -173,10 +174,14 @@ def test_full_field_golden_gpu(scenario):
     assert dev_[2] <= 5e-3 and dev_[3] <= 5e-3
     assert dev_[4:7].max() <= 1e-4 and dev_[7] <= 1e-4
     m = met[FF_SLOTS]
-    # counts that the replaced seeding front end does not move
+    # counts that the replaced seeding front end does not move (the oracle's
+    # own golden test holds the same slots)
     for k, slot in enumerate(FF_SLOTS):
-        if slot in (0, 1, 2, 16, 19, 20, 21, 23):
-            assert m[k] == counts[k], (slot, m[k], counts[k])
+        if slot in (0, 1, 2, 3, 4, 16, 19, 20, 21, 22, 23):
+            assert abs(m[k] - counts[k]) <= (1e-4 if slot == 22 else 0), (slot, m[k], counts[k])
+    assert met[3] + met[4] + met[19] == met[1] + met[20] + met[23]
+    # and against the oracle on the same pair, to the parity contract's bounds
+    parity_check(dev, ref, dfm, [0, 0, 320, 320, 12, 21, 48, 0], 1e-3)
 
 
 # ---- C1: 512^2 speckle, u = 0.4 px, against the oracle's semper_run ---------------
@@ -191,50 +196,159 @@ def workload_module():
     return wl
 
 
-@pytest.mark.parametrize("name", ["c1_512_translate", "c2_2048_affine", "c4_4096_heterogeneous", "t_1024_noisy"])
-def test_workload_full_field_vs_oracle(dev, name):
-    """Each benchmark workload at its full size through the engine against the
-    oracle's run_full_field on the same bytes and stop rule: attempted-subset
-    count identical, point counts within 0.5%, u, v within the reference's
-    cross-toolchain bound (5e-3 px) with 99% within 1e-3 px, strain and corr
-    within 1e-4 (test_full_field_golden.cpp:205-225)."""
+def gpu_run_planes(dev, ref, dfm, prm, w, h, tol, gw, gh):
+    """sdk_engine_run with the per-node report on: (points, out, metrics,
+    timings, node planes (7, gh, gw) u, v, ux, uy, vx, vy, corr)."""
+    hd = dev.sdk_engine_create()
+    assert dev.sdk_engine_set_flag(hd, 12, 1) == 0
+    assert dev.sdk_engine_set_reference(hd, P(ref), 0, w, h, None) == 0
+    cap = (gw + 1) * (gh + 1) * 8
+    out = np.zeros(cap, np.float32)
+    met = np.zeros(24, np.float32)
+    pr = np.array(prm, np.int32)
+    n = dev.sdk_engine_run(hd, P(dfm), 0, P(pr), P(out), out.size, P(met), 24, ctypes.c_float(tol))
+    tm = np.zeros(14, np.float64)
+    dev.sdk_engine_timings(hd, P(tm))
+    planes = np.zeros(7 * gw * gh, np.float32)
+    assert dev.sdk_engine_node_planes(hd, 0, P(planes), planes.size) == planes.size
+    guess = np.zeros(6 * gw * gh, np.float32)
+    assert dev.sdk_engine_node_planes(hd, 2, P(guess), guess.size) == guess.size
+    tag = np.zeros(gw * gh, np.uint8)
+    path = np.zeros(gw * gh, np.uint8)
+    assert dev.sdk_engine_node_report(hd, P(tag), P(path), gw * gh) == gw * gh
+    dev.sdk_engine_destroy(hd)
+    rep = dict(guess=guess.reshape(6, gh, gw), tag=tag.reshape(gh, gw), path=path.reshape(gh, gw))
+    return n, out[: n * 8].reshape(-1, 8).copy(), met, tm, planes.reshape(7, gh, gw), rep
+
+
+def parity_vs_oracle(dev, name, use6=False):
+    """A benchmark workload through parity_check."""
     wl = workload_module()
     cfg = wl.config(name)
     ref, dfm = wl.pair(dev, name)
-    prm = wl.solve_params(name)
-    tol = cfg["tol"]
-    print(f"workload {name} {cfg['w']}x{cfg['h']} tol {tol}", flush=True)
+    prm = list(wl.solve_params(name))
+    prm[7] = int(use6)
+    print(f"workload {name} {cfg['w']}x{cfg['h']} tol {cfg['tol']} use6 {use6}", flush=True)
+    return parity_check(dev, ref, dfm, prm, cfg["tol"])
+
+
+def parity_check(dev, ref, dfm, prm, tol):
+    """A frame pair through the engine and through the oracle's
+    run_full_field on the same bytes and stop rule, held to the tolerances of
+    the MI355X build's parity contract: the point set and grid coordinates
+    identical, slot 0 identical, u, v within 1e-3 px and corr within 1e-4 at
+    every point, and strains within 1e-5 at every point once stop splits are
+    resolved.
+
+    Both sides iterate IC-GN to ||dp|| < tol with FP32 arithmetic in
+    different orders, and the anchors' last-bit differences move the Path A
+    guesses by ~1e-5 px. A node whose ||dp|| lands next to tol can then stop
+    one iteration earlier on one side (a stop split, up to one step of
+    ~tol apart), and a near-flat subset's optimum is only defined to the
+    rounding of its sums. Each node whose u or v differs by more than 1e-5 px
+    must be one of these (helpers.split_analysis): one IC-GN step of the
+    oracle from the earlier side's result lands on the later side's result;
+    or the oracle's own solve from the engine's guess for the node lands on
+    the engine's result (within 1e-4 px, or one step from it: the anchors'
+    last bits move the mesh guesses); or the oracle's IC-GN started from the
+    engine's result stops at its first step. The strains over 1e-5 therefore come from those nodes: with
+    them given the oracle's values, the oracle's median test and VSG over the
+    engine's node field give the oracle's strains within 1e-5, and the
+    engine's strain kernel gives the oracle's VSG of the engine's field.
+    Where a point's strain differs by more than 1e-5, both sides are held
+    against the strain of the FP64-polished field (every node of the point's
+    window polished to the FP64 IC-GN optimum): each side's error there is
+    reported, and the engine's may not exceed the oracle's worst."""
+    h_, w_ = ref.shape
+    use6 = bool(prm[7])
+    dim = prm[5]
     oe = OracleEngine()
     assert oe.set_reference(ref) == 0
-    n_o, pts_o, met_o = oe.run(dfm, prm, tol=tol)
-    h = dev.sdk_engine_create()
-    assert dev.sdk_engine_set_reference(h, P(ref), 0, cfg["w"], cfg["h"], None) == 0
-    cap = (cfg["w"] // cfg["step"] + 1) * (cfg["h"] // cfg["step"] + 1) * 8
-    out = np.zeros(cap, np.float32)
-    met_g = np.zeros(24, np.float32)
-    pr = np.array(prm, np.int32)
-    n_g = dev.sdk_engine_run(h, P(dfm), 0, P(pr), P(out), out.size, P(met_g), 24, ctypes.c_float(tol))
-    tm = np.zeros(14, np.float64)
-    dev.sdk_engine_timings(h, P(tm))
-    dev.sdk_engine_destroy(h)
-    pts_g = out[: n_g * 8].reshape(-1, 8)
+    n_o, pts_o, met_o, pl_o = oe.run_planes(dfm, prm, tol=tol)
+    gh, gw = pl_o.shape[1:]
+    n_g, pts_g, met_g, tm, pl_g, rep = gpu_run_planes(dev, ref, dfm, prm, w_, h_, tol, gw, gh)
     print(f"points gpu {n_g} oracle {n_o}; path B points {met_g[4]:.0f}/{met_o[4]:.0f}, speculative {tm[11]:.0f}, "
           f"further launches {tm[12]:.0f} ({tm[13]:.0f} subsets)", flush=True)
+    # grid, mask and indexing: bit-exact
+    assert met_g[0] == met_o[0]
+    assert n_g == n_o > 0
+    assert np.array_equal(pts_g[:, :2], pts_o[:, :2])
+    res = split_analysis(ref, dfm, prm, pl_g, pl_o, dim, tol, guess_g=rep["guess"], path_g=rep["path"],
+                         tag_g=rep["tag"])
+    # the solved set: identical, but for nodes at the corr acceptance bar
+    # (status 0 and corr <= 0.15) whose corr the two sides round to either
+    # side of it; such a node is not among the output points
+    vd = np.argwhere(res["valid_g"] != res["valid_o"])
+    for y, x in vd:
+        c = max(pl_g[6, y, x], pl_o[6, y, x])
+        assert abs(c - 0.15) <= 1e-6, ((x, y), pl_g[6, y, x], pl_o[6, y, x])
+        assert not ((pts_o[:, 0] == prm[0] + x * prm[4]) & (pts_o[:, 1] == prm[1] + y * prm[4])).any()
+    d = np.abs(pts_g - pts_o)
+    how = res["how"]
+    print(f"max |du,dv| {d[:, 2:4].max():.2e}, strain {d[:, 4:7].max():.2e}, corr {d[:, 7].max():.2e}; "
+          f"solved sets differ at {len(vd)} nodes on the acceptance bar; "
+          f"nodes over 1e-5 px {len(res['nodes'])}: stop splits {how.count('gpu') + how.count('oracle')} "
+          f"(gpu first {how.count('gpu')}, oracle first {how.count('oracle')}), from another guess "
+          f"{how.count('guess')}, converged in the oracle's arithmetic {how.count('converged')}", flush=True)
+    assert d[:, 2:4].max() <= 1e-3, d[:, 2:4].max()
+    assert d[:, 7].max() <= 1e-4, d[:, 7].max()
+    bad = [(n, int(rep["path"][n[1], n[0]]), float(res["duv"][n[1], n[0]])) for n, h in zip(res["nodes"], how)
+           if h == "-"]
+    assert not bad, bad[:10]
+    # strains with those nodes resolved
+    pl = pl_g.copy()
+    for x, y in res["nodes"]:
+        pl[:, y, x] = pl_o[:, y, x]
+    ok, exx, eyy, exy = oracle_vsg(pl, prm[4], prm[6])
+    ix = ((pts_o[:, 0] - prm[0]) / prm[4]).astype(int)
+    iy = ((pts_o[:, 1] - prm[1]) / prm[4]).astype(int)
+    assert ok[iy, ix].all()
+    ds = np.abs(np.stack([exx[iy, ix], eyy[iy, ix], exy[iy, ix]], 1) - pts_o[:, 4:7])
+    # the engine's own strain kernel against the oracle's on the engine's field
+    okg, exg, eyg, exyg = oracle_vsg(pl_g, prm[4], prm[6])
+    dk = np.abs(np.stack([exg[iy, ix], eyg[iy, ix], exyg[iy, ix]], 1) - pts_g[:, 4:7])
+    over = np.flatnonzero(d[:, 4:7].max(1) > 1e-5)
+    print(f"strain: {len(over)} of {n_o} points over 1e-5; with the differing nodes resolved max {ds.max():.2e}; "
+          f"engine VSG vs oracle VSG on the engine's field {dk.max():.2e}", flush=True)
+    assert ds.max() <= 1e-5, ds.max()
+    assert dk.max() <= 1e-6, dk.max()
+    if len(over):
+        # both sides against the strain of the FP64-polished field at those points
+        r = prm[6] / 2.0 / prm[4]
+        rr = int(np.ceil(r))
+        win = set()
+        for k in over:
+            for dy in range(-rr, rr + 1):
+                for dx in range(-rr, rr + 1):
+                    x, y = ix[k] + dx, iy[k] + dy
+                    if 0 <= x < gw and 0 <= y < gh and pl_o[6, y, x] >= 0 and dx * dx + dy * dy <= r * r + 1e-9:
+                        win.add((x, y))
+        win = sorted(win)
+        xs = np.array([x for x, _ in win])
+        ys = np.array([y for _, y in win])
+        si, p64, _ = fp64_polish(ref, dfm, dim, [(prm[0] + x * prm[4], prm[1] + y * prm[4]) for x, y in win],
+                                 pl_o[:6, ys, xs].T.astype(np.float64), use6=use6)
+        pf = pl_o.copy()
+        okp = si[:, 0] == 0
+        pf[:6, ys[okp], xs[okp]] = p64[okp].T.astype(np.float32)
+        _, fx, fy, fxy = oracle_vsg(pf, prm[4], prm[6])
+        e64 = np.stack([fx[iy[over], ix[over]], fy[iy[over], ix[over]], fxy[iy[over], ix[over]]], 1)
+        e_g = np.abs(pts_g[over, 4:7] - e64).max(1)
+        e_o = np.abs(pts_o[over, 4:7] - e64).max(1)
+        print(f"points over 1e-5 vs the FP64-polished strain ({okp.sum()}/{len(win)} window nodes polished): "
+              f"engine max {e_g.max():.2e} mean {e_g.mean():.2e}; oracle max {e_o.max():.2e} mean "
+              f"{e_o.mean():.2e}; engine closer at {(e_g <= e_o).sum()} of {len(over)}", flush=True)
+        assert e_g.max() <= max(e_o.max(), 1e-5) * 1.0 + 1e-6, (e_g.max(), e_o.max())
+    return met_g, met_o, tm
+
+
+@pytest.mark.parametrize("name", ["c1_512_translate", "c2_2048_affine", "c4_4096_heterogeneous", "t_1024_noisy"])
+def test_workload_full_field_vs_oracle(dev, name):
+    """Each benchmark workload at its full size: see parity_vs_oracle."""
+    met_g, met_o, tm = parity_vs_oracle(dev, name)
     if name == "t_1024_noisy":
         # the flood ran, mostly on the speculative first claims
         assert met_o[4] >= 20 and tm[11] >= 100 and tm[12] < tm[11]
-    assert n_g > 0 and abs(n_g - n_o) <= max(2, n_o // 200), (n_g, n_o)
-    assert met_g[0] == met_o[0]
-    assert abs(float(met_g[3] + met_g[4] + met_g[19]) - float(met_o[3] + met_o[4] + met_o[19])) <= max(2, n_o // 200)
-    go = {(a, b): r for a, b, *r in pts_o.tolist()}
-    gg = {(a, b): r for a, b, *r in pts_g.tolist()}
-    common = sorted(set(go) & set(gg))
-    assert len(common) >= n_o - max(2, n_o // 200)
-    d = np.array([np.abs(np.array(go[k]) - np.array(gg[k])) for k in common])
-    assert d[:, :2].max() <= 5e-3, d[:, :2].max()
-    assert (d[:, :2].max(1) <= 1e-3).mean() >= 0.99
-    assert d[:, 2:5].max() <= 1e-4, d[:, 2:5].max()
-    assert d[:, 5].max() <= 1e-4
 
 
 def test_path_b_speculation_is_exact(dev):
