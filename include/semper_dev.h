@@ -33,6 +33,29 @@ SDK_API int sdk_solve_nodes(const float* ref, const void* def, int def_is_f32, i
                             float tol, int n, const int32_t* cxy, const float* guess, const int32_t* modes,
                             int32_t* iout, float* fout);
 
+/* sdk_solve_nodes for callers that cannot allocate device memory (code under
+   HIP graph capture, serving loops); it has no counterpart in the reference.
+   The arguments and results are sdk_solve_nodes', but every pointer is device
+   memory -- ref (w x h floats), def (w x h floats or bytes), cxy (2n int32),
+   guess (6n floats), modes (n int32), iout (5n int32), fout (7n floats) --
+   and the call only enqueues the work on `stream` (a hipStream_t; NULL: the
+   null stream): the outputs are ready when the stream's work is. It calls no
+   allocator: the solver's own buffers (reference planes, per-node pool
+   entries, tasks, results, the fast path's hand-off list) live in one fixed
+   device array of SDK_NOALLOC_SCRATCH_BYTES per GPU, held until the process
+   ends. w x h <= SDK_NOALLOC_MAX_PIXELS and n <= SDK_NOALLOC_MAX_NODES, else
+   SDK_ERR_NOALLOC_CAPACITY and nothing is enqueued. Calls that overlap on one
+   GPU share that array and race: serialise them (one stream, or one call at a
+   time). 0, -3 (bad arguments or a failed launch) or
+   SDK_ERR_NOALLOC_CAPACITY. */
+#define SDK_NOALLOC_MAX_PIXELS (1 << 20)
+#define SDK_NOALLOC_MAX_NODES 16384
+#define SDK_NOALLOC_SCRATCH_BYTES 17105152
+#define SDK_ERR_NOALLOC_CAPACITY (-7)
+SDK_API int sdk_solve_nodes_noalloc(const float* ref, const void* def, int def_is_f32, int w, int h, int dim, int use6,
+                                    int lm, float tol, int n, const int32_t* cxy, const float* guess,
+                                    const int32_t* modes, int32_t* iout, float* fout, void* stream);
+
 /* compute_hessian_only per node centre (subset_precomputer.cpp:42-149). */
 SDK_API int sdk_pool(const float* ref, int w, int h, int dim, int lm, int n, const int32_t* cxy, int32_t* valid,
                      float* stats2, float* hstep);

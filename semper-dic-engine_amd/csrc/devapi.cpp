@@ -10,6 +10,7 @@
 #include "codec.hpp"
 #include "engine.hpp"
 #include "handles.hpp"
+#include "noalloc.hpp"
 #include "semper_c.h"
 #include "semper_dev.h"
 #include "synth.hpp"
@@ -138,6 +139,15 @@ SDK_API int sdk_solve_nodes(const float* ref, const void* def, int def_is_f32, i
     }
     (void)hipStreamDestroy(st);
     return rc;
+}
+
+// sdk_solve_nodes without allocation: device pointers, the caller's stream,
+// the solver's buffers in the library's fixed device array (csrc/noalloc.hip).
+SDK_API int sdk_solve_nodes_noalloc(const float* ref, const void* def, int def_is_f32, int w, int h, int dim, int use6,
+                                    int lm, float tol, int n, const int32_t* cxy, const float* guess,
+                                    const int32_t* modes, int32_t* iout, float* fout, void* stream) {
+    return solve_nodes_noalloc(ref, def, def_is_f32, w, h, dim, use6, lm, tol, n, cxy, guess, modes, iout, fout,
+                               (hipStream_t)stream);
 }
 
 // Hessian pool entries of a host float image at node centres:

@@ -86,6 +86,9 @@ def load_dev(path=LIB_DEV):
     lib.sdk_memcpy.argtypes = [_VP, _VP, ctypes.c_size_t, ctypes.c_int]
     lib.sdk_solve_nodes.argtypes = [_VP, _VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_float, ctypes.c_int, _VP, _VP, _VP, _VP, _VP]
+    lib.sdk_solve_nodes_noalloc.argtypes = [_VP, _VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int, _VP, _VP, _VP,
+                                            _VP, _VP, _VP]
     lib.sdk_pool.argtypes = [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, _VP,
                              _VP, _VP]
     lib.sdk_ref_planes.argtypes = [_VP, _VP, ctypes.c_int, ctypes.c_int, _VP, _VP, _VP]
