@@ -56,10 +56,11 @@ SEMPER_C_API semper_engine* semper_create(void);
 SEMPER_C_API void semper_destroy(semper_engine* eng);
 
 /* Caches the reference (reference semper_c.cpp:105-124): raw 8-bit gray
-   (img_len == w*h) or RGBA (4*w*h) of expected_w x expected_h; mask of the
-   same form or NULL/0 (a pixel below 128 is excluded). The old reference is
-   released first. 0, or SEMPER_ERR_INIT (including encoded image files,
-   which this build does not decode). */
+   (img_len == w*h) or RGBA (4*w*h) of expected_w x expected_h, or an encoded
+   PNG, TIFF, BMP, PNM or JPEG file (decode_any, image_codec.cpp:129-161); mask
+   of the same forms or NULL/0 (a pixel below 128 is excluded). The old
+   reference is released first. 0, or SEMPER_ERR_INIT (including bytes that
+   do not decode, such as WebP). */
 SEMPER_C_API int semper_set_reference(semper_engine* eng, const uint8_t* img, size_t img_len,
                                       const uint8_t* mask, size_t mask_len, int expected_w, int expected_h);
 

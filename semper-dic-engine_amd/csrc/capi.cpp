@@ -2,8 +2,8 @@
 // (include/semper/semper_c.h:83-197, adapters/c/semper_c.cpp) over the MI355X
 // engine. Handles, the one-call-per-handle rule, the cancel window and the
 // exception barrier behave as the reference's. Image bytes are raw gray,
-// raw RGBA or an encoded PNG / TIFF / BMP / PNM file (codec.hpp); JPEG and
-// WebP do not decode here and return SEMPER_ERR_INIT.
+// raw RGBA or an encoded PNG / TIFF / BMP / PNM / JPEG file (codec.hpp);
+// WebP does not decode here and returns SEMPER_ERR_INIT.
 #include <atomic>
 #include <cmath>
 #include <cstring>

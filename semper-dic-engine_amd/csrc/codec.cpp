@@ -580,6 +580,17 @@ Raster decode_tiff(const uint8_t* d, size_t len) {
             for (auto& b : rawc) b = reverse_bits(b);
             src = rawc.data();
         }
+        // Every codec here must deliver the whole chunk, and none can expand
+        // its input beyond a fixed ratio: PackBits 64 (a 2-byte run of 128),
+        // LZW about 3400 (9-bit codes whose strings grow by one byte per
+        // code, up to a 4094-entry table), Deflate 1032. A chunk that
+        // declares more than that is refused before it is allocated.
+        const uint64_t most = comp == 1          ? cnt
+                              : comp == 32773    ? cnt * 64 + 128
+                              : comp == 5        ? cnt * 3641 + 4096
+                              : (comp == 8 || comp == 32946) ? cnt * 1032 + 1024
+                                                             : 0;
+        if ((uint64_t)want > most) return {};
         chunk.assign(want, 0);
         bool ok = false;
         switch (comp) {
@@ -931,75 +942,133 @@ inline uint8_t sat_round(double v) {
 
 }  // namespace
 
-bool has_image_signature(const uint8_t* d, size_t len) {
-    auto starts = [&](const char* sig, size_t n) { return len >= n && std::memcmp(d, sig, n) == 0; };
-    return starts("\x89PNG\r\n\x1a\n", 8) || starts("\xff\xd8\xff", 3) || starts("II*\0", 4) ||
-           starts("MM\0*", 4) || starts("BM", 2) ||
-           (starts("RIFF", 4) && len >= 12 && std::memcmp(d + 8, "WEBP", 4) == 0) ||
-           (len >= 2 && d[0] == 'P' && d[1] >= '1' && d[1] <= '6');
+namespace {
+
+// The containers the reference's imdecode route recognises, told apart by
+// their leading magic bytes (PNG spec 5.2; JPEG T.81 B.1.1.2 SOI followed by
+// a marker; TIFF 6.0 section 2 byte-order mark and 42; BMP file header "BM";
+// RIFF/WEBP; Netpbm "P1".."P6").
+enum class Container { None, Png, Jpeg, Tiff, Bmp, Webp, Pnm };
+
+constexpr uint8_t kPngMagic[8] = {0x89, 'P', 'N', 'G', 0x0D, 0x0A, 0x1A, 0x0A};
+
+bool prefix_is(const uint8_t* d, size_t len, size_t at, const uint8_t* bytes, size_t n) {
+    if (len < at + n) return false;
+    for (size_t k = 0; k < n; ++k)
+        if (d[at + k] != bytes[k]) return false;
+    return true;
 }
+
+Container sniff(const uint8_t* d, size_t len) {
+    if (d == nullptr) return Container::None;
+    static const uint8_t soi[3] = {0xFF, 0xD8, 0xFF};
+    static const uint8_t tiff_le[4] = {'I', 'I', 42, 0}, tiff_be[4] = {'M', 'M', 0, 42};
+    static const uint8_t bm[2] = {'B', 'M'}, riff[4] = {'R', 'I', 'F', 'F'}, webp[4] = {'W', 'E', 'B', 'P'};
+    if (prefix_is(d, len, 0, kPngMagic, 8)) return Container::Png;
+    if (prefix_is(d, len, 0, soi, 3)) return Container::Jpeg;
+    if (prefix_is(d, len, 0, tiff_le, 4) || prefix_is(d, len, 0, tiff_be, 4)) return Container::Tiff;
+    if (prefix_is(d, len, 0, bm, 2)) return Container::Bmp;
+    if (prefix_is(d, len, 0, riff, 4) && prefix_is(d, len, 8, webp, 4)) return Container::Webp;
+    if (len >= 2 && d[0] == 'P' && d[1] >= '1' && d[1] <= '6') return Container::Pnm;
+    return Container::None;
+}
+
+// PNG: the first chunk is IHDR, whose data opens with the width and height as
+// big-endian 32-bit integers (PNG spec 11.2.2).
+bool png_size(const uint8_t* d, size_t len, uint64_t& w, uint64_t& h) {
+    static const uint8_t ihdr[4] = {'I', 'H', 'D', 'R'};
+    if (len < 24 || !prefix_is(d, len, 12, ihdr, 4)) return false;
+    w = be32(d + 16);
+    h = be32(d + 20);
+    return true;
+}
+
+// BMP: after the 14-byte file header, the info header's signed 32-bit width
+// and height (a negative height is a top-down bitmap).
+bool bmp_size(const uint8_t* d, size_t len, uint64_t& w, uint64_t& h) {
+    if (len < 26) return false;
+    const int64_t bw = (int32_t)le32(d + 18), bh = (int32_t)le32(d + 22);
+    w = (uint64_t)(bw < 0 ? -bw : bw);
+    h = (uint64_t)(bh < 0 ? -bh : bh);
+    return true;
+}
+
+// JPEG (T.81 annex B): marker segments after SOI up to the first frame
+// header. Fill bytes 0xFF may precede a marker; SOI, TEM and RST0..7 stand
+// alone; every other marker carries a 16-bit length that counts itself. A
+// frame header (SOF0..15 other than DHT C4, JPG C8 and DAC CC) holds the
+// sample precision, then Y (lines) and X (samples per line). A scan or the
+// end of the image before any frame header leaves the size unknown.
+bool jpeg_size(const uint8_t* d, size_t len, uint64_t& w, uint64_t& h) {
+    auto standalone = [](uint8_t m) { return m == 0xD8 || m == 0x01 || (m >= 0xD0 && m <= 0xD7); };
+    auto frame = [](uint8_t m) { return m >= 0xC0 && m <= 0xCF && m != 0xC4 && m != 0xC8 && m != 0xCC; };
+    size_t at = 2;   // past SOI
+    for (;;) {
+        if (at + 4 > len || d[at] != 0xFF) return false;
+        const uint8_t code = d[at + 1];
+        if (code == 0xFF) {   // a fill byte before the marker
+            at += 1;
+            continue;
+        }
+        if (standalone(code)) {
+            at += 2;
+            continue;
+        }
+        if (code == 0xDA || code == 0xD9) return false;
+        if (frame(code)) {
+            if (at + 9 > len) return false;
+            h = be16(d + at + 5);
+            w = be16(d + at + 7);
+            return true;
+        }
+        const size_t seglen = be16(d + at + 2);
+        if (seglen < 2) return false;
+        at += 2 + seglen;
+    }
+}
+
+// TIFF (6.0, section 2): the first IFD's ImageWidth (256) and ImageLength
+// (257) entries, of type SHORT (3) or LONG (4); the value sits left-justified
+// in the entry's last four bytes in either byte order.
+bool tiff_size(const uint8_t* d, size_t len, uint64_t& w, uint64_t& h) {
+    if (len < 8) return false;
+    const TiffReader t{d, len, d[0] == 'I'};
+    const uint64_t ifd = t.u32(4);
+    if (ifd + 2 > len) return false;
+    const uint64_t entries = t.u16((size_t)ifd);
+    int found = 0;   // bit 0: width, bit 1: height
+    for (uint64_t k = 0; k < entries; ++k) {
+        const uint64_t at = ifd + 2 + 12 * k;
+        if (at + 12 > len) return false;
+        const uint32_t tag = t.u16((size_t)at);
+        if (tag != 256 && tag != 257) continue;
+        const uint32_t type = t.u16((size_t)at + 2);
+        const bool integral = type == 3 || type == 4;
+        const uint64_t v = type == 3 ? t.u16((size_t)at + 8) : type == 4 ? t.u32((size_t)at + 8) : 0;
+        const int bit = tag == 256 ? 1 : 2;
+        if (tag == 256)
+            w = v;
+        else
+            h = v;
+        found = integral ? (found | bit) : (found & ~bit);
+    }
+    return found == 3;
+}
+
+}  // namespace
+
+bool has_image_signature(const uint8_t* d, size_t len) { return sniff(d, len) != Container::None; }
 
 bool header_dimensions(const uint8_t* d, size_t len, uint64_t& w, uint64_t& h) {
     if (d == nullptr) return false;
-    if (len >= 24 && std::memcmp(d, "\x89PNG\r\n\x1a\n", 8) == 0 && std::memcmp(d + 12, "IHDR", 4) == 0) {
-        w = be32(d + 16);
-        h = be32(d + 20);
-        return true;
+    switch (sniff(d, len)) {
+        case Container::Png: return png_size(d, len, w, h);
+        case Container::Bmp: return bmp_size(d, len, w, h);
+        case Container::Tiff: return tiff_size(d, len, w, h);
+        default: break;
     }
-    if (len >= 26 && d[0] == 'B' && d[1] == 'M') {
-        const int32_t bw = (int32_t)le32(d + 18), bh = (int32_t)le32(d + 22);
-        w = (uint64_t)std::llabs((long long)bw);
-        h = (uint64_t)std::llabs((long long)bh);
-        return true;
-    }
-    if (len >= 4 && d[0] == 0xFF && d[1] == 0xD8) {
-        size_t i = 2;
-        while (i + 4 <= len) {
-            if (d[i] != 0xFF) return false;
-            const uint8_t m = d[i + 1];
-            if (m == 0xFF) {
-                ++i;
-                continue;
-            }
-            if (m == 0xD8 || m == 0x01 || (m >= 0xD0 && m <= 0xD7)) {
-                i += 2;
-                continue;
-            }
-            if (m == 0xD9 || m == 0xDA) return false;
-            const size_t seg = be16(d + i + 2);
-            if (m >= 0xC0 && m <= 0xCF && m != 0xC4 && m != 0xC8 && m != 0xCC) {
-                if (i + 9 > len) return false;
-                h = be16(d + i + 5);
-                w = be16(d + i + 7);
-                return true;
-            }
-            if (seg < 2) return false;
-            i += 2 + seg;
-        }
-        return false;
-    }
-    if (len >= 8 && (std::memcmp(d, "II*\0", 4) == 0 || std::memcmp(d, "MM\0*", 4) == 0)) {
-        TiffReader t{d, len, d[0] == 'I'};
-        const size_t ifd = t.u32(4);
-        if ((uint64_t)ifd + 2 > len) return false;
-        const size_t n = t.u16(ifd);
-        bool got_w = false, got_h = false;
-        for (size_t k = 0; k < n; ++k) {
-            const size_t e = ifd + 2 + 12 * k;
-            if ((uint64_t)e + 12 > len) return false;
-            const uint32_t tag = t.u16(e), type = t.u16(e + 2);
-            const uint64_t v = type == 3 ? t.u16(e + 8) : type == 4 ? t.u32(e + 8) : 0;
-            if (tag == 256) {
-                w = v;
-                got_w = type == 3 || type == 4;
-            }
-            if (tag == 257) {
-                h = v;
-                got_h = type == 3 || type == 4;
-            }
-        }
-        return got_w && got_h;
-    }
+    // a JPEG's size needs only SOI and a marker (sniff wants three bytes)
+    if (len >= 4 && d[0] == 0xFF && d[1] == 0xD8) return jpeg_size(d, len, w, h);
     return false;
 }
 

@@ -524,6 +524,24 @@ def test_truncated_tiff_is_refused(lib):
     assert dec(lib, data) is not None and dec(lib, data[:-100]) is None
 
 
+def test_tiff_tiles_beyond_the_data_are_refused_before_allocation(lib):
+    """A few hundred bytes that declare a 1 x 1 image in 65536 x 65536 Deflate
+    tiles: the tile would need 4 GiB, while its coded bytes can inflate to at
+    most 1032 times their size. It is refused without touching that memory
+    (as fast as any other refusal)."""
+    import time
+    data = bytearray(tiff_bytes(np.array([[[7]]], np.uint8), 8, 1, tile=(16, 16), compression=8))
+    n = struct.unpack("<H", data[8:10])[0]
+    for k in range(n):
+        e = 10 + 12 * k
+        if struct.unpack("<H", data[e:e + 2])[0] in (322, 323):
+            data[e + 8:e + 12] = struct.pack("<I", 65536)
+    assert len(data) < 400
+    t0 = time.perf_counter()
+    assert dec(lib, bytes(data)) is None
+    assert time.perf_counter() - t0 < 0.05
+
+
 def test_unsupported_formats_fail(lib):
     """WebP and CMYK JPEG are not decoded here: the call fails, as a file
     that does not decode fails in the reference."""

@@ -29,10 +29,10 @@ def test_oracle_golden_corpus(kind, use6):
     assert n == len(g)
     iv, fv = iv[:n], fv[:n]
     assert (iv[:, 0] == g["x"]).all() and (iv[:, 1] == g["y"]).all()
-    scen = np.array([0] * (n // 2) + [1] * (n - n // 2))
-    # scenario boundary: the corpus lists scenario 0's initialised nodes first
-    nodes0 = sum(1 for _ in range(n))  # placeholder for clarity
-    del nodes0
+    # the corpus lists scenario 0's initialised nodes, then scenario 1's; both
+    # scenarios share the reference image, so the same nodes initialise and
+    # each scenario holds half of the list
+    assert n % 2 == 0 and (iv[: n // 2, :2] == iv[n // 2:, :2]).all()
     flips = 0
     for i in range(n):
         gv = np.array([g[k][i] for k in ("u", "v", "ux", "uy", "vx", "vy")])
@@ -47,7 +47,6 @@ def test_oracle_golden_corpus(kind, use6):
         if g["status"][i] == 0:
             assert agree, (i, gv, fv[i])
     assert flips <= 6
-    del scen
 
 
 @pytest.mark.parametrize("scenario", [0, 1])
