@@ -155,6 +155,16 @@ SDK_API int sdk_engine_node_report(void* h, uint8_t* tag, uint8_t* path, int n);
    count, or -3. */
 SDK_API int sdk_engine_node_planes(void* h, int which, float* out, int n);
 
+/* The global-motion model the last run seeded its anchor lattice with (the
+   replacement of the reference's AKAZE homography, full_field_anchors.cpp:
+   204-279): out12 = {locked, full, a0, a1, a2, a3, a4, a5, cx, cy, patches,
+   pass}, with u = a0 + a1 (x - cx) + a2 (y - cy) and v = a3 + a4 (x - cx) +
+   a5 (y - cy) at reference pixel (x, y); full = an ROI corner moves more than
+   4 px beyond the centre's translation; pass = the search that locked it (1
+   translation, 2 the wider translation search, 3 the rotation-invariant
+   search; 0 none). 0 or -3. */
+SDK_API int sdk_engine_global_model(void* h, double* out12);
+
 /* Graph captures and replays of an engine so far: stats2 = {captures,
    replays}. 0 or -3. */
 SDK_API int sdk_engine_graph_stats(void* h, int32_t* stats2);

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "oracle.hpp"
+#include "seeding.hpp"
 #include "synth.hpp"
 
 using namespace orc;
@@ -33,6 +34,23 @@ void orc_blob_eval(uint64_t seed, int w, int h, float density, float pad, int n,
 }
 
 float orc_hash_u01(uint64_t seed, uint64_t i) { return hash_u01(seed, i); }
+
+// ---- seeding front end -------------------------------------------------------
+
+// The global-motion model run_full_field seeds the anchor lattice with, on a
+// u8 pair and rect (x, y, w, h): out12 = {locked, full, a0..a5, cx, cy,
+// patches, 0}, u = a0 + a1 (x - cx) + a2 (y - cy), v = a3 + a4 (.) + a5 (.).
+void orc_global_motion(const uint8_t* ref, const uint8_t* def, int w, int h, const int32_t* rect, double* out12) {
+    const int L = pyramid_levels(rect[2], rect[3]);
+    std::vector<float> rf(ref, ref + (size_t)w * h), df(def, def + (size_t)w * h);
+    Pyramid pr, pd;
+    build_pyramid(pr, rf.data(), w, h, L);
+    build_pyramid(pd, df.data(), w, h, L);
+    const GlobalModel gm = global_motion(pr, pd, rect[0], rect[1], rect[2], rect[3]);
+    const double v[12] = {gm.locked ? 1.0 : 0.0, gm.full ? 1.0 : 0.0, gm.a[0], gm.a[1], gm.a[2], gm.a[3], gm.a[4],
+                          gm.a[5], gm.cx, gm.cy, (double)gm.patches, 0.0};
+    std::copy(v, v + 12, out12);
+}
 
 // ---- subset level ------------------------------------------------------------
 

@@ -523,6 +523,12 @@ SDK_API int sdk_engine_node_planes(void* h, int which, float* out, int n) {
     return (int)v.size();
 }
 
+SDK_API int sdk_engine_global_model(void* h, double* out12) {
+    if (!h || !out12) return -3;
+    std::memcpy(out12, ((DevEngine*)h)->eng.last_model, sizeof(double) * 12);
+    return 0;
+}
+
 SDK_API int sdk_engine_graph_stats(void* h, int32_t* stats2) {
     if (!h || !stats2) return -3;
     const auto g = ((DevEngine*)h)->eng.graph_stats();

@@ -636,8 +636,8 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
         const auto t_seed = clk::now();
         const int L = std::min(pyramid_levels(p.rect_w, p.rect_h), pyr_ref_levels_);
         // the learned seeder reads level 2 of both pyramids
-        const bool use_cnn = learned_seed && seeder_ready() && pyr_ref_levels_ >= SeederShape::kLevel;
-        const int Lb = use_cnn ? std::max(L, (int)SeederShape::kLevel) : L;
+        const bool cnn_ok = learned_seed && seeder_ready() && pyr_ref_levels_ >= SeederShape::kLevel;
+        const int Lb = cnn_ok ? std::max(L, (int)SeederShape::kLevel) : L;
         seed_stats = SeedStats{};
         PyrDev pd = {};
         pd.r0 = ref_u8_.p;
@@ -659,7 +659,7 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                 pw = nw;
                 ph = nh;
             }
-            if (use_cnn) {
+            if (cnn_ok) {
                 w2 = pd.w[SeederShape::kLevel];
                 h2 = pd.h[SeederShape::kLevel];
             }
@@ -672,28 +672,48 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
         ck(hipMemcpyAsync(centres_d_.p, centres.data(), sizeof(int) * centres.size(), hipMemcpyHostToDevice, st_));
         std::vector<PatchOut> po(npatch);
         GlobalModel gm;
+        int pass = 0;
         for (const int rad : {kCoarseRadius, kCoarseRadiusWide}) {
             pd.coarse_radius = rad;
             ck(launch_patch_search(pd, centres_d_.p, npatch, patch_d_.p, st_));
             ck(hipMemcpyAsync(po.data(), patch_d_.p, sizeof(PatchOut) * npatch, hipMemcpyDeviceToHost, st_));
             ck(hipStreamSynchronize(st_));
             gm = fit_global_model(centres, po, p.rect_x, p.rect_y, p.rect_w, p.rect_h);
+            ++pass;
             if (gm.locked) break;
         }
+        {
+            const double v[12] = {gm.locked ? 1.0 : 0.0, gm.full ? 1.0 : 0.0, gm.a[0], gm.a[1], gm.a[2], gm.a[3],
+                                  gm.a[4], gm.a[5], gm.cx, gm.cy, (double)gm.patches, gm.locked ? (double)pass : 0.0};
+            std::copy(v, v + 12, last_model);
+        }
 
+        // Learned seeding (no reference counterpart) runs the CNN lattice
+        // when the global model finds a large motion (an ROI corner moving
+        // more than 4 px beyond the centre's translation, or no lock). A
+        // near-translation is the reference's plain lattice's case: it seeds
+        // it first, and the CNN lattice follows only if that lattice's
+        // anchors do not converge.
+        bool use_cnn = cnn_ok && !(gm.locked && !gm.full);
         // lattice (solve_anchor_seeds, full_field_anchors.cpp:283-574)
-        int sx, sy;
-        // the learned seeder's lattice is 16 times denser (anchors 4 times
-        // closer): its guesses cost one CNN pass each instead of a coarse
-        // search, and the 8-neighbour median test (+-1 px on residuals) holds
-        // where a field curves between anchors of the plain spacing
-        plan_lattice(gw, gh, sx, sy, use_cnn ? 16 * kAnchorTarget : kAnchorTarget);
+        int sx = 1, sy = 1, LW = 0, LH = 0;
         std::vector<int> lx, ly;
-        for (int x = 0; x < gw; x += sx) lx.push_back(x);
-        if (lx.back() != gw - 1) lx.push_back(gw - 1);
-        for (int y = 0; y < gh; y += sy) ly.push_back(y);
-        if (ly.back() != gh - 1) ly.push_back(gh - 1);
-        const int LW = (int)lx.size(), LH = (int)ly.size();
+        auto plan = [&]() {
+            // the learned seeder's lattice is 16 times denser (anchors 4 times
+            // closer): its guesses cost one CNN pass each instead of a coarse
+            // search, and the 8-neighbour median test (+-1 px on residuals)
+            // holds where a field curves between anchors of the plain spacing
+            plan_lattice(gw, gh, sx, sy, use_cnn ? 16 * kAnchorTarget : kAnchorTarget);
+            lx.clear();
+            ly.clear();
+            for (int x = 0; x < gw; x += sx) lx.push_back(x);
+            if (lx.back() != gw - 1) lx.push_back(gw - 1);
+            for (int y = 0; y < gh; y += sy) ly.push_back(y);
+            if (ly.back() != gh - 1) ly.push_back(gh - 1);
+            LW = (int)lx.size();
+            LH = (int)ly.size();
+        };
+        plan();
         struct Seeds {
             std::vector<AnchorRec> an;
             int quality = 0, attempted = 0, accepted = 0, solved = 0;
@@ -857,7 +877,9 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
         };
         bool have_affine = false;
         double aff[6] = {0, 0, 0, 0, 0, 0}, acx = 0, acy = 0;
-        if (LW >= 2 && LH >= 2) {
+        auto seed_lattice = [&]() {
+            have_affine = false;
+            if (LW < 2 || LH < 2) return;
             if (!gm.full) {
                 solve_lattice(false, seeds);
             } else {
@@ -888,6 +910,21 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                 acx = gm.cx;
                 acy = gm.cy;
             }
+        };
+        seed_lattice();
+        // the plain lattice seeds the field when its anchors converge (the
+        // output gate at 90% of those attempted) and it reaches SPARSE
+        // quality or better; else the CNN lattice takes over
+        int converged = 0;
+        for (const AnchorRec& a : seeds.an) converged += a.output ? 1 : 0;
+        const bool plain_seeds = seeds.quality >= 1 && converged >= 0.9f * (float)std::max(1, seeds.attempted);
+        if (cnn_ok && !use_cnn && !plain_seeds) {
+            use_cnn = true;
+            plan();
+            seeds = Seeds{};
+            seeds.gU = (float)gm.t_u;
+            seeds.gV = (float)gm.t_v;
+            seed_lattice();
         }
         // publish_anchor_results (full_field_anchors.cpp:576-607)
         int anchor_pts = 0;

@@ -269,6 +269,10 @@ public:
     std::vector<float> report_planes_a, report_planes;
     // and the six guess planes (u, v, ux, uy, vx, vy) Path A started from
     std::vector<float> report_guess;
+    // The global-motion model of the last run: locked, full, a0..a5, cx, cy,
+    // patches, the pass that locked it (0 none, 1 translation search, 2 the
+    // wide translation search, 3 the rotation-invariant search).
+    double last_model[12] = {};
     struct GraphStats {
         int captures = 0, replays = 0;
     };

@@ -89,6 +89,7 @@ def load_dev(path=LIB_DEV):
     lib.sdk_solve_nodes_noalloc.argtypes = [_VP, _VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int, _VP, _VP, _VP,
                                             _VP, _VP, _VP]
+    lib.sdk_engine_global_model.argtypes = [_VP, _VP]
     lib.sdk_pool.argtypes = [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, _VP,
                              _VP, _VP]
     lib.sdk_ref_planes.argtypes = [_VP, _VP, ctypes.c_int, ctypes.c_int, _VP, _VP, _VP]

@@ -62,6 +62,7 @@ def oracle():
         lib.orc_vsg_window_node_count.argtypes = [ctypes.c_int, ctypes.c_int]
         lib.orc_blob_eval.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
                                       ctypes.c_int, VP, VP, VP]
+        lib.orc_global_motion.argtypes = [VP, VP, ctypes.c_int, ctypes.c_int, VP, VP]
         lib.orc_hash_u01.restype = ctypes.c_float
         lib.orc_hash_u01.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
         _oracle = lib
@@ -280,6 +281,25 @@ def split_analysis(ref, dfm, prm, planes_g, planes_o, dim, tol, lm=True, agree=1
     res["first"] = [h if h in ("gpu", "oracle") else "-" for h in res["how"]]
     res["split"] = [h in ("gpu", "oracle") for h in res["how"]]
     return res
+
+
+def oracle_global_motion(ref, dfm, rect):
+    """The oracle's global-motion model on a u8 pair: dict of locked, full,
+    a (6: u = a0 + a1 dx + a2 dy, v = a3 + a4 dx + a5 dy about the rect
+    centre), cx, cy, patches."""
+    h, w = ref.shape
+    out = np.zeros(12, np.float64)
+    oracle().orc_global_motion(P(np.ascontiguousarray(ref, np.uint8)), P(np.ascontiguousarray(dfm, np.uint8)), w, h,
+                               P(np.array(rect, np.int32)), P(out))
+    return dict(locked=bool(out[0]), full=bool(out[1]), a=out[2:8].copy(), cx=out[8], cy=out[9],
+                patches=int(out[10]))
+
+
+def model_at(m, x, y):
+    """(u, v) of a global-motion model at reference points (x, y)."""
+    a = m["a"]
+    dx, dy = np.asarray(x, np.float64) - m["cx"], np.asarray(y, np.float64) - m["cy"]
+    return a[0] + a[1] * dx + a[2] * dy, a[3] + a[4] * dx + a[5] * dy
 
 
 def oracle_vsg(planes, step, window):
