@@ -243,10 +243,20 @@ bool consensus_affine(const std::vector<double>& x, const std::vector<double>& y
 GlobalModel global_motion_pass(const Pyramid& ref, const Pyramid& def, int rect_x, int rect_y, int rect_w,
                                int rect_h, int coarse_radius);
 
+// The turn of a model's linear part, atan2(F10 - F01, F00 + F11) in degrees.
+// A translation pass's untouched templates decorrelate long before a quarter
+// turn, so a model it fits with more than 30 degrees is a false lock.
+static bool plausible(const GlobalModel& gm) {
+    const double f00 = 1.0 + gm.a[1], f01 = gm.a[2], f10 = gm.a[4], f11 = 1.0 + gm.a[5];
+    return std::fabs(std::atan2(f10 - f01, f00 + f11) * (180.0 / 3.14159265358979323846)) <= 30.0;
+}
+
 GlobalModel global_motion(const Pyramid& ref, const Pyramid& def, int rect_x, int rect_y,
                           int rect_w, int rect_h) {
     GlobalModel gm = global_motion_pass(ref, def, rect_x, rect_y, rect_w, rect_h, kCoarseRadius);
+    if (gm.locked && !plausible(gm)) gm = GlobalModel{};
     if (!gm.locked) gm = global_motion_pass(ref, def, rect_x, rect_y, rect_w, rect_h, kCoarseRadiusWide);
+    if (gm.locked && !plausible(gm)) gm = GlobalModel{};
     return gm;
 }
 

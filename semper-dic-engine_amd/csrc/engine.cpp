@@ -680,7 +680,62 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             ck(hipStreamSynchronize(st_));
             gm = fit_global_model(centres, po, p.rect_x, p.rect_y, p.rect_w, p.rect_h);
             ++pass;
+            // the translation search's untouched templates decorrelate long
+            // before a quarter turn: a model it fits with a larger turn is a
+            // false lock of mismatched patches
+            if (gm.locked && std::fabs(model_turn_deg(gm)) > kTranslationMaxTurnDeg) gm = GlobalModel{};
             if (gm.locked) break;
+        }
+        // Motions the translation passes cannot follow (a large rotation,
+        // rotation with scale): rotation-invariant matching of 5 x 5 patch
+        // centres over the whole deformed image at the top level, a rough
+        // affine through their consensus, then the translation search again
+        // through that affine's linear map, started at its predicted shifts.
+        if (!gm.locked && pd.w[L] > 2 * kCanonMargin + 2 && pd.h[L] > 2 * kCanonMargin + 2) {
+            const int g5 = 5;
+            std::vector<int> cl;
+            for (int j = 0; j < g5; ++j)
+                for (int i = 0; i < g5; ++i) {
+                    cl.push_back((p.rect_x + (int)std::lround((i + 0.5) * p.rect_w / g5)) >> L);
+                    cl.push_back((p.rect_y + (int)std::lround((j + 0.5) * p.rect_h / g5)) >> L);
+                }
+            const int nc = g5 * g5;
+            const size_t nscores = canon_scores_floats(pd.w[L], pd.h[L], nc);
+            if (!canon_centres_d_.reserve(cl.size()) || !canon_desc_.reserve((size_t)nc * kCanonSamples + nc) ||
+                !canon_scores_.reserve(nscores) || !canon_out_.reserve(nc) || !canon_start_.reserve(2 * (size_t)npatch))
+                return kErrInit;
+            ck(hipMemcpyAsync(canon_centres_d_.p, cl.data(), sizeof(int) * cl.size(), hipMemcpyHostToDevice, st_));
+            ck(launch_canon_match(pd, L, canon_centres_d_.p, nc, canon_desc_.p, canon_desc_.p + (size_t)nc * kCanonSamples,
+                                  canon_scores_.p, canon_out_.p, st_));
+            std::vector<CanonOut> co(nc);
+            ck(hipMemcpyAsync(co.data(), canon_out_.p, sizeof(CanonOut) * nc, hipMemcpyDeviceToHost, st_));
+            ck(hipStreamSynchronize(st_));
+            const double mcx = p.rect_x + 0.5 * p.rect_w, mcy = p.rect_y + 0.5 * p.rect_h;
+            double ra[6];
+            if (canon_affine(cl, co, L, mcx, mcy, ra)) {
+                std::vector<int> start(2 * (size_t)npatch);
+                const double sc = (double)(1 << L);
+                for (int k = 0; k < npatch; ++k) {
+                    const double dx = centres[2 * k] - mcx, dy = centres[2 * k + 1] - mcy;
+                    start[2 * k] = (int)std::lround((ra[0] + ra[1] * dx + ra[2] * dy) / sc);
+                    start[2 * k + 1] = (int)std::lround((ra[3] + ra[4] * dx + ra[5] * dy) / sc);
+                }
+                ck(hipMemcpyAsync(canon_start_.p, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice,
+                                  st_));
+                PyrDev pw = pd;
+                pw.warp = 1;
+                pw.f[0] = 1.0 + ra[1];
+                pw.f[1] = ra[2];
+                pw.f[2] = ra[4];
+                pw.f[3] = 1.0 + ra[5];
+                pw.start = canon_start_.p;
+                pw.coarse_radius = 2;
+                ck(launch_patch_search(pw, centres_d_.p, npatch, patch_d_.p, st_));
+                ck(hipMemcpyAsync(po.data(), patch_d_.p, sizeof(PatchOut) * npatch, hipMemcpyDeviceToHost, st_));
+                ck(hipStreamSynchronize(st_));
+                gm = fit_global_model(centres, po, p.rect_x, p.rect_y, p.rect_w, p.rect_h);
+                pass = 3;
+            }
         }
         {
             const double v[12] = {gm.locked ? 1.0 : 0.0, gm.full ? 1.0 : 0.0, gm.a[0], gm.a[1], gm.a[2], gm.a[3],

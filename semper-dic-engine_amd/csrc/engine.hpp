@@ -388,6 +388,11 @@ private:
     DevBuf<int32_t> celltri_d_, lat_d_;
     DevBuf<int32_t> centres_d_;
     DevBuf<PatchOut> patch_d_;
+    // the rotation-invariant pass: level centres, descriptions and
+    // orientations, scores, best matches, the refinement's start shifts
+    DevBuf<int32_t> canon_centres_d_, canon_start_;
+    DevBuf<float> canon_desc_, canon_scores_;
+    DevBuf<CanonOut> canon_out_;
     Timings tm_;
 };
 

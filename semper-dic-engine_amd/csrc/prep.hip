@@ -369,16 +369,43 @@ __device__ __forceinline__ float lv_at(const PyrDev& p, bool def, int l, int idx
 // rational with at most 6 fractional bits and at most 8 integer bits, so the
 // five sums are exact in double in any order: the score does not depend on
 // how the lanes split the work.
+// The deformed level bilinearly at (x, y), inside [0, w - 1) x [0, h - 1).
+__device__ __forceinline__ double lv_bilinear(const PyrDev& p, int l, double x, double y) {
+    const int w = p.w[l];
+    const int xi = (int)floor(x), yi = (int)floor(y);
+    const double fx = x - xi, fy = y - yi;
+    const int i = yi * w + xi;
+    const double a = lv_at(p, true, l, i), b = lv_at(p, true, l, i + 1);
+    const double c = lv_at(p, true, l, i + w), d = lv_at(p, true, l, i + w + 1);
+    return (a * (1.0 - fx) + b * fx) * (1.0 - fy) + (c * (1.0 - fx) + d * fx) * fy;
+}
+
 __device__ bool zncc(const PyrDev& p, int l, int cx, int cy, int sx, int sy, double& out, double* tstd) {
     const int w = p.w[l], h = p.h[l];
     const int x0 = cx - kPatchHalf, y0 = cy - kPatchHalf, x1 = cx + kPatchHalf, y1 = cy + kPatchHalf;
     if (x0 < 0 || y0 < 0 || x1 > w || y1 > h) return false;
-    if (x0 + sx < 0 || y0 + sy < 0 || x1 + sx > w || y1 + sy > h) return false;
+    if (!p.warp) {
+        if (x0 + sx < 0 || y0 + sy < 0 || x1 + sx > w || y1 + sy > h) return false;
+    } else {
+        // the warped template's corners, with the bilinear footprint, inside
+        for (int k = 0; k < 4; ++k) {
+            const double ox = (k & 1) ? kPatchHalf - 1 : -kPatchHalf, oy = (k & 2) ? kPatchHalf - 1 : -kPatchHalf;
+            const double qx = cx + sx + p.f[0] * ox + p.f[1] * oy, qy = cy + sy + p.f[2] * ox + p.f[3] * oy;
+            if (!(qx >= 0.0 && qy >= 0.0 && qx < (double)(w - 1) && qy < (double)(h - 1))) return false;
+        }
+    }
     const int lane = threadIdx.x & 63, side = 2 * kPatchHalf;
     double sr = 0, sd = 0, srr = 0, sdd = 0, srd = 0;
     for (int k = lane; k < side * side; k += 64) {
         const int y = y0 + k / side, x = x0 + k % side;
-        const double a = lv_at(p, false, l, y * w + x), b = lv_at(p, true, l, (y + sy) * w + x + sx);
+        const double a = lv_at(p, false, l, y * w + x);
+        double b;
+        if (!p.warp) {
+            b = lv_at(p, true, l, (y + sy) * w + x + sx);
+        } else {
+            const double ox = x - cx, oy = y - cy;
+            b = lv_bilinear(p, l, cx + sx + p.f[0] * ox + p.f[1] * oy, cy + sy + p.f[2] * ox + p.f[3] * oy);
+        }
         sr += a;
         sd += b;
         srr += a * a;
@@ -437,6 +464,11 @@ __global__ __launch_bounds__(64 * kPatchWaves) void k_patch(PyrDev p, const int*
         s_sy = 0;
         s_ok = 1;
         s_best = -3.0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && p.start) {
+        s_sx = p.start[2 * pi];
+        s_sy = p.start[2 * pi + 1];
     }
     __syncthreads();
     for (int l = L; l >= 0; --l) {
@@ -504,6 +536,198 @@ __global__ __launch_bounds__(64 * kPatchWaves) void k_patch(PyrDev p, const int*
             o.has_y = s_has[3] && s_has[4];
         }
         out[pi] = o;
+    }
+}
+
+// ---- rotation-invariant patch matching ---------------------------------------
+//
+// A disc of radius kCanonR at level l is described independently of its
+// rotation: its orientation is the angle of its intensity centroid (the
+// first moments of the disc about its centre; a disc turned by t has its
+// centroid turned by t), and the disc is sampled bilinearly on a polar grid
+// (radii 2 .. kCanonR - 1, kCanonAngles angles) turned by that angle, then
+// normalised to zero mean and unit norm. Two discs that differ by a rotation
+// give nearly the same samples; the score of a pair is their dot product.
+
+__device__ __forceinline__ float lvf(const PyrDev& p, bool def, int l, int idx) {
+    return (float)lv_at(p, def, l, idx);
+}
+
+// orientation of the disc at integer (x, y): its centroid's angle
+__device__ float canon_theta(const PyrDev& p, bool def, int l, int x, int y) {
+    const int w = p.w[l];
+    float mx = 0.0f, my = 0.0f;
+    for (int dy = -kCanonR; dy <= kCanonR; ++dy) {
+        const int span = (int)floorf(sqrtf((float)(kCanonR * kCanonR - dy * dy)));
+        const int row = (y + dy) * w + x;
+        for (int dx = -span; dx <= span; ++dx) {
+            const float v = lvf(p, def, l, row + dx);
+            mx += v * (float)dx;
+            my += v * (float)dy;
+        }
+    }
+    return atan2f(my, mx);
+}
+
+__device__ __forceinline__ float lv_bilinear_f(const PyrDev& p, bool def, int l, float x, float y) {
+    const int w = p.w[l];
+    const int xi = (int)floorf(x), yi = (int)floorf(y);
+    const float fx = x - (float)xi, fy = y - (float)yi;
+    const int i = yi * w + xi;
+    const float a = lvf(p, def, l, i), b = lvf(p, def, l, i + 1);
+    const float c = lvf(p, def, l, i + w), d = lvf(p, def, l, i + w + 1);
+    return (a * (1.0f - fx) + b * fx) * (1.0f - fy) + (c * (1.0f - fx) + d * fx) * fy;
+}
+
+// polar sample k (ring-major) of the disc at (x, y) turned by theta
+__device__ __forceinline__ float canon_sample(const PyrDev& p, bool def, int l, int x, int y, float ct, float st,
+                                              int k) {
+    const int ring = k / kCanonAngles, ang = k - ring * kCanonAngles;
+    const float rho = (float)(ring + 2);
+    float sa, ca;
+    sincosf((float)ang * (6.283185307179586f / (float)kCanonAngles), &sa, &ca);
+    // (ca, sa) turned by theta
+    const float dx = rho * (ca * ct - sa * st), dy = rho * (sa * ct + ca * st);
+    return lv_bilinear_f(p, def, l, (float)x + dx, (float)y + dy);
+}
+
+// One wave per reference patch: its orientation and normalised samples.
+__global__ __launch_bounds__(64) void k_canon_ref(PyrDev p, int l, const int* __restrict__ centres, int npatch,
+                                                  float* __restrict__ desc, float* __restrict__ theta) {
+    const int q = blockIdx.x, lane = threadIdx.x;
+    if (q >= npatch) return;
+    const int x = centres[2 * q], y = centres[2 * q + 1];
+    const int w = p.w[l], h = p.h[l];
+    float* d = desc + (size_t)q * kCanonSamples;
+    if (x < kCanonMargin || y < kCanonMargin || x >= w - kCanonMargin || y >= h - kCanonMargin) {
+        for (int k = lane; k < kCanonSamples; k += 64) d[k] = 0.0f;
+        if (lane == 0) theta[q] = 0.0f;
+        return;
+    }
+    const float t = canon_theta(p, false, l, x, y);   // every lane the same value
+    float ct, st;
+    sincosf(t, &st, &ct);
+    float v[(kCanonSamples + 63) / 64];
+    float s = 0.0f, ss = 0.0f;
+#pragma unroll
+    for (int j = 0; j < (kCanonSamples + 63) / 64; ++j) {
+        const int k = lane + 64 * j;
+        v[j] = k < kCanonSamples ? canon_sample(p, false, l, x, y, ct, st, k) : 0.0f;
+        s += v[j];
+    }
+    s = wave_sum(s);
+    const float mean = s / (float)kCanonSamples;
+#pragma unroll
+    for (int j = 0; j < (kCanonSamples + 63) / 64; ++j) {
+        const int k = lane + 64 * j;
+        const float c = k < kCanonSamples ? v[j] - mean : 0.0f;
+        ss += c * c;
+    }
+    ss = wave_sum(ss);
+    const float inv = ss > 1e-12f ? 1.0f / sqrtf(ss) : 0.0f;
+#pragma unroll
+    for (int j = 0; j < (kCanonSamples + 63) / 64; ++j) {
+        const int k = lane + 64 * j;
+        if (k < kCanonSamples) d[k] = (v[j] - mean) * inv;
+    }
+    if (lane == 0) theta[q] = t;
+}
+
+// One thread per deformed position (x, y) of the level, the disc inside: the
+// score against each patch's description, scores[pos * npatch + q]. The
+// patch descriptions are zero-mean and unit-norm, so the correlation is the
+// dot product over the position's own norm.
+__global__ __launch_bounds__(256) void k_canon_scores(PyrDev p, int l, const float* __restrict__ desc, int npatch,
+                                                      float* __restrict__ scores) {
+    __shared__ float sd[kCanonMaxPatches * kCanonSamples];
+    for (int i = threadIdx.x; i < npatch * kCanonSamples; i += blockDim.x) sd[i] = desc[i];
+    __syncthreads();
+    const int w = p.w[l], h = p.h[l];
+    const int cw = w - 2 * kCanonMargin, chh = h - 2 * kCanonMargin;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cw <= 0 || chh <= 0 || i >= cw * chh) return;
+    const int x = kCanonMargin + i % cw, y = kCanonMargin + i / cw;
+    const float t = canon_theta(p, true, l, x, y);
+    float ct, st;
+    sincosf(t, &st, &ct);
+    float dot[kCanonMaxPatches];
+#pragma unroll
+    for (int q = 0; q < kCanonMaxPatches; ++q) dot[q] = 0.0f;
+    float s = 0.0f, ss = 0.0f;
+    for (int k = 0; k < kCanonSamples; ++k) {
+        const float v = canon_sample(p, true, l, x, y, ct, st, k);
+        s += v;
+        ss += v * v;
+#pragma unroll
+        for (int q = 0; q < kCanonMaxPatches; ++q)
+            if (q < npatch) dot[q] += v * sd[q * kCanonSamples + k];
+    }
+    const float var = ss - s * s / (float)kCanonSamples;
+    const float inv = var > 1e-6f ? 1.0f / sqrtf(var) : 0.0f;
+    float* o = scores + (size_t)i * npatch;
+#pragma unroll
+    for (int q = 0; q < kCanonMaxPatches; ++q)
+        if (q < npatch) o[q] = inv > 0.0f ? dot[q] * inv : -2.0f;
+}
+
+// One block per patch: the best position (ties to the lower index), then the
+// best one more than 2 px from it in either axis.
+__global__ __launch_bounds__(256) void k_canon_best(PyrDev p, int l, const float* __restrict__ scores, int npatch,
+                                                    const float* __restrict__ theta_ref, CanonOut* out) {
+    __shared__ float bs[256];
+    __shared__ int bi[256];
+    const int q = blockIdx.x, tid = threadIdx.x;
+    const int w = p.w[l], h = p.h[l];
+    const int cw = w - 2 * kCanonMargin, chh = h - 2 * kCanonMargin;
+    const int n = cw > 0 && chh > 0 ? cw * chh : 0;
+    float best = -3.0f;
+    int idx = -1;
+    for (int i = tid; i < n; i += blockDim.x) {
+        const float v = scores[(size_t)i * npatch + q];
+        if (v > best) {
+            best = v;
+            idx = i;
+        }
+    }
+    bs[tid] = best;
+    bi[tid] = idx;
+    __syncthreads();
+    for (int m = 128; m > 0; m >>= 1) {
+        if (tid < m) {
+            const float a = bs[tid], b = bs[tid + m];
+            const int ia = bi[tid], ib = bi[tid + m];
+            if (b > a || (b == a && ib >= 0 && (ia < 0 || ib < ia))) {
+                bs[tid] = b;
+                bi[tid] = ib;
+            }
+        }
+        __syncthreads();
+    }
+    const int ib = bi[0];
+    const float b0 = bs[0];
+    __syncthreads();
+    const int bx = ib >= 0 ? ib % cw : 0, by = ib >= 0 ? ib / cw : 0;
+    float sec = -3.0f;
+    for (int i = tid; i < n; i += blockDim.x) {
+        const int x = i % cw, y = i / cw;
+        if (abs(x - bx) <= 2 && abs(y - by) <= 2) continue;
+        sec = fmaxf(sec, scores[(size_t)i * npatch + q]);
+    }
+    bs[tid] = sec;
+    __syncthreads();
+    for (int m = 128; m > 0; m >>= 1) {
+        if (tid < m) bs[tid] = fmaxf(bs[tid], bs[tid + m]);
+        __syncthreads();
+    }
+    if (tid == 0) {
+        CanonOut o = {};
+        o.bx = ib >= 0 ? kCanonMargin + bx : -1;
+        o.by = ib >= 0 ? kCanonMargin + by : -1;
+        o.best = b0;
+        o.second = bs[0];
+        o.theta_ref = theta_ref[q];
+        o.theta_def = ib >= 0 ? canon_theta(p, true, l, o.bx, o.by) : 0.0f;
+        out[q] = o;
     }
 }
 
@@ -615,6 +839,23 @@ hipError_t launch_patch_search(const PyrDev& p, const int* centres, int npatch, 
                                hipStream_t st) {
     if (npatch <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_patch, dim3(npatch), dim3(64 * kPatchWaves), 0, st, p, centres, out);
+    return hipGetLastError();
+}
+
+size_t canon_scores_floats(int wl, int hl, int npatch) {
+    const int cw = wl - 2 * kCanonMargin, ch = hl - 2 * kCanonMargin;
+    return cw > 0 && ch > 0 ? (size_t)cw * ch * npatch : 0;
+}
+
+hipError_t launch_canon_match(const PyrDev& p, int l, const int* centres_l, int npatch, float* desc,
+                              float* theta_ref, float* scores, CanonOut* out, hipStream_t st) {
+    if (npatch <= 0 || npatch > kCanonMaxPatches || l < 0 || l > p.levels) return hipErrorInvalidValue;
+    const size_t n = canon_scores_floats(p.w[l], p.h[l], npatch) / npatch;
+    if (n == 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_canon_ref, dim3(npatch), dim3(64), 0, st, p, l, centres_l, npatch, desc, theta_ref);
+    hipLaunchKernelGGL(k_canon_scores, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, l, desc, npatch,
+                       scores);
+    hipLaunchKernelGGL(k_canon_best, dim3(npatch), dim3(256), 0, st, p, l, scores, npatch, theta_ref, out);
     return hipGetLastError();
 }
 

@@ -39,9 +39,9 @@ constexpr double kMinTemplateStd = 2.0;
 constexpr double kMinScore = 0.5;
 // The consensus fit when the plain least squares does not hold half the
 // patches (local motion beyond an affine): deterministic random triples, the
-// affine through the one with the most patches within kConsensusTol px.
+// affine through the one with the most patches within its tolerance (3 px for
+// the translation passes' shifts; seeding.hpp).
 constexpr int kConsensusTrials = 600;
-constexpr double kConsensusTol = 3.0;
 constexpr int kConsensusMinInliers = 6;
 constexpr double kConsensusMinArea = 1.0e4;   // |det| of the triple's design matrix, px^2
 
@@ -183,7 +183,8 @@ bool robust_affine(const std::vector<double>& x, const std::vector<double>& y, c
 }
 
 bool consensus_affine(const std::vector<double>& x, const std::vector<double>& y, const std::vector<double>& u,
-                      const std::vector<double>& v, double cx, double cy, double a[6]) {
+                      const std::vector<double>& v, double cx, double cy, double a[6],
+                      double tol) {
     const int n = (int)x.size();
     const int need = std::max(kConsensusMinInliers, n / 4);
     if (n < need || n < 3) return false;
@@ -212,9 +213,9 @@ bool consensus_affine(const std::vector<double>& x, const std::vector<double>& y
             const double eu = su[0] + su[1] * dx + su[2] * dy - u[q], ev = sv[0] + sv[1] * dx + sv[2] * dy - v[q];
             // a patch clearly beyond the tolerance is out without the hypot
             // (which is within an ulp of the root, far inside the margin)
-            if (eu * eu + ev * ev > kConsensusTol * kConsensusTol * (1.0 + 1e-9)) continue;
+            if (eu * eu + ev * ev > tol * tol * (1.0 + 1e-9)) continue;
             const double e = std::hypot(eu, ev);
-            if (e <= kConsensusTol) {
+            if (e <= tol) {
                 ++in;
                 res += e;
             }
@@ -237,14 +238,14 @@ bool consensus_affine(const std::vector<double>& x, const std::vector<double>& y
         const double dx = x[q] - cx, dy = y[q] - cy;
         const double e = std::hypot(best[0] + best[1] * dx + best[2] * dy - u[q],
                                     best[3] + best[4] * dx + best[5] * dy - v[q]);
-        if (e <= kConsensusTol) {
+        if (e <= tol) {
             xi.push_back(x[q]);
             yi.push_back(y[q]);
             ui.push_back(u[q]);
             vi.push_back(v[q]);
         }
     }
-    return robust_affine(xi, yi, ui, vi, cx, cy, need, kConsensusTol, a);
+    return robust_affine(xi, yi, ui, vi, cx, cy, need, tol, a);
 }
 
 void GlobalModel::at(double x, double y, float g[6]) const {
@@ -255,6 +256,28 @@ void GlobalModel::at(double x, double y, float g[6]) const {
     g[3] = (float)a[2];
     g[4] = (float)a[4];
     g[5] = (float)a[5];
+}
+
+double model_turn_deg(const GlobalModel& gm) {
+    const double f00 = 1.0 + gm.a[1], f01 = gm.a[2], f10 = gm.a[4], f11 = 1.0 + gm.a[5];
+    return std::atan2(f10 - f01, f00 + f11) * (180.0 / 3.14159265358979323846);
+}
+
+bool canon_affine(const std::vector<int>& centres_l, const std::vector<CanonOut>& co, int l, double cx, double cy,
+                  double a[6]) {
+    const double s = (double)(1 << l), half = 0.5 * (s - 1.0);
+    std::vector<double> xs, ys, us, vs;
+    for (size_t k = 0; k < co.size(); ++k) {
+        const CanonOut& o = co[k];
+        if (o.bx < 0 || !(o.best >= 0.5f) || !(o.best - o.second >= 0.05f)) continue;
+        const double rx = centres_l[2 * k] * s + half, ry = centres_l[2 * k + 1] * s + half;
+        xs.push_back(rx);
+        ys.push_back(ry);
+        us.push_back(o.bx * s + half - rx);
+        vs.push_back(o.by * s + half - ry);
+    }
+    if (xs.size() < 3) return false;
+    return consensus_affine(xs, ys, us, vs, cx, cy, a, 2.0 * s);
 }
 
 int pyramid_levels(int rect_w, int rect_h) {

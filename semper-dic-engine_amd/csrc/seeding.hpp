@@ -42,7 +42,8 @@ bool robust_affine(const std::vector<double>& x, const std::vector<double>& y, c
 // The same model from a consensus of random triples (tolerance 3 px, at least
 // max(6, n / 4) patches), refined by robust_affine on the consensus set.
 bool consensus_affine(const std::vector<double>& x, const std::vector<double>& y, const std::vector<double>& u,
-                      const std::vector<double>& v, double cx, double cy, double a[6]);
+                      const std::vector<double>& v, double cx, double cy, double a[6],
+                      double tol = 3.0);
 
 struct GlobalModel {
     bool locked = false, full = false;
@@ -55,6 +56,19 @@ int pyramid_levels(int rect_w, int rect_h);
 void patch_centres(int rect_x, int rect_y, int rect_w, int rect_h, std::vector<int>& cxy);
 GlobalModel fit_global_model(const std::vector<int>& cxy, const std::vector<PatchOut>& po, int rect_x,
                              int rect_y, int rect_w, int rect_h);
+// The turn of a model's linear part F = I + grad u, in degrees: the angle of
+// its rotation factor, atan2(F10 - F01, F00 + F11).
+double model_turn_deg(const GlobalModel& gm);
+// The largest turn a translation pass's model may carry; beyond it the model
+// is a false lock and the rotation-invariant pass decides.
+constexpr double kTranslationMaxTurnDeg = 30.0;
+// The rough affine of the rotation-invariant pass: the matches of the patch
+// centres (level-l pixels, CanonOut) with a clear best (score >= 0.5, 0.05
+// above any position more than 2 px away), in full-resolution pixels (a
+// level-l pixel i is centred on i 2^l + (2^l - 1) / 2), through the consensus
+// fit with a tolerance of 2^(l+1) px. False when it does not hold.
+bool canon_affine(const std::vector<int>& centres_l, const std::vector<CanonOut>& co, int l, double cx, double cy,
+                  double a[6]);
 
 // Two triangles per lattice cell with four kept corners, one with three, in
 // cell order; cell_tri[c] .. cell_tri[c + 1] are cell c's.

@@ -99,6 +99,14 @@ struct PyrDev {
     int w[4], h[4];
     int levels;
     int coarse_radius;   // shifts searched at the coarsest level
+    // The refinement after the rotation-invariant search: the deformed window
+    // is sampled bilinearly through the linear map f (row-major 2 x 2: a
+    // template offset o lands at shift + f o), and each patch's search starts
+    // at start[2 k], start[2 k + 1] (coarsest-level pixels). warp = 0: f is
+    // the identity and every search starts at zero shift.
+    int warp;
+    double f[4];
+    const int* start;
 };
 // Coarse search radii of the global-motion patch search: the first pass, and
 // the wider pass run when the first one's model does not lock.
@@ -110,6 +118,30 @@ struct PatchOut {
 };
 hipError_t launch_patch_search(const PyrDev& p, const int* centres, int npatch, PatchOut* out,
                                hipStream_t st);
+
+// Rotation-invariant patch matching (the global-motion search for motions the
+// translation search cannot follow): each reference patch and every position
+// of the deformed image at one pyramid level is described by its disc turned
+// to the orientation of its intensity centroid and sampled on a polar grid;
+// a patch's match is the position whose description correlates best.
+constexpr int kCanonR = 16;                          // disc radius (level pixels)
+constexpr int kCanonRings = kCanonR - 2;             // sampled radii 2 .. 15
+constexpr int kCanonAngles = 32;
+constexpr int kCanonSamples = kCanonRings * kCanonAngles;   // 448
+constexpr int kCanonMaxPatches = 25;
+constexpr int kCanonMargin = kCanonR + 2;            // candidate positions keep the disc inside
+struct CanonOut {
+    int bx, by;          // best position (level pixels), -1 if none
+    float best, second;  // its score, and the best one more than 2 px away
+    float theta_ref, theta_def;   // the centroid orientations of the pair (rad)
+    int pad[2];
+};
+// Descriptors of npatch reference patches (centres at level l), then the
+// scores of every deformed position and the best per patch. desc holds
+// npatch * kCanonSamples floats, scores (w_l - 2 m) * (h_l - 2 m) * npatch.
+hipError_t launch_canon_match(const PyrDev& p, int l, const int* centres_l, int npatch, float* desc,
+                              float* theta_ref, float* scores, CanonOut* out, hipStream_t st);
+size_t canon_scores_floats(int wl, int hl, int npatch);
 
 // guess.hip
 struct GuessArgs {
