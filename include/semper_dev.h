@@ -108,7 +108,10 @@ SDK_API int sdk_wave_sum_probe(const float* in, int nw, float* out, int32_t* iou
    deformed frame made once per run (default 1; the same field bit for bit)
    rather than unpacking the u8 frame in every gather; flag 14 = a CNN-seeded
    anchor whose IC-GN fails takes the Simplex rescue before the plain anchor
-   solve (default 1; 0: the plain solve at once). */
+   solve (default 1; 0: the plain solve at once); flag 16 = the global-motion
+   search that seeds the anchor lattice (default 1; 0: no search, the lattice
+   and Path C start from a zero shift, as the reference's do when its feature
+   match finds no model). */
 SDK_API int sdk_engine_set_flag(void* h, int flag, int value);
 
 /* Phase timings of the last run, 14 doubles: total, prep, seeding, prepass,

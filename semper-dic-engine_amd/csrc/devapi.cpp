@@ -299,6 +299,9 @@ SDK_API int sdk_engine_set_flag(void* h, int flag, int value) {
         case 14:
             ((DevEngine*)h)->eng.cnn_rescue = value != 0;
             return 0;
+        case 16:
+            ((DevEngine*)h)->eng.global_search = value != 0;
+            return 0;
 
         default:
             return -3;

@@ -674,6 +674,7 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
         GlobalModel gm;
         int pass = 0;
         for (const int rad : {kCoarseRadius, kCoarseRadiusWide}) {
+            if (!global_search) break;
             pd.coarse_radius = rad;
             ck(launch_patch_search(pd, centres_d_.p, npatch, patch_d_.p, st_));
             ck(hipMemcpyAsync(po.data(), patch_d_.p, sizeof(PatchOut) * npatch, hipMemcpyDeviceToHost, st_));
@@ -691,7 +692,7 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
         // centres over the whole deformed image at the top level, a rough
         // affine through their consensus, then the translation search again
         // through that affine's linear map, started at its predicted shifts.
-        if (!gm.locked && pd.w[L] > 2 * kCanonMargin + 2 && pd.h[L] > 2 * kCanonMargin + 2) {
+        if (global_search && !gm.locked && pd.w[L] > 2 * kCanonMargin + 2 && pd.h[L] > 2 * kCanonMargin + 2) {
             const int g5 = 5;
             std::vector<int> cl;
             for (int j = 0; j < g5; ++j)

@@ -215,6 +215,10 @@ public:
     // run; the values are the u8 ones, so the field is the same bit for bit)
     // instead of unpacking four bytes per row of every Keys gather.
     bool def_f32 = true;
+    // The global-motion search that seeds the anchor lattice (on by
+    // default). Off, the lattice and Path C start from a zero shift, as the
+    // reference's do when its feature match finds no model.
+    bool global_search = true;
     // Mixed subset sizes (no reference counterpart): the subset size of each
     // node of a gw x gh grid (row-major), used by runs on a grid of that shape
     // (any other shape, or an empty map: the run's subset everywhere). Each
