@@ -70,8 +70,11 @@ def fp64_accuracy(ref, dfm, subset, pts, use6, k=2000, seed=0):
             "rms_dv": float(np.sqrt((dv ** 2).mean())) if ok.any() else None}
 
 
-PRECISION_VARIANTS = [("fp64_cholesky_step", (2,)), ("fp16_gradients", (3,)), ("mfma_jtj_pool", (4,)),
-                      ("all_three", (2, 3, 4))]
+# (name, ((flag, value), ...)): flag 2 FP64 Cholesky step, flag 3 gradient
+# planes in fp16 (1) or bf16 (2), flag 4 MFMA J^T J pool
+PRECISION_VARIANTS = [("fp64_cholesky_step", ((2, 1),)), ("fp16_gradients", ((3, 1),)),
+                      ("bf16_gradients", ((3, 2),)), ("mfma_jtj_pool", ((4, 1),)),
+                      ("all_three_fp16", ((2, 1), (3, 1), (4, 1))), ("all_three_bf16", ((2, 1), (3, 2), (4, 1)))]
 
 
 def sequence_study(dev, sa, wl):
@@ -135,8 +138,8 @@ def settle_tuner(dev, eng, run_once, limit=16):
 
 
 def precision_study(dev, dref, ddef, prm, cap, tol, ref, dfm, subset, use6, steps):
-    """The precision variants (FP64 Cholesky Gauss-Newton step, fp16 gradient
-    storage, J^T J on MFMA): per variant the first run's Hessian pre-pass (the
+    """The precision variants (FP64 Cholesky Gauss-Newton step, fp16 or bf16
+    gradient storage, J^T J on MFMA): per variant the first run's Hessian pre-pass (the
     pool build), the step time, the converged throughput and the error of
     every output point's (u, v) against the FP64 IC-GN optimum on the same
     2000-node sample as the plain method."""
@@ -145,8 +148,8 @@ def precision_study(dev, dref, ddef, prm, cap, tol, ref, dfm, subset, use6, step
     res = {}
     for name, flags in PRECISION_VARIANTS:
         eng = dev.sdk_engine_create()
-        for f in flags:
-            assert dev.sdk_engine_set_flag(eng, f, 1) == 0
+        for f, v in flags:
+            assert dev.sdk_engine_set_flag(eng, f, v) == 0
         assert dev.sdk_engine_set_reference(eng, dref, 1, w, ref.shape[0], None) == 0
         out = np.zeros(cap, np.float32)
         met = np.zeros(24, np.float32)

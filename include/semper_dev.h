@@ -79,8 +79,8 @@ SDK_API int sdk_wave_sum_probe(const float* in, int nw, float* out, int32_t* iou
    flag 1 = `out` of sdk_engine_run is device memory (default 0);
    precision variants with no reference counterpart (default 0, a change
    rebuilds the Hessian pool on the next run): flag 2 = the Gauss-Newton step
-   by FP64 Cholesky of each pool entry, flag 3 = fp16 storage of the reference
-   gradient planes, flag 4 = J^T J of the pool on MFMA (fp16 inputs, FP32
+   by FP64 Cholesky of each pool entry, flag 3 = 16-bit storage of the
+   reference gradient planes (1: fp16, 2: bf16), flag 4 = J^T J of the pool on MFMA (fp16 inputs, FP32
    accumulation); flag 5 = learned anchor seeding (needs a seeder loaded by
    sdk_engine_load_seeder; -3 otherwise); flag 6 = the launch-shape tuner
    (default 1): the first runs of each (grid, subset size, interpolator, node

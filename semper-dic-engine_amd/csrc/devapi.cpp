@@ -264,7 +264,9 @@ SDK_API int sdk_engine_set_flag(void* h, int flag, int value) {
             ((DevEngine*)h)->eng.prec.chol64 = value != 0;
             return 0;
         case 3:
+            if (value < 0 || value > 2) return -3;
             ((DevEngine*)h)->eng.prec.grad16 = value != 0;
+            ((DevEngine*)h)->eng.prec.grad_bf16 = value == 2;
             return 0;
         case 4:
             ((DevEngine*)h)->eng.prec.mfma_pool = value != 0;

@@ -147,7 +147,7 @@ Engine::~Engine() {
 void Engine::reset() {
     have_ref_ = false;
     grid_valid_ = false;
-    g16_ready_ = false;
+    g16_fmt_ = -1;
     w_ = h_ = 0;
 }
 
@@ -362,14 +362,21 @@ int Engine::ensure_grid(const Params& p, const uint8_t* mask, const std::vector<
     po.mfma = prec.mfma_pool;
     po.dims = dd;
     if (prec.grad16) {
-        if (!g16_ready_) {
+        const int fmt = prec.grad_bf16 ? 1 : 0;
+        if (g16_fmt_ != fmt) {
             if (!gx16_.reserve(n) || !gy16_.reserve(n)) return kErrInit;
-            ck(launch_to_f16(gx_.p, n, gx16_.p, st_));
-            ck(launch_to_f16(gy_.p, n, gy16_.p, st_));
-            g16_ready_ = true;
+            if (fmt == 1) {
+                ck(launch_to_bf16(gx_.p, n, gx16_.p, st_));
+                ck(launch_to_bf16(gy_.p, n, gy16_.p, st_));
+            } else {
+                ck(launch_to_f16(gx_.p, n, gx16_.p, st_));
+                ck(launch_to_f16(gy_.p, n, gy16_.p, st_));
+            }
+            g16_fmt_ = fmt;
         }
         po.gx16 = gx16_.p;
         po.gy16 = gy16_.p;
+        po.bf16 = fmt == 1;
     }
     if (prec.chol64) {
         if (!chol_.reserve((size_t)N * kCholStride)) return kErrInit;
@@ -398,6 +405,7 @@ void Engine::set_precision_args(SolveArgs& a) const {
     if (grid_prec_.grad16) {
         a.refGx16 = gx16_.p;
         a.refGy16 = gy16_.p;
+        a.grad_bf16 = grid_prec_.grad_bf16 ? 1 : 0;
     }
     if (grid_prec_.chol64) a.chol = chol_.p;
 }

@@ -191,9 +191,10 @@ public:
     struct Precision {
         bool chol64 = false;
         bool grad16 = false;
+        bool grad_bf16 = false;   // with grad16: bf16 storage instead of fp16
         bool mfma_pool = false;
         bool operator==(const Precision& o) const {
-            return chol64 == o.chol64 && grad16 == o.grad16 && mfma_pool == o.mfma_pool;
+            return chol64 == o.chol64 && grad16 == o.grad16 && grad_bf16 == o.grad_bf16 && mfma_pool == o.mfma_pool;
         }
     };
     Precision prec;
@@ -349,7 +350,7 @@ private:
     int shape_for(const std::vector<int>& key, bool listed);
     void shape_timed(const std::vector<int>& key, bool listed, int cand, float ms);
     DevBuf<uint16_t> gx16_, gy16_;  // fp16 gradient planes (Precision::grad16)
-    bool g16_ready_ = false;
+    int g16_fmt_ = -1;   // format of the 16-bit gradient planes made (-1 none, 0 fp16, 1 bf16)
     DevBuf<double> chol_;           // FP64 Cholesky factors (Precision::chol64)
     // the precision-variant pointers of a solver launch on the grid pool
     void set_precision_args(SolveArgs& a) const;

@@ -80,12 +80,9 @@ struct GradRead {
     const float* gy;
     const uint16_t* gx16;
     const uint16_t* gy16;
-    __device__ __forceinline__ float x(int a) const {
-        return gx16 ? (float)__builtin_bit_cast(_Float16, gx16[a]) : gx[a];
-    }
-    __device__ __forceinline__ float y(int a) const {
-        return gy16 ? (float)__builtin_bit_cast(_Float16, gy16[a]) : gy[a];
-    }
+    bool bf16;   // the 16-bit copies are bf16 (else fp16)
+    __device__ __forceinline__ float x(int a) const { return gx16 ? half_bits_to_float(gx16[a], bf16) : gx[a]; }
+    __device__ __forceinline__ float y(int a) const { return gy16 ? half_bits_to_float(gy16[a], bf16) : gy[a]; }
 };
 
 typedef _Float16 sdic_half4 __attribute__((ext_vector_type(4)));
@@ -765,7 +762,7 @@ hipError_t launch_pool(const float* I, const float* gx, const float* gy, int w, 
                        void* raw, hipStream_t st, const PoolOpts& o) {
     const int n = gw * gh;
     if (n <= 0) return hipSuccess;
-    const GradRead G{gx, gy, o.gx16, o.gy16};
+    const GradRead G{gx, gy, o.gx16, o.gy16, o.bf16};
     PoolRaw* r = static_cast<PoolRaw*>(raw);
     if (o.mfma)
         hipLaunchKernelGGL(k_pool_sums<true>, dim3(n), dim3(64), 0, st, I, G, w, h, skip, gw, gh, rect_x, rect_y,
@@ -787,6 +784,21 @@ size_t pool_raw_bytes(int n) { return (size_t)n * sizeof(PoolRaw); }
 __global__ void k_to_f16(const float* __restrict__ src, size_t n, uint16_t* __restrict__ dst) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = __builtin_bit_cast(uint16_t, (_Float16)src[i]);
+}
+
+// bf16 copy of a float plane (round to nearest even on the upper 16 bits)
+__global__ void k_to_bf16(const float* __restrict__ src, size_t n, uint16_t* __restrict__ dst) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t u = __float_as_uint(src[i]);
+    const uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);
+    dst[i] = (uint16_t)(r >> 16);
+}
+
+hipError_t launch_to_bf16(const float* src, size_t n, uint16_t* dst, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, n, dst);
+    return hipGetLastError();
 }
 
 hipError_t launch_to_f16(const float* src, size_t n, uint16_t* dst, hipStream_t st) {
@@ -818,7 +830,7 @@ hipError_t launch_u8_to_f32(const uint8_t* src, size_t n, float* dst, hipStream_
 hipError_t launch_pool_list(const float* I, const float* gx, const float* gy, int w, int h, const int* cxy,
                             int n, int dim, bool lm, PoolEntry* pool, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    const GradRead G{gx, gy, nullptr, nullptr};
+    const GradRead G{gx, gy, nullptr, nullptr, false};
     hipLaunchKernelGGL(k_pool_list, dim3(n), dim3(64), 0, st, I, G, w, h, cxy, n, dim, lm ? 1 : 0, pool);
     return hipGetLastError();
 }

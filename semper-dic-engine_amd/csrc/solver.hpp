@@ -28,6 +28,7 @@ struct PoolOpts {
     bool mfma = false;
     const uint16_t* gx16 = nullptr;
     const uint16_t* gy16 = nullptr;
+    bool bf16 = false;   // the 16-bit gradient copies are bf16 (else fp16)
     double* chol = nullptr;
     const int* dims = nullptr;   // per-node subset sizes (mixed subsets; else `dim`)
 };
@@ -47,8 +48,9 @@ struct SolveArgs {
     const float* refI;
     const float* refGx;
     const float* refGy;
-    const uint16_t* refGx16;  // fp16 gradient planes (read instead of refGx / refGy when set)
+    const uint16_t* refGx16;  // 16-bit gradient planes (read instead of refGx / refGy when set)
     const uint16_t* refGy16;
+    int grad_bf16;            // those planes hold bf16 (else fp16)
     const double* chol;       // FP64 Cholesky factors per pool entry (the step when set)
     const void* def;          // uint8 or float plane
     int w, h, dim, lm;
@@ -83,6 +85,8 @@ hipError_t launch_pool_list(const float* I, const float* gx, const float* gy, in
 hipError_t launch_downsample_u8(const uint8_t* src, int w, int h, float* dst, hipStream_t st);
 // fp16 copy of a float plane (round to nearest), as half bits
 hipError_t launch_to_f16(const float* src, size_t n, uint16_t* dst, hipStream_t st);
+// bf16 copy of a float plane (round to nearest even)
+hipError_t launch_to_bf16(const float* src, size_t n, uint16_t* dst, hipStream_t st);
 // u8 -> FP32 copy of a plane (src 4-byte aligned)
 hipError_t launch_u8_to_f32(const uint8_t* src, size_t n, float* dst, hipStream_t st);
 hipError_t launch_downsample_f32(const float* src, int w, int h, float* dst, hipStream_t st);

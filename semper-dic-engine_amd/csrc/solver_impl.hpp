@@ -323,7 +323,7 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
                                           const float* __restrict__ refI,
                                           const float* __restrict__ refGx,
                                           const float* __restrict__ refGy,
-                                          const uint16_t* __restrict__ gx16, const uint16_t* __restrict__ gy16,
+                                          const uint16_t* __restrict__ gx16, const uint16_t* __restrict__ gy16, bool bf16,
                                           const double* __restrict__ chol, const TD* __restrict__ def,
                                           int w, int h, int dim, bool lm, float conv_tol,
                                           WaveLds& L, float* red, SolveOut& out) {
@@ -362,8 +362,8 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
                 const int a = (t.cy + q - half) * w + t.cx + r - half;
                 const float ri = refI[a];
                 nref[j] = (ri - mean_r) * inv_r;
-                gxs[j] = (gx16 ? (float)__builtin_bit_cast(_Float16, gx16[a]) : refGx[a]) * inv_r;
-                gys[j] = (gy16 ? (float)__builtin_bit_cast(_Float16, gy16[a]) : refGy[a]) * inv_r;
+                gxs[j] = (gx16 ? half_bits_to_float(gx16[a], bf16) : refGx[a]) * inv_r;
+                gys[j] = (gy16 ? half_bits_to_float(gy16[a], bf16) : refGy[a]) * inv_r;
                 inmask |= (1ull << j);
                 if (!(ri < kGhostWallThreshold)) refok |= (1ull << j);
                 else bad++;
@@ -963,7 +963,7 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
                                            const float* __restrict__ refI,
                                            const float* __restrict__ refGx,
                                            const float* __restrict__ refGy,
-                                           const uint16_t* __restrict__ gx16, const uint16_t* __restrict__ gy16,
+                                           const uint16_t* __restrict__ gx16, const uint16_t* __restrict__ gy16, bool bf16,
                                            const double* __restrict__ chol,
                                            const TD* __restrict__ def, int w, int h, int dim,
                                            bool lm, float conv_tol, SolveOut& out) {
@@ -994,8 +994,8 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
                 const int a = (t.cy + q - half) * w + t.cx + r - half;
                 const float ri = refI[a];
                 nref[j] = (ri - mean_r) * inv_r;
-                gxs[j] = (gx16 ? (float)__builtin_bit_cast(_Float16, gx16[a]) : refGx[a]) * inv_r;
-                gys[j] = (gy16 ? (float)__builtin_bit_cast(_Float16, gy16[a]) : refGy[a]) * inv_r;
+                gxs[j] = (gx16 ? half_bits_to_float(gx16[a], bf16) : refGx[a]) * inv_r;
+                gys[j] = (gy16 ? half_bits_to_float(gy16[a], bf16) : refGy[a]) * inv_r;
                 bad += (ri < kGhostWallThreshold) ? 1 : 0;
             }
             r += r64;
@@ -1294,7 +1294,7 @@ __global__ __launch_bounds__(256) void k_fast(SolveArgs a) {
     Task t;
     if (!item_task(a, id, t)) return;
     SolveOut o = {};
-    const bool fin = fast_solve<PPL, K6, TD>(t, a.pool, a.refI, a.refGx, a.refGy, a.refGx16, a.refGy16, a.chol,
+    const bool fin = fast_solve<PPL, K6, TD>(t, a.pool, a.refI, a.refGx, a.refGy, a.refGx16, a.refGy16, a.grad_bf16 != 0, a.chol,
                                              (const TD*)a.def, a.w, a.h, a.dim, a.lm != 0, a.tol, o);
     if ((threadIdx.x & 63) != 0) return;
     if (!fin) {
@@ -1320,7 +1320,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2))) vo
         Task t;
         if (!item_task(a, id, t)) continue;
         SolveOut o = {};
-        full_solve<PPL, K6, TD, NW>(t, a.pool, a.refI, a.refGx, a.refGy, a.refGx16, a.refGy16, a.chol,
+        full_solve<PPL, K6, TD, NW>(t, a.pool, a.refI, a.refGx, a.refGy, a.refGx16, a.refGy16, a.grad_bf16 != 0, a.chol,
                                     (const TD*)a.def, a.w, a.h, a.dim, a.lm != 0, a.tol, L, red, o);
         if (threadIdx.x == 0) commit(a, id, o);
         __syncthreads();

@@ -53,6 +53,12 @@ __device__ __forceinline__ double wave_sum_d(double v) {
     return v;
 }
 
+// A 16-bit stored value (a gradient plane's copy) back to float: fp16 bits,
+// or bf16 bits (the upper half of an fp32).
+__device__ __forceinline__ float half_bits_to_float(uint16_t v, bool bf16) {
+    return bf16 ? __uint_as_float((uint32_t)v << 16) : (float)__builtin_bit_cast(_Float16, v);
+}
+
 __device__ __forceinline__ bool finite_f(float f) {
     return (__float_as_uint(f) & 0x7f800000u) != 0x7f800000u;
 }

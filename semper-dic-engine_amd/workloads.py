@@ -5,8 +5,10 @@ px^2 over the image padded by 16 px, signs +-1, mapped linearly with +-3
 standard deviations of the blob field onto [0, 200] (peak 200), 2% (4 grey
 levels) Gaussian noise, rounded to u8. The deformed image samples the continuous field at the analytic inverse
 of the displacement field (fixed-point iteration in FP64), never by
-resampling the reference, so the ground truth is exact. The pair is rendered
-on the GPU by sdk_synth_pair (csrc/synth.hip).
+resampling the reference, so the ground truth is exact -- except the 512^2
+translation, whose deformed frame is the Keys bicubic resampling of the u8
+reference its configuration names. The pairs are rendered on the GPU by
+sdk_synth_pair (csrc/synth.hip).
 """
 import ctypes
 import math
@@ -38,9 +40,10 @@ def bumps_params(seed, w, h, n=8):
 
 
 CONFIGS = {
-    # 512^2, u = 0.4 px rigid translation, step 10, subset 21, window 21
+    # 512^2, u = 0.4 px rigid translation applied by Keys bicubic resampling of
+    # the u8 reference, step 10, subset 21, window 21
     "c1_512_translate": dict(w=512, h=512, seed=1001, kind=AFFINE, p=[0.4, 0.0, 0, 0, 0, 0], step=10, subset=21,
-                             window=21, tol=1e-3),
+                             window=21, tol=1e-3, resample=True),
     # 2048^2 homogeneous affine: rot 2 deg, exx 1%, eyy -0.3%, exy 0.5%, t (12.3, -7.8)
     "c2_2048_affine": dict(w=2048, h=2048, seed=2002, kind=AFFINE, p=affine_params(2.0, 0.01, -0.003, 0.005, 12.3, -7.8),
                            step=5, subset=31, window=31, tol=1e-4),
@@ -63,16 +66,53 @@ CONFIGS = {
 
 
 def config(name):
-    c = dict(CONFIGS[name])
+    analytic = name.endswith("@analytic")
+    c = dict(CONFIGS[name[: -len("@analytic")] if analytic else name])
+    if analytic:
+        c.pop("resample", None)
     if c["p"] is None:
         c["p"] = bumps_params(c["seed"], c["w"], c["h"])
     return c
+
+
+def keys_weights(s):
+    """Keys (a = -0.5) cubic weights of the four taps at -1, 0, 1, 2 for a
+    fraction s in [0, 1)."""
+    s2, s3 = s * s, s * s * s
+    return np.array([-0.5 * s3 + s2 - 0.5 * s, 1.5 * s3 - 2.5 * s2 + 1.0, -1.5 * s3 + 2.0 * s2 + 0.5 * s,
+                     0.5 * s3 - 0.5 * s2])
+
+
+def resample_shift_x(ref, u):
+    """The deformed frame of a rigid shift by u px along x made by Keys
+    bicubic resampling of the u8 reference: D(x, y) = R(x - u, y) with the
+    four taps of each row (edge pixels repeated), computed in double, rounded
+    and clamped to u8. A pixel of the deformed frame at x then shows the
+    reference point x - u, which is the displacement +u."""
+    h, w = ref.shape
+    r = ref.astype(np.float64)
+    x = np.arange(w) - u
+    x0 = np.floor(x).astype(int)
+    wt = keys_weights(x - x0)
+    out = np.zeros((h, w))
+    for k in range(4):
+        out += r[:, np.clip(x0 - 1 + k, 0, w - 1)] * wt[k][None, :]
+    return np.clip(np.rint(out), 0, 255).astype(np.uint8)
 
 
 def pair(dev, name, on_device=False):
     """(ref, def) of a workload: host uint8 arrays, or device pointers
     (allocated with sdk_malloc, to be freed by the caller) when on_device."""
     c = config(name)
+    if c.get("resample"):
+        ra, _ = pair(dev, name + "@analytic")
+        da = resample_shift_x(ra, c["p"][0])
+        if not on_device:
+            return ra, da
+        r, d = dev.sdk_malloc(ra.size), dev.sdk_malloc(da.size)
+        dev.sdk_memcpy(r, ra.ctypes.data_as(ctypes.c_void_p), ra.size, 0)
+        dev.sdk_memcpy(d, da.ctypes.data_as(ctypes.c_void_p), da.size, 0)
+        return r, d
     w, h = c["w"], c["h"]
     prm = np.array(c["p"], np.float64)
     if on_device:

@@ -315,6 +315,43 @@ def oracle_vsg(planes, step, window):
     return ok.astype(bool), exx, eyy, exy
 
 
+def fp64_window_errors(ref, dfm, prm, pts, planes, k=1500, seed=5, use6=False):
+    """Each output point's error against the FP64 IC-GN optimum, on a seeded
+    sample of k points: u and v against the point's own subset polished in
+    FP64 (to ||dp|| < 1e-10 from its result), and the strains against the
+    VSG strain (the oracle's median test and VSG) of the field whose every
+    node in the point's strain window is polished. pts: (n, 8) output points;
+    planes: (7, gh, gw) node planes of the same solve. Returns a dict of
+    arrays (du, dv, dexx, deyy, dexy) over the sample."""
+    rng = np.random.default_rng(seed)
+    sel = np.sort(rng.choice(len(pts), size=min(k, len(pts)), replace=False))
+    step, win = int(prm[4]), int(prm[6])
+    gh, gw = planes.shape[1:]
+    ix = ((pts[sel, 0] - prm[0]) / step).astype(int)
+    iy = ((pts[sel, 1] - prm[1]) / step).astype(int)
+    r = win / 2.0 / step
+    rr = int(np.ceil(r))
+    need = set()
+    for x0, y0 in zip(ix, iy):
+        for dy in range(-rr, rr + 1):
+            for dx in range(-rr, rr + 1):
+                x, y = x0 + dx, y0 + dy
+                if 0 <= x < gw and 0 <= y < gh and planes[6, y, x] >= 0 and dx * dx + dy * dy <= r * r + 1e-9:
+                    need.add((x, y))
+    need = sorted(need)
+    nx = np.array([x for x, _ in need])
+    ny = np.array([y for _, y in need])
+    si, p64, _ = fp64_polish(ref, dfm, int(prm[5]), [(prm[0] + x * step, prm[1] + y * step) for x, y in need],
+                             planes[:6, ny, nx].T.astype(np.float64), use6=use6)
+    ok = si[:, 0] == 0
+    pf = planes.copy()
+    pf[:6, ny[ok], nx[ok]] = p64[ok].T.astype(np.float32)
+    _, ex, ey, exy = oracle_vsg(pf, step, win)
+    return dict(du=np.abs(pts[sel, 2] - pf[0, iy, ix]), dv=np.abs(pts[sel, 3] - pf[1, iy, ix]),
+                dexx=np.abs(pts[sel, 4] - ex[iy, ix]), deyy=np.abs(pts[sel, 5] - ey[iy, ix]),
+                dexy=np.abs(pts[sel, 6] - exy[iy, ix]), polished=int(ok.sum()), needed=len(need))
+
+
 def oracle_solve_nodes(ref, dfm, dim, nodes, guesses, modes, use6=False, lm=False, tol=1e-3):
     h, w = ref.shape
     n = len(nodes)

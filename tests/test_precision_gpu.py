@@ -1,11 +1,11 @@
 """The precision variants of the engine (no reference counterpart): the
-Gauss-Newton step by FP64 Cholesky of each pool entry, fp16 storage of the
-reference gradient planes, and the Hessian pool's J^T J on MFMA (fp16 inputs,
-FP32 accumulation). Each changes the path IC-GN takes, not its fixed point
-(the zero of the ZNSSD gradient), so against the plain FP32 method on the same
-frames it must give the same point set and the same field within the
-reference's cross-toolchain bound, and against the FP64 IC-GN optimum it must
-be no less accurate."""
+Gauss-Newton step by FP64 Cholesky of each pool entry, fp16 or bf16 storage of
+the reference gradient planes, and the Hessian pool's J^T J on MFMA (fp16
+inputs, FP32 accumulation). Each changes the path IC-GN takes, not its fixed
+point (the zero of the ZNSSD gradient), so it must give the plain method's
+point set and be no less accurate than it against the FP64 IC-GN optimum,
+output by output: u, v against each subset's FP64 optimum, and the strains
+against the VSG strain of the FP64-polished field."""
 import ctypes
 import importlib.util
 import os
@@ -13,9 +13,12 @@ import os
 import numpy as np
 import pytest
 
-from helpers import PKG, P, fp64_polish
+from helpers import PKG, P, OracleEngine, fp64_window_errors
 
 pytestmark = pytest.mark.gpu
+
+VARIANTS = {"plain": (), "fp64_cholesky": ((2, 1),), "fp16_gradients": ((3, 1),), "bf16_gradients": ((3, 2),),
+            "mfma_pool": ((4, 1),), "all_fp16": ((2, 1), (3, 1), (4, 1)), "all_bf16": ((2, 1), (3, 2), (4, 1))}
 
 
 @pytest.fixture(scope="module")
@@ -29,66 +32,83 @@ def dev():
     return lib
 
 
-@pytest.fixture(scope="module")
-def case(dev):
+def workloads():
     spec = importlib.util.spec_from_file_location("workloads", os.path.join(PKG, "workloads.py"))
     wl = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(wl)
-    name = "t_1024_noisy"
-    ref, dfm = wl.pair(dev, name)
-    return wl.config(name), ref, dfm, np.array(wl.solve_params(name), np.int32)
+    return wl
 
 
 def run(dev, cfg, ref, dfm, prm, flags):
     h = dev.sdk_engine_create()
-    for f in flags:
-        assert dev.sdk_engine_set_flag(h, f, 1) == 0
+    for f, v in flags:
+        assert dev.sdk_engine_set_flag(h, f, v) == 0
+    assert dev.sdk_engine_set_flag(h, 12, 1) == 0
     assert dev.sdk_engine_set_reference(h, P(ref), 0, cfg["w"], cfg["h"], None) == 0
-    cap = (cfg["w"] // cfg["step"] + 1) * (cfg["h"] // cfg["step"] + 1) * 8
+    gw, gh = cfg["w"] // cfg["step"], cfg["h"] // cfg["step"]
+    cap = (gw + 1) * (gh + 1) * 8
     out = np.zeros(cap, np.float32)
     met = np.zeros(24, np.float32)
-    n = dev.sdk_engine_run(h, P(dfm), 0, P(prm), P(out), cap, P(met), 24, ctypes.c_float(1e-3))
+    n = dev.sdk_engine_run(h, P(dfm), 0, P(prm), P(out), cap, P(met), 24, ctypes.c_float(cfg["tol"]))
     # a second run on the cached pool gives the same field
     out2 = np.zeros(cap, np.float32)
-    n2 = dev.sdk_engine_run(h, P(dfm), 0, P(prm), P(out2), cap, P(met), 24, ctypes.c_float(1e-3))
+    n2 = dev.sdk_engine_run(h, P(dfm), 0, P(prm), P(out2), cap, P(met), 24, ctypes.c_float(cfg["tol"]))
+    planes = np.zeros(7 * gw * gh, np.float32)
+    assert dev.sdk_engine_node_planes(h, 0, P(planes), planes.size) == planes.size
     dev.sdk_engine_destroy(h)
     assert n == n2 > 0 and np.array_equal(out[: n * 8], out2[: n * 8])
-    return n, out[: n * 8].reshape(-1, 8), met
+    return n, out[: n * 8].reshape(-1, 8), met, planes.reshape(7, gh, gw)
 
 
-def fp64_error(ref, dfm, pts, subset, k=300):
-    rng = np.random.default_rng(1)
-    sel = pts[rng.choice(len(pts), size=min(k, len(pts)), replace=False)]
-    p0 = np.zeros((len(sel), 6))
-    p0[:, 0], p0[:, 1] = sel[:, 2], sel[:, 3]
-    si, po, _ = fp64_polish(ref.astype(np.float32), dfm.astype(np.float32), subset,
-                            [(int(x), int(y)) for x, y in sel[:, :2]], p0)
-    ok = si[:, 0] == 0
-    return np.maximum(np.abs(po[ok, 0] - sel[ok, 2]), np.abs(po[ok, 1] - sel[ok, 3]))
-
-
-@pytest.mark.parametrize("flags", [(2,), (3,), (4,), (2, 3, 4)],
-                         ids=["fp64_cholesky", "fp16_gradients", "mfma_pool", "all_three"])
-def test_variant_against_plain_and_fp64(dev, case, flags):
-    cfg, ref, dfm, prm = case
-    n0, p0, m0 = run(dev, cfg, ref, dfm, prm, ())
-    n1, p1, m1 = run(dev, cfg, ref, dfm, prm, flags)
-    print(f"flags {flags}: points {n1} (plain {n0}), iterations per point {m1[8]:.3f} (plain {m0[8]:.3f})",
-          flush=True)
-    assert abs(n1 - n0) <= max(2, n0 // 200) and m1[0] == m0[0]
-    a = {(x, y): r for x, y, *r in p0.tolist()}
-    b = {(x, y): r for x, y, *r in p1.tolist()}
-    common = sorted(set(a) & set(b))
-    assert len(common) >= n0 - max(2, n0 // 200)
-    d = np.array([np.abs(np.array(a[k]) - np.array(b[k])) for k in common])
-    print(f"  max |duv| {d[:, :2].max():.2e}, 99th pct {np.percentile(d[:, :2].max(1), 99):.2e}, "
-          f"max |dstrain| {d[:, 2:5].max():.2e}", flush=True)
-    assert d[:, :2].max() <= 5e-3 and (d[:, :2].max(1) <= 1e-3).mean() >= 0.99
-    assert d[:, 2:5].max() <= 1e-3
-    # no less accurate against the FP64 optimum (the sample's 1e-3 px fraction
-    # within one point of the plain method's)
-    e0, e1 = fp64_error(ref, dfm, p0, int(prm[5])), fp64_error(ref, dfm, p1, int(prm[5]))
-    print(f"  vs FP64: within 1e-3 px {np.mean(e1 <= 1e-3):.4f} (plain {np.mean(e0 <= 1e-3):.4f}), "
-          f"max {e1.max():.2e} (plain {e0.max():.2e})", flush=True)
-    assert np.mean(e1 <= 1e-3) >= np.mean(e0 <= 1e-3) - 1.0 / len(e0) - 1e-9
-    assert e1.max() <= max(2e-3, 1.5 * e0.max())
+@pytest.mark.parametrize("name", ["t_1024_noisy", "c4_4096_heterogeneous"])
+def test_variants_against_plain_oracle_and_fp64(dev, name):
+    """At the noisy 1024^2 frame and at the 4096^2 precision-study frame
+    (heterogeneous field u = 3 sin(2 pi x / 1024) cos(2 pi y / 1536),
+    v = 2 exp(-r^2 / 2 600^2), step 8, subset 31). Per variant, against the
+    FP64 optimum and output by output: the fraction of points within 1e-3 px
+    no more than 0.5% below the plain method's, the largest u, v error at most
+    1.5x the plain method's (or 2e-3 px) and the largest strain error at most
+    1.5x the plain method's (or 2e-5). Against the oracle: its point set up
+    to points at the 0.15 acceptance bar, and every output within the plain
+    method's own distance from the FP64 optimum (u, v at least 2e-3 px,
+    strains at least 5e-5): a variant may stop IC-GN at another iterate of the
+    same ||dp|| < tol ball, never outside it."""
+    wl = workloads()
+    cfg = wl.config(name)
+    ref, dfm = wl.pair(dev, name)
+    prm = np.array(wl.solve_params(name), np.int32)
+    oe = OracleEngine()
+    assert oe.set_reference(ref) == 0
+    n_o, p_o, m_o = oe.run(dfm, prm, tol=cfg["tol"])
+    rows = {}
+    o = {(x, y): r for x, y, *r in p_o.tolist()}
+    for vname, flags in VARIANTS.items():
+        n, pts, met, planes = run(dev, cfg, ref, dfm, prm, flags)
+        e = fp64_window_errors(ref, dfm, prm, pts, planes)
+        g = {(x, y): r for x, y, *r in pts.tolist()}
+        common = sorted(set(g) & set(o))
+        d_o = np.array([np.abs(np.array(g[k]) - np.array(o[k])) for k in common])
+        rows[vname] = (n, pts, met, e, d_o, len(set(g) ^ set(o)))
+        euv = np.maximum(e["du"], e["dv"])
+        es = np.maximum(np.maximum(e["dexx"], e["deyy"]), e["dexy"])
+        print(f"{name} {vname:15s}: {n} points ({rows[vname][5]} not shared with the oracle's {n_o}), "
+              f"{met[8]:.3f} iterations; vs FP64 u,v within 1e-3 {np.mean(euv <= 1e-3):.4f} max {euv.max():.2e} "
+              f"rms {np.sqrt(np.mean(euv ** 2)):.2e}; strain max {es.max():.2e} rms {np.sqrt(np.mean(es ** 2)):.2e}; "
+              f"vs oracle u,v {d_o[:, :2].max():.2e} strain {d_o[:, 2:5].max():.2e}", flush=True)
+    n0, p0, m0, e0, _, _ = rows["plain"]
+    euv0 = np.maximum(e0["du"], e0["dv"])
+    es0 = np.maximum(np.maximum(e0["dexx"], e0["deyy"]), e0["dexy"])
+    for vname, (n, pts, met, e, d_o, nd) in rows.items():
+        # a point whose correlation sits at the 0.15 acceptance bar may land
+        # either side of it
+        assert abs(n - n_o) <= max(2, n_o // 2000) and nd <= max(2, n_o // 1000) and met[0] == m0[0], vname
+        # a variant stops IC-GN at another iterate inside the stop rule's
+        # ball around the optimum, so it sits within the plain method's own
+        # distance from the FP64 optimum of the oracle's field
+        assert d_o[:, :2].max() <= max(2e-3, 1.5 * euv0.max()), vname
+        assert d_o[:, 2:5].max() <= max(5e-5, 1.5 * es0.max()), vname
+        euv = np.maximum(e["du"], e["dv"])
+        es = np.maximum(np.maximum(e["dexx"], e["deyy"]), e["dexy"])
+        assert np.mean(euv <= 1e-3) >= np.mean(euv0 <= 1e-3) - 0.005, vname
+        assert euv.max() <= max(2e-3, 1.5 * euv0.max()), vname
+        assert es.max() <= max(2e-5, 1.5 * es0.max()), vname
