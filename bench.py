@@ -71,10 +71,11 @@ def fp64_accuracy(ref, dfm, subset, pts, use6, k=2000, seed=0):
 
 
 # (name, ((flag, value), ...)): flag 2 FP64 Cholesky step, flag 3 gradient
-# planes in fp16 (1) or bf16 (2), flag 4 MFMA J^T J pool
+# planes in fp16 (1) or bf16 (2), flag 4 MFMA J^T J pool, flag 17 Keys 4x4
+# from the fp16 row-cubic coefficient plane
 PRECISION_VARIANTS = [("fp64_cholesky_step", ((2, 1),)), ("fp16_gradients", ((3, 1),)),
-                      ("bf16_gradients", ((3, 2),)), ("mfma_jtj_pool", ((4, 1),)),
-                      ("all_three_fp16", ((2, 1), (3, 1), (4, 1))), ("all_three_bf16", ((2, 1), (3, 2), (4, 1)))]
+                      ("bf16_gradients", ((3, 2),)), ("mfma_jtj_pool", ((4, 1),)), ("fp16_coefficients", ((17, 1),)),
+                      ("all_fp16", ((2, 1), (3, 1), (4, 1), (17, 1))), ("all_bf16", ((2, 1), (3, 2), (4, 1), (17, 1)))]
 
 
 def sequence_study(dev, sa, wl):
@@ -139,7 +140,7 @@ def settle_tuner(dev, eng, run_once, limit=16):
 
 def precision_study(dev, dref, ddef, prm, cap, tol, ref, dfm, subset, use6, steps):
     """The precision variants (FP64 Cholesky Gauss-Newton step, fp16 or bf16
-    gradient storage, J^T J on MFMA): per variant the first run's Hessian pre-pass (the
+    gradient storage, J^T J on MFMA, fp16 coefficient plane): per variant the first run's Hessian pre-pass (the
     pool build), the step time, the converged throughput and the error of
     every output point's (u, v) against the FP64 IC-GN optimum on the same
     2000-node sample as the plain method."""
@@ -440,6 +441,8 @@ def main():
     ap.add_argument("--no-sequence", action="store_true", help="skip the 64-frame incremental sequence")
     ap.add_argument("--u8-deformed", action="store_true",
                     help="the solver samples the u8 deformed frame instead of its FP32 copy (same field)")
+    ap.add_argument("--coef-plane", action="store_true",
+                    help="Keys 4x4 from the fp16 row-cubic coefficient plane of each frame")
     ap.add_argument("--tol", type=float, default=None, help="IC-GN stop rule ||dp|| < tol (default: the workload's)")
     ap.add_argument("--learned-seed", action="store_true",
                     help="the main line's engine seeds its anchors with the learned seeder (the committed weights)")
@@ -473,6 +476,8 @@ def main():
         assert dev.sdk_engine_set_flag(eng, 5, 1) == 0
     if args.u8_deformed:
         assert dev.sdk_engine_set_flag(eng, 13, 0) == 0
+    if args.coef_plane:
+        assert dev.sdk_engine_set_flag(eng, 17, 1) == 0
     assert dev.sdk_engine_set_reference(eng, dref, 1, w, h, None) == 0
     gw, gh = w // cfg["step"], h // cfg["step"]
     cap = gw * gh * 8

@@ -75,6 +75,15 @@ SDK_API int sdk_engine_run(void* h, const uint8_t* def, int on_device, const int
    network). */
 SDK_API int sdk_wave_sum_probe(const float* in, int nw, float* out, int32_t* iout);
 
+/* The subset solver's Keys 4x4 sampling forms (no reference counterpart;
+   the interpolant is interpolate_bicubic, image_processor.cpp:188-242) at n
+   points of a u8 frame (host memory): form 0 the tap weights over the FP32
+   plane, 1 the row-cubic coefficient plane of flag 17, 2 the tap weights over
+   the u8 plane. Every point must lie inside the solver's guard, 4 <= x < w - 5
+   and 4 <= y < h - 5, and w h < 2^28; 0 or -3. */
+SDK_API int sdk_interp_probe(const uint8_t* img, int w, int h, int form, int n, const float* xs, const float* ys,
+                             float* out);
+
 /* Engine switches: flag 0 = speculative Path B first claims (default 1);
    flag 1 = `out` of sdk_engine_run is device memory (default 0);
    precision variants with no reference counterpart (default 0, a change
@@ -111,7 +120,13 @@ SDK_API int sdk_wave_sum_probe(const float* in, int nw, float* out, int32_t* iou
    solve (default 1; 0: the plain solve at once); flag 16 = the global-motion
    search that seeds the anchor lattice (default 1; 0: no search, the lattice
    and Path C start from a zero shift, as the reference's do when its feature
-   match finds no model). */
+   match finds no model); flag 17 = Keys 4x4 from a row-cubic coefficient
+   plane (default 0): each run makes, from the u8 frame, four fp16 per pixel
+   (the Keys cubic of the cell to the pixel's right on its row, exact for u8
+   taps), and the fast path's interior samples read one 8-byte entry per
+   stencil row and evaluate it by Horner, instead of weighting four taps (the
+   same interpolant, other roundings; Keys 6x6, flag 13 off and frames of
+   2^28 pixels or more keep the direct form). */
 SDK_API int sdk_engine_set_flag(void* h, int flag, int value);
 
 /* Phase timings of the last run, 14 doubles: total, prep, seeding, prepass,

@@ -251,6 +251,29 @@ SDK_API int sdk_wave_sum_probe(const float* in, int nw, float* out, int32_t* iou
     return 0;
 }
 
+SDK_API int sdk_interp_probe(const uint8_t* img, int w, int h, int form, int n, const float* xs, const float* ys,
+                             float* out) {
+    if (!img || !xs || !ys || !out || w < 16 || h < 16 || n <= 0 || form < 0 || form > 2 ||
+        (size_t)w * h >= ((size_t)1 << 28))
+        return -3;
+    for (int i = 0; i < n; ++i)   // the solver's guard: the stencil lies inside the plane
+        if (!(xs[i] >= 4.0f && xs[i] < (float)(w - 5) && ys[i] >= 4.0f && ys[i] < (float)(h - 5))) return -3;
+    const size_t npx = (size_t)w * h;
+    Dev<uint8_t> d8(npx);
+    Dev<float> df(npx), dx(n), dy(n), dout(n);
+    Dev<uint2> dc(npx);
+    if (!d8.p || !df.p || !dx.p || !dy.p || !dout.p || !dc.p) return -3;
+    if (hipMemcpy(d8.p, img, npx, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dx.p, xs, sizeof(float) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dy.p, ys, sizeof(float) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        launch_u8_to_f32(d8.p, npx, df.p, nullptr) != hipSuccess ||
+        launch_row_cubic(d8.p, w, h, dc.p, nullptr) != hipSuccess ||
+        launch_interp_probe(df.p, dc.p, d8.p, w, h, form, n, dx.p, dy.p, dout.p, nullptr) != hipSuccess ||
+        hipMemcpy(out, dout.p, sizeof(float) * n, hipMemcpyDeviceToHost) != hipSuccess)
+        return -3;
+    return 0;
+}
+
 SDK_API int sdk_engine_set_flag(void* h, int flag, int value) {
     if (!h) return -3;
     switch (flag) {
@@ -303,6 +326,9 @@ SDK_API int sdk_engine_set_flag(void* h, int flag, int value) {
             return 0;
         case 16:
             ((DevEngine*)h)->eng.global_search = value != 0;
+            return 0;
+        case 17:
+            ((DevEngine*)h)->eng.def_coef = value != 0;
             return 0;
 
         default:

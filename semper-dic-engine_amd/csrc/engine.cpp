@@ -453,7 +453,7 @@ int Engine::solve_tasks_dim(const std::vector<Task>& tasks, std::vector<SolveOut
     a.refGx = gx_.p;
     a.refGy = gy_.p;
     set_precision_args(a);
-    a.def = def_plane();
+    set_def_args(a);
     a.w = w_;
     a.h = h_;
     a.dim = dim;
@@ -535,7 +535,7 @@ int Engine::match_anchors(std::vector<AnchorMatch>& reqs, int subset, bool use6,
             a.refI = cI;
             a.refGx = cgx;
             a.refGy = cgy;
-            a.def = def_plane();
+            set_def_args(a);
             a.w = w_;
             a.h = h_;
             a.dim = subset;
@@ -588,6 +588,12 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             if (!def_f32_.reserve(npx)) return kErrInit;
             ck(launch_u8_to_f32(def_u8_.p, npx, def_f32_.p, st_));
             def_f32_ready_ = true;
+        }
+        def_coef_ready_ = false;
+        if (def_coef && def_f32_ready_ && npx < ((size_t)1 << 28)) {
+            if (!def_coef_.reserve(npx)) return kErrInit;
+            ck(launch_row_cubic(def_u8_.p, w_, h_, def_coef_.p, st_));
+            def_coef_ready_ = true;
         }
         tm_.prep = ms_since(t_prep);
         const int gw = p.rect_w / p.step, gh = p.rect_h / p.step, N = gw * gh;
@@ -1184,7 +1190,7 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             a.refGx = gx_.p;
             a.refGy = gy_.p;
             set_precision_args(a);
-            a.def = def_plane();
+            set_def_args(a);
             a.w = w_;
             a.h = h_;
             a.lm = 1;
@@ -1203,7 +1209,8 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             for (size_t ci = 0; ci < nc; ++ci) {
                 const DimClass& c = classes_[ci];
                 const bool listed = c.off >= 0;
-                keys[ci] = {gw, gh, c.dim, p.use6 ? 1 : 0, listed ? 1 : 0, c.n, def_f32_ready_ ? 1 : 0};
+                keys[ci] = {gw, gh, c.dim, p.use6 ? 1 : 0, listed ? 1 : 0, c.n,
+                            (def_f32_ready_ ? 1 : 0) + (def_coef_ready_ ? 2 : 0)};
                 shapes[ci] = shape_for(keys[ci], listed);
                 a.items = listed ? class_nodes_d_.p + c.off : nullptr;
                 a.nitems = c.n;

@@ -216,6 +216,13 @@ public:
     // run; the values are the u8 ones, so the field is the same bit for bit)
     // instead of unpacking four bytes per row of every Keys gather.
     bool def_f32 = true;
+    // Keys 4x4 from a row-cubic coefficient plane of the deformed frame (no
+    // reference counterpart): made once per run (four fp16 per pixel, exact
+    // for u8 frames), read by the fast path's interior samples with one
+    // 8-byte load per stencil row and a Horner cubic, instead of four taps
+    // weighted. Needs def_f32 (the full solver keeps the FP32 plane) and a
+    // frame of under 2^28 pixels (32-bit byte offsets).
+    bool def_coef = false;
     // The global-motion search that seeds the anchor lattice (on by
     // default). Off, the lattice and Path C start from a zero shift, as the
     // reference's do when its feature match finds no model.
@@ -312,6 +319,12 @@ private:
     DevBuf<float> def_f32_;      // the deformed frame as FP32, when def_f32
     bool def_f32_ready_ = false;
     const void* def_plane() const { return def_f32_ready_ ? (const void*)def_f32_.p : (const void*)def_u8_.p; }
+    DevBuf<uint2> def_coef_;     // row-cubic coefficient plane of the frame, when def_coef
+    bool def_coef_ready_ = false;
+    void set_def_args(SolveArgs& a) const {
+        a.def = def_plane();
+        a.defc = def_coef_ready_ ? def_coef_.p : nullptr;
+    }
     DevBuf<float> I_, gx_, gy_;
     DevBuf<float> pyr_r_[4], pyr_d_[4];
     int pyr_ref_levels_ = 0;

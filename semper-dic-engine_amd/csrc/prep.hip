@@ -827,6 +827,34 @@ hipError_t launch_u8_to_f32(const uint8_t* src, size_t n, float* dst, hipStream_
     return hipGetLastError();
 }
 
+// Row-cubic coefficients of the Keys a = -0.5 kernel (image_processor.cpp:
+// 188-195): a Keys row read at x = xi + s is sum_k w_k(s) p_k over the taps
+// p_0..p_3 at xi - 1 .. xi + 2, the cubic c0 + c1 s + c2 s^2 + c3 s^3 with
+//   c0 = p1, c1 = (p2 - p0) / 2, c2 = p0 - 5/2 p1 + 2 p2 - 1/2 p3,
+//   c3 = -1/2 p0 + 3/2 p1 - 3/2 p2 + 1/2 p3.
+// For u8 taps each c is a multiple of 1/2 with |c| <= 765, so both the FP32
+// arithmetic here and the fp16 storage are exact. Pixel (x, y) holds the
+// cubic of the cell [x, x + 1) on row y as four fp16 (8 bytes, c0 lowest);
+// the taps are clamped at the row ends (those cells lie inside the 4 px
+// guard and are never read by the solver).
+__global__ void k_row_cubic(const uint8_t* __restrict__ img, int w, uint2* __restrict__ out) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= w) return;
+    const uint8_t* r = img + (size_t)blockIdx.y * w;
+    const float p0 = r[x > 0 ? x - 1 : 0], p1 = r[x], p2 = r[x + 1 < w ? x + 1 : w - 1],
+                p3 = r[x + 2 < w ? x + 2 : w - 1];
+    const float c0 = p1, c1 = 0.5f * (p2 - p0), c2 = (p0 - 2.5f * p1) + (2.0f * p2 - 0.5f * p3),
+                c3 = (1.5f * p1 - 0.5f * p0) + (0.5f * p3 - 1.5f * p2);
+    const auto bits = [](float v) { return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)v); };
+    out[(size_t)blockIdx.y * w + x] = make_uint2(bits(c0) | bits(c1) << 16, bits(c2) | bits(c3) << 16);
+}
+
+hipError_t launch_row_cubic(const uint8_t* img, int w, int h, uint2* out, hipStream_t st) {
+    if (w <= 0 || h <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_row_cubic, dim3((unsigned)((w + 255) / 256), (unsigned)h), dim3(256), 0, st, img, w, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_pool_list(const float* I, const float* gx, const float* gy, int w, int h, const int* cxy,
                             int n, int dim, bool lm, PoolEntry* pool, hipStream_t st) {
     if (n <= 0) return hipSuccess;

@@ -8,6 +8,7 @@ hipError_t launch_solve_bu8(const SolveArgs& a, int ppl, hipStream_t st);
 hipError_t launch_solve_ku8(const SolveArgs& a, int ppl, hipStream_t st);
 hipError_t launch_solve_bf32(const SolveArgs& a, int ppl, hipStream_t st);
 hipError_t launch_solve_kf32(const SolveArgs& a, int ppl, hipStream_t st);
+hipError_t launch_solve_cf32(const SolveArgs& a, int ppl, hipStream_t st);
 
 int ppl_for(int dim) {
     if (dim < 3 || dim > kMaxSubset) return -1;
@@ -27,6 +28,7 @@ hipError_t launch_solve(const SolveArgs& a, bool use6, bool def_f32, hipStream_t
     const int ppl = ppl_for(a.dim);
     if (ppl < 0) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, a.nbail, 1);
+    if (def_f32 && a.defc && !use6) return launch_solve_cf32(a, ppl, st);
     if (def_f32) return use6 ? launch_solve_kf32(a, ppl, st) : launch_solve_bf32(a, ppl, st);
     return use6 ? launch_solve_ku8(a, ppl, st) : launch_solve_bu8(a, ppl, st);
 }

@@ -53,6 +53,7 @@ struct SolveArgs {
     int grad_bf16;            // those planes hold bf16 (else fp16)
     const double* chol;       // FP64 Cholesky factors per pool entry (the step when set)
     const void* def;          // uint8 or float plane
+    const uint2* defc;        // row-cubic coefficient plane of the same frame (Keys 4x4 fast path), or null
     int w, h, dim, lm;
     float tol;
     // launch shape: subsets (waves) per fast-path workgroup (0 or 1: one),
@@ -66,6 +67,8 @@ int ppl_for(int dim);
 // solver_*.hip: one translation unit per (interpolator, deformed pixel type).
 // Launches the fast path over the work items, then the full solver over the
 // items it handed on. a.bail must hold nitems ints; a.nbail is zeroed here.
+// With a.defc set (and Keys 4x4 over an FP32 plane), the fast path's
+// interior samples read the coefficient plane instead.
 hipError_t launch_solve(const SolveArgs& a, bool use6, bool def_f32, hipStream_t st);
 
 // prep.hip
@@ -89,6 +92,13 @@ hipError_t launch_to_f16(const float* src, size_t n, uint16_t* dst, hipStream_t 
 hipError_t launch_to_bf16(const float* src, size_t n, uint16_t* dst, hipStream_t st);
 // u8 -> FP32 copy of a plane (src 4-byte aligned)
 hipError_t launch_u8_to_f32(const uint8_t* src, size_t n, float* dst, hipStream_t st);
+// Row-cubic coefficient plane of a u8 frame: per pixel the Keys cubic of the
+// cell to its right on its row, four fp16 (exact for u8 taps)
+hipError_t launch_row_cubic(const uint8_t* img, int w, int h, uint2* out, hipStream_t st);
+// solver_cf32.hip: the fast path's Keys 4x4 sampling forms at given points
+// (0: FP32 plane, 1: coefficient plane, 2: u8 plane), for tests
+hipError_t launch_interp_probe(const float* f32, const uint2* coef, const uint8_t* u8, int w, int h, int form,
+                               int n, const float* xs, const float* ys, float* out, hipStream_t st);
 hipError_t launch_downsample_f32(const float* src, int w, int h, float* dst, hipStream_t st);
 
 // Global-motion patch search: per patch the best integer shift after the

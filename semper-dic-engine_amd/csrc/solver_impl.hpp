@@ -34,6 +34,16 @@
 
 #pragma clang fp contract(off)
 
+// pixels per lane whose Keys 4x4 gathers the fast path issues together: over
+// the FP32 plane (16 dwords a pixel), and over the u8 or coefficient plane
+// (4 or 8 dwords a pixel)
+#ifndef SDIC_K4_BATCH_F32
+#define SDIC_K4_BATCH_F32 4
+#endif
+#ifndef SDIC_K4_BATCH_NARROW
+#define SDIC_K4_BATCH_NARROW 4
+#endif
+
 namespace sdic {
 
 namespace {
@@ -231,6 +241,101 @@ __device__ __forceinline__ float interp_buf(__amdgpu_buffer_rsrc_t rs, int w, fl
 #pragma unroll
         for (int k = 1; k < R; ++k) rv = __builtin_fmaf(px[k], wx[k], rv);
         val = __builtin_fmaf(rv, wy[j], val);
+    }
+    return val;
+}
+
+// Keys 4x4 from the row-cubic coefficient plane (four fp16 per pixel: the
+// cubic in s of the cell to the pixel's right, exact for u8 taps): one 8-byte
+// load per stencil row, the row's value by Horner in dx, the four rows
+// combined with the Keys weights in dy. The same interpolant as interp_buf;
+// the roundings differ (Horner against the tap-weight sum).
+typedef _Float16 sdic_h4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float interp_coef(__amdgpu_buffer_rsrc_t rs, int w, float x, float y) {
+    const int xi = (int)x, yi = (int)y;
+    const float dx = __builtin_amdgcn_fractf(x), dy = __builtin_amdgcn_fractf(y);
+    const int base = ((yi - 1) * w + xi) * 8;
+    float wy[4];
+    keys4_fma(dy, wy);
+    float val = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const auto b = __builtin_amdgcn_raw_buffer_load_b64(rs, base, j * w * 8, 0);
+        const sdic_h4 c = __builtin_bit_cast(sdic_h4, b);
+        float rv = __builtin_fmaf((float)c.w, dx, (float)c.z);
+        rv = __builtin_fmaf(rv, dx, (float)c.y);
+        rv = __builtin_fmaf(rv, dx, (float)c.x);
+        val = __builtin_fmaf(rv, wy[j], val);
+    }
+    return val;
+}
+
+// Keys 4x4 in two phases, so that a wave keeps the gathers of several pixels
+// in flight: k4_issue computes the cell and issues the four stencil-row loads
+// (the raw dwords: 1 per row from the u8 plane, 4 from the FP32 plane, 2 from
+// the coefficient plane), k4_eval combines them with exactly the operations
+// of interp_buf / interp_coef (so the value is theirs bit for bit).
+struct K4Rows {
+    uint32_t r[4][4];
+    float dx, dy;
+};
+
+template <typename TD, bool COEF>
+__device__ __forceinline__ void k4_issue(__amdgpu_buffer_rsrc_t rs, int w, float x, float y, K4Rows& k) {
+    const int xi = (int)x, yi = (int)y;
+    k.dx = __builtin_amdgcn_fractf(x);
+    k.dy = __builtin_amdgcn_fractf(y);
+    if constexpr (COEF) {
+        const int base = ((yi - 1) * w + xi) * 8;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const auto b = __builtin_amdgcn_raw_buffer_load_b64(rs, base, j * w * 8, 0);
+            k.r[j][0] = (uint32_t)b[0];
+            k.r[j][1] = (uint32_t)b[1];
+        }
+    } else if constexpr (sizeof(TD) == 1) {
+        const int base = (yi - 1) * w + xi - 1;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) k.r[j][0] = __builtin_amdgcn_raw_buffer_load_b32(rs, base, j * w, 0);
+    } else {
+        const int base = ((yi - 1) * w + xi - 1) * 4;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const auto b = __builtin_amdgcn_raw_buffer_load_b128(rs, base, j * w * 4, 0);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) k.r[j][c] = (uint32_t)b[c];
+        }
+    }
+}
+
+template <typename TD, bool COEF>
+__device__ __forceinline__ float k4_eval(const K4Rows& k) {
+    float wy[4];
+    keys4_fma(k.dy, wy);
+    float val = 0.0f;
+    if constexpr (COEF) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const sdic_h4 c = __builtin_bit_cast(sdic_h4, make_uint2(k.r[j][0], k.r[j][1]));
+            float rv = __builtin_fmaf((float)c.w, k.dx, (float)c.z);
+            rv = __builtin_fmaf(rv, k.dx, (float)c.y);
+            rv = __builtin_fmaf(rv, k.dx, (float)c.x);
+            val = __builtin_fmaf(rv, wy[j], val);
+        }
+    } else {
+        float wx[4];
+        keys4_fma(k.dx, wx);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float px[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                px[c] = sizeof(TD) == 1 ? ub(k.r[j][0], c) : __uint_as_float(k.r[j][c]);
+            float rv = px[0] * wx[0];
+#pragma unroll
+            for (int c = 1; c < 4; ++c) rv = __builtin_fmaf(px[c], wx[c], rv);
+            val = __builtin_fmaf(rv, wy[j], val);
+        }
     }
     return val;
 }
@@ -958,15 +1063,15 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
 // item goes to the full solver, which starts it again from scratch -- as soon
 // as a masked iteration, a Simplex rescue or a coarse search would be needed,
 // so the committed results are exactly full_solve's.
-template <int PPL, bool K6, typename TD>
+template <int PPL, bool K6, typename TD, bool COEF>
 __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __restrict__ pool,
                                            const float* __restrict__ refI,
                                            const float* __restrict__ refGx,
                                            const float* __restrict__ refGy,
                                            const uint16_t* __restrict__ gx16, const uint16_t* __restrict__ gy16, bool bf16,
                                            const double* __restrict__ chol,
-                                           const TD* __restrict__ def, int w, int h, int dim,
-                                           bool lm, float conv_tol, SolveOut& out) {
+                                           const TD* __restrict__ def, const uint2* __restrict__ defc,
+                                           int w, int h, int dim, bool lm, float conv_tol, SolveOut& out) {
     const int lane = threadIdx.x & 63;
     const PoolEntry* pe = pool + __builtin_amdgcn_readfirstlane(t.pool);
     out.init = 0;
@@ -989,7 +1094,10 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
             gxs[j] = 0.0f;
             gys[j] = 0.0f;
             val[j] = 0.0f;
-            xy[j] = sdic_h2{(_Float16)(r - half), (_Float16)(q - half)};
+            // a lane past the subset's last pixel samples the centre, so the
+            // interior loop runs without a branch per pixel (its sample is
+            // left out of every sum)
+            xy[j] = lane + 64 * j < n ? sdic_h2{(_Float16)(r - half), (_Float16)(q - half)} : sdic_h2{0, 0};
             if (lane + 64 * j < n) {
                 const int a = (t.cy + q - half) * w + t.cx + r - half;
                 const float ri = refI[a];
@@ -1024,7 +1132,8 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
     const double cxd = (double)t.cx, cyd = (double)t.cy, hd = (double)half;
     const float cxf = (float)t.cx, cyf = (float)t.cy;
     const __amdgpu_buffer_rsrc_t rs =
-        __builtin_amdgcn_make_buffer_rsrc((void*)def, (short)0, (int)((size_t)w * h * sizeof(TD)), 0x00020000);
+        COEF ? __builtin_amdgcn_make_buffer_rsrc((void*)defc, (short)0, (int)((size_t)w * h * 8), 0x00020000)
+             : __builtin_amdgcn_make_buffer_rsrc((void*)def, (short)0, (int)((size_t)w * h * sizeof(TD)), 0x00020000);
     float W[9] = {1.0f + t.g[2], t.g[3], t.g[0], t.g[4], 1.0f + t.g[5], t.g[1], 0.0f, 0.0f, 1.0f};
     float final_score = 1.0f;
     int status = 1, iters = kIcgnMaxIter;
@@ -1048,9 +1157,35 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
         if (interior) {
             // Inside the margin the warp is evaluated in FP32: it can differ from
             // the FP64 position by one float ulp, far inside the 1 px margin.
+            // No branch per pixel, and (Keys 4x4) the loads of kBatch pixels
+            // issued before any is evaluated, so their gathers overlap.
+            if constexpr (!K6) {
+                constexpr int kBatch = (COEF || sizeof(TD) == 1) ? SDIC_K4_BATCH_NARROW : SDIC_K4_BATCH_F32;
+                static_assert(PPL % kBatch == 0, "batch must divide the pixels per lane");
 #pragma unroll
-            for (int j = 0; j < PPL; ++j) {
-                if (lane + 64 * j < n) {
+                for (int j0 = 0; j0 < PPL; j0 += kBatch) {
+                    K4Rows kr[kBatch];
+#pragma unroll
+                    for (int b = 0; b < kBatch; ++b) {
+                        sdic_h2 c = xy[j0 + b];
+                        asm volatile("" : "+v"(c));
+                        const float x = (float)c.x, y = (float)c.y;
+                        const float fx = cxf + __builtin_fmaf(W[0], x, __builtin_fmaf(W[1], y, W[2]));
+                        const float fy = cyf + __builtin_fmaf(W[3], x, __builtin_fmaf(W[4], y, W[5]));
+                        k4_issue<TD, COEF>(rs, w, fx, fy, kr[b]);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int b = 0; b < kBatch; ++b) {
+                        const int j = j0 + b;
+                        const float v = k4_eval<TD, COEF>(kr[b]);
+                        val[j] = v;
+                        sum += lane + 64 * j < n ? v : 0.0f;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < PPL; ++j) {
                     sdic_h2 c = xy[j];
                     asm volatile("" : "+v"(c));
                     const float x = (float)c.x, y = (float)c.y;
@@ -1058,7 +1193,7 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
                     const float fy = cyf + __builtin_fmaf(W[3], x, __builtin_fmaf(W[4], y, W[5]));
                     const float v = interp_buf<K6, TD>(rs, w, fx, fy);
                     val[j] = v;
-                    sum += v;
+                    sum += lane + 64 * j < n ? v : 0.0f;
                 }
             }
         } else {
@@ -1102,22 +1237,22 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
         if (dstd < 1e-5f) dstd = 1.0f;
         const float inv = 1.0f / dstd;
         float err = 0.0f, dp[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        // branch-free: a lane past the last pixel has d = 0 (and zero
+        // gradients and offsets), so it adds exact zeros
 #pragma unroll
         for (int j = 0; j < PPL; ++j) {
-            if (lane + 64 * j < n) {
-                sdic_h2 c = xy[j];
-                asm volatile("" : "+v"(c));
-                const float x = (float)c.x, y = (float)c.y;
-                const float d = nref[j] - (val[j] - mean) * inv;
-                err = __builtin_fmaf(d, d, err);
-                const float gd = gxs[j] * d, hdd = gys[j] * d;
-                dp[0] += gd;
-                dp[1] += hdd;
-                dp[2] = __builtin_fmaf(gd, x, dp[2]);
-                dp[3] = __builtin_fmaf(gd, y, dp[3]);
-                dp[4] = __builtin_fmaf(hdd, x, dp[4]);
-                dp[5] = __builtin_fmaf(hdd, y, dp[5]);
-            }
+            sdic_h2 c = xy[j];
+            asm volatile("" : "+v"(c));
+            const float x = (float)c.x, y = (float)c.y;
+            const float d = lane + 64 * j < n ? nref[j] - (val[j] - mean) * inv : 0.0f;
+            err = __builtin_fmaf(d, d, err);
+            const float gd = gxs[j] * d, hdd = gys[j] * d;
+            dp[0] += gd;
+            dp[1] += hdd;
+            dp[2] = __builtin_fmaf(gd, x, dp[2]);
+            dp[3] = __builtin_fmaf(gd, y, dp[3]);
+            dp[4] = __builtin_fmaf(hdd, x, dp[4]);
+            dp[5] = __builtin_fmaf(hdd, y, dp[5]);
         }
         err = wave_sum(err);
 #pragma unroll
@@ -1285,7 +1420,7 @@ __device__ __forceinline__ int tile_order(int k, int gw, int gh, int T) {
 
 // Fast path, one wave per work item (a.wpb items per workgroup); items it
 // cannot finish are appended to a.bail for k_slow.
-template <int PPL, bool K6, typename TD>
+template <int PPL, bool K6, typename TD, bool COEF = false>
 __global__ __launch_bounds__(256) void k_fast(SolveArgs a) {
     const int wpb = (int)(blockDim.x >> 6);
     const int k = xcd_remap(blockIdx.x, gridDim.x) * wpb + (int)(threadIdx.x >> 6);
@@ -1294,8 +1429,9 @@ __global__ __launch_bounds__(256) void k_fast(SolveArgs a) {
     Task t;
     if (!item_task(a, id, t)) return;
     SolveOut o = {};
-    const bool fin = fast_solve<PPL, K6, TD>(t, a.pool, a.refI, a.refGx, a.refGy, a.refGx16, a.refGy16, a.grad_bf16 != 0, a.chol,
-                                             (const TD*)a.def, a.w, a.h, a.dim, a.lm != 0, a.tol, o);
+    const bool fin = fast_solve<PPL, K6, TD, COEF>(t, a.pool, a.refI, a.refGx, a.refGy, a.refGx16, a.refGy16,
+                                                   a.grad_bf16 != 0, a.chol, (const TD*)a.def, a.defc, a.w, a.h,
+                                                   a.dim, a.lm != 0, a.tol, o);
     if ((threadIdx.x & 63) != 0) return;
     if (!fin) {
         a.bail[atomicAdd(a.nbail, 1)] = id;

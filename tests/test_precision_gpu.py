@@ -1,7 +1,7 @@
 """The precision variants of the engine (no reference counterpart): the
 Gauss-Newton step by FP64 Cholesky of each pool entry, fp16 or bf16 storage of
-the reference gradient planes, and the Hessian pool's J^T J on MFMA (fp16
-inputs, FP32 accumulation). Each changes the path IC-GN takes, not its fixed
+the reference gradient planes, the Hessian pool's J^T J on MFMA (fp16 inputs,
+FP32 accumulation) and Keys 4x4 from the fp16 row-cubic coefficient plane. Each changes the path IC-GN takes, not its fixed
 point (the zero of the ZNSSD gradient), so it must give the plain method's
 point set and be no less accurate than it against the FP64 IC-GN optimum,
 output by output: u, v against each subset's FP64 optimum, and the strains
@@ -18,7 +18,8 @@ from helpers import PKG, P, OracleEngine, fp64_window_errors
 pytestmark = pytest.mark.gpu
 
 VARIANTS = {"plain": (), "fp64_cholesky": ((2, 1),), "fp16_gradients": ((3, 1),), "bf16_gradients": ((3, 2),),
-            "mfma_pool": ((4, 1),), "all_fp16": ((2, 1), (3, 1), (4, 1)), "all_bf16": ((2, 1), (3, 2), (4, 1))}
+            "mfma_pool": ((4, 1),), "fp16_coefficients": ((17, 1),), "all_fp16": ((2, 1), (3, 1), (4, 1), (17, 1)),
+            "all_bf16": ((2, 1), (3, 2), (4, 1), (17, 1))}
 
 
 @pytest.fixture(scope="module")
