@@ -140,7 +140,7 @@ def settle_tuner(dev, eng, run_once, limit=16):
 
 def precision_study(dev, dref, ddef, prm, cap, tol, ref, dfm, subset, use6, steps):
     """The precision variants (FP64 Cholesky Gauss-Newton step, fp16 or bf16
-    gradient storage, J^T J on MFMA, fp16 coefficient plane): per variant the first run's Hessian pre-pass (the
+    gradient storage, J^T J on MFMA, the fp16 coefficient plane): per variant the first run's Hessian pre-pass (the
     pool build), the step time, the converged throughput and the error of
     every output point's (u, v) against the FP64 IC-GN optimum on the same
     2000-node sample as the plain method."""
@@ -442,7 +442,7 @@ def main():
     ap.add_argument("--u8-deformed", action="store_true",
                     help="the solver samples the u8 deformed frame instead of its FP32 copy (same field)")
     ap.add_argument("--coef-plane", action="store_true",
-                    help="Keys 4x4 from the fp16 row-cubic coefficient plane of each frame")
+                    help="Keys 4x4 from the fp16 row-cubic coefficient plane of each frame (flag 17)")
     ap.add_argument("--tol", type=float, default=None, help="IC-GN stop rule ||dp|| < tol (default: the workload's)")
     ap.add_argument("--learned-seed", action="store_true",
                     help="the main line's engine seeds its anchors with the learned seeder (the committed weights)")

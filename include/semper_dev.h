@@ -80,7 +80,7 @@ SDK_API int sdk_wave_sum_probe(const float* in, int nw, float* out, int32_t* iou
    points of a u8 frame (host memory): form 0 the tap weights over the FP32
    plane, 1 the row-cubic coefficient plane of flag 17, 2 the tap weights over
    the u8 plane. Every point must lie inside the solver's guard, 4 <= x < w - 5
-   and 4 <= y < h - 5, and w h < 2^28; 0 or -3. */
+   and 4 <= y < h - 5, and w h < 2^27; 0 or -3. */
 SDK_API int sdk_interp_probe(const uint8_t* img, int w, int h, int form, int n, const float* xs, const float* ys,
                              float* out);
 
@@ -121,12 +121,13 @@ SDK_API int sdk_interp_probe(const uint8_t* img, int w, int h, int form, int n, 
    search that seeds the anchor lattice (default 1; 0: no search, the lattice
    and Path C start from a zero shift, as the reference's do when its feature
    match finds no model); flag 17 = Keys 4x4 from a row-cubic coefficient
-   plane (default 0): each run makes, from the u8 frame, four fp16 per pixel
-   (the Keys cubic of the cell to the pixel's right on its row, exact for u8
-   taps), and the fast path's interior samples read one 8-byte entry per
-   stencil row and evaluate it by Horner, instead of weighting four taps (the
-   same interpolant, other roundings; Keys 6x6, flag 13 off and frames of
-   2^28 pixels or more keep the direct form). */
+   plane (default 0): each run makes, from the u8 frame, 16 bytes per pixel
+   (the Keys cubics of the cell to the pixel's right on its row and the next,
+   four fp16 each, exact for u8 taps), and the fast path's interior samples
+   read two such entries per sample and evaluate each row by Horner, instead
+   of one load and four weighted taps per stencil row (the same interpolant,
+   other roundings; Keys 6x6, flag 13 off and frames of 2^27 pixels or more
+   keep the direct form). */
 SDK_API int sdk_engine_set_flag(void* h, int flag, int value);
 
 /* Phase timings of the last run, 14 doubles: total, prep, seeding, prepass,

@@ -254,14 +254,14 @@ SDK_API int sdk_wave_sum_probe(const float* in, int nw, float* out, int32_t* iou
 SDK_API int sdk_interp_probe(const uint8_t* img, int w, int h, int form, int n, const float* xs, const float* ys,
                              float* out) {
     if (!img || !xs || !ys || !out || w < 16 || h < 16 || n <= 0 || form < 0 || form > 2 ||
-        (size_t)w * h >= ((size_t)1 << 28))
+        (size_t)w * h >= kCoefMaxPixels)
         return -3;
     for (int i = 0; i < n; ++i)   // the solver's guard: the stencil lies inside the plane
         if (!(xs[i] >= 4.0f && xs[i] < (float)(w - 5) && ys[i] >= 4.0f && ys[i] < (float)(h - 5))) return -3;
     const size_t npx = (size_t)w * h;
     Dev<uint8_t> d8(npx);
     Dev<float> df(npx), dx(n), dy(n), dout(n);
-    Dev<uint2> dc(npx);
+    Dev<uint4> dc(npx);
     if (!d8.p || !df.p || !dx.p || !dy.p || !dout.p || !dc.p) return -3;
     if (hipMemcpy(d8.p, img, npx, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(dx.p, xs, sizeof(float) * n, hipMemcpyHostToDevice) != hipSuccess ||

@@ -590,7 +590,7 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             def_f32_ready_ = true;
         }
         def_coef_ready_ = false;
-        if (def_coef && def_f32_ready_ && npx < ((size_t)1 << 28)) {
+        if (def_coef && def_f32_ready_ && npx < kCoefMaxPixels) {
             if (!def_coef_.reserve(npx)) return kErrInit;
             ck(launch_row_cubic(def_u8_.p, w_, h_, def_coef_.p, st_));
             def_coef_ready_ = true;
@@ -1157,12 +1157,14 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
         // ---- Path A: the grid launch ----------------------------------------
         const auto t_a = clk::now();
         {
+            if (!stat_slots_.reserve((size_t)kStatSlots * kStatStride)) throw HipFail{hipErrorOutOfMemory};
             ck(hipMemsetAsync(counters_.p, 0, sizeof(int32_t) * 8, st_));
+            ck(launch_zero(stat_slots_.p, kStatSlots * kStatStride, st_));
             SolveArgs a = {};
             a.nitems = N;
             a.bail = bail_.p;
             a.nbail = counters_.p + 8;
-            a.stats = counters_.p;
+            a.stats = stat_slots_.p;
             a.grid.grid = g;
             a.grid.in_mesh = in_mesh;
             a.grid.gu = gu;
@@ -1222,6 +1224,7 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                 ck(hipEventRecord(class_ev_[ci + 1], st_));
             }
             int sa[9];
+            ck(launch_sum_stats(stat_slots_.p, counters_.p, st_));
             ck(hipMemcpyAsync(sa, counters_.p, sizeof(sa), hipMemcpyDeviceToHost, st_));
             ck(hipStreamSynchronize(st_));
             float kms = 0;

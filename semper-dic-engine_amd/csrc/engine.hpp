@@ -217,11 +217,13 @@ public:
     // instead of unpacking four bytes per row of every Keys gather.
     bool def_f32 = true;
     // Keys 4x4 from a row-cubic coefficient plane of the deformed frame (no
-    // reference counterpart): made once per run (four fp16 per pixel, exact
-    // for u8 frames), read by the fast path's interior samples with one
-    // 8-byte load per stencil row and a Horner cubic, instead of four taps
-    // weighted. Needs def_f32 (the full solver keeps the FP32 plane) and a
-    // frame of under 2^28 pixels (32-bit byte offsets).
+    // reference counterpart; off by default): made once per run (per pixel the
+    // cubics of two rows, four fp16 each, exact for u8 frames), read by the
+    // fast path's interior samples with two 16-byte loads per sample and a
+    // Horner cubic per stencil row, instead of four row loads and four taps
+    // weighted (half the vector-memory instructions, a quarter fewer
+    // vector-ALU ones). Needs def_f32 (the full solver keeps the FP32 plane)
+    // and a frame of under 2^27 pixels (32-bit byte offsets).
     bool def_coef = false;
     // The global-motion search that seeds the anchor lattice (on by
     // default). Off, the lattice and Path C start from a zero shift, as the
@@ -319,7 +321,8 @@ private:
     DevBuf<float> def_f32_;      // the deformed frame as FP32, when def_f32
     bool def_f32_ready_ = false;
     const void* def_plane() const { return def_f32_ready_ ? (const void*)def_f32_.p : (const void*)def_u8_.p; }
-    DevBuf<uint2> def_coef_;     // row-cubic coefficient plane of the frame, when def_coef
+    DevBuf<uint4> def_coef_;     // row-cubic coefficient plane of the frame, when def_coef
+    DevBuf<int32_t> stat_slots_; // Path A's counter slots (kStatSlots x kStatStride)
     bool def_coef_ready_ = false;
     void set_def_args(SolveArgs& a) const {
         a.def = def_plane();

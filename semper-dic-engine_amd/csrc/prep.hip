@@ -50,27 +50,32 @@ __global__ void k_prep_ref(const T* __restrict__ img, const uint8_t* __restrict_
 }
 
 // One thread per grid node: skip = inside the boundary buffer d/2 + 4 + 15,
-// or any subset pixel masked (< 128).
+// or any subset pixel masked (< 128). The kept nodes are counted once per
+// wave (a ballot), not by one same-address atomic per node.
 __global__ void k_scan(const uint8_t* __restrict__ mask, int w, int h, int rect_x, int rect_y, int step,
                        int gw, int gh, int dim, const int* __restrict__ dims, uint8_t* __restrict__ skip,
                        int* count) {
     const int node = blockIdx.x * blockDim.x + threadIdx.x;
-    if (node >= gw * gh) return;
-    const int rx = rect_x + (node % gw) * step, ry = rect_y + (node / gw) * step;
-    const int half = (dims ? dims[node] : dim) / 2, buf = half + 4 + 15;
-    bool s = rx - buf < 0 || rx + buf >= w || ry - buf < 0 || ry + buf >= h;
-    if (!s && mask) {
-        for (int dy = -half; dy <= half && !s; ++dy) {
-            const uint8_t* row = mask + (int64_t)(ry + dy) * w + rx;
-            for (int dx = -half; dx <= half; ++dx)
-                if (row[dx] < 128) {
-                    s = true;
-                    break;
-                }
+    const bool live = node < gw * gh;
+    bool s = true;
+    if (live) {
+        const int rx = rect_x + (node % gw) * step, ry = rect_y + (node / gw) * step;
+        const int half = (dims ? dims[node] : dim) / 2, buf = half + 4 + 15;
+        s = rx - buf < 0 || rx + buf >= w || ry - buf < 0 || ry + buf >= h;
+        if (!s && mask) {
+            for (int dy = -half; dy <= half && !s; ++dy) {
+                const uint8_t* row = mask + (int64_t)(ry + dy) * w + rx;
+                for (int dx = -half; dx <= half; ++dx)
+                    if (row[dx] < 128) {
+                        s = true;
+                        break;
+                    }
+            }
         }
+        skip[node] = s ? 1 : 0;
     }
-    skip[node] = s ? 1 : 0;
-    if (!s) atomicAdd(count, 1);
+    const uint64_t kept = __ballot(live && !s);
+    if ((threadIdx.x & 63) == 0 && kept) atomicAdd(count, (int)__popcll(kept));
 }
 
 // The reference planes a pool pass reads: float gradients, or their fp16
@@ -833,25 +838,33 @@ hipError_t launch_u8_to_f32(const uint8_t* src, size_t n, float* dst, hipStream_
 //   c0 = p1, c1 = (p2 - p0) / 2, c2 = p0 - 5/2 p1 + 2 p2 - 1/2 p3,
 //   c3 = -1/2 p0 + 3/2 p1 - 3/2 p2 + 1/2 p3.
 // For u8 taps each c is a multiple of 1/2 with |c| <= 765, so both the FP32
-// arithmetic here and the fp16 storage are exact. Pixel (x, y) holds the
-// cubic of the cell [x, x + 1) on row y as four fp16 (8 bytes, c0 lowest);
-// the taps are clamped at the row ends (those cells lie inside the 4 px
-// guard and are never read by the solver).
-__global__ void k_row_cubic(const uint8_t* __restrict__ img, int w, uint2* __restrict__ out) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= w) return;
-    const uint8_t* r = img + (size_t)blockIdx.y * w;
+// arithmetic here and the fp16 storage are exact. Entry (x, y) holds the
+// cubics of the cell [x, x + 1) on rows y and y + 1, four fp16 each (16
+// bytes, c0 lowest, row y first), so a 4 x 4 stencil is two 16-byte loads
+// (entries (x, y - 1) and (x, y + 1)). Taps are clamped at the plane's edges
+// (those cells lie inside the 4 px guard and are never read by the solver).
+__device__ __forceinline__ uint2 row_cubic(const uint8_t* __restrict__ r, int x, int w) {
     const float p0 = r[x > 0 ? x - 1 : 0], p1 = r[x], p2 = r[x + 1 < w ? x + 1 : w - 1],
                 p3 = r[x + 2 < w ? x + 2 : w - 1];
     const float c0 = p1, c1 = 0.5f * (p2 - p0), c2 = (p0 - 2.5f * p1) + (2.0f * p2 - 0.5f * p3),
                 c3 = (1.5f * p1 - 0.5f * p0) + (0.5f * p3 - 1.5f * p2);
     const auto bits = [](float v) { return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)v); };
-    out[(size_t)blockIdx.y * w + x] = make_uint2(bits(c0) | bits(c1) << 16, bits(c2) | bits(c3) << 16);
+    return make_uint2(bits(c0) | bits(c1) << 16, bits(c2) | bits(c3) << 16);
 }
 
-hipError_t launch_row_cubic(const uint8_t* img, int w, int h, uint2* out, hipStream_t st) {
+__global__ void k_row_cubic_pairs(const uint8_t* __restrict__ img, int w, int h, uint4* __restrict__ out) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= w) return;
+    const int y = blockIdx.y;
+    const uint2 a = row_cubic(img + (size_t)y * w, x, w);
+    const uint2 b = row_cubic(img + (size_t)(y + 1 < h ? y + 1 : h - 1) * w, x, w);
+    out[(size_t)y * w + x] = make_uint4(a.x, a.y, b.x, b.y);
+}
+
+hipError_t launch_row_cubic(const uint8_t* img, int w, int h, uint4* out, hipStream_t st) {
     if (w <= 0 || h <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_row_cubic, dim3((unsigned)((w + 255) / 256), (unsigned)h), dim3(256), 0, st, img, w, out);
+    hipLaunchKernelGGL(k_row_cubic_pairs, dim3((unsigned)((w + 255) / 256), (unsigned)h), dim3(256), 0, st, img, w,
+                       h, out);
     return hipGetLastError();
 }
 

@@ -24,6 +24,21 @@ __global__ void k_zero_int(int* p, int n) {
     if (i < n) p[i] = 0;
 }
 
+// one workgroup: thread c < 5 adds counter c over the slots (integer sums, so
+// the order does not matter)
+__global__ void k_sum_stats(const int* __restrict__ slots, int* __restrict__ out) {
+    const int c = threadIdx.x;
+    if (c >= 5) return;
+    int s = 0;
+    for (int k = 0; k < kStatSlots; ++k) s += slots[k * kStatStride + c];
+    out[c] = s;
+}
+
+hipError_t launch_sum_stats(const int* slots, int* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_sum_stats, dim3(1), dim3(64), 0, st, slots, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_solve(const SolveArgs& a, bool use6, bool def_f32, hipStream_t st) {
     const int ppl = ppl_for(a.dim);
     if (ppl < 0) return hipErrorInvalidValue;

@@ -43,7 +43,9 @@ struct SolveArgs {
     int* bail;                // ids the fast path hands to the full solver
     int* nbail;               // their count (device)
     GridArgs grid;
-    int* stats;               // grid mode: points, iterations, simplex calls, saved, dead
+    int* stats;               // grid mode: kStatSlots slots of kStatStride ints (points, iterations,
+                              // simplex calls, saved, dead), a workgroup counting into slot
+                              // blockIdx.x % kStatSlots; launch_sum_stats adds them up
     const PoolEntry* pool;
     const float* refI;
     const float* refGx;
@@ -53,13 +55,21 @@ struct SolveArgs {
     int grad_bf16;            // those planes hold bf16 (else fp16)
     const double* chol;       // FP64 Cholesky factors per pool entry (the step when set)
     const void* def;          // uint8 or float plane
-    const uint2* defc;        // row-cubic coefficient plane of the same frame (Keys 4x4 fast path), or null
+    const uint4* defc;        // row-cubic coefficient plane of the same frame (Keys 4x4 fast path), or null
     int w, h, dim, lm;
     float tol;
     // launch shape: subsets (waves) per fast-path workgroup (0 or 1: one),
     // and Path A's node order (0: row-major; T > 0: T x T tiles)
     int wpb, tile;
 };
+
+// Path A's counters are spread over slots of one 128-byte line each: one
+// shared counter would take two same-address atomics per subset, which the L2
+// serializes (about 4 ms of a 4096^2 frame's 262k subsets).
+constexpr int kStatSlots = 512, kStatStride = 32;
+// Sums the five counters of the slots into out[0..4] (after zeroing nothing:
+// out is overwritten).
+hipError_t launch_sum_stats(const int* slots, int* out, hipStream_t st);
 
 // Pixels-per-lane class of a subset size (-1: larger than kMaxSubset).
 int ppl_for(int dim);
@@ -92,12 +102,14 @@ hipError_t launch_to_f16(const float* src, size_t n, uint16_t* dst, hipStream_t 
 hipError_t launch_to_bf16(const float* src, size_t n, uint16_t* dst, hipStream_t st);
 // u8 -> FP32 copy of a plane (src 4-byte aligned)
 hipError_t launch_u8_to_f32(const uint8_t* src, size_t n, float* dst, hipStream_t st);
-// Row-cubic coefficient plane of a u8 frame: per pixel the Keys cubic of the
-// cell to its right on its row, four fp16 (exact for u8 taps)
-hipError_t launch_row_cubic(const uint8_t* img, int w, int h, uint2* out, hipStream_t st);
+// Row-cubic coefficient plane of a u8 frame: per pixel the Keys cubics of the
+// cell to its right on its row and the next, four fp16 each (exact for u8
+// taps); kCoefMaxPixels bounds the frame (32-bit byte offsets)
+constexpr size_t kCoefMaxPixels = (size_t)1 << 27;
+hipError_t launch_row_cubic(const uint8_t* img, int w, int h, uint4* out, hipStream_t st);
 // solver_cf32.hip: the fast path's Keys 4x4 sampling forms at given points
 // (0: FP32 plane, 1: coefficient plane, 2: u8 plane), for tests
-hipError_t launch_interp_probe(const float* f32, const uint2* coef, const uint8_t* u8, int w, int h, int form,
+hipError_t launch_interp_probe(const float* f32, const uint4* coef, const uint8_t* u8, int w, int h, int form,
                                int n, const float* xs, const float* ys, float* out, hipStream_t st);
 hipError_t launch_downsample_f32(const float* src, int w, int h, float* dst, hipStream_t st);
 

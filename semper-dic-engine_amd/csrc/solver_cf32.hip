@@ -35,7 +35,7 @@ hipError_t launch_solve_cf32(const SolveArgs& a, int ppl, hipStream_t st) {
 // the tap weights over the FP32 plane, 1 the row-cubic coefficient plane,
 // 2 the tap weights over the u8 plane (the caller keeps every point inside
 // the solver's 4 px guard).
-__global__ void k_interp_probe(const float* __restrict__ f32, const uint2* __restrict__ coef,
+__global__ void k_interp_probe(const float* __restrict__ f32, const uint4* __restrict__ coef,
                                const uint8_t* __restrict__ u8, int w, int h, int form, int n,
                                const float* __restrict__ xs, const float* __restrict__ ys, float* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -43,7 +43,7 @@ __global__ void k_interp_probe(const float* __restrict__ f32, const uint2* __res
     const float x = xs[i], y = ys[i];
     const int npx = w * h;
     if (form == 1) {
-        const auto rs = __builtin_amdgcn_make_buffer_rsrc((void*)coef, (short)0, npx * 8, 0x00020000);
+        const auto rs = __builtin_amdgcn_make_buffer_rsrc((void*)coef, (short)0, npx * 16, 0x00020000);
         out[i] = interp_coef(rs, w, x, y);
     } else if (form == 0) {
         const auto rs = __builtin_amdgcn_make_buffer_rsrc((void*)f32, (short)0, npx * 4, 0x00020000);
@@ -54,7 +54,7 @@ __global__ void k_interp_probe(const float* __restrict__ f32, const uint2* __res
     }
 }
 
-hipError_t launch_interp_probe(const float* f32, const uint2* coef, const uint8_t* u8, int w, int h, int form,
+hipError_t launch_interp_probe(const float* f32, const uint4* coef, const uint8_t* u8, int w, int h, int form,
                                int n, const float* xs, const float* ys, float* out, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_interp_probe, dim3((n + 255) / 256), dim3(256), 0, st, f32, coef, u8, w, h, form, n, xs, ys,

@@ -38,7 +38,7 @@
 // the FP32 plane (16 dwords a pixel), and over the u8 or coefficient plane
 // (4 or 8 dwords a pixel)
 #ifndef SDIC_K4_BATCH_F32
-#define SDIC_K4_BATCH_F32 4
+#define SDIC_K4_BATCH_F32 2
 #endif
 #ifndef SDIC_K4_BATCH_NARROW
 #define SDIC_K4_BATCH_NARROW 4
@@ -245,36 +245,42 @@ __device__ __forceinline__ float interp_buf(__amdgpu_buffer_rsrc_t rs, int w, fl
     return val;
 }
 
-// Keys 4x4 from the row-cubic coefficient plane (four fp16 per pixel: the
-// cubic in s of the cell to the pixel's right, exact for u8 taps): one 8-byte
-// load per stencil row, the row's value by Horner in dx, the four rows
-// combined with the Keys weights in dy. The same interpolant as interp_buf;
-// the roundings differ (Horner against the tap-weight sum).
+// Keys 4x4 from the row-cubic coefficient plane (per pixel the cubics in s of
+// the cell to its right on its row and the next, four fp16 each, exact for
+// u8 taps): two 16-byte loads per sample (rows yi - 1, yi and yi + 1, yi + 2),
+// each row's value by Horner in dx, the four rows combined with the Keys
+// weights in dy. The same interpolant as interp_buf; the roundings differ
+// (Horner against the tap-weight sum).
 typedef _Float16 sdic_h4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float coef_row(uint32_t lo, uint32_t hi, float dx) {
+    const sdic_h4 c = __builtin_bit_cast(sdic_h4, make_uint2(lo, hi));
+    float rv = __builtin_fmaf((float)c.w, dx, (float)c.z);
+    rv = __builtin_fmaf(rv, dx, (float)c.y);
+    return __builtin_fmaf(rv, dx, (float)c.x);
+}
+
 __device__ __forceinline__ float interp_coef(__amdgpu_buffer_rsrc_t rs, int w, float x, float y) {
     const int xi = (int)x, yi = (int)y;
     const float dx = __builtin_amdgcn_fractf(x), dy = __builtin_amdgcn_fractf(y);
-    const int base = ((yi - 1) * w + xi) * 8;
+    const int base = ((yi - 1) * w + xi) * 16;
     float wy[4];
     keys4_fma(dy, wy);
+    const auto a = __builtin_amdgcn_raw_buffer_load_b128(rs, base, 0, 0);
+    const auto b = __builtin_amdgcn_raw_buffer_load_b128(rs, base, 2 * w * 16, 0);
     float val = 0.0f;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const auto b = __builtin_amdgcn_raw_buffer_load_b64(rs, base, j * w * 8, 0);
-        const sdic_h4 c = __builtin_bit_cast(sdic_h4, b);
-        float rv = __builtin_fmaf((float)c.w, dx, (float)c.z);
-        rv = __builtin_fmaf(rv, dx, (float)c.y);
-        rv = __builtin_fmaf(rv, dx, (float)c.x);
-        val = __builtin_fmaf(rv, wy[j], val);
-    }
+    val = __builtin_fmaf(coef_row((uint32_t)a[0], (uint32_t)a[1], dx), wy[0], val);
+    val = __builtin_fmaf(coef_row((uint32_t)a[2], (uint32_t)a[3], dx), wy[1], val);
+    val = __builtin_fmaf(coef_row((uint32_t)b[0], (uint32_t)b[1], dx), wy[2], val);
+    val = __builtin_fmaf(coef_row((uint32_t)b[2], (uint32_t)b[3], dx), wy[3], val);
     return val;
 }
 
 // Keys 4x4 in two phases, so that a wave keeps the gathers of several pixels
-// in flight: k4_issue computes the cell and issues the four stencil-row loads
-// (the raw dwords: 1 per row from the u8 plane, 4 from the FP32 plane, 2 from
-// the coefficient plane), k4_eval combines them with exactly the operations
-// of interp_buf / interp_coef (so the value is theirs bit for bit).
+// in flight: k4_issue computes the cell and issues the stencil loads (the raw
+// dwords: 1 per row from the u8 plane, 4 per row from the FP32 plane, two
+// 16-byte row pairs from the coefficient plane), k4_eval combines them with
+// exactly the operations of interp_buf / interp_coef (so the value is theirs
+// bit for bit).
 struct K4Rows {
     uint32_t r[4][4];
     float dx, dy;
@@ -286,12 +292,13 @@ __device__ __forceinline__ void k4_issue(__amdgpu_buffer_rsrc_t rs, int w, float
     k.dx = __builtin_amdgcn_fractf(x);
     k.dy = __builtin_amdgcn_fractf(y);
     if constexpr (COEF) {
-        const int base = ((yi - 1) * w + xi) * 8;
+        const int base = ((yi - 1) * w + xi) * 16;
+        const auto a = __builtin_amdgcn_raw_buffer_load_b128(rs, base, 0, 0);
+        const auto b = __builtin_amdgcn_raw_buffer_load_b128(rs, base, 2 * w * 16, 0);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const auto b = __builtin_amdgcn_raw_buffer_load_b64(rs, base, j * w * 8, 0);
-            k.r[j][0] = (uint32_t)b[0];
-            k.r[j][1] = (uint32_t)b[1];
+        for (int c = 0; c < 4; ++c) {
+            k.r[0][c] = (uint32_t)a[c];
+            k.r[1][c] = (uint32_t)b[c];
         }
     } else if constexpr (sizeof(TD) == 1) {
         const int base = (yi - 1) * w + xi - 1;
@@ -314,14 +321,10 @@ __device__ __forceinline__ float k4_eval(const K4Rows& k) {
     keys4_fma(k.dy, wy);
     float val = 0.0f;
     if constexpr (COEF) {
+        // rows yi - 1, yi in k.r[0], rows yi + 1, yi + 2 in k.r[1]
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const sdic_h4 c = __builtin_bit_cast(sdic_h4, make_uint2(k.r[j][0], k.r[j][1]));
-            float rv = __builtin_fmaf((float)c.w, k.dx, (float)c.z);
-            rv = __builtin_fmaf(rv, k.dx, (float)c.y);
-            rv = __builtin_fmaf(rv, k.dx, (float)c.x);
-            val = __builtin_fmaf(rv, wy[j], val);
-        }
+        for (int j = 0; j < 4; ++j)
+            val = __builtin_fmaf(coef_row(k.r[j >> 1][2 * (j & 1)], k.r[j >> 1][2 * (j & 1) + 1], k.dx), wy[j], val);
     } else {
         float wx[4];
         keys4_fma(k.dx, wx);
@@ -1070,7 +1073,7 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
                                            const float* __restrict__ refGy,
                                            const uint16_t* __restrict__ gx16, const uint16_t* __restrict__ gy16, bool bf16,
                                            const double* __restrict__ chol,
-                                           const TD* __restrict__ def, const uint2* __restrict__ defc,
+                                           const TD* __restrict__ def, const uint4* __restrict__ defc,
                                            int w, int h, int dim, bool lm, float conv_tol, SolveOut& out) {
     const int lane = threadIdx.x & 63;
     const PoolEntry* pe = pool + __builtin_amdgcn_readfirstlane(t.pool);
@@ -1085,27 +1088,23 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
     sdic_h2 xy[PPL];   // the pixel's offset from the centre, (x, y), exact in fp16
     const float mean_r = pe->mean, inv_r = pe->inv_std;
     const int q64 = 64 / dim, r64 = 64 % dim;
+    // pixels j < kFull exist in every lane: launch_solve picks the smallest
+    // class, so the subset has more pixels than 64 lanes of the next smaller
+    constexpr int kFull = PPL == 4 ? 0 : PPL == 8 ? 4 : PPL == 12 ? 8 : PPL == 16 ? 12 : PPL == 24 ? 16
+                        : PPL == 32 ? 24 : PPL == 48 ? 32 : 48;
     int bad = 0;
     {
+        // Branch-free, so that all 3 PPL loads are in flight together: a lane
+        // past the subset's last pixel reads the centre (and keeps zeros) and
+        // samples the centre in the iterations (its sample is left out of
+        // every sum).
+        int at[PPL];
         int q = lane / dim, r = lane % dim;
 #pragma unroll
         for (int j = 0; j < PPL; ++j) {
-            nref[j] = 0.0f;
-            gxs[j] = 0.0f;
-            gys[j] = 0.0f;
-            val[j] = 0.0f;
-            // a lane past the subset's last pixel samples the centre, so the
-            // interior loop runs without a branch per pixel (its sample is
-            // left out of every sum)
-            xy[j] = lane + 64 * j < n ? sdic_h2{(_Float16)(r - half), (_Float16)(q - half)} : sdic_h2{0, 0};
-            if (lane + 64 * j < n) {
-                const int a = (t.cy + q - half) * w + t.cx + r - half;
-                const float ri = refI[a];
-                nref[j] = (ri - mean_r) * inv_r;
-                gxs[j] = (gx16 ? half_bits_to_float(gx16[a], bf16) : refGx[a]) * inv_r;
-                gys[j] = (gy16 ? half_bits_to_float(gy16[a], bf16) : refGy[a]) * inv_r;
-                bad += (ri < kGhostWallThreshold) ? 1 : 0;
-            }
+            const bool in = j < kFull || lane + 64 * j < n;
+            xy[j] = in ? sdic_h2{(_Float16)(r - half), (_Float16)(q - half)} : sdic_h2{0, 0};
+            at[j] = in ? (t.cy + q - half) * w + t.cx + r - half : t.cy * w + t.cx;
             r += r64;
             q += q64;
             if (r >= dim) {
@@ -1113,10 +1112,44 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
                 q++;
             }
         }
+#pragma unroll
+        for (int j = 0; j < PPL; ++j) nref[j] = refI[at[j]];
+        if (gx16) {
+#pragma unroll
+            for (int j = 0; j < PPL; ++j) {
+                gxs[j] = half_bits_to_float(gx16[at[j]], bf16);
+                gys[j] = half_bits_to_float(gy16[at[j]], bf16);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < PPL; ++j) {
+                gxs[j] = refGx[at[j]];
+                gys[j] = refGy[at[j]];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < PPL; ++j) {
+            const bool in = j < kFull || lane + 64 * j < n;
+            const float ri = nref[j];
+            bad += (in && ri < kGhostWallThreshold) ? 1 : 0;
+            nref[j] = in ? (ri - mean_r) * inv_r : 0.0f;
+            gxs[j] = in ? gxs[j] * inv_r : 0.0f;
+            gys[j] = in ? gys[j] * inv_r : 0.0f;
+            val[j] = 0.0f;
+        }
     }
     // Any Ghost Wall pixel makes every iteration a masked one.
     if (wave_sum_i(bad) != 0) return false;
     const float* __restrict__ hs = pe->hstep;   // wave-uniform: scalar loads
+    // Lane-parallel pieces of the per-iteration bookkeeping: lane l holds row
+    // (l & 7) of the step matrix (rows 6 and 7 repeat row 0) and tests warped
+    // corner (l & 3) of the subset.
+    float hrow[6];
+    {
+        const int hr = (lane & 7) < 6 ? (lane & 7) : 0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) hrow[c] = hs[hr * 6 + c];
+    }
     const double* __restrict__ ch =
         chol ? chol + (size_t)__builtin_amdgcn_readfirstlane(t.pool) * kCholStride : nullptr;
     const bool use_chol = ch != nullptr && !__builtin_isnan(ch[0]);
@@ -1132,7 +1165,7 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
     const double cxd = (double)t.cx, cyd = (double)t.cy, hd = (double)half;
     const float cxf = (float)t.cx, cyf = (float)t.cy;
     const __amdgpu_buffer_rsrc_t rs =
-        COEF ? __builtin_amdgcn_make_buffer_rsrc((void*)defc, (short)0, (int)((size_t)w * h * 8), 0x00020000)
+        COEF ? __builtin_amdgcn_make_buffer_rsrc((void*)defc, (short)0, (int)((size_t)w * h * 16), 0x00020000)
              : __builtin_amdgcn_make_buffer_rsrc((void*)def, (short)0, (int)((size_t)w * h * sizeof(TD)), 0x00020000);
     float W[9] = {1.0f + t.g[2], t.g[3], t.g[0], t.g[4], 1.0f + t.g[5], t.g[1], 0.0f, 0.0f, 1.0f};
     float final_score = 1.0f;
@@ -1143,15 +1176,15 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
         const double w0 = W[0], w1 = W[1], w2 = W[2], w3 = W[3], w4 = W[4], w5 = W[5];
         // Interior: the four warped corners lie 1 px inside the 4 px guard, so
         // (the map being affine) every pixel does, and no pixel test is needed.
-        bool interior = true;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const double x = (k & 1) ? hd : -hd, y = (k & 2) ? hd : -hd;
+        // each lane tests one of the four corners; interior when none is out
+        bool interior;
+        {
+            const double x = (lane & 1) ? hd : -hd, y = (lane & 2) ? hd : -hd;
             const float fx = warped_fma(cxd, w0, w1, w2, x, y), fy = warped_fma(cyd, w3, w4, w5, x, y);
-            interior = interior && fx + 0.5f >= 5.0f && fx + 0.5f < (float)(w - 5) && fy + 0.5f >= 5.0f &&
-                       fy + 0.5f < (float)(h - 5);
+            const bool in = fx + 0.5f >= 5.0f && fx + 0.5f < (float)(w - 5) && fy + 0.5f >= 5.0f &&
+                            fy + 0.5f < (float)(h - 5);
+            interior = __builtin_amdgcn_ballot_w64(!in) == 0;
         }
-        interior = __builtin_amdgcn_readfirstlane(interior ? 1 : 0) != 0;
         float sum = 0.0f;
         int nv = n;
         if (interior) {
@@ -1180,7 +1213,7 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
                         const int j = j0 + b;
                         const float v = k4_eval<TD, COEF>(kr[b]);
                         val[j] = v;
-                        sum += lane + 64 * j < n ? v : 0.0f;
+                        sum += (j < kFull || lane + 64 * j < n) ? v : 0.0f;
                     }
                 }
             } else {
@@ -1231,7 +1264,7 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
 #pragma unroll
         for (int j = 0; j < PPL; ++j) {
             const float d = val[j] - mean;
-            s2 = (lane + 64 * j < n) ? __builtin_fmaf(d, d, s2) : s2;
+            s2 = (j < kFull || lane + 64 * j < n) ? __builtin_fmaf(d, d, s2) : s2;
         }
         float dstd = sqrtf(wave_sum(s2) / (float)nv);
         if (dstd < 1e-5f) dstd = 1.0f;
@@ -1244,7 +1277,7 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
             sdic_h2 c = xy[j];
             asm volatile("" : "+v"(c));
             const float x = (float)c.x, y = (float)c.y;
-            const float d = lane + 64 * j < n ? nref[j] - (val[j] - mean) * inv : 0.0f;
+            const float d = (j < kFull || lane + 64 * j < n) ? nref[j] - (val[j] - mean) * inv : 0.0f;
             err = __builtin_fmaf(d, d, err);
             const float gd = gxs[j] * d, hdd = gys[j] * d;
             dp[0] += gd;
@@ -1262,13 +1295,15 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
         if (use_chol) {
             chol64_step(ch, dp, d);
         } else {
+            // row (lane & 7) of the step on each lane, in the order of the
+            // row-by-row product, then the six rows read back
+            float acc = 0.0f;
 #pragma unroll
-            for (int r = 0; r < 6; ++r) {
-                float acc = 0.0f;
+            for (int c = 0; c < 6; ++c) acc += hrow[c] * dp[c];
+            const float dr = -acc;
 #pragma unroll
-                for (int c = 0; c < 6; ++c) acc += hs[r * 6 + c] * dp[c];
-                d[r] = -acc;
-            }
+            for (int r = 0; r < 6; ++r)
+                d[r] = __uint_as_float((uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(dr), r));
         }
         const float dW[9] = {1.0f + d[2], d[3], d[0], d[4], 1.0f + d[5], d[1], 0.0f, 0.0f, 1.0f};
         const float det = dW[0] * dW[4] - dW[1] * dW[3];
@@ -1282,9 +1317,8 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
             done = true;
             break;
         }
-        float dWi[9], Wn[9];
-        inv3(dW, dWi);
-        mul3(W, dWi, Wn);
+        float Wn[9];
+        compose_inverse_affine(W, dW, Wn);
 #pragma unroll
         for (int k = 0; k < 9; ++k) fin = fin && finite_f(Wn[k]);
         if (!fin) {
@@ -1345,8 +1379,10 @@ __device__ __forceinline__ bool item_task(const SolveArgs& a, int id, Task& t) {
         return true;
     }
     const GridArgs& g = a.grid;
-    if (!g.in_mesh[id] || g.grid.solved[id]) return false;
+    // the three bytes read together (one memory round trip, not three)
+    const uint8_t in_mesh = g.in_mesh[id], solved = g.grid.solved[id];
     const int tag = g.grid.tag[id];
+    if (!in_mesh || solved) return false;
     if (g.skip_model && tag == kMeshHomography) return false;
     t.cx = g.rect_x + (id % g.gw) * g.step;
     t.cy = g.rect_y + (id / g.gw) * g.step;
@@ -1364,7 +1400,8 @@ __device__ __forceinline__ bool item_task(const SolveArgs& a, int id, Task& t) {
 
 // Lane 0 commits a work item: the result slot in task mode; in grid mode the
 // accepted node into the grid, and the Path A counters (points, iterations,
-// Simplex calls / saved / dead) with integer atomics.
+// Simplex calls / saved / dead) with integer atomics into the workgroup's
+// counter slot.
 __device__ __forceinline__ void commit(const SolveArgs& a, int id, const SolveOut& o) {
     if (a.tasks) {
         a.out[id] = o;
@@ -1373,10 +1410,11 @@ __device__ __forceinline__ void commit(const SolveArgs& a, int id, const SolveOu
     if (!o.init) return;
     GridDev gd = a.grid.grid;
     const bool acc = o.status == 0 && o.corr <= kCorrAccept;
-    atomicAdd(&a.stats[1], o.iters);
+    int* const st = a.stats + (blockIdx.x % kStatSlots) * kStatStride;
+    atomicAdd(&st[1], o.iters);
     if (o.simplex) {
-        atomicAdd(&a.stats[2], 1);
-        atomicAdd(acc ? &a.stats[3] : &a.stats[4], 1);
+        atomicAdd(&st[2], 1);
+        atomicAdd(acc ? &st[3] : &st[4], 1);
     }
     gd.iters[id] = o.iters;
     gd.simplex[id] = o.simplex ? 1 : 0;
@@ -1389,7 +1427,7 @@ __device__ __forceinline__ void commit(const SolveArgs& a, int id, const SolveOu
         gd.vy[id] = o.p[5];
         gd.corr[id] = o.corr;
         gd.solved[id] = 1;
-        atomicAdd(&a.stats[0], 1);
+        atomicAdd(&st[0], 1);
     } else {
         gd.corr[id] = kCorrInvalid;
     }
@@ -1420,8 +1458,13 @@ __device__ __forceinline__ int tile_order(int k, int gw, int gh, int T) {
 
 // Fast path, one wave per work item (a.wpb items per workgroup); items it
 // cannot finish are appended to a.bail for k_slow.
+#ifdef SDIC_FAST_MIN_WAVES
+#define SDIC_FAST_ATTR __attribute__((amdgpu_waves_per_eu(SDIC_FAST_MIN_WAVES)))
+#else
+#define SDIC_FAST_ATTR
+#endif
 template <int PPL, bool K6, typename TD, bool COEF = false>
-__global__ __launch_bounds__(256) void k_fast(SolveArgs a) {
+__global__ __launch_bounds__(256) SDIC_FAST_ATTR void k_fast(SolveArgs a) {
     const int wpb = (int)(blockDim.x >> 6);
     const int k = xcd_remap(blockIdx.x, gridDim.x) * wpb + (int)(threadIdx.x >> 6);
     if (k >= a.nitems) return;

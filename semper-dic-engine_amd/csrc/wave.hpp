@@ -77,6 +77,29 @@ __device__ inline void inv3(const float* a, float* r) {
         for (int j = 0; j < 3; ++j) r[i * 3 + j] = cof(j, i) * inv;
 }
 
+// W * inv3(A) for affine 3x3 matrices (last rows 0 0 1), with the products
+// inv3 and mul3 form for the entries that are not multiplied by an exact zero
+// of those last rows, in their order: the same bits as mul3(W, inv3(A)) for
+// finite entries, up to the sign of an entry that is zero.
+__device__ __forceinline__ void compose_inverse_affine(const float* W, const float* a, float* r) {
+    const float det = a[4] * a[0] + -a[1] * a[3];
+    const float inv = 1.0f / det;
+    // inv3's r[0..5] and r[8] (r[6] = r[7] = 0)
+    const float i0 = a[4] * inv, i1 = -a[1] * inv, i2 = (a[1] * a[5] - a[2] * a[4]) * inv;
+    const float i3 = -a[3] * inv, i4 = a[0] * inv, i5 = (a[2] * a[3] - a[0] * a[5]) * inv;
+    const float i8 = det * inv;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const float w0 = W[i * 3], w1 = W[i * 3 + 1], w2 = W[i * 3 + 2];
+        r[i * 3] = w0 * i0 + w1 * i3;
+        r[i * 3 + 1] = w0 * i1 + w1 * i4;
+        r[i * 3 + 2] = w0 * i2 + w1 * i5 + w2 * i8;
+    }
+    r[6] = 0.0f;
+    r[7] = 0.0f;
+    r[8] = W[8] * i8;
+}
+
 __device__ inline void mul3(const float* a, const float* b, float* r) {
 #pragma unroll
     for (int i = 0; i < 3; ++i)

@@ -98,16 +98,20 @@ def test_every_shape_gives_the_same_field(env, name):
 @pytest.mark.parametrize("use6", [0, 1], ids=["keys4", "keys6"])
 def test_fp32_deformed_plane_gives_the_same_field(env, use6):
     """The solver sampling an FP32 copy of the deformed frame (flag 13, the
-    default) against sampling the u8 frame: the copy holds the same values,
-    and both kernels gather and sum in the same order, so the field, the
-    point count and the metrics are the same bit for bit."""
+    default) against sampling the u8 frame, both by tap weights (flag 17 off:
+    the coefficient plane is made from the FP32 run only): the copy holds the
+    same values, and both kernels gather and sum in the same order, so the
+    field, the point count and the metrics are the same bit for bit. Without
+    the FP32 copy the coefficient plane is not used, whatever flag 17 says."""
     dev, wl = env
     name = "t_1024_noisy"
     cfg = wl.config(name)
     ref, dfm = wl.pair(dev, name)
     prm = np.array(wl.solve_params(name), np.int32)
     prm[7] = use6
-    [(a, ma)], _ = run(dev, cfg, ref, dfm, prm, [(13, 0)])
-    [(b, mb)], _ = run(dev, cfg, ref, dfm, prm, [(13, 1)])
+    [(a, ma)], _ = run(dev, cfg, ref, dfm, prm, [(13, 0), (17, 0)])
+    [(b, mb)], _ = run(dev, cfg, ref, dfm, prm, [(13, 1), (17, 0)])
+    [(c, mc)], _ = run(dev, cfg, ref, dfm, prm, [(13, 0), (17, 1)])
     print(f"use6 {use6}: {len(a) // 8} points", flush=True)
     assert np.array_equal(a, b) and np.array_equal(ma[COUNTS], mb[COUNTS])
+    assert np.array_equal(a, c) and np.array_equal(ma[COUNTS], mc[COUNTS])
