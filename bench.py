@@ -294,7 +294,7 @@ def copy_probe():
     print("copy probe: 256 MiB read, 256 MiB written per dispatch", flush=True)
 
 
-def frames_in_flight(dev, sa, dref, ddef, prm, cap, tol, w, h, steps, k, expect, learned=True):
+def frames_in_flight(dev, sa, dref, ddef, prm, cap, tol, w, h, steps, k, expect, learned=True, flags=()):
     """One frame pair solved with k frames in flight: k engines, each with its
     own stream and the same cached reference, driven by k host threads
     (ctypes releases the GIL for the call), so that one frame's host phases
@@ -306,6 +306,8 @@ def frames_in_flight(dev, sa, dref, ddef, prm, cap, tol, w, h, steps, k, expect,
     engs, outs, mets = [], [], []
     for _ in range(k):
         e = dev.sdk_engine_create()
+        for f, v in flags:
+            assert dev.sdk_engine_set_flag(e, f, v) == 0
         if learned:
             if dev.sdk_engine_load_seeder(e, sa.SEEDER_WEIGHTS.encode()) != 0:
                 raise RuntimeError("seeder weights missing: " + sa.SEEDER_WEIGHTS)
@@ -474,10 +476,10 @@ def main():
         if dev.sdk_engine_load_seeder(eng, sa.SEEDER_WEIGHTS.encode()) != 0:
             raise RuntimeError("seeder weights missing: " + sa.SEEDER_WEIGHTS)
         assert dev.sdk_engine_set_flag(eng, 5, 1) == 0
-    if args.u8_deformed:
-        assert dev.sdk_engine_set_flag(eng, 13, 0) == 0
-    if args.coef_plane:
-        assert dev.sdk_engine_set_flag(eng, 17, 1) == 0
+    # the main line's engine switches, applied to every engine of the same solve
+    engine_flags = ([(13, 0)] if args.u8_deformed else []) + ([(17, 1)] if args.coef_plane else [])
+    for f, v in engine_flags:
+        assert dev.sdk_engine_set_flag(eng, f, v) == 0
     assert dev.sdk_engine_set_reference(eng, dref, 1, w, h, None) == 0
     gw, gh = w // cfg["step"], h // cfg["step"]
     cap = gw * gh * 8
@@ -524,7 +526,8 @@ def main():
     if n > 0:
         for k in (2, 3):
             in_flight[str(k)] = frames_in_flight(dev, sa, dref, ddef, prm, cap, tol, w, h, max(4, args.steps), k,
-                                                 (int(n), out[: int(n) * 8].copy()), learned=args.learned_seed)
+                                                 (int(n), out[: int(n) * 8].copy()), learned=args.learned_seed,
+                                                 flags=engine_flags)
     solved = float(met[3] + met[4] + met[19])
     n = int(n)
     shape = np.zeros(2, np.int32)
@@ -558,6 +561,8 @@ def main():
         # the node planes of the same solve, from an engine with the node report on
         e2 = dev.sdk_engine_create()
         assert dev.sdk_engine_set_flag(e2, 12, 1) == 0
+        for f, v in engine_flags:
+            assert dev.sdk_engine_set_flag(e2, f, v) == 0
         if args.learned_seed:
             dev.sdk_engine_load_seeder(e2, sa.SEEDER_WEIGHTS.encode())
             dev.sdk_engine_set_flag(e2, 5, 1)

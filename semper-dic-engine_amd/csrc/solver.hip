@@ -1,6 +1,7 @@
 // Dispatch of the subset solver over its instantiations, and the pixels-per-lane
 // class of a subset size.
 #include "solver.hpp"
+#include "wave.hpp"
 
 namespace sdic {
 
@@ -24,18 +25,18 @@ __global__ void k_zero_int(int* p, int n) {
     if (i < n) p[i] = 0;
 }
 
-// one workgroup: thread c < 5 adds counter c over the slots (integer sums, so
-// the order does not matter)
+// one wave per counter: each lane adds kStatSlots / 64 slots, then the wave
+// sum (integer sums, so the order does not matter)
 __global__ void k_sum_stats(const int* __restrict__ slots, int* __restrict__ out) {
-    const int c = threadIdx.x;
-    if (c >= 5) return;
+    const int c = threadIdx.x >> 6, lane = threadIdx.x & 63;
     int s = 0;
-    for (int k = 0; k < kStatSlots; ++k) s += slots[k * kStatStride + c];
-    out[c] = s;
+    for (int k = lane; k < kStatSlots; k += 64) s += slots[k * kStatStride + c];
+    s = wave_sum_i(s);
+    if (lane == 0) out[c] = s;
 }
 
 hipError_t launch_sum_stats(const int* slots, int* out, hipStream_t st) {
-    hipLaunchKernelGGL(k_sum_stats, dim3(1), dim3(64), 0, st, slots, out);
+    hipLaunchKernelGGL(k_sum_stats, dim3(1), dim3(5 * 64), 0, st, slots, out);
     return hipGetLastError();
 }
 

@@ -194,7 +194,7 @@ __device__ __forceinline__ float interp_buf(__amdgpu_buffer_rsrc_t rs, int w, fl
     const int xi = (int)x, yi = (int)y;
     const float dx = __builtin_amdgcn_fractf(x), dy = __builtin_amdgcn_fractf(y);
     constexpr int R = K6 ? 6 : 4, H = K6 ? 2 : 1;
-    const int base = ((yi - H) * w + xi - H) * (int)sizeof(TD);
+    const int base = ((int)__umul24((unsigned)(yi - H), (unsigned)w) + xi - H) * (int)sizeof(TD);
     float wx[R], wy[R];
     if constexpr (!K6) {
         keys4_fma(dx, wx);
@@ -262,7 +262,7 @@ __device__ __forceinline__ float coef_row(uint32_t lo, uint32_t hi, float dx) {
 __device__ __forceinline__ float interp_coef(__amdgpu_buffer_rsrc_t rs, int w, float x, float y) {
     const int xi = (int)x, yi = (int)y;
     const float dx = __builtin_amdgcn_fractf(x), dy = __builtin_amdgcn_fractf(y);
-    const int base = ((yi - 1) * w + xi) * 16;
+    const int base = ((int)__umul24((unsigned)(yi - 1), (unsigned)w) + xi) * 16;
     float wy[4];
     keys4_fma(dy, wy);
     const auto a = __builtin_amdgcn_raw_buffer_load_b128(rs, base, 0, 0);
@@ -275,6 +275,8 @@ __device__ __forceinline__ float interp_coef(__amdgpu_buffer_rsrc_t rs, int w, f
     return val;
 }
 
+// (Row offsets use the full-rate 24-bit multiply: the row and the width are
+// below 2^24 and their product, a pixel index, below 2^31.)
 // Keys 4x4 in two phases, so that a wave keeps the gathers of several pixels
 // in flight: k4_issue computes the cell and issues the stencil loads (the raw
 // dwords: 1 per row from the u8 plane, 4 per row from the FP32 plane, two
@@ -292,7 +294,7 @@ __device__ __forceinline__ void k4_issue(__amdgpu_buffer_rsrc_t rs, int w, float
     k.dx = __builtin_amdgcn_fractf(x);
     k.dy = __builtin_amdgcn_fractf(y);
     if constexpr (COEF) {
-        const int base = ((yi - 1) * w + xi) * 16;
+        const int base = ((int)__umul24((unsigned)(yi - 1), (unsigned)w) + xi) * 16;
         const auto a = __builtin_amdgcn_raw_buffer_load_b128(rs, base, 0, 0);
         const auto b = __builtin_amdgcn_raw_buffer_load_b128(rs, base, 2 * w * 16, 0);
 #pragma unroll
@@ -301,11 +303,11 @@ __device__ __forceinline__ void k4_issue(__amdgpu_buffer_rsrc_t rs, int w, float
             k.r[1][c] = (uint32_t)b[c];
         }
     } else if constexpr (sizeof(TD) == 1) {
-        const int base = (yi - 1) * w + xi - 1;
+        const int base = (int)__umul24((unsigned)(yi - 1), (unsigned)w) + xi - 1;
 #pragma unroll
         for (int j = 0; j < 4; ++j) k.r[j][0] = __builtin_amdgcn_raw_buffer_load_b32(rs, base, j * w, 0);
     } else {
-        const int base = ((yi - 1) * w + xi - 1) * 4;
+        const int base = ((int)__umul24((unsigned)(yi - 1), (unsigned)w) + xi - 1) * 4;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const auto b = __builtin_amdgcn_raw_buffer_load_b128(rs, base, j * w * 4, 0);
@@ -1269,7 +1271,19 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
         float dstd = sqrtf(wave_sum(s2) / (float)nv);
         if (dstd < 1e-5f) dstd = 1.0f;
         const float inv = 1.0f / dstd;
-        float err = 0.0f, dp[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        float dp[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        // The ZNSSD of this iteration's sample, sum d^2 / n: only the iteration
+        // that reports it (the converged or the last one) forms it, in the
+        // same order as the residual loop below.
+        const auto znssd = [&]() {
+            float e = 0.0f;
+#pragma unroll
+            for (int j = 0; j < PPL; ++j) {
+                const float d = (j < kFull || lane + 64 * j < n) ? nref[j] - (val[j] - mean) * inv : 0.0f;
+                e = __builtin_fmaf(d, d, e);
+            }
+            return wave_sum(e) / (float)nv;
+        };
         // branch-free: a lane past the last pixel has d = 0 (and zero
         // gradients and offsets), so it adds exact zeros
 #pragma unroll
@@ -1278,7 +1292,6 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
             asm volatile("" : "+v"(c));
             const float x = (float)c.x, y = (float)c.y;
             const float d = (j < kFull || lane + 64 * j < n) ? nref[j] - (val[j] - mean) * inv : 0.0f;
-            err = __builtin_fmaf(d, d, err);
             const float gd = gxs[j] * d, hdd = gys[j] * d;
             dp[0] += gd;
             dp[1] += hdd;
@@ -1287,10 +1300,8 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
             dp[4] = __builtin_fmaf(hdd, x, dp[4]);
             dp[5] = __builtin_fmaf(hdd, y, dp[5]);
         }
-        err = wave_sum(err);
 #pragma unroll
         for (int k = 0; k < 6; ++k) dp[k] = wave_sum(dp[k]);
-        final_score = err / (float)nv;
         float d[6];
         if (use_chol) {
             chol64_step(ch, dp, d);
@@ -1335,6 +1346,7 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
         for (int k = 0; k < 6; ++k) nrm += d[k] * d[k];
         if (sqrtf(nrm) < conv_tol) {
             iters = it + 1;
+            final_score = znssd();
             if (!textured || !finite_f(final_score)) {
                 status = 1;
                 corr = 2.0f;
@@ -1345,6 +1357,7 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
             done = true;
             break;
         }
+        if (it == kIcgnMaxIter - 1) final_score = znssd();
     }
     if (!done) {
         status = 1;
