@@ -42,8 +42,10 @@ SD_FLOP_PER_PIXEL = 8.0
 PEAK_FP32_TFLOPS = 157.3
 # HBM bytes per Path A launch from two rocprofv3 --pmc runs (FETCH_SIZE x 2.0,
 # WRITE_SIZE x 1.0, the factors a 256 MiB copy kernel reads under the same
-# counters); measured per workload, recorded with the kernel it was measured on
-TRAFFIC_BYTES = {"c4_4096_heterogeneous": 178023.6 * 1024 * 2.0 + 24360.2 * 1024 * 1.0}
+# counters); measured per workload with the default engine switches, recorded
+# with the kernel it was measured on (null for a workload or switch set not
+# measured)
+TRAFFIC_BYTES = {"c4_4096_heterogeneous": 214073.7 * 1024 * 2.0 + 24362.3 * 1024 * 1.0}
 
 
 def fp64_accuracy(ref, dfm, subset, pts, use6, k=2000, seed=0):
@@ -616,7 +618,8 @@ def main():
         "roofline": {"bound": "valu", "compute_roof": "FP32 vector ALU (MI355X FP32 peak 157.3 TFLOP/s)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": TRAFFIC_BYTES.get(args.workload), "traffic_unit": "bytes per launch",
+                     "traffic": None if (engine_flags or args.learned_seed) else TRAFFIC_BYTES.get(args.workload),
+                     "traffic_unit": "bytes per launch",
                      "kernel": "Path A subset solver launch (k_fast + k_slow)", "avg_launch_ms": avg_kms,
                      "iterations_per_launch": float(np.mean(kern_iters)), "launch_shape": launch_shape},
         "cpu_baseline": cpu,
