@@ -84,6 +84,20 @@ SDK_API int sdk_wave_sum_probe(const float* in, int nw, float* out, int32_t* iou
 SDK_API int sdk_interp_probe(const uint8_t* img, int w, int h, int form, int n, const float* xs, const float* ys,
                              float* out);
 
+/* The incremental session's FP64 composition kernel (no reference
+   counterpart; the sampling is sample_increment, incremental_math.cpp:61-123)
+   on host arrays, for tests: an increment field (7 planes of fgw x fgh, u v ux
+   uy vx vy corr, and the median test's flags, grid origin (frx, fry) and
+   step) sampled at every alive node of a gw x gh grid at (rx + i step + uf,
+   ry + j step + vf), the sample added to u64 / v64 in double. Per node:
+   flags (bit 0 sampled, bit 1 by the first-order expansion), the new totals
+   in double and rounded once, and the sample's corr (other nodes: flags 0,
+   outputs unspecified). 0 or -3. */
+SDK_API int sdk_compose64_probe(const float* planes, const uint8_t* med, int fgw, int fgh, int frx, int fry,
+                                int fstep, const float* uf, const float* vf, const double* u64, const double* v64,
+                                const uint8_t* alive, int gw, int gh, int rx, int ry, int step, uint8_t* flags,
+                                double* ou64, double* ov64, float* ouf, float* ovf, float* ocorr);
+
 /* Engine switches: flag 0 = speculative Path B first claims (default 1);
    flag 1 = `out` of sdk_engine_run is device memory (default 0);
    precision variants with no reference counterpart (default 0, a change

@@ -251,6 +251,40 @@ SDK_API int sdk_wave_sum_probe(const float* in, int nw, float* out, int32_t* iou
     return 0;
 }
 
+SDK_API int sdk_compose64_probe(const float* planes, const uint8_t* med, int fgw, int fgh, int frx, int fry,
+                                int fstep, const float* uf, const float* vf, const double* u64, const double* v64,
+                                const uint8_t* alive, int gw, int gh, int rx, int ry, int step, uint8_t* flags,
+                                double* ou64, double* ov64, float* ouf, float* ovf, float* ocorr) {
+    if (!planes || !med || !uf || !vf || !u64 || !v64 || !alive || !flags || !ou64 || !ov64 || !ouf || !ovf ||
+        !ocorr || fgw <= 0 || fgh <= 0 || fstep <= 0 || gw <= 0 || gh <= 0 || step <= 0)
+        return -3;
+    const size_t NN = (size_t)fgw * fgh, n0 = (size_t)gw * gh;
+    Dev<float> dpl(7 * NN), dtf(4 * n0), dcorr(n0);
+    Dev<uint8_t> dmed(NN), dfl(2 * n0);
+    Dev<double> dt64(4 * n0);
+    if (!dpl.p || !dtf.p || !dcorr.p || !dmed.p || !dfl.p || !dt64.p) return -3;
+    if (hipMemcpy(dpl.p, planes, sizeof(float) * 7 * NN, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dmed.p, med, NN, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dtf.p, uf, sizeof(float) * n0, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dtf.p + n0, vf, sizeof(float) * n0, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dt64.p, u64, sizeof(double) * n0, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dt64.p + n0, v64, sizeof(double) * n0, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dfl.p, alive, n0, hipMemcpyHostToDevice) != hipSuccess)
+        return -3;
+    const ComposeField f{dpl.p, dmed.p, fgw, fgh, frx, fry, fstep};
+    const ComposeTotals t{dtf.p, dtf.p + n0, dt64.p, dt64.p + n0, dfl.p, gw, gh, rx, ry, step};
+    const ComposeOut o{dt64.p + 2 * n0, dt64.p + 3 * n0, dtf.p + 2 * n0, dtf.p + 3 * n0, dcorr.p, dfl.p + n0};
+    if (launch_sample_compose64(f, t, o, nullptr) != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(flags, dfl.p + n0, n0, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(ou64, dt64.p + 2 * n0, sizeof(double) * n0, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(ov64, dt64.p + 3 * n0, sizeof(double) * n0, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(ouf, dtf.p + 2 * n0, sizeof(float) * n0, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(ovf, dtf.p + 3 * n0, sizeof(float) * n0, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(ocorr, dcorr.p, sizeof(float) * n0, hipMemcpyDeviceToHost) != hipSuccess)
+        return -3;
+    return 0;
+}
+
 SDK_API int sdk_interp_probe(const uint8_t* img, int w, int h, int form, int n, const float* xs, const float* ys,
                              float* out) {
     if (!img || !xs || !ys || !out || w < 16 || h < 16 || n <= 0 || form < 0 || form > 2 ||

@@ -1739,6 +1739,7 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
         std::memcpy(hc, pin_counts_.p, sizeof(hc));
         if (keep_field) {
             // the chain leaves the planes and the valid flags as they were
+            last_dev_field = ComposeField{gplanes_.p, med, gw, gh, p.rect_x, p.rect_y, p.step};
             last_field = Field{};
             last_field.gw = gw;
             last_field.gh = gh;

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "compose.hpp"
 #include "seeder.hpp"
 #include "solver.hpp"
 
@@ -302,6 +303,9 @@ public:
     // over several subset sizes reports its last size's hand-offs
     std::vector<std::pair<int, int>> launch_log;
     Field last_field;
+    // The same field on the device when keep_field (the node planes and the
+    // median test's flags; valid until the next run or reference change).
+    ComposeField last_dev_field;
 
     // Median test, VSG strain and row-major packing of a node field given on
     // the host (f.ok marks the measured nodes; corr is reported as is). The

@@ -368,6 +368,43 @@ int IncrementalSession::set_region_map(int rw, int rh, const std::vector<int>& d
     return 0;
 }
 
+int IncrementalSession::compose_on_device(std::vector<uint8_t>& flags, std::vector<double>& u64,
+                                          std::vector<double>& v64, std::vector<float>& uf, std::vector<float>& vf,
+                                          std::vector<float>& corr) {
+    const size_t n0 = (size_t)gw_ * gh_;
+    if (!c64_uf_.reserve(4 * n0) || !c64_u64_.reserve(4 * n0) || !c64_corr_.reserve(n0) ||
+        !c64_flags_.reserve(2 * n0))
+        return kErrInit;
+    const hipStream_t st = eng_->stream();
+    float* uf_d = c64_uf_.p;
+    double* u64_d = c64_u64_.p;
+    uint8_t* fl_d = c64_flags_.p;
+    bool ok = hipMemcpyAsync(uf_d, ub_.data(), sizeof(float) * n0, hipMemcpyHostToDevice, st) == hipSuccess &&
+              hipMemcpyAsync(uf_d + n0, vb_.data(), sizeof(float) * n0, hipMemcpyHostToDevice, st) == hipSuccess &&
+              hipMemcpyAsync(u64_d, ub64_.data(), sizeof(double) * n0, hipMemcpyHostToDevice, st) == hipSuccess &&
+              hipMemcpyAsync(u64_d + n0, vb64_.data(), sizeof(double) * n0, hipMemcpyHostToDevice, st) ==
+                  hipSuccess &&
+              hipMemcpyAsync(fl_d, alive_.data(), n0, hipMemcpyHostToDevice, st) == hipSuccess;
+    if (!ok) return kErrInit;
+    ComposeTotals t{uf_d, uf_d + n0, u64_d, u64_d + n0, fl_d, gw_, gh_, p0_.rect_x, p0_.rect_y, p0_.step};
+    ComposeOut o{u64_d + 2 * n0, u64_d + 3 * n0, uf_d + 2 * n0, uf_d + 3 * n0, c64_corr_.p, fl_d + n0};
+    if (launch_sample_compose64(eng_->last_dev_field, t, o, st) != hipSuccess) return kErrInit;
+    flags.resize(n0);
+    u64.resize(n0);
+    v64.resize(n0);
+    uf.resize(n0);
+    vf.resize(n0);
+    corr.resize(n0);
+    ok = hipMemcpyAsync(flags.data(), fl_d + n0, n0, hipMemcpyDeviceToHost, st) == hipSuccess &&
+         hipMemcpyAsync(u64.data(), u64_d + 2 * n0, sizeof(double) * n0, hipMemcpyDeviceToHost, st) == hipSuccess &&
+         hipMemcpyAsync(v64.data(), u64_d + 3 * n0, sizeof(double) * n0, hipMemcpyDeviceToHost, st) == hipSuccess &&
+         hipMemcpyAsync(uf.data(), uf_d + 2 * n0, sizeof(float) * n0, hipMemcpyDeviceToHost, st) == hipSuccess &&
+         hipMemcpyAsync(vf.data(), uf_d + 3 * n0, sizeof(float) * n0, hipMemcpyDeviceToHost, st) == hipSuccess &&
+         hipMemcpyAsync(corr.data(), c64_corr_.p, sizeof(float) * n0, hipMemcpyDeviceToHost, st) == hipSuccess &&
+         hipStreamSynchronize(st) == hipSuccess;
+    return ok ? 0 : kErrInit;
+}
+
 int IncrementalSession::alive_count() const {
     int n = 0;
     for (uint8_t a : alive_) n += a ? 1 : 0;
@@ -534,6 +571,15 @@ int IncrementalSession::process(const uint8_t* def, float* out, int out_capacity
         const Engine::Field inc_field = eng_->last_field;
         const NodeField inc = node_field(inc_field);
         const GapDistance gd = gap_distance(inc);
+        // with FP64 composition, the sampling and the double sums of every
+        // followed node run on the device (coasting nodes stay below)
+        std::vector<uint8_t> g_flags;
+        std::vector<double> g_u64, g_v64;
+        std::vector<float> g_uf, g_vf, g_corr;
+        if (c64_) {
+            const int rcc = compose_on_device(g_flags, g_u64, g_v64, g_uf, g_vf, g_corr);
+            if (rcc != 0) return rcc;
+        }
         Engine::Field tot;
         tot.gw = gw_;
         tot.gh = gh_;
@@ -555,12 +601,26 @@ int IncrementalSession::process(const uint8_t* def, float* out, int out_capacity
                 if (!alive_[i]) continue;
                 const float x0 = (float)(p0_.rect_x + gx * p0_.step), y0 = (float)(p0_.rect_y + gy * p0_.step);
                 const float px = x0 + ub_[i], py = y0 + vb_[i];
-                const Sample smp = sample_increment(inc, px, py);
+                Sample smp;
+                if (c64_) {
+                    smp.ok = (g_flags[i] & 1) != 0;
+                    smp.taylor = (g_flags[i] & 2) != 0;
+                    smp.corr = g_corr[i];
+                } else {
+                    smp = sample_increment(inc, px, py);
+                }
                 if (!smp.ok) {
                     coast(i, inc, px, py);
                     continue;
                 }
-                add_increment(i, smp.u, smp.v);
+                if (c64_) {
+                    U64[i] = g_u64[i];
+                    V64[i] = g_v64[i];
+                    U[i] = g_uf[i];
+                    V[i] = g_vf[i];
+                } else {
+                    add_increment(i, smp.u, smp.v);
+                }
                 corr_of[i] = smp.corr;
                 need[i] = anchor_need(inc, gd, px, py, subset);
                 if (coast_travel_[i] > trusted_travel) {

@@ -92,6 +92,17 @@ private:
     std::vector<float> ub_, vb_, coast_travel_;
     std::vector<double> ub64_, vb64_;   // the same totals in double, when c64_
     bool c64_ = false;                   // compose64 as it was at init
+    // The device side of the FP64 composition: the totals in (float, then
+    // double) and out, the followed nodes, the per-node sample flags and corr.
+    DevBuf<float> c64_uf_;               // 4 n0: uf vf in, uf vf out
+    DevBuf<double> c64_u64_;             // 4 n0: u64 v64 in, u64 v64 out
+    DevBuf<float> c64_corr_;
+    DevBuf<uint8_t> c64_flags_;          // 2 n0: alive in, flags out
+    // Samples every followed node's increment and adds it to its total in
+    // double on the device (launch_sample_compose64); host copies of the
+    // results for the node loop.
+    int compose_on_device(std::vector<uint8_t>& flags, std::vector<double>& u64, std::vector<double>& v64,
+                          std::vector<float>& uf, std::vector<float>& vf, std::vector<float>& corr);
     std::vector<uint8_t> alive_;
     std::map<size_t, Anchor> anchors_;
     int ref_index_ = 0, frames_since_update_ = 0, ref_reported_ = -1, adopted_reported_ = -1, frame_count_ = 0;

@@ -103,6 +103,8 @@ def load_dev(path=LIB_DEV):
     lib.sdk_engine_set_flag.argtypes = [_VP, ctypes.c_int, ctypes.c_int]
     lib.sdk_wave_sum_probe.argtypes = [_VP, ctypes.c_int, _VP, _VP]
     lib.sdk_interp_probe.argtypes = [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, _VP, _VP]
+    lib.sdk_compose64_probe.argtypes = ([_VP, _VP] + [ctypes.c_int] * 5 + [_VP] * 5 + [ctypes.c_int] * 5 +
+                                        [_VP] * 6)
     lib.sdk_synth_frame.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP,
                                     ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_float, _VP, ctypes.c_int]
     lib.sdk_synth_pair.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, ctypes.c_int,
