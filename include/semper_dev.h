@@ -141,7 +141,10 @@ SDK_API int sdk_compose64_probe(const float* planes, const uint8_t* med, int fgw
    read two such entries per sample and evaluate each row by Horner, instead
    of one load and four weighted taps per stencil row (the same interpolant,
    other roundings; Keys 6x6, flag 13 off and frames of 2^27 pixels or more
-   keep the direct form). */
+   keep the direct form); flag 18 = the layered Path B's frontier (sources,
+   breadth-first search, claims, updates) on the device (default 1; for reach
+   <= 16 on one subset size; 0: on the host; the field is the same either
+   way). */
 SDK_API int sdk_engine_set_flag(void* h, int flag, int value);
 
 /* Phase timings of the last run, 14 doubles: total, prep, seeding, prepass,

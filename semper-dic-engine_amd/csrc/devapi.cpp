@@ -364,6 +364,9 @@ SDK_API int sdk_engine_set_flag(void* h, int flag, int value) {
         case 17:
             ((DevEngine*)h)->eng.def_coef = value != 0;
             return 0;
+        case 18:
+            ((DevEngine*)h)->eng.flood_on_device = value != 0;
+            return 0;
 
         default:
             return -3;

@@ -468,6 +468,66 @@ int Engine::solve_tasks_dim(const std::vector<Task>& tasks, std::vector<SolveOut
     return 0;
 }
 
+int Engine::solve_tasks_device(const Task* tasks, int n, SolveOut* out, bool use6, int dim, float tol) {
+    if (n <= 0) return 0;
+    if (!bail_.reserve(n) || !counters_.reserve(32)) return kErrInit;
+    SolveArgs a = {};
+    a.tasks = tasks;
+    a.out = out;
+    a.nitems = n;
+    a.bail = bail_.p;
+    a.nbail = counters_.p + 8;
+    a.pool = pool_.p;
+    a.refI = I_.p;
+    a.refGx = gx_.p;
+    a.refGy = gy_.p;
+    set_precision_args(a);
+    set_def_args(a);
+    a.w = w_;
+    a.h = h_;
+    a.dim = dim;
+    a.lm = 1;
+    a.tol = tol;
+    ck(launch_solve(a, use6, def_f32_ready_, st_));
+    return 0;
+}
+
+FloodDev Engine::flood_dev(int gw, int gh) {
+    const size_t NN = (size_t)gw * gh;
+    FloodDev f;
+    if (!fl_u8_.reserve(4 * NN) || !fl_w_.reserve(7 * NN) || !fl_int_.reserve(9 * NN) || !fl_key_.reserve(3 * NN) ||
+        !fl_tasks_.reserve(NN) || !fl_res_.reserve(3 * NN) || !fl_counters_.reserve(kFcCount) ||
+        !pin_fl_.reserve(kFcCount + 1))
+        throw HipFail{hipErrorOutOfMemory};
+    const size_t temp = flood_sort_temp_bytes((int)NN);
+    if (!fl_sort_temp_.reserve(temp)) throw HipFail{hipErrorOutOfMemory};
+    f.gw = gw;
+    f.gh = gh;
+    f.ok = fl_u8_.p;
+    f.fin = fl_u8_.p + NN;
+    f.committed = fl_u8_.p + 2 * NN;
+    f.tried = (int8_t*)(fl_u8_.p + 3 * NN);
+    f.w = fl_w_.p;
+    int* ip = fl_int_.p;
+    f.stamp = ip;
+    f.dist = ip + NN;
+    f.cand = ip + 2 * NN;
+    f.sources = ip + 3 * NN;
+    f.level_a = ip + 4 * NN;
+    f.level_b = ip + 5 * NN;
+    f.reached = ip + 6 * NN;
+    f.tdist = ip + 7 * NN;
+    f.commit_nodes = ip + 8 * NN;
+    f.key = fl_key_.p;
+    f.skeys = fl_key_.p + NN;
+    f.skeys_sorted = fl_key_.p + 2 * NN;
+    f.tasks = fl_tasks_.p;
+    f.commit_res = fl_res_.p + NN;
+    f.commit_list = fl_res_.p + 2 * NN;
+    f.counters = fl_counters_.p;
+    return f;
+}
+
 // match_anchor (src/pipeline/incremental_math.cpp:237-272) for a batch of
 // stored subsets. The patches are laid side by side on a canvas as wide as the
 // frame, so the subset solver reads the canvas as its reference plane (the
@@ -1264,18 +1324,23 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
         ck(hipStreamSynchronize(st_));
         // with every node solved or skipped there is nothing to claim, and no global seed to try
         if (open_nodes > 0) {
-            // the node planes and flags, through page-locked staging
-            // (and the accepted flags, solved with a real corr, made on the device
-            // into the valid-flag buffer, which the end of the run rewrites)
+            // the accepted flags (solved with a real corr), made on the device into
+            // the valid-flag buffer, which the end of the run rewrites; the host's
+            // floods read them and the node planes through page-locked staging
             if (!pin_planes_.reserve(7 * NN) || !pin_flags_.reserve(2 * NN)) throw HipFail{hipErrorOutOfMemory};
             float* const hp = pin_planes_.p;
             const uint8_t* const hsolved = pin_flags_.p;
             const uint8_t* const haccepted = pin_flags_.p + NN;
             ck(launch_valid(g, valid_d, N, st_));
-            ck(hipMemcpyAsync(hp, gplanes_.p, sizeof(float) * 7 * NN, hipMemcpyDeviceToHost, st_));
-            ck(hipMemcpyAsync(pin_flags_.p, g.solved, NN, hipMemcpyDeviceToHost, st_));
-            ck(hipMemcpyAsync(pin_flags_.p + NN, valid_d, NN, hipMemcpyDeviceToHost, st_));
-            ck(hipStreamSynchronize(st_));
+            bool fetched = false;
+            const auto fetch = [&]() {
+                if (fetched) return;
+                ck(hipMemcpyAsync(hp, gplanes_.p, sizeof(float) * 7 * NN, hipMemcpyDeviceToHost, st_));
+                ck(hipMemcpyAsync(pin_flags_.p, g.solved, NN, hipMemcpyDeviceToHost, st_));
+                ck(hipMemcpyAsync(pin_flags_.p + NN, valid_d, NN, hipMemcpyDeviceToHost, st_));
+                ck(hipStreamSynchronize(st_));
+                fetched = true;
+            };
             const float* hu = hp;
             const float* hv = hu + NN;
             const float* hux = hv + NN;
@@ -1286,7 +1351,54 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             const int dx4[4] = {1, -1, 0, 0}, dy4[4] = {0, 0, 1, -1};
             bool flooded = false;
             const int reach = layered_path_b > 0 ? layered_path_b : use_cnn ? kLayeredReachLearned : 0;
-            if (reach > 0) {
+            if (reach > 0 && flood_on_device && reach <= kFloodDeviceMaxReach && grid_dims_.empty()) {
+                // Layered Path B with its frontier on the device (flood.hpp): the
+                // same layers, claims and seeds as the host's below, with two
+                // small read-backs per layer (the source and task counts)
+                const FloodDev f = flood_dev(gw, gh);
+                ck(launch_flood_init(f, valid_d, g.solved, gplanes_.p, reach, st_));
+                int32_t* const pc = pin_fl_.p;
+                int pending = 0;   // the last layer's task count, its hand-off count still to read
+                for (int layer = 0;; ++layer) {
+                    if (cancel.requested()) return kErrCancelled;
+                    ck(launch_flood_sources(f, st_));
+                    ck(hipMemcpyAsync(pc, f.counters, sizeof(int32_t) * kFcCount, hipMemcpyDeviceToHost, st_));
+                    ck(hipMemcpyAsync(pc + kFcCount, counters_.p + 8, sizeof(int32_t), hipMemcpyDeviceToHost, st_));
+                    ck(hipStreamSynchronize(st_));
+                    if (pending > 0) launch_log.emplace_back(pending, pc[kFcCount]);
+                    pending = 0;
+                    ticker.set(anchor_pts + st_a.points + pc[kFcPoints]);
+                    const int ns = pc[kFcSources];
+                    if (ns == 0) break;
+                    flooded = true;
+                    ck(launch_flood_sort(f, ns, fl_sort_temp_.p, fl_sort_temp_.n, st_));
+                    ck(launch_flood_search(f, ns, reach, layer + 1, p.rect_x, p.rect_y, p.step, use_cnn, st_));
+                    ck(hipMemcpyAsync(pc, f.counters, sizeof(int32_t) * kFcCount, hipMemcpyDeviceToHost, st_));
+                    ck(hipStreamSynchronize(st_));
+                    const int nt = pc[kFcTasks];
+                    if (nt == 0) break;
+                    if (solve_tasks_device(f.tasks, nt, fl_res_.p, p.use6, p.subset, tol) != 0) return kErrInit;
+                    tm_.path_b_launches++;
+                    tm_.path_b_miss += nt;
+                    ck(launch_flood_update(f, fl_res_.p, nt, st_));
+                    pending = nt;
+                }
+                // one entry per claimed node, holding its last claim
+                ck(launch_flood_commit_list(f, st_));
+                ck(hipMemcpyAsync(pc, f.counters, sizeof(int32_t) * kFcCount, hipMemcpyDeviceToHost, st_));
+                ck(hipStreamSynchronize(st_));
+                st_b.points += pc[kFcPoints];
+                st_b.iters += pc[kFcIters];
+                st_b.calls += pc[kFcCalls];
+                st_b.saved += pc[kFcSaved];
+                st_b.dead += pc[kFcDead];
+                if (pc[kFcCommitted] > 0) {
+                    ck(launch_apply(g, f.commit_nodes, f.commit_list, pc[kFcCommitted], -1, true, st_));
+                    ck(hipStreamSynchronize(st_));
+                }
+                ticker.set(anchor_pts + st_a.points + st_b.points);
+            } else if (reach > 0) {
+                fetch();
                 // Layered Path B. ok: accepted (solved before Path B with a real corr, or
                 // accepted here), w: their warps (the seven planes of hp, updated in
                 // place: the reference flood below runs only when no layer ran); fin: solved,
@@ -1449,6 +1561,7 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             // the reference's flood; the layered one hands over only when no node was
             // accepted before it (the flood's global seeds start it)
             if (!flooded) {
+                fetch();
                 struct SeedNode {
                     int x, y;
                     float u, v, ux, uy, vx, vy, corr;

@@ -18,6 +18,7 @@
 
 #include "common.hpp"
 #include "compose.hpp"
+#include "flood.hpp"
 #include "seeder.hpp"
 #include "solver.hpp"
 
@@ -166,6 +167,10 @@ public:
     // Solve Path B's first claims speculatively in one launch (off: every round
     // is solved when it is reached; the committed field is the same either way).
     bool speculate_path_b = true;
+    // The layered Path B's frontier (sources, search, tasks, updates) on the
+    // device (on by default, for reach <= 16 on one subset size; off: on the
+    // host; the field is the same either way).
+    bool flood_on_device = true;
     // Layered Path B (no reference counterpart; the default in learned
     // seeding, with reach 8): instead of the reference's priority flood, which
     // claims up to 32 nodes per round in corr order, each layer claims at once
@@ -399,6 +404,22 @@ private:
         int epoch = 0;   // stamps below it belong to earlier layers or runs
     };
     FloodScratch flood_;
+    // The device frontier of the layered Path B (flood.hpp): per-node flags,
+    // warps, search state and results, the sort's scratch, the counters.
+    DevBuf<uint8_t> fl_u8_;            // ok, fin, committed (3 NN), tried (NN)
+    DevBuf<float> fl_w_;               // 7 NN
+    DevBuf<int> fl_int_;               // stamp dist cand sources level_a level_b reached tdist commit_nodes (9 NN)
+    DevBuf<unsigned long long> fl_key_;   // key, skeys, skeys_sorted (3 NN)
+    DevBuf<Task> fl_tasks_;
+    DevBuf<SolveOut> fl_res_;          // a layer's results, per-node results, the commit list (3 NN)
+    DevBuf<int> fl_counters_;
+    DevBuf<uint8_t> fl_sort_temp_;
+    PinnedBuf<int32_t> pin_fl_;        // the counters and the hand-off count read back
+    FloodDev flood_dev(int gw, int gh);
+    // The subset solver over tasks already on the device (no host copies):
+    // enqueued on the engine's stream; the hand-off count lands in
+    // counters_[8].
+    int solve_tasks_device(const Task* tasks, int n, SolveOut* out, bool use6, int dim, float tol);
     PinnedBuf<Task> pin_tasks_;        // a task launch's upload
     PinnedBuf<SolveOut> pin_res_;      // and its results
     PinnedBuf<int32_t> pin_handed_;    // and its hand-off count

@@ -167,3 +167,69 @@ def test_learned_frame_repeats_bit_for_bit(env):
         else:
             assert pts.shape == first.shape and np.array_equal(pts.view(np.uint32), first.view(np.uint32))
     dev.sdk_engine_destroy(h)
+
+
+def run_flags(dev, cfg, ref, dfm, prm, flags, seeder=None):
+    h = dev.sdk_engine_create()
+    if seeder is not None:
+        assert dev.sdk_engine_load_seeder(h, seeder) == 0
+    for f, v in flags:
+        assert dev.sdk_engine_set_flag(h, f, v) == 0
+    assert dev.sdk_engine_set_reference(h, P(ref), 0, cfg["w"], cfg["h"], None) == 0
+    cap = (cfg["w"] // cfg["step"] + 1) * (cfg["h"] // cfg["step"] + 1) * 8
+    out = np.zeros(cap, np.float32)
+    met = np.zeros(24, np.float32)
+    tm = np.zeros(14, np.float64)
+    n = dev.sdk_engine_run(h, P(dfm), 0, P(prm), P(out), cap, P(met), 24, ctypes.c_float(cfg["tol"]))
+    dev.sdk_engine_timings(h, P(tm))
+    log = np.zeros(2 * 256, np.int32)
+    nl = dev.sdk_engine_launch_log(h, P(log), 256)
+    dev.sdk_engine_destroy(h)
+    assert n > 0, n
+    return out[: n * 8].copy(), met, tm, log[: 2 * min(nl, 256)].copy()
+
+
+@pytest.mark.parametrize("reach", [1, 4, 16])
+def test_device_frontier_is_the_host_frontier(env, reach):
+    """The layered flood's frontier on the device (flag 18, the default: the
+    sources sorted in the flood's pop order, the breadth-first search's seeds
+    by the FIFO tie-break, the claims and the updates) against the same layers
+    run on the host: the same field bit for bit, the same metrics, the same
+    solver launches with the same work items and hand-offs, on the noisy 1024^2
+    frame where the flood has real work."""
+    dev, wl = env
+    name = "t_1024_noisy"
+    cfg = wl.config(name)
+    ref, dfm = wl.pair(dev, name)
+    prm = np.array(wl.solve_params(name), np.int32)
+    a, ma, ta, la = run_flags(dev, cfg, ref, dfm, prm, [(8, reach), (18, 1)])
+    b, mb, tb, lb = run_flags(dev, cfg, ref, dfm, prm, [(8, reach), (18, 0)])
+    print(f"reach {reach}: {len(a) // 8} points, Path B {int(ma[4])} solved in {int(ta[12])} launches "
+          f"({int(ta[13])} claims)", flush=True)
+    assert ma[4] > 50
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert np.array_equal(ma[[0, 1, 2, 3, 4, 5, 6, 7, 8, 16, 19, 20, 21, 22, 23]],
+                          mb[[0, 1, 2, 3, 4, 5, 6, 7, 8, 16, 19, 20, 21, 22, 23]])
+    assert ta[12] == tb[12] and ta[13] == tb[13]
+    assert np.array_equal(la, lb)
+
+
+def test_device_frontier_on_the_learned_frame(env):
+    """The learned-seeding workload at full size (4096^2 bumps, reach 16 by
+    default under learned seeding): the device frontier's field equals the
+    host frontier's bit for bit, with the same metrics and launches."""
+    dev, wl = env
+    sa = _load("semper_amd", os.path.join(PKG, "semper_amd.py"))
+    name = "c5_4096_bumps"
+    cfg = wl.config(name)
+    ref, dfm = wl.pair(dev, name)
+    prm = np.array(wl.solve_params(name), np.int32)
+    w = sa.SEEDER_WEIGHTS.encode()
+    a, ma, ta, la = run_flags(dev, cfg, ref, dfm, prm, [(5, 1), (18, 1)], seeder=w)
+    b, mb, tb, lb = run_flags(dev, cfg, ref, dfm, prm, [(5, 1), (18, 0)], seeder=w)
+    print(f"learned frame: {len(a) // 8} points, Path B {int(ma[4])} solved in {int(ta[12])} launches, "
+          f"Path B ms {ta[6]:.2f} (device) {tb[6]:.2f} (host)", flush=True)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert np.array_equal(ma[[0, 1, 2, 3, 4, 5, 6, 7, 8]], mb[[0, 1, 2, 3, 4, 5, 6, 7, 8]])
+    assert ta[12] == tb[12] and ta[13] == tb[13]
+    assert np.array_equal(la, lb)
