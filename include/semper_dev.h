@@ -220,6 +220,9 @@ SDK_API int sdk_incremental_graph_stats(struct semper_incremental* inc, int32_t*
    the field fill, the mask warp, the warped mask's bounds and the adoption
    of the frame as the new reference. 0 or -3. */
 SDK_API int sdk_incremental_timings(struct semper_incremental* inc, double* t8);
+/* The session engine's sdk_engine_timings for its last run (the frame's
+   solve against the current reference). 0, or -3 before init. */
+SDK_API int sdk_incremental_engine_timings(struct semper_incremental* inc, double* out14);
 
 /* The tuner's latest timing: chosen2 = {subsets per workgroup, tile side (0:
    row-major)} of least mean time so far, ms6 = the mean Path A launch time

@@ -373,14 +373,17 @@ SDK_API int sdk_engine_set_flag(void* h, int flag, int value) {
     }
 }
 
-SDK_API int sdk_engine_timings(void* h, double* out14) {
-    if (!h) return -3;
-    const Timings& t = ((DevEngine*)h)->eng.timings();
+static void timings14(const Timings& t, double* out14) {
     const double v[14] = {t.total,  t.prep,   t.seeding,       t.prepass,
                           t.mesh,   t.path_a, t.path_b,        t.strain,
                           t.kernel_path_a,    (double)t.path_a_items, (double)t.path_a_iters,
                           (double)t.path_b_spec, (double)t.path_b_launches, (double)t.path_b_miss};
     std::memcpy(out14, v, sizeof(v));
+}
+
+SDK_API int sdk_engine_timings(void* h, double* out14) {
+    if (!h || !out14) return -3;
+    timings14(((DevEngine*)h)->eng.timings(), out14);
     return 0;
 }
 
@@ -617,6 +620,12 @@ SDK_API int sdk_incremental_set_flag(semper_incremental* inc, int flag, int valu
 SDK_API int sdk_incremental_timings(semper_incremental* inc, double* t8) {
     if (!inc || !t8) return -3;
     for (int i = 0; i < 8; ++i) t8[i] = inc->session.timing[i];
+    return 0;
+}
+
+SDK_API int sdk_incremental_engine_timings(semper_incremental* inc, double* out14) {
+    if (!inc || !out14 || !inc->session.engine()) return -3;
+    timings14(inc->session.engine()->timings(), out14);
     return 0;
 }
 

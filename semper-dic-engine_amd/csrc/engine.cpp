@@ -1978,11 +1978,15 @@ int Engine::strain_pack(const Field& f, int window, float* out, int out_capacity
         run_chain(strain_chain_, key, [&] {
             ck(hipMemcpyAsync(u, pin_in_.p, sizeof(float) * 2 * NN, hipMemcpyHostToDevice, st_));
             ck(hipMemcpyAsync(ok0, pin_ok_in_.p, NN, hipMemcpyHostToDevice, st_));
-            ck(launch_zero(counters_.p + 12, 1, st_));
+            // one rejection count per pass: a pass after one that rejected
+            // nothing copies the flags
+            int* pass_fail = counters_.p + 16;
+            ck(launch_zero(pass_fail, kOutlierMaxPasses, st_));
             uint8_t* src = ok0;
             uint8_t* dst = ok1;
             for (int pass = 0; pass < kOutlierMaxPasses; ++pass) {
-                ck(launch_outlier_pass(gw, gh, u, v, src, dst, nullptr, counters_.p + 12, st_));
+                ck(launch_outlier_pass(gw, gh, u, v, src, dst, pass > 0 ? pass_fail + pass - 1 : nullptr,
+                                       pass_fail + pass, st_));
                 src = dst;
                 dst = (dst == ok1) ? ok2 : ok1;
             }
@@ -1990,12 +1994,14 @@ int Engine::strain_pack(const Field& f, int window, float* out, int out_capacity
                           st_));
             ck(hipMemcpyAsync(pin_out_.p, exx, sizeof(float) * 3 * NN, hipMemcpyDeviceToHost, st_));
             ck(hipMemcpyAsync(pin_ok_out_.p, src, NN, hipMemcpyDeviceToHost, st_));
-            ck(hipMemcpyAsync(pin_counts_.p + 12, counters_.p + 12, sizeof(int32_t), hipMemcpyDeviceToHost, st_));
+            ck(hipMemcpyAsync(pin_counts_.p + 16, pass_fail, sizeof(int32_t) * kOutlierMaxPasses,
+                              hipMemcpyDeviceToHost, st_));
         });
         ck(hipStreamSynchronize(st_));
         const float* hs = pin_out_.p;
         const uint8_t* hok = pin_ok_out_.p;
-        const int rej = pin_counts_.p[12];
+        int rej = 0;
+        for (int pass = 0; pass < kOutlierMaxPasses; ++pass) rej += pin_counts_.p[16 + pass];
         int valid = 0, post = 0;
         bool trunc = false;
         for (int y = 0; y < gh && !trunc; ++y)

@@ -72,6 +72,8 @@ public:
     // mask warp, the warped mask's bounds and the adoption of the frame.
     double timing[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     Engine::GraphStats graph_stats() const { return eng_ ? eng_->graph_stats() : Engine::GraphStats{}; }
+    // the session's engine (null before init)
+    const Engine* engine() const { return eng_.get(); }
 
 private:
     std::vector<int> region_map_;

@@ -49,9 +49,9 @@ __global__ void k_prep_ref(const T* __restrict__ img, const uint8_t* __restrict_
     }
 }
 
-// One thread per grid node: skip = inside the boundary buffer d/2 + 4 + 15,
-// or any subset pixel masked (< 128). The kept nodes are counted once per
-// wave (a ballot), not by one same-address atomic per node.
+// One thread per grid node (no mask): skip = inside the boundary buffer
+// d/2 + 4 + 15. The kept nodes are counted once per wave (a ballot), not by
+// one same-address atomic per node.
 __global__ void k_scan(const uint8_t* __restrict__ mask, int w, int h, int rect_x, int rect_y, int step,
                        int gw, int gh, int dim, const int* __restrict__ dims, uint8_t* __restrict__ skip,
                        int* count) {
@@ -62,20 +62,56 @@ __global__ void k_scan(const uint8_t* __restrict__ mask, int w, int h, int rect_
         const int rx = rect_x + (node % gw) * step, ry = rect_y + (node / gw) * step;
         const int half = (dims ? dims[node] : dim) / 2, buf = half + 4 + 15;
         s = rx - buf < 0 || rx + buf >= w || ry - buf < 0 || ry + buf >= h;
-        if (!s && mask) {
-            for (int dy = -half; dy <= half && !s; ++dy) {
-                const uint8_t* row = mask + (int64_t)(ry + dy) * w + rx;
-                for (int dx = -half; dx <= half; ++dx)
-                    if (row[dx] < 128) {
-                        s = true;
-                        break;
-                    }
-            }
-        }
         skip[node] = s ? 1 : 0;
     }
     const uint64_t kept = __ballot(live && !s);
     if ((threadIdx.x & 63) == 0 && kept) atomicAdd(count, (int)__popcll(kept));
+}
+
+// The same with a mask: skip also when any subset pixel is masked (< 128).
+// A workgroup takes 64 consecutive nodes, each of its four waves every
+// fourth of them, and a node's d x d window is read by the wave's 64 lanes
+// together; one count per workgroup.
+constexpr int kScanNodesPerGroup = 64;
+__global__ __launch_bounds__(256) void k_scan_masked(const uint8_t* __restrict__ mask, int w, int h, int rect_x,
+                                                     int rect_y, int step, int gw, int gh, int dim,
+                                                     const int* __restrict__ dims, uint8_t* __restrict__ skip,
+                                                     int* count) {
+    __shared__ int kept_w[4];
+    const int lane = (int)(threadIdx.x & 63), wid = (int)(threadIdx.x >> 6);
+    const int n = gw * gh, base = (int)blockIdx.x * kScanNodesPerGroup;
+    int kept = 0;
+    for (int k = wid; k < kScanNodesPerGroup; k += 4) {
+        const int node = base + k;
+        if (node >= n) break;
+        const int rx = rect_x + (node % gw) * step, ry = rect_y + (node / gw) * step;
+        const int d = dims ? dims[node] : dim, half = d / 2, buf = half + 4 + 15;
+        bool s = rx - buf < 0 || rx + buf >= w || ry - buf < 0 || ry + buf >= h;
+        if (!s) {
+            // pixel i of the window at (q, r) = divmod(i, d), advanced by divmod(64, d)
+            const int q64 = 64 / d, r64 = 64 % d;
+            int q = lane / d, r = lane % d;
+            bool bad = false;
+            for (int i = lane; i < d * d; i += 64) {
+                bad |= mask[(int64_t)(ry + q - half) * w + rx + r - half] < 128;
+                r += r64;
+                q += q64;
+                if (r >= d) {
+                    r -= d;
+                    q++;
+                }
+            }
+            s = __ballot(bad) != 0;
+        }
+        if (lane == 0) skip[node] = s ? 1 : 0;
+        kept += s ? 0 : 1;
+    }
+    if (lane == 0) kept_w[wid] = kept;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int t = kept_w[0] + kept_w[1] + kept_w[2] + kept_w[3];
+        if (t) atomicAdd(count, t);
+    }
 }
 
 // The reference planes a pool pass reads: float gradients, or their fp16
@@ -757,8 +793,13 @@ hipError_t launch_prep_ref_f32(const float* img, int w, int h, float* I, float* 
 hipError_t launch_scan(const uint8_t* mask, int w, int h, int rect_x, int rect_y, int step, int gw, int gh,
                        int dim, uint8_t* skip, int* count, hipStream_t st, const int* dims) {
     const int n = gw * gh;
-    hipLaunchKernelGGL(k_scan, dim3((n + 255) / 256), dim3(256), 0, st, mask, w, h, rect_x, rect_y, step, gw,
-                       gh, dim, dims, skip, count);
+    if (n <= 0) return hipSuccess;
+    if (mask)
+        hipLaunchKernelGGL(k_scan_masked, dim3((n + kScanNodesPerGroup - 1) / kScanNodesPerGroup), dim3(256), 0, st,
+                           mask, w, h, rect_x, rect_y, step, gw, gh, dim, dims, skip, count);
+    else
+        hipLaunchKernelGGL(k_scan, dim3((n + 255) / 256), dim3(256), 0, st, mask, w, h, rect_x, rect_y, step, gw,
+                           gh, dim, dims, skip, count);
     return hipGetLastError();
 }
 

@@ -6,6 +6,8 @@
 //              1-norm rcond, Green-Lagrange strain (strain_calculator.cpp:27-158)
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include "common.hpp"
 #include "solver.hpp"
 
@@ -17,30 +19,76 @@ namespace {
 
 constexpr int kNb = 24;   // 5x5 minus the centre
 
-// Median of the `valid` entries of v[0..kNb), by rank (ties by index), as
-// median_inplace: the middle value, or the mean of the middle pair.
-__device__ __forceinline__ float median_masked(const float (&v)[kNb], uint32_t valid, int k) {
+// Batcher's merge-exchange network (Knuth, TAOCP vol. 3, 5.2.2, Algorithm M)
+// for kNb keys, as a list of compare-exchange pairs fixed at compile time.
+struct SortNet {
+    int n = 0;
+    uint8_t a[160] = {}, b[160] = {};
+};
+constexpr SortNet merge_exchange(int keys) {
+    SortNet s;
+    int t = 0;
+    while ((1 << t) < keys) ++t;
+    for (int p = 1 << (t - 1); p > 0; p >>= 1) {
+        int q = 1 << (t - 1), r = 0, d = p;
+        while (d > 0) {
+            for (int i = 0; i + d < keys; ++i)
+                if ((i & p) == r) {
+                    s.a[s.n] = (uint8_t)i;
+                    s.b[s.n] = (uint8_t)(i + d);
+                    ++s.n;
+                }
+            d = q - p;
+            q >>= 1;
+            r = p;
+        }
+    }
+    return s;
+}
+constexpr SortNet kNet = merge_exchange(kNb);
+static_assert(kNet.n <= 160, "network larger than its table");
+
+// One compare-exchange of the network: its pair is a template argument, so the
+// keys stay in registers.
+template <int C>
+__device__ __forceinline__ void cmp_exchange(float (&v)[kNb]) {
+    constexpr int i = kNet.a[C], j = kNet.b[C];
+    const float lo = fminf(v[i], v[j]), hi = fmaxf(v[i], v[j]);
+    v[i] = lo;
+    v[j] = hi;
+}
+template <int... C>
+__device__ __forceinline__ void sort_net(float (&v)[kNb], std::integer_sequence<int, C...>) {
+    (cmp_exchange<C>(v), ...);
+}
+
+// The median of the `valid` entries of v (median_inplace: the middle value,
+// or the mean of the middle pair). The invalid entries become +inf and the
+// network sorts ascending, so the k valid values lead in order; an order
+// statistic is the same value whichever selection finds it.
+__device__ __forceinline__ float median_masked(float (&v)[kNb], uint32_t valid, int k) {
+#pragma unroll
+    for (int i = 0; i < kNb; ++i)
+        if (!((valid >> i) & 1)) v[i] = __builtin_inff();
+    sort_net(v, std::make_integer_sequence<int, kNet.n>{});
     const int r_hi = k / 2, r_lo = (k - 1) / 2;
     float hi = 0.0f, lo = 0.0f;
 #pragma unroll
     for (int i = 0; i < kNb; ++i) {
-        if (!((valid >> i) & 1)) continue;
-        int r = 0;
-#pragma unroll
-        for (int j = 0; j < kNb; ++j)
-            r += ((valid >> j) & 1) && (v[j] < v[i] || (v[j] == v[i] && j < i));
-        if (r == r_hi) hi = v[i];
-        if (r == r_lo) lo = v[i];
+        hi = (i == r_hi) ? v[i] : hi;
+        lo = (i == r_lo) ? v[i] : lo;
     }
     return (k & 1) ? hi : 0.5f * (lo + hi);
 }
 
 __device__ __forceinline__ float norm_residual(float c, const float (&nb)[kNb], uint32_t valid, int k) {
-    const float med = median_masked(nb, valid, k);
-    float sc[kNb];
+    float w[kNb];
 #pragma unroll
-    for (int i = 0; i < kNb; ++i) sc[i] = fabsf(nb[i] - med);
-    const float spread = median_masked(sc, valid, k);
+    for (int i = 0; i < kNb; ++i) w[i] = nb[i];
+    const float med = median_masked(w, valid, k);
+#pragma unroll
+    for (int i = 0; i < kNb; ++i) w[i] = fabsf(nb[i] - med);
+    const float spread = median_masked(w, valid, k);
     return fabsf(c - med) / (spread + kOutlierEpsPx);
 }
 
@@ -59,25 +107,22 @@ __global__ void k_outlier(int gw, int gh, const float* __restrict__ u, const flo
         return;
     }
     const int x = idx % gw, y = idx / gw;
+    // the 24 neighbours, loaded unconditionally (a slot off the grid reads the
+    // node itself) so that the keys stay in registers; `valid` marks the ones
+    // that count
     float nu[kNb], nv[kNb];
     uint32_t valid = 0;
-    int k = 0, slot = 0;
 #pragma unroll
-    for (int dy = -2; dy <= 2; ++dy)
-#pragma unroll
-        for (int dx = -2; dx <= 2; ++dx) {
-            if (dx == 0 && dy == 0) continue;
-            const int nx = x + dx, ny = y + dy;
-            nu[slot] = 0.0f;
-            nv[slot] = 0.0f;
-            if (nx >= 0 && nx < gw && ny >= 0 && ny < gh && ok_in[ny * gw + nx]) {
-                nu[slot] = u[ny * gw + nx];
-                nv[slot] = v[ny * gw + nx];
-                valid |= (1u << slot);
-                k++;
-            }
-            slot++;
-        }
+    for (int s = 0; s < kNb; ++s) {
+        const int t = s < kNb / 2 ? s : s + 1;   // the 5x5 window, row-major, without its centre
+        const int nx = x + t % 5 - 2, ny = y + t / 5 - 2;
+        const bool in = nx >= 0 && nx < gw && ny >= 0 && ny < gh;
+        const int j = in ? ny * gw + nx : idx;
+        nu[s] = u[j];
+        nv[s] = v[j];
+        if (in && ok_in[j]) valid |= (1u << s);
+    }
+    const int k = __builtin_popcount(valid);
     bool fail = false;
     if (k >= kOutlierMinNeighbours)
         fail = norm_residual(u[idx], nu, valid, k) > kOutlierThreshold ||
