@@ -345,6 +345,31 @@ __device__ __forceinline__ float k4_eval(const K4Rows& k) {
     return val;
 }
 
+// interp<false>'s arithmetic (keys4 weights, products summed in stencil
+// order, no fused multiply-adds) on rows loaded by k4_issue from a u8 or
+// FP32 plane: its value bit for bit (the fraction fract(x) equals
+// x - (int)x for the positive positions inside the guard).
+template <typename TD>
+__device__ __forceinline__ float k4_eval_ref(const K4Rows& k) {
+    float wx[4], wy[4];
+    keys4(k.dx, wx);
+    keys4(k.dy, wy);
+    float val = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        float px[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) px[c] = sizeof(TD) == 1 ? ub(k.r[j][0], c) : __uint_as_float(k.r[j][c]);
+        float rv = 0.0f;
+        rv += px[0] * wx[0];
+        rv += px[1] * wx[1];
+        rv += px[2] * wx[2];
+        rv += px[3] * wx[3];
+        val += rv * wy[j];
+    }
+    return val;
+}
+
 // The warp position of `warped` with the same roundings: a*x is exact in
 // double (a 24-bit by 6-bit product), so fma(a, x, b*y) rounds exactly as
 // a*x + b*y does.
@@ -363,9 +388,7 @@ struct Res {
 
 struct WaveLds {
     float H0[36];     // the pool's step matrix
-    float P[7 * 6];   // Simplex vertices
-    float Y[7];
-    int idx[7];
+    float P[7 * 6];   // Simplex vertices (vertex k, coordinate j at 6 k + j)
 };
 
 // Sums over the NW waves that solve one subset together (the full solver):
@@ -451,6 +474,12 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
     const int half = dim / 2;
     if (t.cx - half < 0 || t.cx + half >= w || t.cy - half < 0 || t.cy + half >= h) return;
     const int n = dim * dim;
+    // the deformed plane as a buffer resource (the Keys 4x4 gathers)
+    const __amdgpu_buffer_rsrc_t drs =
+        __builtin_amdgcn_make_buffer_rsrc((void*)def, (short)0, (int)((size_t)w * h * sizeof(TD)), 0x00020000);
+    // slots whose gathers are issued together: the raw rows of a u8 sample
+    // take 4 registers, of an FP32 sample 16
+    constexpr int kSlowBatch = sizeof(TD) == 1 ? (SL < 4 ? SL : 4) : (SL < 2 ? SL : 2);
 
     // ---- prologue: the subset in registers --------------------------------
     float nref[SL], gxs[SL], gys[SL], val[SL];
@@ -581,9 +610,26 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
     float W[9];
     int it = 0;
     float final_score = 1.0f;
-    float q[6];             // point under evaluation (Simplex)
-    float pb[6], pr[6], pe_[6], pw[6];
+    // The Simplex state, the same in every wave, held lane-wise: coordinate
+    // j of a point on lane j (j < 6) -- the point under evaluation qv, the
+    // centroid, the reflected, the worst and the expanded or contracted
+    // point; the vertices in the wave's LDS copy (vertex k, coordinate j at
+    // L.P[6 k + j], each lane reading and writing only its own coordinate);
+    // and the score of vertex k on lane k (Yl), so that the ranks of all
+    // vertices come from one pass of lane-parallel compares.
+    float Yl = 0.0f;
+    float qv = 0.0f, pbv = 0.0f, prv = 0.0f, pev = 0.0f, pwv = 0.0f;
     float yr = 0.0f, yw = 0.0f, y0 = 0.0f, y2 = 0.0f;
+    const int pj = lane < 6 ? lane : 0;   // this lane's coordinate slot
+    const auto ldP = [&](int k) -> float { return L.P[k * 6 + pj]; };
+    const auto stP = [&](int k, float v) {
+        if (lane < 6) L.P[k * 6 + pj] = v;
+    };
+    const auto lane_f = [&](float v, int j) -> float {
+        return __uint_as_float((uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(v), j));
+    };
+    const auto setY = [&](int k, float v) { Yl = (lane == k) ? v : Yl; };
+
     Res first = {}, res = {};
     int trig = 0;
 
@@ -604,13 +650,17 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
     auto begin_simplex = [&](const float* s0, int dims) {
         D = dims;
         np = dims + 1;
-        for (int k = 0; k < 42; ++k) L.P[k] = 0.0f;
-        for (int j = 0; j < 6; ++j) L.P[j] = (j < D) ? s0[j] : 0.0f;
+        float s0v = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) s0v = (lane == j) ? s0[j] : s0v;
+        const float p0 = (lane < D) ? s0v : 0.0f;
+#pragma unroll
+        for (int k = 1; k < 7; ++k) stP(k, 0.0f);
+        stP(0, p0);
         s_stage = S_INIT;
         s_k = 0;
         s_iter = 0;
-#pragma unroll
-        for (int j = 0; j < 6; ++j) q[j] = L.P[j];
+        qv = p0;
         phase = PH_SIMPLEX;
         simplex = 1;
     };
@@ -623,48 +673,55 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
 
     // stable ascending order of the vertex scores: rank of each vertex
     auto simplex_iter_begin = [&]() -> bool {
-        int* idx = L.idx;
-        for (int i = 0; i < np; ++i) {
-            int r = 0;
-            for (int k = 0; k < np; ++k) r += (L.Y[k] < L.Y[i]) || (L.Y[k] == L.Y[i] && k < i);
-            idx[r] = i;
+        // rank of vertex `lane` among the np (ties by index), then the vertex of each rank
+        int rank = 0;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            const float yk = lane_f(Yl, k);
+            rank += (k < np) && ((yk < Yl) || (yk == Yl && k < lane));
         }
-        s_best = idx[0];
-        s_worst = idx[np - 1];
-        y0 = L.Y[idx[0]];
-        yw = L.Y[s_worst];
-        y2 = L.Y[idx[np - 2]];
+        const auto of_rank = [&](int r) -> int {
+            return (int)__builtin_ctzll(__builtin_amdgcn_ballot_w64(lane < np && rank == r));
+        };
+        s_best = of_rank(0);
+        s_worst = of_rank(np - 1);
+        y0 = lane_f(Yl, s_best);
+        yw = lane_f(Yl, s_worst);
+        y2 = lane_f(Yl, of_rank(np - 2));
         if (fabsf(y0 - yw) < 1e-5f) return false;
-#pragma unroll
-        for (int j = 0; j < 6; ++j) pb[j] = 0.0f;
-        for (int i = 0; i < D; ++i)
-#pragma unroll
-            for (int j = 0; j < 6; ++j)
-                if (j < D) pb[j] += L.P[idx[i] * 6 + j];
-#pragma unroll
-        for (int j = 0; j < 6; ++j) {
-            if (j < D) pb[j] /= (float)D;
-            pw[j] = L.P[s_worst * 6 + j];
-            pr[j] = (j < D) ? pb[j] + 1.0f * (pb[j] - pw[j]) : 0.0f;
-            q[j] = pr[j];
+        // the centroid of all but the worst, summed in rank order
+        const bool in_dim = lane < D;
+        pbv = 0.0f;
+        for (int r = 0; r < D; ++r) {
+            const float v = ldP(of_rank(r));
+            if (in_dim) pbv += v;
         }
+        if (in_dim) pbv /= (float)D;
+        pwv = ldP(s_worst);
+        prv = in_dim ? pbv + 1.0f * (pbv - pwv) : 0.0f;
+        qv = prv;
         s_stage = S_REFLECT;
         return true;
     };
-    auto replace_worst = [&](const float* pt, float yv) {
-#pragma unroll
-        for (int j = 0; j < 6; ++j)
-            if (j < D) L.P[s_worst * 6 + j] = pt[j];
-        L.Y[s_worst] = yv;
+    auto replace_worst = [&](float ptv, float yv) {
+        if (lane < D) stP(s_worst, ptv);
+        setY(s_worst, yv);
     };
     // Simplex done: best vertex, then the IC-GN run it seeds.
     auto finish_simplex = [&]() {
         int best = 0;
-        for (int k = 1; k < np; ++k)
-            if (L.Y[k] < L.Y[best]) best = k;
+        float yb = lane_f(Yl, 0);
+        for (int k = 1; k < np; ++k) {
+            const float yk = lane_f(Yl, k);
+            if (yk < yb) {
+                best = k;
+                yb = yk;
+            }
+        }
+        const float bv = ldP(best);
         float p[6];
 #pragma unroll
-        for (int j = 0; j < 6; ++j) p[j] = (j < D) ? L.P[best * 6 + j] : 0.0f;
+        for (int j = 0; j < 6; ++j) p[j] = (j < D) ? lane_f(bv, j) : 0.0f;
         begin_icgn(p);
     };
     // one IC-GN run ended with r
@@ -707,6 +764,9 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
 #pragma unroll
             for (int k = 0; k < 6; ++k) Wc[k] = W[k];
         } else {
+            float q[6];
+#pragma unroll
+            for (int j = 0; j < 6; ++j) q[j] = lane_f(qv, j);
             Wc[0] = 1.0f + q[2];
             Wc[1] = q[3];
             Wc[2] = q[0];
@@ -725,28 +785,73 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
             // loop and the register file, not the ALUs, becomes the limit.
             int qq = q0, rq = r0;
             asm volatile("" : "+v"(qq), "+v"(rq));
+            if constexpr (!K6) {
+                // Keys 4x4 in two phases (k4_issue, then the reference's
+                // arithmetic on the loaded rows), SB slots' gathers in flight
+                // at once; a slot off the subset or outside the guard loads
+                // the subset's centre and keeps a zero.
 #pragma unroll
-            for (int j = 0; j < SL; ++j) {
-                val[j] = 0.0f;
-                if ((inmask >> j) & 1) {
-                    const float x = (float)(rq - half), y = (float)(qq - half);
-                    const float fx = warped(t.cx, Wc[0], Wc[1], Wc[2], x, y);
-                    const float fy = warped(t.cy, Wc[3], Wc[4], Wc[5], x, y);
-                    if (inside_guard(fx, fy, w, h)) {
-                        const float v = interp<K6>(def, w, fx, fy);
-                        val[j] = v;
-                        if ((refok >> j) & 1) {
-                            ok |= (1ull << j);
-                            sum += v;
-                            cnt++;
+                for (int j0 = 0; j0 < SL; j0 += kSlowBatch) {
+                    K4Rows kr[kSlowBatch];
+                    bool inb[kSlowBatch];
+#pragma unroll
+                    for (int b = 0; b < kSlowBatch; ++b) {
+                        const int j = j0 + b;
+                        inb[b] = false;
+                        if (j < SL) {
+                            const float x = (float)(rq - half), y = (float)(qq - half);
+                            const float fx = warped(t.cx, Wc[0], Wc[1], Wc[2], x, y);
+                            const float fy = warped(t.cy, Wc[3], Wc[4], Wc[5], x, y);
+                            inb[b] = ((inmask >> j) & 1) && inside_guard(fx, fy, w, h);
+                            k4_issue<TD, false>(drs, w, inb[b] ? fx : (float)t.cx, inb[b] ? fy : (float)t.cy, kr[b]);
+                            rq += r64;
+                            qq += q64;
+                            if (rq >= dim) {
+                                rq -= dim;
+                                qq++;
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int b = 0; b < kSlowBatch; ++b) {
+                        const int j = j0 + b;
+                        if (j >= SL) continue;
+                        val[j] = 0.0f;
+                        if (inb[b]) {
+                            const float v = k4_eval_ref<TD>(kr[b]);
+                            val[j] = v;
+                            if ((refok >> j) & 1) {
+                                ok |= (1ull << j);
+                                sum += v;
+                                cnt++;
+                            }
                         }
                     }
                 }
-                rq += r64;
-                qq += q64;
-                if (rq >= dim) {
-                    rq -= dim;
-                    qq++;
+            } else {
+#pragma unroll
+                for (int j = 0; j < SL; ++j) {
+                    val[j] = 0.0f;
+                    if ((inmask >> j) & 1) {
+                        const float x = (float)(rq - half), y = (float)(qq - half);
+                        const float fx = warped(t.cx, Wc[0], Wc[1], Wc[2], x, y);
+                        const float fy = warped(t.cy, Wc[3], Wc[4], Wc[5], x, y);
+                        if (inside_guard(fx, fy, w, h)) {
+                            const float v = interp<K6>(def, w, fx, fy);
+                            val[j] = v;
+                            if ((refok >> j) & 1) {
+                                ok |= (1ull << j);
+                                sum += v;
+                                cnt++;
+                            }
+                        }
+                    }
+                    rq += r64;
+                    qq += q64;
+                    if (rq >= dim) {
+                        rq -= dim;
+                        qq++;
+                    }
                 }
             }
         }
@@ -785,72 +890,64 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
             }
             // ---- Nelder-Mead step (:531-651) --------------------------------
             bool next_iter = false;
+            const bool in_dim = lane < D;
             if (s_stage == S_INIT) {
-                L.Y[s_k] = score;
+                setY(s_k, score);
                 s_k++;
                 if (s_k < np) {
-#pragma unroll
-                    for (int j = 0; j < 6; ++j) {
-                        float v = L.P[j];
-                        if (j == s_k - 1) v += (j < 2) ? 2.0f : 0.01f;
-                        L.P[s_k * 6 + j] = (j < D) ? v : 0.0f;
-                        q[j] = L.P[s_k * 6 + j];
-                    }
+                    // vertex s_k: vertex 0 with coordinate s_k - 1 moved by its scale
+                    float v = ldP(0);
+                    if (lane == s_k - 1) v += (lane < 2) ? 2.0f : 0.01f;
+                    const float pk = in_dim ? v : 0.0f;
+                    stP(s_k, pk);
+                    qv = pk;
                 } else {
                     if (!simplex_iter_begin()) finish_simplex();
                 }
             } else if (s_stage == S_REFLECT) {
                 yr = score;
                 if (y0 <= yr && yr < y2) {
-                    replace_worst(pr, yr);
+                    replace_worst(prv, yr);
                     next_iter = true;
                 } else if (yr < y0) {
-#pragma unroll
-                    for (int j = 0; j < 6; ++j) {
-                        pe_[j] = (j < D) ? pb[j] + 2.0f * (pr[j] - pb[j]) : 0.0f;
-                        q[j] = pe_[j];
-                    }
+                    pev = in_dim ? pbv + 2.0f * (prv - pbv) : 0.0f;
+                    qv = pev;
                     s_stage = S_EXPAND;
                 } else {
                     const bool outside = yr < yw;
-#pragma unroll
-                    for (int j = 0; j < 6; ++j) {
-                        pe_[j] = (j < D) ? pb[j] + 0.5f * ((outside ? pr[j] : pw[j]) - pb[j]) : 0.0f;
-                        q[j] = pe_[j];
-                    }
+                    pev = in_dim ? pbv + 0.5f * ((outside ? prv : pwv) - pbv) : 0.0f;
+                    qv = pev;
                     s_stage = S_CONTRACT;
                 }
             } else if (s_stage == S_EXPAND) {
-                if (score < yr) replace_worst(pe_, score);
-                else replace_worst(pr, yr);
+                if (score < yr) replace_worst(pev, score);
+                else replace_worst(prv, yr);
                 next_iter = true;
             } else if (s_stage == S_CONTRACT) {
                 if (score < fminf(yr, yw)) {
-                    replace_worst(pe_, score);
+                    replace_worst(pev, score);
                     next_iter = true;
                 } else {
                     // shrink every vertex but the best towards it, one evaluation each
                     s_k = (s_best == 0) ? 1 : 0;
-#pragma unroll
-                    for (int j = 0; j < 6; ++j) {
-                        const float b = L.P[s_best * 6 + j];
-                        if (j < D) L.P[s_k * 6 + j] = b + 0.5f * (L.P[s_k * 6 + j] - b);
-                        q[j] = L.P[s_k * 6 + j];
-                    }
+                    const float b = ldP(s_best);
+                    float pk = ldP(s_k);
+                    if (in_dim) pk = b + 0.5f * (pk - b);
+                    stP(s_k, pk);
+                    qv = pk;
                     s_stage = S_SHRINK;
                 }
             } else {  // S_SHRINK
-                L.Y[s_k] = score;
+                setY(s_k, score);
                 int nk = s_k + 1;
                 if (nk == s_best) nk++;
                 if (nk < np) {
                     s_k = nk;
-#pragma unroll
-                    for (int j = 0; j < 6; ++j) {
-                        const float b = L.P[s_best * 6 + j];
-                        if (j < D) L.P[s_k * 6 + j] = b + 0.5f * (L.P[s_k * 6 + j] - b);
-                        q[j] = L.P[s_k * 6 + j];
-                    }
+                    const float b = ldP(s_best);
+                    float pk = ldP(s_k);
+                    if (in_dim) pk = b + 0.5f * (pk - b);
+                    stP(s_k, pk);
+                    qv = pk;
                 } else {
                     next_iter = true;
                 }
@@ -1500,7 +1597,7 @@ __global__ __launch_bounds__(256) SDIC_FAST_ATTR void k_fast(SolveArgs a) {
 // striding over the list (its length is read on the device, so no host round
 // trip).
 template <int PPL, bool K6, typename TD, int NW>
-__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2))) void k_slow(SolveArgs a) {
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1))) void k_slow(SolveArgs a) {
     // each wave keeps the solver's uniform state in its own copy; the waves
     // meet only in the group sums
     __shared__ WaveLds Ls[NW];
