@@ -1019,30 +1019,26 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
         float d[6];
         if (masked) {
             group_sum_n<NW, 21>(hh, red, par);
-            // masked Hessian (:373-413): 2x2 test, LM, rcond, inverse. The sums
-            // are wave-uniform, so every lane runs the same sequence in its own
-            // registers (the pool precompute's, hence the same bits).
-            float Hd[36], Hm[36];
+            // masked Hessian (:373-413): 2x2 test, LM, rcond, inverse, with the
+            // 6x6 spread one entry per lane (lane 6 r + c) and the same
+            // operations per entry as the pool precompute's register routines
+            const int hr = lane / 6, hc = lane - 6 * (lane / 6);
+            float hL;
             {
-                int k = 0;
+                // entry (r, c) of the symmetric matrix is sum k of its upper triangle
+                const int a0 = hr < hc ? hr : hc, b0 = hr < hc ? hc : hr;
+                const int tk = a0 * 6 - (a0 * (a0 - 1)) / 2 + (b0 - a0);
+                hL = hh[0];
 #pragma unroll
-                for (int a = 0; a < 6; ++a)
-#pragma unroll
-                    for (int b = a; b < 6; ++b) {
-                        Hd[a * 6 + b] = hh[k];
-                        Hd[b * 6 + a] = hh[k];
-                        ++k;
-                    }
+                for (int k = 1; k < 21; ++k) hL = (tk == k) ? hh[k] : hL;
             }
-            const float det2 = Hd[0] * Hd[7] - Hd[6] * Hd[1];
-            const float nrm2 = Hd[0] * Hd[0] + Hd[1] * Hd[1] + Hd[6] * Hd[6] + Hd[7] * Hd[7];
+            const float det2 = hh[0] * hh[6] - hh[1] * hh[1];
+            const float nrm2 = hh[0] * hh[0] + hh[1] * hh[1] + hh[1] * hh[1] + hh[6] * hh[6];
             const float cond2 = (fabsf(det2) > 1e-12f) ? (nrm2 / fabsf(det2)) : 1.0e13f;
             bool flag = cond2 > 1.0e12f;
+            float hmv = 0.0f;   // row `lane` of the step matrix times dp, on lanes 0..5
             if (!flag) {
-                if (lm) {
-                    Hd[0] += kLmAlpha;
-                    Hd[7] += kLmAlpha;
-                }
+                if (lm && (lane == 0 || lane == 7)) hL += kLmAlpha;
                 if constexpr (NW > 1) {
                     // the rcond test on wave 0 and the inverse on wave 1 at once
                     // (the same sequences, so the same bits); every wave takes
@@ -1051,22 +1047,33 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
                     float* xch = red + 2 * NW * kGroupSlots;
                     const int wid = (int)(threadIdx.x >> 6);
                     if (wid == 0) {
-                        const bool wc = well_conditioned_reg(Hd, half);
+                        const bool wc = well_conditioned_lanes(hL, half, lane);
                         if (lane == 0) xch[36] = wc ? 1.0f : 0.0f;
                     } else if (wid == 1) {
-                        gj_inverse_reg(Hd, Hm);
-                        if (lane == 0) {
-#pragma unroll
-                            for (int q = 0; q < 36; ++q) xch[q] = Hm[q];
-                        }
+                        float mL = hL, mR = (hr == hc) ? 1.0f : 0.0f;
+                        gj_inverse_lanes(mL, mR, lane);
+                        if (lane < 36) xch[lane] = mR;
                     }
                     __syncthreads();
                     flag = xch[36] == 0.0f;
+                    if (!flag) {
+                        const int rr = lane < 6 ? lane : 0;
+                        float acc = 0.0f;
 #pragma unroll
-                    for (int q = 0; q < 36; ++q) Hm[q] = xch[q];
+                        for (int c = 0; c < 6; ++c) acc += xch[rr * 6 + c] * dp[c];
+                        hmv = -acc;
+                    }
                 } else {
-                    flag = !well_conditioned_reg(Hd, half);
-                    if (!flag) gj_inverse_reg(Hd, Hm);
+                    flag = !well_conditioned_lanes(hL, half, lane);
+                    if (!flag) {
+                        float mL = hL, mR = (hr == hc) ? 1.0f : 0.0f;
+                        gj_inverse_lanes(mL, mR, lane);
+                        const int rr = lane < 6 ? lane : 0;
+                        float acc = 0.0f;
+#pragma unroll
+                        for (int c = 0; c < 6; ++c) acc += lane_fetch(mR, rr * 6 + c) * dp[c];
+                        hmv = -acc;
+                    }
                 }
             }
             if (flag) {
@@ -1075,12 +1082,7 @@ __device__ __forceinline__ void full_solve(const Task& t, const PoolEntry* __res
             }
             final_score = err / (float)nv;
 #pragma unroll
-            for (int r = 0; r < 6; ++r) {
-                float acc = 0.0f;
-#pragma unroll
-                for (int c = 0; c < 6; ++c) acc += Hm[r * 6 + c] * dp[c];
-                d[r] = -acc;
-            }
+            for (int r = 0; r < 6; ++r) d[r] = lane_read(hmv, r);
         } else {
             final_score = err / (float)nv;
             if (use_chol) {
@@ -1595,9 +1597,11 @@ __global__ __launch_bounds__(256) SDIC_FAST_ATTR void k_fast(SolveArgs a) {
 
 // Full solver over the bail list: a fixed grid of NW-wave workgroups, each
 // striding over the list (its length is read on the device, so no host round
-// trip).
+// trip). Two waves per SIMD up to 32 pixels per lane (a list of a few hundred
+// subsets is solved in one round); the two largest classes take one, with
+// AGPRs as spill room instead of scratch.
 template <int PPL, bool K6, typename TD, int NW>
-__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(1))) void k_slow(SolveArgs a) {
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(PPL <= 32 ? 2 : 1))) void k_slow(SolveArgs a) {
     // each wave keeps the solver's uniform state in its own copy; the waves
     // meet only in the group sums
     __shared__ WaveLds Ls[NW];

@@ -196,6 +196,85 @@ __device__ __forceinline__ bool well_conditioned_reg(const float* H, int half) {
     return 1.0f / (a * b) > 1e-6f;
 }
 
+// ---- the same two routines with the matrix spread over a wave --------------
+// Lane l = 6 r + c (l < 36) holds entry (r, c) of a 6x6 in `mL` and, for the
+// Gauss-Jordan sweep, entry (r, 6 + c) of the augmented half in `mR`; lanes
+// 36..63 carry values nobody reads. Each entry sees exactly the operations of
+// the register routines above, in their order, so the results are theirs bit
+// for bit; a pivot step costs a few cross-lane moves instead of a sweep over
+// 72 registers in every lane.
+__device__ __forceinline__ float lane_read(float v, int l) {
+    return __uint_as_float((uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(v), l));
+}
+__device__ __forceinline__ float lane_fetch(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v)));
+}
+
+// gj_inverse_reg on (mL | mR) = (A | I): the inverse ends in mR.
+__device__ __forceinline__ void gj_inverse_lanes(float& mL, float& mR, int lane) {
+    const int r = lane / 6, c = lane - 6 * r;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        int piv = k;
+        float best = fabsf(lane_read(mL, 6 * k + k));
+#pragma unroll
+        for (int q = k + 1; q < 6; ++q) {
+            const float v = fabsf(lane_read(mL, 6 * q + k));
+            if (v > best) {
+                best = v;
+                piv = q;
+            }
+        }
+        if (piv != k) {
+            const int src = (r == k) ? 6 * piv + c : (r == piv) ? 6 * k + c : lane;
+            mL = lane_fetch(mL, src);
+            mR = lane_fetch(mR, src);
+        }
+        const float ip = 1.0f / lane_read(mL, 6 * k + k);
+        if (r == k) {
+            mL *= ip;
+            mR *= ip;
+        }
+        // this row's factor m[r][k] and the pivot row's entries of this column
+        const float f = lane_fetch(mL, 6 * r + k);
+        const float kL = lane_fetch(mL, 6 * k + c), kR = lane_fetch(mR, 6 * k + c);
+        if (r != k) {
+            mL = (f == 0.0f) ? mL : mL - f * kL;
+            mR = (f == 0.0f) ? mR : mR - f * kR;
+        }
+    }
+}
+
+// norm1_reg of the matrix in v: column sums on lanes 0..5 (rows in order),
+// then their maximum in column order (uniform).
+__device__ __forceinline__ float norm1_lanes(float v, int lane) {
+    const int c = lane < 6 ? lane : 0;
+    float s = 0.0f;
+#pragma unroll
+    for (int r = 0; r < 6; ++r) s += fabsf(lane_fetch(v, 6 * r + c));
+    float best = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const float sq = lane_read(s, q);
+        best = sq > best ? sq : best;
+    }
+    return best;
+}
+
+// well_conditioned_reg of the matrix in hL.
+__device__ __forceinline__ bool well_conditioned_lanes(float hL, int half, int lane) {
+    const int r = lane / 6, c = lane - 6 * r;
+    const float s = half > 1 ? 1.0f / (float)half : 1.0f;
+    const float dr = r < 2 ? 1.0f : s, dc = c < 2 ? 1.0f : s;
+    const float hn = dr * hL * dc;
+    if (__builtin_amdgcn_ballot_w64(lane < 36 && !finite_f(hn)) != 0) return false;
+    float mL = hn, mR = (r == c) ? 1.0f : 0.0f;
+    gj_inverse_lanes(mL, mR, lane);
+    if (__builtin_amdgcn_ballot_w64(lane < 36 && !finite_f(mR)) != 0) return false;
+    const float a = norm1_lanes(hn, lane), b = norm1_lanes(mR, lane);
+    return 1.0f / (a * b) > 1e-6f;
+}
+
 // FP64 Cholesky factor of a symmetric positive definite 6x6 (float input,
 // converted exactly): L row-major lower triangle at i (i + 1) / 2 + j, its
 // diagonal stored as the reciprocal. False when a pivot is not positive.
