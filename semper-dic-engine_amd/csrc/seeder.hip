@@ -69,12 +69,21 @@ __device__ inline float bicubic_u8(const uint8_t* __restrict__ img, int w, int h
 }
 
 // One training pair per sample n: a random level-2 centre c2, a random affine
-// warp about the patch centre P = 4 c2 - 0.5 (full resolution), the reference
-// patch as the engine's pyramid makes it (two 2x2 box averages) and the
-// deformed patch from the warped, noisy, u8-rounded full-resolution samples.
-// Targets are scaled (u / kUvScale, gradients / kGradScale).
+// warp about the patch centre P = 4 c2 - 0.5 (full resolution) and, on a
+// `het_frac` share of the pairs, a smooth heterogeneous field on top of it --
+// one Gaussian bump per displacement component, of amplitude up to
+// `bump_amp` px and width `bump_min` .. `bump_max` px, centred within one
+// width of P -- the reference patch as the engine's pyramid makes it (two
+// 2x2 box averages) and the deformed patch from the warped, noisy,
+// u8-rounded full-resolution samples. The target is the field's first-order
+// warp at P (the affine's plus the bumps' values and gradients there),
+// scaled (u / kUvScale, gradients / kGradScale).
+struct PairBumps {
+    float het_frac, amp, wmin, wmax;
+};
 __global__ void k_pairs(const uint8_t* __restrict__ ref, int w, int h, int B, uint64_t seed, uint64_t step,
-                        float max_shift, float max_grad, float noise, float* __restrict__ x, float* __restrict__ t) {
+                        float max_shift, float max_grad, float noise, PairBumps bm, float* __restrict__ x,
+                        float* __restrict__ t) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= B * S::kPatch * S::kPatch) return;
     const int n = i / (S::kPatch * S::kPatch), px = i % S::kPatch, py = (i / S::kPatch) % S::kPatch;
@@ -91,6 +100,18 @@ __global__ void k_pairs(const uint8_t* __restrict__ ref, int w, int h, int B, ui
     const float i00 = a11 / det, i01 = -a01 / det, i10 = -a10 / det, i11 = a00 / det;
     const float Px = 4.0f * c2x - 0.5f, Py = 4.0f * c2y - 0.5f;
     const int fx0 = 4 * (c2x - S::kPatch / 2 + px), fy0 = 4 * (c2y - S::kPatch / 2 + py);
+    // the bumps: h_u(X) = au g(X - Cu), h_v(X) = av g(X - Cv), g = exp(-|.|^2 / 2 s^2)
+    const bool het = rnd(seed, id, 8) < bm.het_frac;
+    const float au = het ? (2.0f * rnd(seed, id, 9) - 1.0f) * bm.amp : 0.0f;
+    const float av = het ? (2.0f * rnd(seed, id, 10) - 1.0f) * bm.amp : 0.0f;
+    const float su = bm.wmin + rnd(seed, id, 11) * (bm.wmax - bm.wmin);
+    const float sv = bm.wmin + rnd(seed, id, 12) * (bm.wmax - bm.wmin);
+    const float cux = Px + (2.0f * rnd(seed, id, 13) - 1.0f) * su, cuy = Py + (2.0f * rnd(seed, id, 14) - 1.0f) * su;
+    const float cvx = Px + (2.0f * rnd(seed, id, 15) - 1.0f) * sv, cvy = Py + (2.0f * rnd(seed, id, 16) - 1.0f) * sv;
+    const auto bump = [&](float a, float sg, float cx, float cy, float X, float Y) {
+        const float ex = X - cx, ey = Y - cy;
+        return a * expf(-(ex * ex + ey * ey) / (2.0f * sg * sg));
+    };
     // reference: the pyramid's two box levels
     float l1[4];
 #pragma unroll
@@ -108,8 +129,20 @@ __global__ void k_pairs(const uint8_t* __restrict__ ref, int w, int h, int B, ui
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int xx = fx0 + 2 * (q & 1) + (r & 1), yy = fy0 + 2 * (q >> 1) + (r >> 1);
-            const float dx = (float)xx - Px - tu, dy = (float)yy - Py - tv;
-            const float X = Px + i00 * dx + i01 * dy, Y = Py + i10 * dx + i11 * dy;
+            // X with x = X + t + G (X - P) + h(X): the affine inverse, then
+            // fixed-point steps on the bumps (a contraction: their gradients
+            // stay far below one)
+            float X = 0.0f, Y = 0.0f, hu = 0.0f, hv = 0.0f;
+#pragma unroll 1
+            for (int itx = 0; itx < (het ? 6 : 1); ++itx) {
+                const float dx = (float)xx - Px - tu - hu, dy = (float)yy - Py - tv - hv;
+                X = Px + i00 * dx + i01 * dy;
+                Y = Py + i10 * dx + i11 * dy;
+                if (het) {
+                    hu = bump(au, su, cux, cuy, X, Y);
+                    hv = bump(av, sv, cvx, cvy, X, Y);
+                }
+            }
             float v = bicubic_u8(ref, w, h, X, Y) + noise * gauss(seed ^ 0x5EEDull, id, (uint64_t)(yy * 8191 + xx));
             v = fminf(fmaxf(rintf(v), 0.0f), 255.0f);
             s4[r] = v;
@@ -121,13 +154,17 @@ __global__ void k_pairs(const uint8_t* __restrict__ ref, int w, int h, int B, ui
     x[base] = rv;
     x[base + S::kPatch * S::kPatch] = dv;
     if (px == 0 && py == 0) {
+        // the field's value and gradient at P
+        const float hu0 = bump(au, su, cux, cuy, Px, Py), hv0 = bump(av, sv, cvx, cvy, Px, Py);
+        const float hux = -hu0 * (Px - cux) / (su * su), huy = -hu0 * (Py - cuy) / (su * su);
+        const float hvx = -hv0 * (Px - cvx) / (sv * sv), hvy = -hv0 * (Py - cvy) / (sv * sv);
         float* tt = t + (size_t)n * S::kOut;
-        tt[0] = tu / S::kUvScale;
-        tt[1] = tv / S::kUvScale;
-        tt[2] = g00 / S::kGradScale;
-        tt[3] = g01 / S::kGradScale;
-        tt[4] = g10 / S::kGradScale;
-        tt[5] = g11 / S::kGradScale;
+        tt[0] = (tu + hu0) / S::kUvScale;
+        tt[1] = (tv + hv0) / S::kUvScale;
+        tt[2] = (g00 + hux) / S::kGradScale;
+        tt[3] = (g01 + huy) / S::kGradScale;
+        tt[4] = (g10 + hvx) / S::kGradScale;
+        tt[5] = (g11 + hvy) / S::kGradScale;
     }
 }
 
@@ -743,7 +780,8 @@ int Seeder::train(const uint8_t* ref, int w, int h, const SeederTrainConfig& cfg
         double p1 = 1.0, p2 = 1.0;
         for (int s = 0; s < cfg.steps; ++s) {
             hipLaunchKernelGGL(k_pairs, dim3(blocks((size_t)B * 4096, 256)), dim3(256), 0, d.st, ref, w, h, B, cfg.seed,
-                               (uint64_t)s, cfg.max_shift, cfg.max_grad, cfg.noise, d.x, d.t);
+                               (uint64_t)s, cfg.max_shift, cfg.max_grad, cfg.noise,
+                               PairBumps{cfg.het_frac, cfg.bump_amp, cfg.bump_min, cfg.bump_max}, d.x, d.t);
             hipLaunchKernelGGL(k_normalize, dim3(2 * B), dim3(256), 0, d.st, d.x);
             d.forward(B);
             d.backward(B);

@@ -48,6 +48,11 @@ struct SeederTrainConfig {
     float max_shift = 28.0f;      // full-resolution px, per component
     float max_grad = 0.12f;
     float noise = 4.0f;           // grey levels on the deformed samples
+    // smooth heterogeneous fields on top of the affine: the share of pairs
+    // with them, the bump amplitude (px, per component) and width range (px)
+    float het_frac = 0.5f;
+    float bump_amp = 8.0f;
+    float bump_min = 100.0f, bump_max = 400.0f;
     uint64_t seed = 1;
 };
 

@@ -5,10 +5,13 @@
 
 The pairs are the engine's own: a 2048 x 2048 reference of its synthetic
 speckle (the workloads' statistics, noise 4 grey levels), cut into patches at
-pyramid level 2 and warped by random affine maps (shift up to 28 px per
-component, displacement gradients up to 0.12) through a Keys bicubic
-interpolator, with fresh noise on the deformed samples. 8000 steps of 256 are
-2.05 M pairs. The forward and backward passes, the loss and Adam are the
+pyramid level 2 and warped through a Keys bicubic interpolator by random
+affine maps (shift up to 28 px per component, displacement gradients up to
+0.12), on half of the pairs plus a smooth heterogeneous field (a Gaussian
+bump per displacement component, up to 8 px, 100-400 px wide, centred
+within one width of the patch), with fresh noise on the deformed samples.
+The target is the field's first-order warp at the patch centre. 8000 steps
+of 256 are 2.05 M pairs. The forward and backward passes, the loss and Adam are the
 library's own HIP kernels (csrc/seeder.hip). Prints the mean loss of every
 100 steps.
 """
