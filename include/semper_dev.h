@@ -223,6 +223,10 @@ SDK_API int sdk_incremental_timings(struct semper_incremental* inc, double* t8);
 /* The session engine's sdk_engine_timings for its last run (the frame's
    solve against the current reference). 0, or -3 before init. */
 SDK_API int sdk_incremental_engine_timings(struct semper_incremental* inc, double* out14);
+/* The last frame's re-anchoring: out2 = {stored subsets matched on the frame
+   (nodes that coasted past a quarter subset and came back), of those
+   accepted}. 0 or -3. */
+SDK_API int sdk_incremental_anchor_stats(struct semper_incremental* inc, int32_t* out2);
 
 /* The tuner's latest timing: chosen2 = {subsets per workgroup, tile side (0:
    row-major)} of least mean time so far, ms6 = the mean Path A launch time

@@ -629,6 +629,13 @@ SDK_API int sdk_incremental_engine_timings(semper_incremental* inc, double* out1
     return 0;
 }
 
+SDK_API int sdk_incremental_anchor_stats(semper_incremental* inc, int32_t* out2) {
+    if (!inc || !out2) return -3;
+    out2[0] = inc->session.reanchor_requests;
+    out2[1] = inc->session.reanchor_matched;
+    return 0;
+}
+
 SDK_API int sdk_incremental_graph_stats(semper_incremental* inc, int32_t* stats2) {
     if (!inc || !stats2) return -3;
     const auto g = inc->session.graph_stats();

@@ -535,13 +535,29 @@ FloodDev Engine::flood_dev(int gw, int gh) {
 // gradients are the patch's own) and the last run's frame as its deformed
 // plane; a subset centred at (cx, cy) on the canvas starts at the node's guess
 // less its offset from the patch centre.
-int Engine::match_anchors(std::vector<AnchorMatch>& reqs, int subset, bool use6, float tol) {
+int Engine::cut_patches(const std::vector<int32_t>& entries, int side, uint8_t* store) {
+    const int n = (int)(entries.size() / 3);
+    if (n <= 0) return 0;
+    if (!def_u8_.p || !store || side <= 0) return kErrInit;
+    try {
+        if (!patch_list_.reserve(entries.size())) return kErrInit;
+        ck(hipMemcpyAsync(patch_list_.p, entries.data(), sizeof(int32_t) * entries.size(), hipMemcpyHostToDevice, st_));
+        ck(launch_cut_patches(def_u8_.p, w_, patch_list_.p, n, side, store, st_));
+        // the entries' host buffer may go before the copy would otherwise run
+        ck(hipStreamSynchronize(st_));
+        return 0;
+    } catch (const HipFail&) {
+        return kErrInit;
+    }
+}
+
+int Engine::match_anchors(std::vector<AnchorMatch>& reqs, const uint8_t* store, int subset, bool use6, float tol) {
     for (auto& r : reqs) r.ok = false;
     if (reqs.empty()) return 0;
-    if (w_ <= 0 || h_ <= 0 || !def_u8_.p) return kErrInit;
+    if (w_ <= 0 || h_ <= 0 || !def_u8_.p || !store) return kErrInit;
     const int P = reqs[0].side;
     for (const auto& r : reqs)
-        if (r.side != P || !r.patch) return kErrInit;
+        if (r.side != P || r.slot < 0) return kErrInit;
     const int cols = w_ / P, rows_max = h_ / P;
     if (cols < 1 || rows_max < 1 || ppl_for(subset) < 0) return 0;   // no patch fits the frame: nothing matches
     const int per_batch = cols * rows_max;
@@ -550,14 +566,12 @@ int Engine::match_anchors(std::vector<AnchorMatch>& reqs, int subset, bool use6,
             const int n = (int)std::min<size_t>(per_batch, reqs.size() - first);
             const int rows = (n + cols - 1) / cols, Hc = rows * P;
             const size_t npx = (size_t)w_ * Hc;
-            std::vector<uint8_t> canvas(npx, 0);
-            std::vector<int32_t> cxy(2 * (size_t)n);
+            std::vector<int32_t> cxy(2 * (size_t)n), slots(n);
             std::vector<Task> tasks(n);
             for (int k = 0; k < n; ++k) {
                 const AnchorMatch& r = reqs[first + k];
                 const int ox = (k % cols) * P, oy = (k / cols) * P;
-                for (int y = 0; y < P; ++y)
-                    std::memcpy(&canvas[(size_t)(oy + y) * w_ + ox], r.patch + (size_t)y * P, (size_t)P);
+                slots[k] = r.slot;
                 Task t = {};
                 t.cx = ox + P / 2;
                 t.cy = oy + P / 2;
@@ -580,7 +594,9 @@ int Engine::match_anchors(std::vector<AnchorMatch>& reqs, int subset, bool use6,
             float* cI = planes.p;
             float* cgx = cI + npx;
             float* cgy = cgx + npx;
-            ck(hipMemcpyAsync(dcan.p, canvas.data(), npx, hipMemcpyHostToDevice, st_));
+            if (!patch_list_.reserve((size_t)n)) return kErrInit;
+            ck(hipMemcpyAsync(patch_list_.p, slots.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st_));
+            ck(launch_place_patches(store, P, patch_list_.p, n, cols, w_, rows, dcan.p, st_));
             ck(hipMemcpyAsync(dcxy.p, cxy.data(), sizeof(int32_t) * 2 * n, hipMemcpyHostToDevice, st_));
             ck(hipMemcpyAsync(tasks_d_.p, tasks.data(), sizeof(Task) * n, hipMemcpyHostToDevice, st_));
             ck(launch_prep_ref_u8(dcan.p, nullptr, w_, Hc, cI, cgx, cgy, st_));

@@ -153,16 +153,21 @@ public:
     // was cut from, the node at (ax + dx, ay + dy) there, and the guess of
     // where the node is now.
     struct AnchorMatch {
-        const uint8_t* patch = nullptr;
+        int slot = -1;            // the patch's slot in the caller's device store
         int side = 0, ax = 0, ay = 0;
         float dx = 0, dy = 0, guess_x = 0, guess_y = 0, max_shift = 0;
         bool ok = false;          // out: converged, ZNSSD <= 0.15, within max_shift
         float x = 0, y = 0, corr = 0;
     };
     // Matches every request with the subset solver (NO_SEARCH with the Simplex
-    // rescue, LM damping) on the deformed frame of the last run. Returns 0 or
-    // a negative code.
-    int match_anchors(std::vector<AnchorMatch>& reqs, int subset, bool use6, float tol);
+    // rescue, LM damping) on the deformed frame of the last run; the patches
+    // are slots of `store` (side x side bytes each, device memory). Returns 0
+    // or a negative code.
+    int match_anchors(std::vector<AnchorMatch>& reqs, const uint8_t* store, int subset, bool use6, float tol);
+    // Copies side x side blocks of the last run's deformed frame into slots of
+    // a device store: 3 ints per block (slot, left column, top row), each
+    // block inside the frame. Enqueued on the engine's stream. 0 or kErrInit.
+    int cut_patches(const std::vector<int32_t>& entries, int side, uint8_t* store);
     bool keep_field = false;
     // Solve Path B's first claims speculatively in one launch (off: every round
     // is solved when it is reached; the committed field is the same either way).
@@ -428,6 +433,7 @@ private:
     DevBuf<float> pack_out_;       // packed output points, 8 floats each
     DevBuf<int32_t> pack_blocks_;  // per-block write counts of the pack
     DevBuf<Task> tasks_d_;
+    DevBuf<int32_t> patch_list_;    // cut_patches entries, match_anchors slots
     DevBuf<SolveOut> out_d_;
     DevBuf<int32_t> bail_, counters_, nodes_d_;
     DevBuf<TriDev> tris_d_;

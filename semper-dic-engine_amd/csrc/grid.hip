@@ -62,6 +62,41 @@ __global__ void k_count_open(GridDev g, int n, int* __restrict__ count) {
 
 __global__ void k_zero1(int* p) { *p = 0; }
 
+// Anchor patches of the incremental session: entry e = (slot, x0, y0) copies
+// the side x side block of the frame whose top-left pixel is (x0, y0) into
+// store slot `slot` (row-major), one workgroup per entry.
+__global__ void k_cut_patches(const uint8_t* __restrict__ frame, int w, const int32_t* __restrict__ entries,
+                              int side, uint8_t* __restrict__ store) {
+    const int e = blockIdx.x;
+    const int slot = entries[3 * e], x0 = entries[3 * e + 1], y0 = entries[3 * e + 2];
+    const int n = side * side;
+    uint8_t* dst = store + (size_t)slot * n;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int y = i / side, x = i - y * side;
+        dst[i] = frame[(size_t)(y0 + y) * w + x0 + x];
+    }
+}
+
+// The matching canvas: patch k of the list (store slot slots[k]) at column
+// k % cols, row k / cols of side x side cells on a w-wide canvas whose other
+// bytes are zero (cleared by k_fill_u8 first).
+__global__ void k_place_patches(const uint8_t* __restrict__ store, int side, const int32_t* __restrict__ slots,
+                                int cols, int w, uint8_t* __restrict__ canvas) {
+    const int k = blockIdx.x;
+    const int n = side * side;
+    const uint8_t* src = store + (size_t)slots[k] * n;
+    const int ox = (k % cols) * side, oy = (k / cols) * side;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int y = i / side, x = i - y * side;
+        canvas[(size_t)(oy + y) * w + ox + x] = src[i];
+    }
+}
+
+__global__ void k_fill_u8(uint8_t* p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 __global__ void k_zero_n(int* p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
@@ -108,6 +143,21 @@ hipError_t launch_apply(GridDev g, const int* nodes, const SolveOut* res, int n,
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_apply, dim3((n + 255) / 256), dim3(256), 0, st, g, nodes, res, n, tag,
                        accepted_only ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_cut_patches(const uint8_t* frame, int w, const int32_t* entries, int n, int side, uint8_t* store,
+                              hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cut_patches, dim3(n), dim3(256), 0, st, frame, w, entries, side, store);
+    return hipGetLastError();
+}
+
+hipError_t launch_place_patches(const uint8_t* store, int side, const int32_t* slots, int n, int cols, int w,
+                                int rows, uint8_t* canvas, hipStream_t st) {
+    const size_t npx = (size_t)w * rows * side;
+    if (npx > 0) hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, st, canvas, npx);
+    if (n > 0) hipLaunchKernelGGL(k_place_patches, dim3(n), dim3(256), 0, st, store, side, slots, cols, w, canvas);
     return hipGetLastError();
 }
 

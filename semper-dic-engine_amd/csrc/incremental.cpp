@@ -337,6 +337,12 @@ int prefix_len(int metrics_len) {
 
 }  // namespace
 
+void IncrementalSession::drop_anchors() {
+    for (Anchor& a : anchors_) a.slot = -1;
+    free_slots_.clear();
+    slots_used_ = 0;
+}
+
 void IncrementalSession::reset() {
     eng_.reset();
     init_ = false;
@@ -352,6 +358,8 @@ void IncrementalSession::reset() {
     coast_travel_.clear();
     alive_.clear();
     anchors_.clear();
+    free_slots_.clear();
+    slots_used_ = anchor_side_ = 0;
     ref_index_ = frames_since_update_ = frame_count_ = 0;
     ref_reported_ = adopted_reported_ = -1;
     pk_ = Params{};
@@ -485,6 +493,7 @@ int IncrementalSession::process(const uint8_t* def, float* out, int out_capacity
                                 Cancel& cancel, const std::function<void(int)>& progress, FrameInfo* info) {
     const auto t_all = sclk::now();
     for (double& t : timing) t = 0.0;
+    reanchor_requests = reanchor_matched = 0;
     if (!init_ || !eng_ || !out || !def) return kErrInit;
     FrameInfo fi;
     fi.reference_frame = ref_index_;
@@ -562,7 +571,7 @@ int IncrementalSession::process(const uint8_t* def, float* out, int out_capacity
             // nothing on this reference's grid can be solved, now or on any later frame
             fi.lost_this_frame = alive_count();
             std::fill(alive_.begin(), alive_.end(), 0);
-            anchors_.clear();
+            drop_anchors();
             return nothing_to_follow(fi, metrics, metrics_len, progress, info);
         }
         timing[1] = ms_from(t_run);
@@ -624,19 +633,19 @@ int IncrementalSession::process(const uint8_t* def, float* out, int out_capacity
                 corr_of[i] = smp.corr;
                 need[i] = anchor_need(inc, gd, px, py, subset);
                 if (coast_travel_[i] > trusted_travel) {
-                    const auto a = anchors_.find(i);
-                    if (need[i] == AnchorNeed::kKeep || a == anchors_.end()) {
+                    const Anchor* a = i < anchors_.size() && anchors_[i].slot >= 0 ? &anchors_[i] : nullptr;
+                    if (need[i] == AnchorNeed::kKeep || !a) {
                         carried[i] = 1;   // unconfirmed: moves on, unreported
                         coasting++;
                         continue;
                     }
                     Engine::AnchorMatch m;
-                    m.patch = a->second.patch.data();
-                    m.side = a->second.side;
-                    m.ax = a->second.ax;
-                    m.ay = a->second.ay;
-                    m.dx = a->second.dx;
-                    m.dy = a->second.dy;
+                    m.slot = a->slot;
+                    m.side = anchor_side_;
+                    m.ax = a->ax;
+                    m.ay = a->ay;
+                    m.dx = a->dx;
+                    m.dy = a->dy;
                     m.guess_x = x0 + U[i];
                     m.guess_y = y0 + V[i];
                     m.max_shift = std::max(kReanchorMinShiftPx, kReanchorMaxShiftFraction * coast_travel_[i]);
@@ -650,7 +659,9 @@ int IncrementalSession::process(const uint8_t* def, float* out, int out_capacity
                 if (smp.taylor) fi.taylor_fallbacks++;
             }
         if (!ra.empty()) {
-            if (eng_->match_anchors(ra, subset, p0_.use6, tol) != 0) return kErrInit;
+            if (eng_->match_anchors(ra, patch_store_.p, subset, p0_.use6, tol) != 0) return kErrInit;
+            reanchor_requests = (int)ra.size();
+            for (const Engine::AnchorMatch& m : ra) reanchor_matched += m.ok ? 1 : 0;
             for (size_t k = 0; k < ra.size(); ++k) {
                 const size_t i = ra_node[k];
                 if (!ra[k].ok) {
@@ -804,27 +815,66 @@ int IncrementalSession::process(const uint8_t* def, float* out, int out_capacity
         timing[6] = ms_from(t_box);
         // the anchors (make_anchor, incremental_math.cpp:224-235), updated in
         // place: nothing after this point can fail and leave the session whole
+        // (a failed launch below ends the session)
+        if (anchors_.size() != n0) {
+            drop_anchors();
+            anchors_.assign(n0, Anchor{});
+        }
+        const int r_anchor = subset / 2 + kAnchorMarginPx;
+        anchor_side_ = 2 * r_anchor + 1;
+        std::vector<int32_t> cuts;   // (slot, x0, y0) per refreshed anchor
         for (size_t i = 0; i < n0; ++i) {
+            Anchor& an = anchors_[i];
             if (!carried[i] || (valid[i] && need[i] == AnchorNeed::kDrop)) {
-                anchors_.erase(i);
+                if (an.slot >= 0) free_slots_.push_back(an.slot);
+                an.slot = -1;
             } else if (valid[i] && need[i] == AnchorNeed::kRefresh) {
                 // on the frame becoming the reference
                 const float x = (float)(p0_.rect_x + (int)(i % gw_) * p0_.step) + U[i];
                 const float y = (float)(p0_.rect_y + (int)(i / gw_) * p0_.step) + V[i];
                 const int ax = (int)std::lround(x), ay = (int)std::lround(y);
-                const int r = subset / 2 + kAnchorMarginPx;
+                const int r = r_anchor;
                 if (ax - r >= 0 && ay - r >= 0 && ax + r < w_ && ay + r < h_) {
-                    Anchor& an = anchors_[i];
-                    an.side = 2 * r + 1;
+                    if (an.slot < 0) {
+                        if (!free_slots_.empty()) {
+                            an.slot = free_slots_.back();
+                            free_slots_.pop_back();
+                        } else {
+                            an.slot = slots_used_++;
+                        }
+                    }
                     an.ax = ax;
                     an.ay = ay;
                     an.dx = x - (float)ax;
                     an.dy = y - (float)ay;
-                    an.patch.resize((size_t)an.side * an.side);
-                    for (int yy = 0; yy < an.side; ++yy)
-                        std::memcpy(&an.patch[(size_t)yy * an.side], def + (size_t)(ay - r + yy) * w_ + (ax - r),
-                                    (size_t)an.side);
+                    cuts.push_back(an.slot);
+                    cuts.push_back(ax - r);
+                    cuts.push_back(ay - r);
                 }
+            }
+        }
+        if (!cuts.empty()) {
+            // the store grows by doubling, keeping the slots it holds
+            const size_t per = (size_t)anchor_side_ * anchor_side_;
+            const size_t need_bytes = (size_t)slots_used_ * per;
+            if (patch_store_.n < need_bytes) {
+                DevBuf<uint8_t> grown;
+                const size_t cap = std::max(need_bytes, std::max<size_t>(64 * per, 2 * patch_store_.n));
+                bool ok = grown.reserve(cap);
+                if (ok && patch_store_.p && patch_store_.n > 0)
+                    ok = hipMemcpyAsync(grown.p, patch_store_.p, patch_store_.n, hipMemcpyDeviceToDevice,
+                                        eng_->stream()) == hipSuccess;
+                if (!ok) {
+                    init_ = false;
+                    return kErrInit;
+                }
+                std::swap(grown.p, patch_store_.p);
+                std::swap(grown.n, patch_store_.n);
+                // the old buffer goes when `grown` does, after the copy: hipFree waits for it
+            }
+            if (eng_->cut_patches(cuts, anchor_side_, patch_store_.p) != 0) {
+                init_ = false;
+                return kErrInit;
             }
         }
         const auto t_adopt = sclk::now();

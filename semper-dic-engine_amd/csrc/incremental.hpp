@@ -71,6 +71,10 @@ public:
     // total field), the reference update, and within it the field fill, the
     // mask warp, the warped mask's bounds and the adoption of the frame.
     double timing[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // Stored subsets the last process call matched on its frame (nodes that
+    // coasted past a quarter subset and came back), and of those the ones
+    // accepted.
+    int reanchor_requests = 0, reanchor_matched = 0;
     Engine::GraphStats graph_stats() const { return eng_ ? eng_->graph_stats() : Engine::GraphStats{}; }
     // the session's engine (null before init)
     const Engine* engine() const { return eng_.get(); }
@@ -78,9 +82,13 @@ public:
 private:
     std::vector<int> region_map_;
     int region_w_ = 0, region_h_ = 0;
+    // A stored subset (make_anchor, incremental_math.cpp:224-235): its gray
+    // patch of side anchor_side_ around (ax, ay) lives in slot `slot` of the
+    // device store patch_store_ (-1: the node has none), cut from the frame
+    // that became the reference by one launch per update.
     struct Anchor {
-        std::vector<uint8_t> patch;   // side x side gray around (ax, ay)
-        int side = 0, ax = 0, ay = 0;
+        int slot = -1;
+        int ax = 0, ay = 0;
         float dx = 0, dy = 0;
     };
 
@@ -106,7 +114,11 @@ private:
     int compose_on_device(std::vector<uint8_t>& flags, std::vector<double>& u64, std::vector<double>& v64,
                           std::vector<float>& uf, std::vector<float>& vf, std::vector<float>& corr);
     std::vector<uint8_t> alive_;
-    std::map<size_t, Anchor> anchors_;
+    std::vector<Anchor> anchors_;         // per frame-0 node
+    std::vector<int32_t> free_slots_;     // store slots released by dropped anchors
+    int slots_used_ = 0, anchor_side_ = 0;
+    DevBuf<uint8_t> patch_store_;         // slots x side x side bytes
+    void drop_anchors();
     int ref_index_ = 0, frames_since_update_ = 0, ref_reported_ = -1, adopted_reported_ = -1, frame_count_ = 0;
     Params pk_;                       // the grid on the current reference (k > 0)
     std::vector<uint8_t> mask_k_;

@@ -204,6 +204,14 @@ hipError_t launch_vsg(int gw, int gh, int step, int window, int expected, const 
 
 // grid.hip (per-node bookkeeping)
 hipError_t launch_grid_init(GridDev g, const uint8_t* skip, int n, hipStream_t st);
+// Incremental-session anchor patches: n entries (slot, x0, y0) cut side x side
+// blocks of a w-wide u8 frame into a slot store; and the matching canvas, the
+// store slots of a list placed in rows of `cols` cells on a zeroed w x
+// (rows side) canvas.
+hipError_t launch_cut_patches(const uint8_t* frame, int w, const int32_t* entries, int n, int side, uint8_t* store,
+                              hipStream_t st);
+hipError_t launch_place_patches(const uint8_t* store, int side, const int32_t* slots, int n, int cols, int w,
+                                int rows, uint8_t* canvas, hipStream_t st);
 // Write solve results into the grid: with accepted_only, only results with
 // status 0 and corr <= kCorrAccept mark their node solved (Path B commits);
 // otherwise every listed result does (published anchors, with tag).

@@ -133,6 +133,7 @@ def load_dev(path=LIB_DEV):
     lib.sdk_incremental_graph_stats.argtypes = [_VP, _VP]
     lib.sdk_incremental_timings.argtypes = [_VP, _VP]
     lib.sdk_incremental_engine_timings.argtypes = [_VP, _VP]
+    lib.sdk_incremental_anchor_stats.argtypes = [_VP, _VP]
     return lib
 
 

@@ -91,6 +91,53 @@ def test_session_vs_oracle(sa, seq, interval):
         assert np.sqrt(np.mean((p_g[:, 2] - u) ** 2 + (p_g[:, 3] - v) ** 2)) <= 0.1
 
 
+def test_reanchoring_vs_oracle(sa):
+    """Nodes that coast past a quarter subset and come back are matched by
+    their stored subsets (match_anchor, incremental_math.cpp:237-272) on the
+    device patch store, and the session stays frame by frame with the
+    oracle's. The reference frame has a blank 32 px square at its centre, so
+    the nodes around it keep stored subsets; frames 2-5 blank a 112 px square
+    over them while the speckle drifts 4 px right and 2 px down per frame, so
+    they coast 16 px and are matched again on frame 7."""
+    dev = sa.load_dev()
+    T, tx = 8, 32.0
+    ref, frames, _ = speckle_sequence(T, W, H, e_max=0.04, theta_max=0.0, tx_max=tx, ty_max=tx / 2)
+    ref = ref.copy()
+    frames = [f.copy() for f in frames]
+    c = W // 2
+    ref[c - 16:c + 16, c - 16:c + 16] = 128
+    for t, f in enumerate(frames, 1):
+        dx, dy = int(round(tx * t / T)), int(round(tx / 2 * t / T))
+        f[c - 16 + dy:c + 16 + dy, c - 16 + dx:c + 16 + dx] = 128
+        if 2 <= t <= 5:
+            f[c - 56 + dy:c + 56 + dy, c - 56 + dx:c + 56 + dx] = 128
+    prm = centred_params(W, H, 192)
+    s = sa.Incremental(dev)
+    assert s.init(ref, params(sa, prm), 1) == 0
+    o = OracleIncremental()
+    assert o.init(ref, prm, 1) == 0
+    st = np.zeros(2, np.int32)
+    requested = matched = 0
+    for t, f in enumerate(frames, 1):
+        n_g, p_g, _, info = s.process(f)
+        assert dev.sdk_incremental_anchor_stats(s.h, st.ctypes.data_as(sa.ctypes.c_void_p)) == 0
+        requested += int(st[0])
+        matched += int(st[1])
+        n_o, p_o, _, i_o = o.process(f)
+        i_g = [info.reference_frame, info.reference_updated, info.tracked_points, info.frame0_points,
+               info.taylor_fallbacks, info.lost_this_frame]
+        print(f"frame {t}: gpu {n_g} oracle {n_o} re-anchored {st.tolist()} info {i_g}", flush=True)
+        assert i_g == i_o.tolist()
+        assert n_g == n_o > 0 and np.array_equal(p_g[:, :2], p_o[:, :2])
+        d = np.abs(p_g - p_o)
+        assert d[:, 2:4].max() <= 1e-3 and d[:, 7].max() <= 1e-4
+        # strains: 1e-5, plus what a stop split of the two sides' IC-GN (u, v
+        # apart by d px) moves a VSG slope over neighbours one step away on
+        # either side, d / step
+        assert d[:, 4:7].max() <= 1e-5 + 1.1 * d[:, 2:4].max() / prm[4]
+    assert requested >= 50 and matched >= 0.9 * requested
+
+
 def test_lost_nodes_never_return(sa):
     ref, frames, _ = speckle_sequence(8, W, H, e_max=0.02, theta_max=1.0, tx_max=100.0)
     inc = run_gpu(sa, ref, frames, centred_params(W, H, 160), 1)
