@@ -37,6 +37,22 @@ def _load(name, path):
     return mod
 
 
+_PINNED = []
+
+
+def pinned_f32(dev, n):
+    """A float32 array of n in page-locked host memory (the output buffer of the
+    timed runs: the packed points reach it by a direct copy from the device);
+    held until the process ends."""
+    p = dev.sdk_host_alloc(4 * n)
+    if not p:
+        raise MemoryError("sdk_host_alloc failed")
+    a = np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctypes.c_float)), shape=(n,))
+    a[:] = 0.0
+    _PINNED.append(p)
+    return a
+
+
 FLOP_PER_PIXEL_ITER = {False: 109.0, True: 227.0}
 SD_FLOP_PER_PIXEL = 8.0
 PEAK_FP32_TFLOPS = 157.3
@@ -80,7 +96,7 @@ PRECISION_VARIANTS = [("fp64_cholesky_step", ((2, 1),)), ("fp16_gradients", ((3,
                       ("all_fp16", ((2, 1), (3, 1), (4, 1), (17, 1))), ("all_bf16", ((2, 1), (3, 2), (4, 1), (17, 1)))]
 
 
-def sequence_study(dev, sa, wl):
+def sequence_study(dev, sa, wl, frames_limit=None):
     """The batched / incremental sequence: 64 frames of 1024^2 speckle under a
     uniaxial stretch ramping 0 -> 20% (Poisson 0.3), each frame with its own
     noise, through the incremental session at update interval 1. The mixed
@@ -90,7 +106,8 @@ def sequence_study(dev, sa, wl):
     error against the analytic field."""
     name = "c3_1024_mixed"
     cfg = wl.config(name)
-    frames = [wl.sequence_frame(dev, name, k) for k in range(cfg["frames"])]
+    nframes = cfg["frames"] if frames_limit is None else min(cfg["frames"], frames_limit)
+    frames = [wl.sequence_frame(dev, name, k) for k in range(nframes)]
     P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
     res = {"workload": f"{name}: {cfg['frames']} frames {cfg['w']}x{cfg['h']}, exx 0 -> {cfg['ramp']}, Poisson "
                        f"{cfg['p'][1]}, step {cfg['step']}, strain window {cfg['window']}, update interval 1",
@@ -106,16 +123,23 @@ def sequence_study(dev, sa, wl):
                       1) == 0
         ms, npts = [], []
         pts = None
+        t8s, t14s = [], []
         for f in frames[1:]:
             t0 = time.perf_counter()
             n, pts, _, _ = s.process(f)
             ms.append((time.perf_counter() - t0) * 1e3)
             npts.append(int(n))
+            t8, t14 = np.zeros(8), np.zeros(14)
+            dev.sdk_incremental_timings(s.h, P(t8))
+            dev.sdk_incremental_engine_timings(s.h, P(t14))
+            t8s.append(t8)
+            t14s.append(t14)
         stats = np.zeros(2, np.int32)
         dev.sdk_incremental_graph_stats(s.h, P(stats))
         s.close()
-        k = cfg["frames"] - 1
+        k = nframes - 1
         exx = wl.sequence_strain(name, k)
+        t8m, t14m = np.mean(t8s, 0), np.mean(t14s, 0)
         u = exx * (pts[:, 0] - cfg["w"] / 2.0)
         v = -cfg["p"][1] * exx * (pts[:, 1] - cfg["h"] / 2.0)
         err = np.hypot(pts[:, 2] - u, pts[:, 3] - v)
@@ -124,7 +148,13 @@ def sequence_study(dev, sa, wl):
                       "points_first": npts[0], "points_last": npts[-1],
                       "last_frame_rms_px": float(np.sqrt(np.mean(err ** 2))),
                       "last_frame_median_px": float(np.median(err)),
-                      "graph_captures": int(stats[0]), "graph_replays": int(stats[1])}
+                      "graph_captures": int(stats[0]), "graph_replays": int(stats[1]),
+                      "session_phases_ms": {k_: round(float(v), 4) for k_, v in zip(
+                          ["total", "engine_run", "composition", "reference_update", "field_fill", "mask_warp",
+                           "mask_bounds", "adopt"], t8m)},
+                      "engine_phases_ms": {k_: round(float(v), 4) for k_, v in zip(
+                          ["total", "prep", "seeding", "prepass", "mesh", "path_a", "path_b", "strain",
+                           "path_a_kernel"], t14m[:9])}}
     return res
 
 
@@ -154,7 +184,7 @@ def precision_study(dev, dref, ddef, prm, cap, tol, ref, dfm, subset, use6, step
         for f, v in flags:
             assert dev.sdk_engine_set_flag(eng, f, v) == 0
         assert dev.sdk_engine_set_reference(eng, dref, 1, w, ref.shape[0], None) == 0
-        out = np.zeros(cap, np.float32)
+        out = pinned_f32(dev, cap)
         met = np.zeros(24, np.float32)
         tm = np.zeros(14, np.float64)
         n = dev.sdk_engine_run(eng, ddef, 1, P(prm), P(out), cap, P(met), 24, tol)
@@ -315,7 +345,7 @@ def frames_in_flight(dev, sa, dref, ddef, prm, cap, tol, w, h, steps, k, expect,
                 raise RuntimeError("seeder weights missing: " + sa.SEEDER_WEIGHTS)
             assert dev.sdk_engine_set_flag(e, 5, 1) == 0
         assert dev.sdk_engine_set_reference(e, dref, 1, w, h, None) == 0
-        out = np.zeros(cap, np.float32)
+        out = pinned_f32(dev, cap)
         met = np.zeros(24, np.float32)
         run = lambda e=e, out=out, met=met: dev.sdk_engine_run(e, ddef, 1, P(prm), P(out), cap, P(met), 24, tol)
         run()
@@ -377,7 +407,7 @@ def learned_seed_study(dev, sa, wl, steps, threads, name="c5_4096_bumps", in_fli
                 raise RuntimeError("seeder weights missing: " + sa.SEEDER_WEIGHTS)
             assert dev.sdk_engine_set_flag(eng, 5, 1) == 0
         assert dev.sdk_engine_set_reference(eng, dref, 1, w, h, None) == 0
-        out = np.zeros(cap, np.float32)
+        out = pinned_f32(dev, cap)
         met = np.zeros(24, np.float32)
         n = dev.sdk_engine_run(eng, ddef, 1, P(prm), P(out), cap, P(met), 24, tol)   # warm (pool, pyramids)
         settle_tuner(dev, eng, lambda: dev.sdk_engine_run(eng, ddef, 1, P(prm), P(out), cap, P(met), 24, tol))
@@ -450,6 +480,12 @@ def main():
     ap.add_argument("--tol", type=float, default=None, help="IC-GN stop rule ||dp|| < tol (default: the workload's)")
     ap.add_argument("--learned-seed", action="store_true",
                     help="the main line's engine seeds its anchors with the learned seeder (the committed weights)")
+    ap.add_argument("--sequence-only", type=int, default=0, metavar="FRAMES",
+                    help="only run the incremental sequence study on its first FRAMES frames (for a profiler) "
+                         "and print its JSON")
+    ap.add_argument("--reference-only", type=int, default=0, metavar="RUNS",
+                    help="only set the workload's reference and solve its frame RUNS times, each after a fresh "
+                         "set_reference (the reference precompute under a profiler), and print the prepass times")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB copy kernel three times (the calibration of the HBM traffic "
                          "counters) and exit")
@@ -466,9 +502,29 @@ def main():
     dev = sa.load_dev()
     if dev.sdk_device_count() < 1:
         raise RuntimeError("no GPU visible")
+    if args.sequence_only:
+        print(json.dumps({"sequence": sequence_study(dev, sa, wl, frames_limit=args.sequence_only)}))
+        return
     cfg = wl.config(args.workload)
     w, h = cfg["w"], cfg["h"]
     prm = np.array(wl.solve_params(args.workload), np.int32)
+    if args.reference_only:
+        dref, ddef = wl.pair(dev, args.workload, on_device=True)
+        eng = dev.sdk_engine_create()
+        cap = (w // cfg["step"]) * (h // cfg["step"]) * 8
+        out = np.zeros(cap, np.float32)
+        met = np.zeros(24, np.float32)
+        tm = np.zeros(14, np.float64)
+        P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        res = []
+        for _ in range(args.reference_only):
+            assert dev.sdk_engine_set_reference(eng, dref, 1, w, h, None) == 0
+            n = dev.sdk_engine_run(eng, ddef, 1, P(prm), P(out), cap, P(met), 24, ctypes.c_float(cfg["tol"]))
+            assert n > 0, n
+            dev.sdk_engine_timings(eng, P(tm))
+            res.append({"points": int(n), "prepass_ms": float(tm[3]), "total_ms": float(tm[0])})
+        print(json.dumps({"reference_only": args.workload, "runs": res}))
+        return
     use6 = bool(prm[7])
     t0 = time.perf_counter()
     dref, ddef = wl.pair(dev, args.workload, on_device=True)
@@ -485,7 +541,7 @@ def main():
     assert dev.sdk_engine_set_reference(eng, dref, 1, w, h, None) == 0
     gw, gh = w // cfg["step"], h // cfg["step"]
     cap = gw * gh * 8
-    out = np.zeros(cap, np.float32)
+    out = pinned_f32(dev, cap)
     met = np.zeros(24, np.float32)
     tm = np.zeros(14, np.float64)
     P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
@@ -520,6 +576,13 @@ def main():
     for _ in range(nh):
         dev.sdk_engine_run(eng, P(host_def), 0, P(prm), P(out), cap, P(met_h), 24, tol)
     host_ms = (time.perf_counter() - t1) * 1000.0 / nh
+    # and with the points written to pageable host memory (a numpy buffer):
+    # the copy then goes through the runtime's staging; reported beside value
+    out_pg = np.zeros(cap, np.float32)
+    t1 = time.perf_counter()
+    for _ in range(nh):
+        dev.sdk_engine_run(eng, ddef, 1, P(prm), P(out_pg), cap, P(met_h), 24, tol)
+    pageable_ms = (time.perf_counter() - t1) * 1000.0 / nh
     # restore the device-input run's output for the checks below
     n = dev.sdk_engine_run(eng, ddef, 1, P(prm), P(out), cap, P(met), 24, tol)
     # the same frame with 2 and 3 frames in flight (engines on their own
@@ -614,7 +677,8 @@ def main():
                                f"strain window {cfg['window']}, {'Keys 6x6' if use6 else 'Keys 4x4'} IC-GN",
                    "method": "learned-seed GPU" if args.learned_seed else "plain GPU",
                    "attempted_subsets": int(met[0]), "solved_subsets": int(solved), "output_points": int(n),
-                   "pair_generation_s": round(t_gen, 3), "stop_rule": tol_v},
+                   "pair_generation_s": round(t_gen, 3), "stop_rule": tol_v,
+                   "output_buffer": "page-locked host memory"},
         "roofline": {"bound": "valu", "compute_roof": "FP32 vector ALU (MI355X FP32 peak 157.3 TFLOP/s)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS,
@@ -625,6 +689,10 @@ def main():
         "cpu_baseline": cpu,
         "accuracy_vs_fp64": acc,
         "parity_vs_oracle": parity,
+        "host_output_pageable": {"ms_per_step": pageable_ms,
+                                 "value": float(solved * (acc["within_1e-3px"] if acc else 1.0) / (pageable_ms / 1000.0)),
+                                 "note": "points written to pageable host memory instead of the page-locked buffer "
+                                         "of the timed steps"},
         "host_input": {"ms_per_step": host_ms, "value": float(solved * (acc["within_1e-3px"] if acc else 1.0)
                                                                / (host_ms / 1000.0)),
                        "note": "deformed frame passed in host memory (pageable), PCIe copy included"},

@@ -25,6 +25,11 @@ extern "C" {
 SDK_API int sdk_device_count(void);
 SDK_API void* sdk_malloc(size_t bytes);
 SDK_API void sdk_free(void* p);
+/* Page-locked host memory (hipHostMalloc): an output buffer of semper_run /
+   sdk_engine_run in it is written by a direct copy from the device, without
+   the staging that pageable memory takes. NULL on failure. */
+SDK_API void* sdk_host_alloc(size_t bytes);
+SDK_API void sdk_host_free(void* p);
 /* kind 0: host to device, 1: device to host, 2: device to device */
 SDK_API int sdk_memcpy(void* dst, const void* src, size_t bytes, int kind);
 

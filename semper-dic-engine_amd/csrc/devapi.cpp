@@ -55,6 +55,16 @@ SDK_API void sdk_free(void* p) {
     if (p) (void)hipFree(p);
 }
 
+SDK_API void* sdk_host_alloc(size_t bytes) {
+    void* p = nullptr;
+    if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) return nullptr;
+    return p;
+}
+
+SDK_API void sdk_host_free(void* p) {
+    if (p) (void)hipHostFree(p);
+}
+
 SDK_API int sdk_memcpy(void* dst, const void* src, size_t bytes, int kind) {
     const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
     return (int)hipMemcpy(dst, src, bytes, k);
@@ -366,6 +376,9 @@ SDK_API int sdk_engine_set_flag(void* h, int flag, int value) {
             return 0;
         case 18:
             ((DevEngine*)h)->eng.flood_on_device = value != 0;
+            return 0;
+        case 19:
+            ((DevEngine*)h)->eng.cnn_always = value != 0;
             return 0;
 
         default:

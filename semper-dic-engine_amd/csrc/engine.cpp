@@ -840,7 +840,7 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
         // near-translation is the reference's plain lattice's case: it seeds
         // it first, and the CNN lattice follows only if that lattice's
         // anchors do not converge.
-        bool use_cnn = cnn_ok && !(gm.locked && !gm.full);
+        bool use_cnn = cnn_ok && (cnn_always || !(gm.locked && !gm.full));
         // lattice (solve_anchor_seeds, full_field_anchors.cpp:283-574)
         int sx = 1, sy = 1, LW = 0, LH = 0;
         std::vector<int> lx, ly;

@@ -215,6 +215,10 @@ public:
     // fails), and an anchor whose refined solve fails is solved again the
     // plain way.
     bool learned_seed = false;
+    // Learned seeding runs the CNN lattice on every frame (off: only where
+    // the global model finds a large motion, or the plain lattice's anchors
+    // do not converge).
+    bool cnn_always = false;
     // The seeder's inference: the fp16 MFMA kernel (true) or the FP32
     // forward pass it is trained with.
     bool seeder_mfma = true;

@@ -83,6 +83,9 @@ def load_dev(path=LIB_DEV):
     lib.sdk_malloc.restype = _VP
     lib.sdk_malloc.argtypes = [ctypes.c_size_t]
     lib.sdk_free.argtypes = [_VP]
+    lib.sdk_host_alloc.restype = _VP
+    lib.sdk_host_alloc.argtypes = [ctypes.c_size_t]
+    lib.sdk_host_free.argtypes = [_VP]
     lib.sdk_memcpy.argtypes = [_VP, _VP, ctypes.c_size_t, ctypes.c_int]
     lib.sdk_solve_nodes.argtypes = [_VP, _VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_float, ctypes.c_int, _VP, _VP, _VP, _VP, _VP]
