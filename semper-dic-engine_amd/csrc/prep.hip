@@ -50,11 +50,13 @@ __global__ void k_prep_ref(const T* __restrict__ img, const uint8_t* __restrict_
 }
 
 // One thread per grid node (no mask): skip = inside the boundary buffer
-// d/2 + 4 + 15. The kept nodes are counted once per wave (a ballot), not by
-// one same-address atomic per node.
-__global__ void k_scan(const uint8_t* __restrict__ mask, int w, int h, int rect_x, int rect_y, int step,
-                       int gw, int gh, int dim, const int* __restrict__ dims, uint8_t* __restrict__ skip,
-                       int* count) {
+// d/2 + 4 + 15. The kept nodes are counted by a ballot per wave and a sum
+// per 1024-node workgroup: one same-address atomic per workgroup.
+constexpr int kScanBlock = 1024;
+__global__ __launch_bounds__(kScanBlock) void k_scan(int w, int h, int rect_x, int rect_y, int step, int gw, int gh,
+                                                     int dim, const int* __restrict__ dims,
+                                                     uint8_t* __restrict__ skip, int* count) {
+    __shared__ int kept_w[kScanBlock / 64];
     const int node = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = node < gw * gh;
     bool s = true;
@@ -65,7 +67,13 @@ __global__ void k_scan(const uint8_t* __restrict__ mask, int w, int h, int rect_
         skip[node] = s ? 1 : 0;
     }
     const uint64_t kept = __ballot(live && !s);
-    if ((threadIdx.x & 63) == 0 && kept) atomicAdd(count, (int)__popcll(kept));
+    if ((threadIdx.x & 63) == 0) kept_w[threadIdx.x >> 6] = (int)__popcll(kept);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int k = 0; k < kScanBlock / 64; ++k) t += kept_w[k];
+        if (t) atomicAdd(count, t);
+    }
 }
 
 // The same with a mask: skip also when any subset pixel is masked (< 128).
@@ -141,24 +149,131 @@ __device__ bool pool_sums(const float* __restrict__ I, GradRead G, int w, int h,
     const int half = dim / 2, n = dim * dim;
     if (dim < 3 || (dim & 1) == 0 || cx - half < 0 || cx + half >= w || cy - half < 0 || cy + half >= h)
         return false;
+    // pixel i = lane + 64 k of the subset at (q, r) = divmod(i, dim), advanced
+    // by divmod(64, dim) per step instead of divided anew; each pass reads
+    // kChunk of a lane's pixels before it uses any (their loads in flight
+    // together) and adds them in pixel order
+    constexpr int kChunk = 4;
+    const int q0 = lane / dim, r0 = lane % dim, q64 = 64 / dim, r64 = 64 % dim;
+    const int base = (cy - half) * w + cx - half;
+    const auto offsets = [&](int i, int& q, int& r, int (&off)[kChunk], int (&oxy)[kChunk][2], bool (&in)[kChunk]) {
+#pragma unroll
+        for (int c = 0; c < kChunk; ++c) {
+            in[c] = i + 64 * c < n;
+            off[c] = in[c] ? q * w + r : 0;
+            oxy[c][0] = r - half;
+            oxy[c][1] = q - half;
+            r += r64;
+            q += q64;
+            if (r >= dim) {
+                r -= dim;
+                q++;
+            }
+        }
+    };
+    if constexpr (!MFMA) {
+        // a subset of up to 64 kHeld pixels: its intensities and gradients
+        // loaded into registers at once, then the three passes over them
+        constexpr int kHeld = 16;
+        if (n <= 64 * kHeld) {
+            float v[kHeld], gxv[kHeld], gyv[kHeld];
+            int oxy[kHeld][2];
+            bool in[kHeld];
+            {
+                int q = q0, r = r0;
+#pragma unroll
+                for (int c = 0; c < kHeld; ++c) {
+                    in[c] = lane + 64 * c < n;
+                    const int off = in[c] ? q * w + r : 0;
+                    oxy[c][0] = r - half;
+                    oxy[c][1] = q - half;
+                    v[c] = I[base + off];
+                    gxv[c] = G.x(base + off);
+                    gyv[c] = G.y(base + off);
+                    r += r64;
+                    q += q64;
+                    if (r >= dim) {
+                        r -= dim;
+                        q++;
+                    }
+                }
+            }
+            float sum = 0.0f;
+            int cnt = 0;
+#pragma unroll
+            for (int c = 0; c < kHeld; ++c) {
+                if (!in[c] || v[c] < kGhostWallThreshold) continue;
+                sum += v[c];
+                cnt++;
+            }
+            sum = wave_sum(sum);
+            cnt = wave_sum_i(cnt);
+            if (cnt < n * 0.5f) return false;
+            mean = sum / (float)cnt;
+            float ss = 0.0f;
+#pragma unroll
+            for (int c = 0; c < kHeld; ++c) {
+                if (!in[c] || v[c] < kGhostWallThreshold) continue;
+                const float d = v[c] - mean;
+                ss += d * d;
+            }
+            ss = wave_sum(ss);
+            float sd = sqrtf(ss / (float)cnt);
+            if (sd < 1e-5f) sd = 1.0f;
+            inv = 1.0f / sd;
+#pragma unroll
+            for (int k = 0; k < 21; ++k) hh[k] = 0.0f;
+#pragma unroll
+            for (int c = 0; c < kHeld; ++c) {
+                if (!in[c] || v[c] < kGhostWallThreshold) continue;
+                const float g0 = gxv[c] * inv, g1 = gyv[c] * inv;
+                const float fx = (float)oxy[c][0], fy = (float)oxy[c][1];
+                const float s6[6] = {g0, g1, g0 * fx, g0 * fy, g1 * fx, g1 * fy};
+                int k = 0;
+#pragma unroll
+                for (int rr = 0; rr < 6; ++rr)
+#pragma unroll
+                    for (int cc = rr; cc < 6; ++cc) hh[k++] += s6[rr] * s6[cc];
+            }
+#pragma unroll
+            for (int k = 0; k < 21; ++k) hh[k] = wave_sum(hh[k]);
+            return true;
+        }
+    }
     float sum = 0.0f;
     int cnt = 0;
-    for (int i = lane; i < n; i += 64) {
-        const float r = I[(cy + i / dim - half) * w + cx + i % dim - half];
-        if (r < kGhostWallThreshold) continue;
-        sum += r;
-        cnt++;
+    for (int i = lane, q = q0, r = r0; i < n; i += 64 * kChunk) {
+        int off[kChunk], oxy[kChunk][2];
+        bool in[kChunk];
+        offsets(i, q, r, off, oxy, in);
+        float v[kChunk];
+#pragma unroll
+        for (int c = 0; c < kChunk; ++c) v[c] = I[base + off[c]];
+#pragma unroll
+        for (int c = 0; c < kChunk; ++c) {
+            if (!in[c] || v[c] < kGhostWallThreshold) continue;
+            sum += v[c];
+            cnt++;
+        }
     }
     sum = wave_sum(sum);
     cnt = wave_sum_i(cnt);
     if (cnt < n * 0.5f) return false;
     mean = sum / (float)cnt;
     float ss = 0.0f;
-    for (int i = lane; i < n; i += 64) {
-        const float r = I[(cy + i / dim - half) * w + cx + i % dim - half];
-        if (r < kGhostWallThreshold) continue;
-        const float d = r - mean;
-        ss += d * d;
+    for (int i = lane, q = q0, r = r0; i < n; i += 64 * kChunk) {
+        int off[kChunk], oxy[kChunk][2];
+        bool in[kChunk];
+        offsets(i, q, r, off, oxy, in);
+        float v[kChunk];
+#pragma unroll
+        for (int c = 0; c < kChunk; ++c) v[c] = I[base + off[c]];
+#pragma unroll
+        for (int c = 0; c < kChunk; ++c) {
+            if (!in[c] || v[c] < kGhostWallThreshold) continue;
+            const float d = v[c] - mean;
+            ss += d * d;
+        }
     }
     ss = wave_sum(ss);
     float sd = sqrtf(ss / (float)cnt);
@@ -167,18 +282,29 @@ __device__ bool pool_sums(const float* __restrict__ I, GradRead G, int w, int h,
     if constexpr (!MFMA) {
 #pragma unroll
         for (int k = 0; k < 21; ++k) hh[k] = 0.0f;
-        for (int i = lane; i < n; i += 64) {
-            const int oy = i / dim - half, ox = i % dim - half;
-            const int a = (cy + oy) * w + cx + ox;
-            if (I[a] < kGhostWallThreshold) continue;
-            const float g0 = G.x(a) * inv, g1 = G.y(a) * inv;
-            const float fx = (float)ox, fy = (float)oy;
-            const float s[6] = {g0, g1, g0 * fx, g0 * fy, g1 * fx, g1 * fy};
-            int k = 0;
+        for (int i = lane, q = q0, r = r0; i < n; i += 64 * kChunk) {
+            int off[kChunk], oxy[kChunk][2];
+            bool in[kChunk];
+            offsets(i, q, r, off, oxy, in);
+            float v[kChunk], gxv[kChunk], gyv[kChunk];
 #pragma unroll
-            for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < kChunk; ++c) {
+                v[c] = I[base + off[c]];
+                gxv[c] = G.x(base + off[c]);
+                gyv[c] = G.y(base + off[c]);
+            }
 #pragma unroll
-                for (int c = r; c < 6; ++c) hh[k++] += s[r] * s[c];
+            for (int c = 0; c < kChunk; ++c) {
+                if (!in[c] || v[c] < kGhostWallThreshold) continue;
+                const float g0 = gxv[c] * inv, g1 = gyv[c] * inv;
+                const float fx = (float)oxy[c][0], fy = (float)oxy[c][1];
+                const float s6[6] = {g0, g1, g0 * fx, g0 * fy, g1 * fx, g1 * fy};
+                int k = 0;
+#pragma unroll
+                for (int rr = 0; rr < 6; ++rr)
+#pragma unroll
+                    for (int cc = rr; cc < 6; ++cc) hh[k++] += s6[rr] * s6[cc];
+            }
         }
 #pragma unroll
         for (int k = 0; k < 21; ++k) hh[k] = wave_sum(hh[k]);
@@ -798,8 +924,8 @@ hipError_t launch_scan(const uint8_t* mask, int w, int h, int rect_x, int rect_y
         hipLaunchKernelGGL(k_scan_masked, dim3((n + kScanNodesPerGroup - 1) / kScanNodesPerGroup), dim3(256), 0, st,
                            mask, w, h, rect_x, rect_y, step, gw, gh, dim, dims, skip, count);
     else
-        hipLaunchKernelGGL(k_scan, dim3((n + 255) / 256), dim3(256), 0, st, mask, w, h, rect_x, rect_y, step, gw,
-                           gh, dim, dims, skip, count);
+        hipLaunchKernelGGL(k_scan, dim3((n + kScanBlock - 1) / kScanBlock), dim3(kScanBlock), 0, st, w, h, rect_x,
+                           rect_y, step, gw, gh, dim, dims, skip, count);
     return hipGetLastError();
 }
 
