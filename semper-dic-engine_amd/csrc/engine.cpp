@@ -141,6 +141,10 @@ Engine::~Engine() {
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
     for (hipEvent_t e : class_ev_) (void)hipEventDestroy(e);
+    for (hipStream_t s : class_st_) {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+    }
     if (st_) (void)hipStreamDestroy(st_);
 }
 
@@ -410,24 +414,85 @@ void Engine::set_precision_args(SolveArgs& a) const {
     if (grid_prec_.chol64) a.chol = chol_.p;
 }
 
+void Engine::ensure_class_streams(size_t nc) {
+    while (class_ev_.size() < 2 * nc + 2) {
+        hipEvent_t e;
+        ck(hipEventCreate(&e));
+        class_ev_.push_back(e);
+    }
+    while (class_st_.size() + 1 < nc) {
+        hipStream_t s2;
+        ck(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+        class_st_.push_back(s2);
+    }
+    if (!class_nbail_.reserve(std::max<size_t>(nc, 1))) throw HipFail{hipErrorOutOfMemory};
+}
+
 int Engine::solve_tasks(const std::vector<Task>& tasks, std::vector<SolveOut>& out, bool use6, int dim, float tol) {
     if (grid_dims_.empty()) return solve_tasks_dim(tasks, out, use6, dim, tol);
-    // mixed sizes: one launch per size among the tasks' nodes
+    // mixed sizes: the tasks grouped by their node's size, one launch per
+    // size, the groups after the first on streams of their own (forked from
+    // the engine's stream and joined back), with one upload and one download
     out.assign(tasks.size(), SolveOut{});
+    struct Seg {
+        int dim, off, n;
+    };
+    std::vector<Seg> segs;
+    std::vector<Task> sorted;
+    std::vector<size_t> at;
     for (const DimClass& c : classes_) {
-        std::vector<Task> sub;
-        std::vector<size_t> at;
+        const int off = (int)sorted.size();
         for (size_t k = 0; k < tasks.size(); ++k)
             if (tasks[k].pool >= 0 && (size_t)tasks[k].pool < grid_dims_.size() && grid_dims_[tasks[k].pool] == c.dim) {
-                sub.push_back(tasks[k]);
+                sorted.push_back(tasks[k]);
                 at.push_back(k);
             }
-        if (sub.empty()) continue;
-        std::vector<SolveOut> r;
-        const int rc = solve_tasks_dim(sub, r, use6, c.dim, tol);
-        if (rc != 0) return rc;
-        for (size_t i = 0; i < at.size(); ++i) out[at[i]] = r[i];
+        if ((int)sorted.size() > off) segs.push_back(Seg{c.dim, off, (int)sorted.size() - off});
     }
+    if (segs.empty()) return 0;
+    const int n = (int)sorted.size();
+    const size_t ns = segs.size();
+    if (!tasks_d_.reserve(n) || !out_d_.reserve(n) || !bail_.reserve(n) || !counters_.reserve(32)) return kErrInit;
+    if (!pin_tasks_.reserve(n) || !pin_res_.reserve(n) || !pin_handed_.reserve(ns)) return kErrInit;
+    ensure_class_streams(ns);
+    std::memcpy(pin_tasks_.p, sorted.data(), sizeof(Task) * n);
+    ck(hipMemcpyAsync(tasks_d_.p, pin_tasks_.p, sizeof(Task) * n, hipMemcpyHostToDevice, st_));
+    hipEvent_t ev_fork = class_ev_[2 * ns];
+    ck(hipEventRecord(ev_fork, st_));
+    for (size_t i = 0; i < ns; ++i) {
+        hipStream_t cs = i == 0 ? st_ : class_st_[i - 1];
+        if (i > 0) ck(hipStreamWaitEvent(cs, ev_fork, 0));
+        SolveArgs a = {};
+        a.tasks = tasks_d_.p + segs[i].off;
+        a.out = out_d_.p + segs[i].off;
+        a.nitems = segs[i].n;
+        a.bail = bail_.p + segs[i].off;
+        a.nbail = i == 0 ? counters_.p + 8 : class_nbail_.p + i;
+        a.pool = pool_.p;
+        a.refI = I_.p;
+        a.refGx = gx_.p;
+        a.refGy = gy_.p;
+        set_precision_args(a);
+        set_def_args(a);
+        a.w = w_;
+        a.h = h_;
+        a.dim = segs[i].dim;
+        a.lm = 1;
+        a.tol = tol;
+        ck(launch_solve(a, use6, def_f32_ready_, cs));
+        if (i > 0) {
+            ck(hipEventRecord(class_ev_[2 * i + 1], cs));
+            ck(hipStreamWaitEvent(st_, class_ev_[2 * i + 1], 0));
+        }
+    }
+    ck(hipMemcpyAsync(pin_res_.p, out_d_.p, sizeof(SolveOut) * n, hipMemcpyDeviceToHost, st_));
+    ck(hipMemcpyAsync(pin_handed_.p, counters_.p + 8, sizeof(int32_t), hipMemcpyDeviceToHost, st_));
+    if (ns > 1)
+        ck(hipMemcpyAsync(pin_handed_.p + 1, class_nbail_.p + 1, sizeof(int32_t) * (ns - 1), hipMemcpyDeviceToHost,
+                          st_));
+    ck(hipStreamSynchronize(st_));
+    for (int j = 0; j < n; ++j) out[at[j]] = pin_res_.p[j];
+    for (size_t i = 0; i < ns; ++i) launch_log.emplace_back(segs[i].n, pin_handed_.p[i]);
     return 0;
 }
 
@@ -1274,42 +1339,57 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             a.lm = 1;
             a.tol = tol;
             // one launch per subset size (one in all unless sizes are mixed),
-            // each in the shape the tuner gives it and timed on its own
+            // each in the shape the tuner gives it and timed on its own; with
+            // several sizes, the classes after the first run at once on streams
+            // of their own, forked from the engine's stream and joined back
+            // into it, each with its own hand-off list (its nodes' share of
+            // the bail array) and counter, so that one class's slow subsets
+            // overlap the others' launches
             const size_t nc = classes_.size();
-            while (class_ev_.size() < nc + 1) {
-                hipEvent_t e;
-                ck(hipEventCreate(&e));
-                class_ev_.push_back(e);
-            }
+            ensure_class_streams(nc);
             std::vector<std::vector<int>> keys(nc);
             std::vector<int> shapes(nc);
-            ck(hipEventRecord(class_ev_[0], st_));
+            hipEvent_t ev_fork = class_ev_[2 * nc], ev_done = class_ev_[2 * nc + 1];
+            ck(hipEventRecord(ev_fork, st_));
             for (size_t ci = 0; ci < nc; ++ci) {
                 const DimClass& c = classes_[ci];
                 const bool listed = c.off >= 0;
+                hipStream_t cs = ci == 0 ? st_ : class_st_[ci - 1];
+                if (ci > 0) ck(hipStreamWaitEvent(cs, ev_fork, 0));
                 keys[ci] = {gw, gh, c.dim, p.use6 ? 1 : 0, listed ? 1 : 0, c.n,
                             (def_f32_ready_ ? 1 : 0) + (def_coef_ready_ ? 2 : 0)};
                 shapes[ci] = shape_for(keys[ci], listed);
-                a.items = listed ? class_nodes_d_.p + c.off : nullptr;
-                a.nitems = c.n;
-                a.dim = c.dim;
-                a.grid.dim = c.dim;
-                a.wpb = kShapes[shapes[ci]].wpb;
-                a.tile = listed ? 0 : kShapes[shapes[ci]].tile;
-                ck(launch_solve(a, p.use6, def_f32_ready_, st_));
-                ck(hipEventRecord(class_ev_[ci + 1], st_));
+                SolveArgs ac = a;
+                ac.items = listed ? class_nodes_d_.p + c.off : nullptr;
+                ac.nitems = c.n;
+                ac.dim = c.dim;
+                ac.grid.dim = c.dim;
+                ac.bail = bail_.p + (listed ? c.off : 0);
+                ac.nbail = ci == 0 ? counters_.p + 8 : class_nbail_.p + ci;
+                ac.wpb = kShapes[shapes[ci]].wpb;
+                ac.tile = listed ? 0 : kShapes[shapes[ci]].tile;
+                ck(hipEventRecord(class_ev_[2 * ci], cs));
+                ck(launch_solve(ac, p.use6, def_f32_ready_, cs));
+                ck(hipEventRecord(class_ev_[2 * ci + 1], cs));
             }
+            for (size_t ci = 1; ci < nc; ++ci) ck(hipStreamWaitEvent(st_, class_ev_[2 * ci + 1], 0));
+            ck(hipEventRecord(ev_done, st_));
             int sa[9];
+            std::vector<int32_t> handed(nc, 0);
             ck(launch_sum_stats(stat_slots_.p, counters_.p, st_));
             ck(hipMemcpyAsync(sa, counters_.p, sizeof(sa), hipMemcpyDeviceToHost, st_));
+            if (nc > 1)
+                ck(hipMemcpyAsync(handed.data() + 1, class_nbail_.p + 1, sizeof(int32_t) * (nc - 1),
+                                  hipMemcpyDeviceToHost, st_));
             ck(hipStreamSynchronize(st_));
             float kms = 0;
-            ck(hipEventElapsedTime(&kms, class_ev_[0], class_ev_[nc]));
+            ck(hipEventElapsedTime(&kms, ev_fork, ev_done));
             for (size_t ci = 0; ci < nc; ++ci) {
                 float cms = 0;
-                ck(hipEventElapsedTime(&cms, class_ev_[ci], class_ev_[ci + 1]));
+                ck(hipEventElapsedTime(&cms, class_ev_[2 * ci], class_ev_[2 * ci + 1]));
                 shape_timed(keys[ci], classes_[ci].off >= 0, shapes[ci], cms);
             }
+            for (size_t ci = 1; ci < nc; ++ci) sa[8] += handed[ci];
             tm_.kernel_path_a = kms;
             tm_.path_a_items = N;
             launch_log.emplace_back((int)N, sa[8]);

@@ -314,7 +314,7 @@ public:
     SeedStats seed_stats;
     // (work items, items the fast path handed to the full solver) of each
     // subset-solver launch of the last run, in launch order; a Path A launch
-    // over several subset sizes reports its last size's hand-offs
+    // over several subset sizes reports the hand-offs of all of them
     std::vector<std::pair<int, int>> launch_log;
     Field last_field;
     // The same field on the device when keep_field (the node planes and the
@@ -378,7 +378,11 @@ private:
         std::vector<float> samples[6];
     };
     std::map<std::vector<int>, TuneState> tuned_;
-    std::vector<hipEvent_t> class_ev_;     // one event before and after each class launch
+    std::vector<hipEvent_t> class_ev_;     // before and after each class launch, then the fork and the join
+    std::vector<hipStream_t> class_st_;    // the streams of the Path A classes after the first
+    DevBuf<int32_t> class_nbail_;          // their hand-off counters
+    // events, streams and counters for launches over nc subset sizes at once
+    void ensure_class_streams(size_t nc);
     // The candidate for the next Path A launch of a size class (index into the
     // candidate list), and the report of a finished timing.
     int shape_for(const std::vector<int>& key, bool listed);
