@@ -9,7 +9,9 @@
 // exact hit, bilinear over four valid corners, else the first-order
 // expansion from the nearest valid node of the 4 x 4 block within 1.5 cells),
 // in the same float operations; the increment is then added to the total in
-// double and the reported total is that sum rounded once.
+// double and the reported total is that sum rounded once -- or, for a session
+// with float totals (no double totals given), added to the float total as the
+// reference does.
 #include <hip/hip_runtime.h>
 
 #include "compose.hpp"
@@ -89,11 +91,17 @@ __global__ void k_sample_compose64(ComposeField f, ComposeTotals t, ComposeOut o
         }
     }
     if (!ok) return;
-    const double U64 = t.u64[k] + (double)su, V64 = t.v64[k] + (double)sv;
-    o.u64[k] = U64;
-    o.v64[k] = V64;
-    o.uf[k] = (float)U64;
-    o.vf[k] = (float)V64;
+    if (t.u64) {
+        const double U64 = t.u64[k] + (double)su, V64 = t.v64[k] + (double)sv;
+        o.u64[k] = U64;
+        o.v64[k] = V64;
+        o.uf[k] = (float)U64;
+        o.vf[k] = (float)V64;
+    } else {
+        // the reference's float totals: the increment added in float
+        o.uf[k] = t.uf[k] + su;
+        o.vf[k] = t.vf[k] + sv;
+    }
     o.corr[k] = sc;
     o.flags[k] = (uint8_t)(1 | (taylor ? 2 : 0));
 }

@@ -26,8 +26,9 @@ struct ComposeTotals {
 };
 
 // Per node: flags bit 0 sampled, bit 1 by the first-order expansion; for a
-// sampled node the new totals in double and rounded once, and the sample's
-// corr (other nodes: flags 0, the rest untouched).
+// sampled node the new totals in double and rounded once (in float when the
+// totals carry no double: u64 and v64 null in ComposeTotals and here), and
+// the sample's corr (other nodes: flags 0, the rest untouched).
 struct ComposeOut {
     double* u64 = nullptr;
     double* v64 = nullptr;

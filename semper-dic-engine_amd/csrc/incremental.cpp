@@ -18,7 +18,6 @@ namespace {
 
 constexpr int kErrRoi = -2, kErrInit = -3, kErrStrainWindow = -4;
 constexpr float kCorrInvalidF = -1.0f;
-constexpr float kTaylorMaxCells = 1.5f;
 constexpr int kMaskMaxSamplesPerPx = 8;
 constexpr int kCoastRadiusCells = 8;
 constexpr int kCoastMinNodes = 6;
@@ -65,59 +64,6 @@ struct Sample {
     bool ok = false, taylor = false;
     float u = 0, v = 0, corr = 0;
 };
-
-// sample_increment (incremental_math.cpp:61-123)
-Sample sample_increment(const NodeField& f, float x, float y) {
-    Sample s;
-    if (f.gw <= 0 || f.gh <= 0 || f.step <= 0) return s;
-    const float gx = (x - (float)f.rx) / (float)f.step, gy = (y - (float)f.ry) / (float)f.step;
-    const int i0 = (int)std::floor(gx), j0 = (int)std::floor(gy);
-    const float fx = gx - (float)i0, fy = gy - (float)j0;
-    const auto& V = *f.valid;
-    if (fx == 0.0f && fy == 0.0f && f.in(i0, j0) && V[f.idx(i0, j0)]) {
-        const size_t k = f.idx(i0, j0);
-        s.ok = true;
-        s.u = (*f.u)[k];
-        s.v = (*f.v)[k];
-        s.corr = (*f.corr)[k];
-        return s;
-    }
-    if (f.in(i0, j0) && f.in(i0 + 1, j0 + 1)) {
-        const size_t a = f.idx(i0, j0), b = f.idx(i0 + 1, j0), c = f.idx(i0, j0 + 1), d = f.idx(i0 + 1, j0 + 1);
-        if (V[a] && V[b] && V[c] && V[d]) {
-            const float wa = (1 - fx) * (1 - fy), wb = fx * (1 - fy), wc = (1 - fx) * fy, wd = fx * fy;
-            s.ok = true;
-            s.u = wa * (*f.u)[a] + wb * (*f.u)[b] + wc * (*f.u)[c] + wd * (*f.u)[d];
-            s.v = wa * (*f.v)[a] + wb * (*f.v)[b] + wc * (*f.v)[c] + wd * (*f.v)[d];
-            s.corr = std::max(std::max((*f.corr)[a], (*f.corr)[b]), std::max((*f.corr)[c], (*f.corr)[d]));
-            return s;
-        }
-    }
-    // first-order expansion from the nearest valid node of the 4 x 4 block (ties: row-major first)
-    const float max_d = kTaylorMaxCells * (float)f.step;
-    float best = std::numeric_limits<float>::infinity();
-    int bi = -1, bj = -1;
-    for (int j = j0 - 1; j <= j0 + 2; ++j)
-        for (int i = i0 - 1; i <= i0 + 2; ++i) {
-            if (!f.in(i, j) || !V[f.idx(i, j)]) continue;
-            const float dx = x - f.nx(i), dy = y - f.ny(j);
-            const float d2 = dx * dx + dy * dy;
-            if (d2 < best) {
-                best = d2;
-                bi = i;
-                bj = j;
-            }
-        }
-    if (bi < 0 || best > max_d * max_d) return s;
-    const size_t k = f.idx(bi, bj);
-    const float dx = x - f.nx(bi), dy = y - f.ny(bj);
-    s.ok = true;
-    s.taylor = true;
-    s.u = (*f.u)[k] + (*f.ux)[k] * dx + (*f.uy)[k] * dy;
-    s.v = (*f.v)[k] + (*f.vx)[k] * dx + (*f.vy)[k] * dy;
-    s.corr = (*f.corr)[k];
-    return s;
-}
 
 double det3(const double m[3][3]) {
     return m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
@@ -389,23 +335,26 @@ int IncrementalSession::compose_on_device(std::vector<uint8_t>& flags, std::vect
     uint8_t* fl_d = c64_flags_.p;
     bool ok = hipMemcpyAsync(uf_d, ub_.data(), sizeof(float) * n0, hipMemcpyHostToDevice, st) == hipSuccess &&
               hipMemcpyAsync(uf_d + n0, vb_.data(), sizeof(float) * n0, hipMemcpyHostToDevice, st) == hipSuccess &&
-              hipMemcpyAsync(u64_d, ub64_.data(), sizeof(double) * n0, hipMemcpyHostToDevice, st) == hipSuccess &&
-              hipMemcpyAsync(u64_d + n0, vb64_.data(), sizeof(double) * n0, hipMemcpyHostToDevice, st) ==
-                  hipSuccess &&
               hipMemcpyAsync(fl_d, alive_.data(), n0, hipMemcpyHostToDevice, st) == hipSuccess;
+    if (ok && c64_)
+        ok = hipMemcpyAsync(u64_d, ub64_.data(), sizeof(double) * n0, hipMemcpyHostToDevice, st) == hipSuccess &&
+             hipMemcpyAsync(u64_d + n0, vb64_.data(), sizeof(double) * n0, hipMemcpyHostToDevice, st) == hipSuccess;
     if (!ok) return kErrInit;
-    ComposeTotals t{uf_d, uf_d + n0, u64_d, u64_d + n0, fl_d, gw_, gh_, p0_.rect_x, p0_.rect_y, p0_.step};
-    ComposeOut o{u64_d + 2 * n0, u64_d + 3 * n0, uf_d + 2 * n0, uf_d + 3 * n0, c64_corr_.p, fl_d + n0};
+    ComposeTotals t{uf_d, uf_d + n0, c64_ ? u64_d : nullptr, c64_ ? u64_d + n0 : nullptr, fl_d, gw_, gh_,
+                    p0_.rect_x, p0_.rect_y, p0_.step};
+    ComposeOut o{c64_ ? u64_d + 2 * n0 : nullptr, c64_ ? u64_d + 3 * n0 : nullptr, uf_d + 2 * n0, uf_d + 3 * n0,
+                 c64_corr_.p, fl_d + n0};
     if (launch_sample_compose64(eng_->last_dev_field, t, o, st) != hipSuccess) return kErrInit;
     flags.resize(n0);
-    u64.resize(n0);
-    v64.resize(n0);
+    u64.resize(c64_ ? n0 : 0);
+    v64.resize(c64_ ? n0 : 0);
     uf.resize(n0);
     vf.resize(n0);
     corr.resize(n0);
-    ok = hipMemcpyAsync(flags.data(), fl_d + n0, n0, hipMemcpyDeviceToHost, st) == hipSuccess &&
-         hipMemcpyAsync(u64.data(), u64_d + 2 * n0, sizeof(double) * n0, hipMemcpyDeviceToHost, st) == hipSuccess &&
-         hipMemcpyAsync(v64.data(), u64_d + 3 * n0, sizeof(double) * n0, hipMemcpyDeviceToHost, st) == hipSuccess &&
+    if (c64_)
+        ok = hipMemcpyAsync(u64.data(), u64_d + 2 * n0, sizeof(double) * n0, hipMemcpyDeviceToHost, st) == hipSuccess &&
+             hipMemcpyAsync(v64.data(), u64_d + 3 * n0, sizeof(double) * n0, hipMemcpyDeviceToHost, st) == hipSuccess;
+    ok = ok && hipMemcpyAsync(flags.data(), fl_d + n0, n0, hipMemcpyDeviceToHost, st) == hipSuccess &&
          hipMemcpyAsync(uf.data(), uf_d + 2 * n0, sizeof(float) * n0, hipMemcpyDeviceToHost, st) == hipSuccess &&
          hipMemcpyAsync(vf.data(), uf_d + 3 * n0, sizeof(float) * n0, hipMemcpyDeviceToHost, st) == hipSuccess &&
          hipMemcpyAsync(corr.data(), c64_corr_.p, sizeof(float) * n0, hipMemcpyDeviceToHost, st) == hipSuccess &&
@@ -580,12 +529,13 @@ int IncrementalSession::process(const uint8_t* def, float* out, int out_capacity
         const Engine::Field inc_field = eng_->last_field;
         const NodeField inc = node_field(inc_field);
         const GapDistance gd = gap_distance(inc);
-        // with FP64 composition, the sampling and the double sums of every
-        // followed node run on the device (coasting nodes stay below)
+        // the sampling of every followed node's increment and its sum with
+        // the node's total run on the device, in float as the reference adds
+        // them or in double with FP64 composition (coasting nodes stay below)
         std::vector<uint8_t> g_flags;
         std::vector<double> g_u64, g_v64;
         std::vector<float> g_uf, g_vf, g_corr;
-        if (c64_) {
+        {
             const int rcc = compose_on_device(g_flags, g_u64, g_v64, g_uf, g_vf, g_corr);
             if (rcc != 0) return rcc;
         }
@@ -611,13 +561,9 @@ int IncrementalSession::process(const uint8_t* def, float* out, int out_capacity
                 const float x0 = (float)(p0_.rect_x + gx * p0_.step), y0 = (float)(p0_.rect_y + gy * p0_.step);
                 const float px = x0 + ub_[i], py = y0 + vb_[i];
                 Sample smp;
-                if (c64_) {
-                    smp.ok = (g_flags[i] & 1) != 0;
-                    smp.taylor = (g_flags[i] & 2) != 0;
-                    smp.corr = g_corr[i];
-                } else {
-                    smp = sample_increment(inc, px, py);
-                }
+                smp.ok = (g_flags[i] & 1) != 0;
+                smp.taylor = (g_flags[i] & 2) != 0;
+                smp.corr = g_corr[i];
                 if (!smp.ok) {
                     coast(i, inc, px, py);
                     continue;
@@ -625,11 +571,9 @@ int IncrementalSession::process(const uint8_t* def, float* out, int out_capacity
                 if (c64_) {
                     U64[i] = g_u64[i];
                     V64[i] = g_v64[i];
-                    U[i] = g_uf[i];
-                    V[i] = g_vf[i];
-                } else {
-                    add_increment(i, smp.u, smp.v);
                 }
+                U[i] = g_uf[i];
+                V[i] = g_vf[i];
                 corr_of[i] = smp.corr;
                 need[i] = anchor_need(inc, gd, px, py, subset);
                 if (coast_travel_[i] > trusted_travel) {
