@@ -459,6 +459,7 @@ int Engine::solve_tasks(const std::vector<Task>& tasks, std::vector<SolveOut>& o
     ck(hipMemcpyAsync(tasks_d_.p, pin_tasks_.p, sizeof(Task) * n, hipMemcpyHostToDevice, st_));
     hipEvent_t ev_fork = class_ev_[2 * ns];
     ck(hipEventRecord(ev_fork, st_));
+    std::vector<SolveArgs> sargs(ns);
     for (size_t i = 0; i < ns; ++i) {
         hipStream_t cs = i == 0 ? st_ : class_st_[i - 1];
         if (i > 0) ck(hipStreamWaitEvent(cs, ev_fork, 0));
@@ -479,7 +480,12 @@ int Engine::solve_tasks(const std::vector<Task>& tasks, std::vector<SolveOut>& o
         a.dim = segs[i].dim;
         a.lm = 1;
         a.tol = tol;
-        ck(launch_solve(a, use6, def_f32_ready_, cs));
+        sargs[i] = a;
+        ck(launch_solve(a, use6, def_f32_ready_, cs, kSolveFast));
+    }
+    for (size_t i = 0; i < ns; ++i) {
+        hipStream_t cs = i == 0 ? st_ : class_st_[i - 1];
+        ck(launch_solve(sargs[i], use6, def_f32_ready_, cs, kSolveFull));
         if (i > 0) {
             ck(hipEventRecord(class_ev_[2 * i + 1], cs));
             ck(hipStreamWaitEvent(st_, class_ev_[2 * i + 1], 0));
@@ -1349,8 +1355,12 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
             ensure_class_streams(nc);
             std::vector<std::vector<int>> keys(nc);
             std::vector<int> shapes(nc);
+            std::vector<SolveArgs> cargs(nc);
             hipEvent_t ev_fork = class_ev_[2 * nc], ev_done = class_ev_[2 * nc + 1];
             ck(hipEventRecord(ev_fork, st_));
+            // every class's fast path first, then every class's full solver:
+            // streams that share a hardware queue then never hold a fast path
+            // behind another class's slow subsets
             for (size_t ci = 0; ci < nc; ++ci) {
                 const DimClass& c = classes_[ci];
                 const bool listed = c.off >= 0;
@@ -1368,8 +1378,13 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                 ac.nbail = ci == 0 ? counters_.p + 8 : class_nbail_.p + ci;
                 ac.wpb = kShapes[shapes[ci]].wpb;
                 ac.tile = listed ? 0 : kShapes[shapes[ci]].tile;
+                cargs[ci] = ac;
                 ck(hipEventRecord(class_ev_[2 * ci], cs));
-                ck(launch_solve(ac, p.use6, def_f32_ready_, cs));
+                ck(launch_solve(ac, p.use6, def_f32_ready_, cs, kSolveFast));
+            }
+            for (size_t ci = 0; ci < nc; ++ci) {
+                hipStream_t cs = ci == 0 ? st_ : class_st_[ci - 1];
+                ck(launch_solve(cargs[ci], p.use6, def_f32_ready_, cs, kSolveFull));
                 ck(hipEventRecord(class_ev_[2 * ci + 1], cs));
             }
             for (size_t ci = 1; ci < nc; ++ci) ck(hipStreamWaitEvent(st_, class_ev_[2 * ci + 1], 0));

@@ -5,11 +5,11 @@
 
 namespace sdic {
 
-hipError_t launch_solve_bu8(const SolveArgs& a, int ppl, hipStream_t st);
-hipError_t launch_solve_ku8(const SolveArgs& a, int ppl, hipStream_t st);
-hipError_t launch_solve_bf32(const SolveArgs& a, int ppl, hipStream_t st);
-hipError_t launch_solve_kf32(const SolveArgs& a, int ppl, hipStream_t st);
-hipError_t launch_solve_cf32(const SolveArgs& a, int ppl, hipStream_t st);
+hipError_t launch_solve_bu8(const SolveArgs& a, int ppl, hipStream_t st, int phases);
+hipError_t launch_solve_ku8(const SolveArgs& a, int ppl, hipStream_t st, int phases);
+hipError_t launch_solve_bf32(const SolveArgs& a, int ppl, hipStream_t st, int phases);
+hipError_t launch_solve_kf32(const SolveArgs& a, int ppl, hipStream_t st, int phases);
+hipError_t launch_solve_cf32(const SolveArgs& a, int ppl, hipStream_t st, int phases);
 
 int ppl_for(int dim) {
     if (dim < 3 || dim > kMaxSubset) return -1;
@@ -40,13 +40,13 @@ hipError_t launch_sum_stats(const int* slots, int* out, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_solve(const SolveArgs& a, bool use6, bool def_f32, hipStream_t st) {
+hipError_t launch_solve(const SolveArgs& a, bool use6, bool def_f32, hipStream_t st, int phases) {
     const int ppl = ppl_for(a.dim);
     if (ppl < 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, a.nbail, 1);
-    if (def_f32 && a.defc && !use6) return launch_solve_cf32(a, ppl, st);
-    if (def_f32) return use6 ? launch_solve_kf32(a, ppl, st) : launch_solve_bf32(a, ppl, st);
-    return use6 ? launch_solve_ku8(a, ppl, st) : launch_solve_bu8(a, ppl, st);
+    if (phases & kSolveFast) hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, a.nbail, 1);
+    if (def_f32 && a.defc && !use6) return launch_solve_cf32(a, ppl, st, phases);
+    if (def_f32) return use6 ? launch_solve_kf32(a, ppl, st, phases) : launch_solve_bf32(a, ppl, st, phases);
+    return use6 ? launch_solve_ku8(a, ppl, st, phases) : launch_solve_bu8(a, ppl, st, phases);
 }
 
 }  // namespace sdic

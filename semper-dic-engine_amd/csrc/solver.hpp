@@ -79,7 +79,11 @@ int ppl_for(int dim);
 // items it handed on. a.bail must hold nitems ints; a.nbail is zeroed here.
 // With a.defc set (and Keys 4x4 over an FP32 plane), the fast path's
 // interior samples read the coefficient plane instead.
-hipError_t launch_solve(const SolveArgs& a, bool use6, bool def_f32, hipStream_t st);
+// phases: the fast path (with the hand-off counter's clear) and the full
+// solver, or one of them (the full solver after the fast path of the same
+// arguments on the same stream).
+constexpr int kSolveFast = 1, kSolveFull = 2, kSolveBoth = 3;
+hipError_t launch_solve(const SolveArgs& a, bool use6, bool def_f32, hipStream_t st, int phases = kSolveBoth);
 
 // prep.hip
 hipError_t launch_prep_ref_u8(const uint8_t* img, const uint8_t* mask, int w, int h, float* I, float* gx,

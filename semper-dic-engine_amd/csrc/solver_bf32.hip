@@ -3,7 +3,7 @@
 
 namespace sdic {
 
-hipError_t launch_solve_bf32(const SolveArgs& a, int ppl, hipStream_t st) {
+hipError_t launch_solve_bf32(const SolveArgs& a, int ppl, hipStream_t st, int phases) {
     if (a.nitems <= 0) return hipSuccess;
     const int wpb = a.wpb == 2 || a.wpb == 4 ? a.wpb : 1;
     const dim3 block(64 * kSlowWaves), fast_block(64 * wpb), fast_grid((a.nitems + wpb - 1) / wpb),
@@ -11,8 +11,8 @@ hipError_t launch_solve_bf32(const SolveArgs& a, int ppl, hipStream_t st) {
     switch (ppl) {
 #define SDIC_CASE(P)                                                          \
     case P:                                                                   \
-        hipLaunchKernelGGL((k_fast<P, false, float>), fast_grid, fast_block, 0, st, a); \
-        hipLaunchKernelGGL((k_slow<P, false, float, kSlowWaves>), slow_grid, block, 0, st, a); \
+        if (phases & 1) hipLaunchKernelGGL((k_fast<P, false, float>), fast_grid, fast_block, 0, st, a); \
+        if (phases & 2) hipLaunchKernelGGL((k_slow<P, false, float, kSlowWaves>), slow_grid, block, 0, st, a); \
         break;
         SDIC_CASE(4)
         SDIC_CASE(8)
