@@ -1,6 +1,9 @@
 // Dispatch of the subset solver over its instantiations, and the pixels-per-lane
 // class of a subset size.
 #include "solver.hpp"
+
+#include <cmath>
+
 #include "wave.hpp"
 
 namespace sdic {
@@ -40,7 +43,20 @@ hipError_t launch_sum_stats(const int* slots, int* out, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_solve(const SolveArgs& a, bool use6, bool def_f32, hipStream_t st, int phases) {
+// The least float x with sqrtf(x) >= tol (correctly rounded, as on the
+// device): sqrtf is monotone, so sqrtf(x) < tol exactly when x < the result.
+static float stop_threshold(float tol) {
+    if (!(tol > 0.0f)) return tol == 0.0f ? 0.0f : tol;   // 0: never below; NaN: never
+    if (std::isinf(tol)) return tol;
+    float x = (float)((double)tol * (double)tol);
+    while (x > 0.0f && std::sqrt(std::nextafter(x, 0.0f)) >= tol) x = std::nextafter(x, 0.0f);
+    while (std::sqrt(x) < tol) x = std::nextafter(x, INFINITY);
+    return x;
+}
+
+hipError_t launch_solve(const SolveArgs& a_in, bool use6, bool def_f32, hipStream_t st, int phases) {
+    SolveArgs a = a_in;
+    a.stop2 = stop_threshold(a.tol);
     const int ppl = ppl_for(a.dim);
     if (ppl < 0) return hipErrorInvalidValue;
     if (phases & kSolveFast) hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, a.nbail, 1);

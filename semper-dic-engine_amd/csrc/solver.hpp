@@ -58,6 +58,9 @@ struct SolveArgs {
     const uint4* defc;        // row-cubic coefficient plane of the same frame (Keys 4x4 fast path), or null
     int w, h, dim, lm;
     float tol;
+    // set by launch_solve: the least float whose correctly rounded square root
+    // is at least tol, so that sqrtf(x) < tol exactly when x < stop2
+    float stop2;
     // launch shape: subsets (waves) per fast-path workgroup (0 or 1: one),
     // and Path A's node order (0: row-major; T > 0: T x T tiles)
     int wpb, tile;

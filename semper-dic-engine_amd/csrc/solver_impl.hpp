@@ -1175,7 +1175,7 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
                                            const uint16_t* __restrict__ gx16, const uint16_t* __restrict__ gy16, bool bf16,
                                            const double* __restrict__ chol,
                                            const TD* __restrict__ def, const uint4* __restrict__ defc,
-                                           int w, int h, int dim, bool lm, float conv_tol, SolveOut& out) {
+                                           int w, int h, int dim, bool lm, float stop2, SolveOut& out) {
     const int lane = threadIdx.x & 63;
     const PoolEntry* pe = pool + __builtin_amdgcn_readfirstlane(t.pool);
     out.init = 0;
@@ -1199,22 +1199,33 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
         // past the subset's last pixel reads the centre (and keeps zeros) and
         // samples the centre in the iterations (its sample is left out of
         // every sum).
+        // Pixel (q, r) of the subset (row q, column r) is q w + r past its
+        // top-left pixel; the offsets advance by additions only.
         int at[PPL];
-        int q = lane / dim, r = lane % dim;
+        int q = lane / dim, r = lane - (lane / dim) * dim;
+        const int ctr = t.cy * w + t.cx;
+        const int adv = q64 * w + r64, wrap = w - dim;
+        int off = ctr - half * w - half + q * w + r;
 #pragma unroll
         for (int j = 0; j < PPL; ++j) {
             const bool in = j < kFull || lane + 64 * j < n;
             xy[j] = in ? sdic_h2{(_Float16)(r - half), (_Float16)(q - half)} : sdic_h2{0, 0};
-            at[j] = in ? (t.cy + q - half) * w + t.cx + r - half : t.cy * w + t.cx;
+            at[j] = in ? off : ctr;
             r += r64;
             q += q64;
+            off += adv;
             if (r >= dim) {
                 r -= dim;
                 q++;
+                off += wrap;
             }
         }
+        // the three planes through buffer resources: one 32-bit byte offset per
+        // pixel serves all three loads
+        const int plane_bytes = (int)((size_t)w * h * sizeof(float));
+        const __amdgpu_buffer_rsrc_t rI = __builtin_amdgcn_make_buffer_rsrc((void*)refI, (short)0, plane_bytes, 0x00020000);
 #pragma unroll
-        for (int j = 0; j < PPL; ++j) nref[j] = refI[at[j]];
+        for (int j = 0; j < PPL; ++j) nref[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rI, at[j] * 4, 0, 0));
         if (gx16) {
 #pragma unroll
             for (int j = 0; j < PPL; ++j) {
@@ -1222,10 +1233,14 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
                 gys[j] = half_bits_to_float(gy16[at[j]], bf16);
             }
         } else {
+            const __amdgpu_buffer_rsrc_t rX =
+                __builtin_amdgcn_make_buffer_rsrc((void*)refGx, (short)0, plane_bytes, 0x00020000);
+            const __amdgpu_buffer_rsrc_t rY =
+                __builtin_amdgcn_make_buffer_rsrc((void*)refGy, (short)0, plane_bytes, 0x00020000);
 #pragma unroll
             for (int j = 0; j < PPL; ++j) {
-                gxs[j] = refGx[at[j]];
-                gys[j] = refGy[at[j]];
+                gxs[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rX, at[j] * 4, 0, 0));
+                gys[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rY, at[j] * 4, 0, 0));
             }
         }
 #pragma unroll
@@ -1264,7 +1279,18 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
     const bool textured = fminf(wave_sum(tgx), wave_sum(tgy)) > 0.0f;
     const int min_valid = (int)((float)n * kMinValidPixelFraction);
     const double cxd = (double)t.cx, cyd = (double)t.cy, hd = (double)half;
-    const float cxf = (float)t.cx, cyf = (float)t.cy;
+    const float cxf = (float)t.cx, cyf = (float)t.cy, hf = (float)half;
+    // a / n correctly rounded without a division (a sum over the subset's n
+    // pixels; a sampled iteration reaching the sums has all n valid): with
+    // rn = RN(1 / n), q = RN(a rn) is within an ulp of a / n, the residual
+    // a - q n is exact as an FMA, and q + (a - q n) rn rounds to RN(a / n),
+    // the same bits as the division (checked exhaustively for every odd side
+    // 3 .. 83 and every non-negative finite float a; the sums are never negative)
+    const float fn = (float)n, rn = 1.0f / fn;
+    const auto div_n = [&](float a) {
+        const float q = a * rn;
+        return __builtin_fmaf(__builtin_fmaf(-q, fn, a), rn, q);
+    };
     const __amdgpu_buffer_rsrc_t rs =
         COEF ? __builtin_amdgcn_make_buffer_rsrc((void*)defc, (short)0, (int)((size_t)w * h * 16), 0x00020000)
              : __builtin_amdgcn_make_buffer_rsrc((void*)def, (short)0, (int)((size_t)w * h * sizeof(TD)), 0x00020000);
@@ -1277,9 +1303,20 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
         const double w0 = W[0], w1 = W[1], w2 = W[2], w3 = W[3], w4 = W[4], w5 = W[5];
         // Interior: the four warped corners lie 1 px inside the 4 px guard, so
         // (the map being affine) every pixel does, and no pixel test is needed.
-        // each lane tests one of the four corners; interior when none is out
+        // each lane tests one of the four corners; interior when none is out.
+        // The corners in FP32 first: where they are within the frame they are
+        // within 0.02 px of the FP64 corners, so corners 1 px further inside
+        // than the test asks pass it in FP64 too, and only a subset near the
+        // guard takes the FP64 test.
         bool interior;
         {
+            const float xs = (lane & 1) ? hf : -hf, ys = (lane & 2) ? hf : -hf;
+            const float gx = cxf + __builtin_fmaf(W[0], xs, __builtin_fmaf(W[1], ys, W[2]));
+            const float gy = cyf + __builtin_fmaf(W[3], xs, __builtin_fmaf(W[4], ys, W[5]));
+            const bool sure = gx >= 5.5f && gx < (float)(w - 7) && gy >= 5.5f && gy < (float)(h - 7);
+            interior = __builtin_amdgcn_ballot_w64(!sure) == 0;
+        }
+        if (!interior) {
             const double x = (lane & 1) ? hd : -hd, y = (lane & 2) ? hd : -hd;
             const float fx = warped_fma(cxd, w0, w1, w2, x, y), fy = warped_fma(cyd, w3, w4, w5, x, y);
             const bool in = fx + 0.5f >= 5.0f && fx + 0.5f < (float)(w - 5) && fy + 0.5f >= 5.0f &&
@@ -1360,14 +1397,14 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
             if (nv != n) return false;   // a masked iteration
         }
         sum = wave_sum(sum);
-        const float mean = sum / (float)nv;
+        const float mean = div_n(sum);
         float s2 = 0.0f;
 #pragma unroll
         for (int j = 0; j < PPL; ++j) {
             const float d = val[j] - mean;
             s2 = (j < kFull || lane + 64 * j < n) ? __builtin_fmaf(d, d, s2) : s2;
         }
-        float dstd = sqrtf(wave_sum(s2) / (float)nv);
+        float dstd = sqrtf(div_n(wave_sum(s2)));
         if (dstd < 1e-5f) dstd = 1.0f;
         const float inv = 1.0f / dstd;
         float dp[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
@@ -1381,7 +1418,7 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
                 const float d = (j < kFull || lane + 64 * j < n) ? nref[j] - (val[j] - mean) * inv : 0.0f;
                 e = __builtin_fmaf(d, d, e);
             }
-            return wave_sum(e) / (float)nv;
+            return div_n(wave_sum(e));
         };
         // branch-free: a lane past the last pixel has d = 0 (and zero
         // gradients and offsets), so it adds exact zeros
@@ -1399,8 +1436,7 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
             dp[4] = __builtin_fmaf(hdd, x, dp[4]);
             dp[5] = __builtin_fmaf(hdd, y, dp[5]);
         }
-#pragma unroll
-        for (int k = 0; k < 6; ++k) dp[k] = wave_sum(dp[k]);
+        wave_sum6(dp);
         float d[6];
         if (use_chol) {
             chol64_step(ch, dp, d);
@@ -1443,7 +1479,7 @@ __device__ __forceinline__ bool fast_solve(const Task& t, const PoolEntry* __res
         float nrm = 0.0f;
 #pragma unroll
         for (int k = 0; k < 6; ++k) nrm += d[k] * d[k];
-        if (sqrtf(nrm) < conv_tol) {
+        if (nrm < stop2) {   // sqrtf(nrm) < tol
             iters = it + 1;
             final_score = znssd();
             if (!textured || !finite_f(final_score)) {
@@ -1586,7 +1622,7 @@ __global__ __launch_bounds__(256) SDIC_FAST_ATTR void k_fast(SolveArgs a) {
     SolveOut o = {};
     const bool fin = fast_solve<PPL, K6, TD, COEF>(t, a.pool, a.refI, a.refGx, a.refGy, a.refGx16, a.refGy16,
                                                    a.grad_bf16 != 0, a.chol, (const TD*)a.def, a.defc, a.w, a.h,
-                                                   a.dim, a.lm != 0, a.tol, o);
+                                                   a.dim, a.lm != 0, a.stop2, o);
     if ((threadIdx.x & 63) != 0) return;
     if (!fin) {
         a.bail[atomicAdd(a.nbail, 1)] = id;

@@ -36,6 +36,41 @@ __device__ __forceinline__ float wave_sum(float v) {
     return __uint_as_float(h[0]) + __uint_as_float(h[1]);
 }
 
+// Six sums at once, each the same bits as wave_sum of that value: the four
+// row steps per value, then the swap steps on pairs of values. A
+// v_permlane16_swap of values a and b leaves the row pairs (0, 1) and (2, 3)
+// of a in rows 0 and 2 and those of b in rows 1 and 3, so one swap and one add
+// serve two values; a v_permlane32_swap of two such results leaves the totals
+// of four values in rows 0 .. 3. Each total is read from a lane that holds it.
+__device__ __forceinline__ void wave_sum6(float (&v)[6]) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        float x = v[k];
+        x += __uint_as_float(dpp_u<0xB1>(__float_as_uint(x)));
+        x += __uint_as_float(dpp_u<0x4E>(__float_as_uint(x)));
+        x += __uint_as_float(dpp_u<0x141>(__float_as_uint(x)));
+        x += __uint_as_float(dpp_u<0x140>(__float_as_uint(x)));
+        v[k] = x;
+    }
+    float s[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v[2 * i]), __float_as_uint(v[2 * i + 1]),
+                                                        false, false);
+        s[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
+    const auto h = __builtin_amdgcn_permlane32_swap(__float_as_uint(s[0]), __float_as_uint(s[1]), false, false);
+    const uint32_t t = __float_as_uint(__uint_as_float(h[0]) + __uint_as_float(h[1]));
+    const auto g = __builtin_amdgcn_permlane32_swap(__float_as_uint(s[2]), __float_as_uint(s[2]), false, false);
+    const uint32_t u = __float_as_uint(__uint_as_float(g[0]) + __uint_as_float(g[1]));
+    v[0] = __uint_as_float((uint32_t)__builtin_amdgcn_readlane((int)t, 0));
+    v[1] = __uint_as_float((uint32_t)__builtin_amdgcn_readlane((int)t, 16));
+    v[2] = __uint_as_float((uint32_t)__builtin_amdgcn_readlane((int)t, 32));
+    v[3] = __uint_as_float((uint32_t)__builtin_amdgcn_readlane((int)t, 48));
+    v[4] = __uint_as_float((uint32_t)__builtin_amdgcn_readlane((int)u, 0));
+    v[5] = __uint_as_float((uint32_t)__builtin_amdgcn_readlane((int)u, 16));
+}
+
 __device__ __forceinline__ int wave_sum_i(int v) {
     v += (int)dpp_u<0xB1>((uint32_t)v);
     v += (int)dpp_u<0x4E>((uint32_t)v);
