@@ -477,6 +477,8 @@ def main():
                     help="the solver samples the u8 deformed frame instead of its FP32 copy (same field)")
     ap.add_argument("--coef-plane", action="store_true",
                     help="Keys 4x4 from the fp16 row-cubic coefficient plane of each frame (flag 17)")
+    ap.add_argument("--flag", action="append", default=[], metavar="F=V",
+                    help="another engine switch for the main line's engines (sdk_engine_set_flag F V)")
     ap.add_argument("--tol", type=float, default=None, help="IC-GN stop rule ||dp|| < tol (default: the workload's)")
     ap.add_argument("--learned-seed", action="store_true",
                     help="the main line's engine seeds its anchors with the learned seeder (the committed weights)")
@@ -535,7 +537,8 @@ def main():
             raise RuntimeError("seeder weights missing: " + sa.SEEDER_WEIGHTS)
         assert dev.sdk_engine_set_flag(eng, 5, 1) == 0
     # the main line's engine switches, applied to every engine of the same solve
-    engine_flags = ([(13, 0)] if args.u8_deformed else []) + ([(17, 1)] if args.coef_plane else [])
+    engine_flags = ([(13, 0)] if args.u8_deformed else []) + ([(17, 1)] if args.coef_plane else []) + \
+        [tuple(int(x) for x in f.split("=")) for f in args.flag]
     for f, v in engine_flags:
         assert dev.sdk_engine_set_flag(eng, f, v) == 0
     assert dev.sdk_engine_set_reference(eng, dref, 1, w, h, None) == 0
