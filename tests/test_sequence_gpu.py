@@ -80,12 +80,22 @@ def test_sequence_mixed_sizes_against_truth_and_plain_sessions(env):
     print(f"mixed, frame {K}: {len(last)} points (frame 1: {first}), rms {np.sqrt(np.mean(err ** 2)):.3f} px, "
           f"median {np.median(err):.3f}, mean |dExx| {np.mean(np.abs(last[:, 4] - Exx)):.4f}, "
           f"|dEyy| {np.mean(np.abs(last[:, 5] - Eyy)):.4f}", flush=True)
-    assert np.sqrt(np.mean(err ** 2)) <= 1.0 and np.median(err) <= 0.5
+    # 63 updates of IC-GN's per-increment bias; the size-15 regions carry most
+    # of it (0.52 px rms, median 0.28 when measured)
+    assert np.sqrt(np.mean(err ** 2)) <= 0.6 and np.median(err) <= 0.35
     assert abs(np.median(last[:, 4]) - Exx) <= 0.01 and abs(np.median(last[:, 5]) - Eyy) <= 0.01
     reg = region_of(cfg, last)
     sizes = wl.region_map(NAME).ravel()
     for s in sorted(set(sizes.tolist())):
         plain = session(sa, dev, wl, cfg, frames, subset=s)[-1]
+        if s == 21:
+            # the reference's long-horizon bound for update interval 1 at
+            # subset 21 (test_incremental.cpp:619, 0.45 px on rms u), here on
+            # the rms of |(du, dv)|
+            up, vp, _, _ = truth(wl, cfg, K, plain)
+            ep = np.sqrt(np.mean((plain[:, 2] - up) ** 2 + (plain[:, 3] - vp) ** 2))
+            print(f"plain subset 21, frame {K}: {len(plain)} points, rms {ep:.3f} px", flush=True)
+            assert ep <= 0.45, ep
         pr = region_of(cfg, plain)
         for r in np.nonzero(sizes == s)[0]:
             a, b = last[reg == r], plain[pr == r]
