@@ -149,7 +149,9 @@ SDK_API int sdk_compose64_probe(const float* planes, const uint8_t* med, int fgw
    keep the direct form); flag 18 = the layered Path B's frontier (sources,
    breadth-first search, claims, updates) on the device (default 1; for reach
    <= 16 on one subset size; 0: on the host; the field is the same either
-   way). */
+   way); flag 19 = under learned seeding (flag 5), the CNN lattice on every
+   frame (default 0: only where the global model finds more than a
+   near-translation or the plain lattice's anchors do not converge). */
 SDK_API int sdk_engine_set_flag(void* h, int flag, int value);
 
 /* Phase timings of the last run, 14 doubles: total, prep, seeding, prepass,
