@@ -829,7 +829,9 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
         std::vector<int> centres;
         patch_centres(p.rect_x, p.rect_y, p.rect_w, p.rect_h, centres);
         const int npatch = (int)centres.size() / 2;
-        if (!centres_d_.reserve(centres.size()) || !patch_d_.reserve(npatch)) return kErrInit;
+        if (!centres_d_.reserve(centres.size()) || !patch_d_.reserve(npatch) ||
+            !patch_scratch_.reserve(patch_scratch_bytes(npatch)))
+            return kErrInit;
         ck(hipMemcpyAsync(centres_d_.p, centres.data(), sizeof(int) * centres.size(), hipMemcpyHostToDevice, st_));
         std::vector<PatchOut> po(npatch);
         GlobalModel gm;
@@ -837,7 +839,7 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
         for (const int rad : {kCoarseRadius, kCoarseRadiusWide}) {
             if (!global_search) break;
             pd.coarse_radius = rad;
-            ck(launch_patch_search(pd, centres_d_.p, npatch, patch_d_.p, st_));
+            ck(launch_patch_search(pd, centres_d_.p, npatch, patch_d_.p, patch_scratch_.p, st_));
             ck(hipMemcpyAsync(po.data(), patch_d_.p, sizeof(PatchOut) * npatch, hipMemcpyDeviceToHost, st_));
             ck(hipStreamSynchronize(st_));
             gm = fit_global_model(centres, po, p.rect_x, p.rect_y, p.rect_w, p.rect_h);
@@ -892,7 +894,7 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                 pw.f[3] = 1.0 + ra[5];
                 pw.start = canon_start_.p;
                 pw.coarse_radius = 2;
-                ck(launch_patch_search(pw, centres_d_.p, npatch, patch_d_.p, st_));
+                ck(launch_patch_search(pw, centres_d_.p, npatch, patch_d_.p, patch_scratch_.p, st_));
                 ck(hipMemcpyAsync(po.data(), patch_d_.p, sizeof(PatchOut) * npatch, hipMemcpyDeviceToHost, st_));
                 ck(hipStreamSynchronize(st_));
                 gm = fit_global_model(centres, po, p.rect_x, p.rect_y, p.rect_w, p.rect_h);

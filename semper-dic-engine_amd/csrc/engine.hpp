@@ -448,6 +448,7 @@ private:
     DevBuf<int32_t> celltri_d_, lat_d_;
     DevBuf<int32_t> centres_d_;
     DevBuf<PatchOut> patch_d_;
+    DevBuf<uint8_t> patch_scratch_;   // the coarsest level's shared candidates
     // the rotation-invariant pass: level centres, descriptions and
     // orientations, scores, best matches, the refinement's start shifts
     DevBuf<int32_t> canon_centres_d_, canon_start_;

@@ -614,9 +614,50 @@ __device__ __forceinline__ bool better(const Cand& a, const Cand& b) {
 // over the waves, and better() is a total order, so the choice is the serial
 // scan's.
 constexpr int kPatchWaves = 16;
+// The coarsest level holds most of the candidates ((2 r + 1)^2 against 25 at
+// each finer level): it is shared by kPatchSplit blocks per patch, so that
+// the GPU's CUs, not one per patch, scan it. Each block keeps the best of its
+// share by better(), and the patch's block takes the best of those, the same
+// candidate as one block's scan (a total order has one maximum).
+constexpr int kPatchSplit = 4;
 
+__global__ __launch_bounds__(64 * kPatchWaves) void k_patch_coarse(PyrDev p, const int* __restrict__ centres,
+                                                                   Cand* __restrict__ cand) {
+    __shared__ Cand red[kPatchWaves];
+    const int pi = blockIdx.x, part = blockIdx.y, lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int px = centres[2 * pi], py = centres[2 * pi + 1];
+    const int l = p.levels;
+    const int rad = p.coarse_radius, side = 2 * rad + 1, ns = side * side;
+    const int csx = p.start ? p.start[2 * pi] : 0, csy = p.start ? p.start[2 * pi + 1] : 0;
+    Cand best;
+    best.any = 0;
+    for (int k = part * kPatchWaves + wid; k < ns; k += kPatchWaves * kPatchSplit) {
+        const int sy = csy - rad + k / side, sx = csx - rad + k % side;
+        double sc;
+        if (!zncc(p, l, px >> l, py >> l, sx, sy, sc, nullptr)) continue;
+        Cand c;
+        c.s = sc;
+        c.r2 = (long)sx * sx + (long)sy * sy;
+        c.order = k;
+        c.sx = sx;
+        c.sy = sy;
+        c.any = 1;
+        if (better(c, best)) best = c;
+    }
+    if (lane == 0) red[wid] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Cand b = red[0];
+        for (int k = 1; k < kPatchWaves; ++k)
+            if (better(red[k], b)) b = red[k];
+        cand[pi * kPatchSplit + part] = b;
+    }
+}
+
+// coarse: the coarsest level's candidates from k_patch_coarse, or null (this
+// block scans that level too)
 __global__ __launch_bounds__(64 * kPatchWaves) void k_patch(PyrDev p, const int* __restrict__ centres,
-                                                            PatchOut* out) {
+                                                            const Cand* __restrict__ coarse, PatchOut* out) {
     __shared__ Cand red[kPatchWaves];
     __shared__ int s_sx, s_sy, s_ok;
     __shared__ double s_best;
@@ -637,6 +678,22 @@ __global__ __launch_bounds__(64 * kPatchWaves) void k_patch(PyrDev p, const int*
     __syncthreads();
     for (int l = L; l >= 0; --l) {
         if (!s_ok) break;
+        if (l == L && coarse) {
+            if (threadIdx.x == 0) {
+                Cand b = coarse[pi * kPatchSplit];
+                for (int k = 1; k < kPatchSplit; ++k)
+                    if (better(coarse[pi * kPatchSplit + k], b)) b = coarse[pi * kPatchSplit + k];
+                if (!b.any) {
+                    s_ok = 0;
+                } else {
+                    s_best = b.s;
+                    s_sx = b.sx * (l > 0 ? 2 : 1);
+                    s_sy = b.sy * (l > 0 ? 2 : 1);
+                }
+            }
+            __syncthreads();
+            continue;
+        }
         const int rad = (l == L) ? p.coarse_radius : kFineRadius;
         const int side = 2 * rad + 1, ns = side * side;
         const int csx = s_sx, csy = s_sy;
@@ -1055,10 +1112,18 @@ hipError_t launch_downsample_f32(const float* src, int w, int h, float* dst, hip
     return hipGetLastError();
 }
 
-hipError_t launch_patch_search(const PyrDev& p, const int* centres, int npatch, PatchOut* out,
+size_t patch_scratch_bytes(int npatch) { return npatch > 0 ? (size_t)npatch * kPatchSplit * sizeof(Cand) : 0; }
+
+hipError_t launch_patch_search(const PyrDev& p, const int* centres, int npatch, PatchOut* out, void* scratch,
                                hipStream_t st) {
     if (npatch <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_patch, dim3(npatch), dim3(64 * kPatchWaves), 0, st, p, centres, out);
+    const int side = 2 * p.coarse_radius + 1;
+    // split only a coarsest level of more candidates than the waves of one block take in a few rounds
+    Cand* coarse = scratch && side * side > 4 * kPatchWaves ? (Cand*)scratch : nullptr;
+    if (coarse)
+        hipLaunchKernelGGL(k_patch_coarse, dim3(npatch, kPatchSplit), dim3(64 * kPatchWaves), 0, st, p, centres,
+                           coarse);
+    hipLaunchKernelGGL(k_patch, dim3(npatch), dim3(64 * kPatchWaves), 0, st, p, centres, (const Cand*)coarse, out);
     return hipGetLastError();
 }
 

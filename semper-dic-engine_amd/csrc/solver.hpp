@@ -149,8 +149,12 @@ struct PatchOut {
     double best, tstd, xm, xp, ym, yp;
     int has_x, has_y, pad2[2];
 };
-hipError_t launch_patch_search(const PyrDev& p, const int* centres, int npatch, PatchOut* out,
+hipError_t launch_patch_search(const PyrDev& p, const int* centres, int npatch, PatchOut* out, void* scratch,
                                hipStream_t st);
+// Bytes of the scratch launch_patch_search needs for npatch patches (the
+// coarsest level's candidates of the blocks that share a patch; with a null
+// scratch one block searches each patch alone).
+size_t patch_scratch_bytes(int npatch);
 
 // Rotation-invariant patch matching (the global-motion search for motions the
 // translation search cannot follow): each reference patch and every position
