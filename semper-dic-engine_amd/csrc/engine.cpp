@@ -539,6 +539,78 @@ int Engine::solve_tasks_dim(const std::vector<Task>& tasks, std::vector<SolveOut
     return 0;
 }
 
+int Engine::solve_tasks_spec(const std::vector<Task>& t1, const std::vector<Task>& t2, std::vector<SolveOut>& o1,
+                             std::vector<SolveOut>& o2, std::vector<uint8_t>& handed, bool use6, int dim,
+                             float tol) {
+    const int n = (int)t1.size();
+    o1.assign(n, SolveOut{});
+    o2.assign(n, SolveOut{});
+    handed.assign(n, 0);
+    if (n == 0) return 0;
+    if ((int)t2.size() != n) return kErrInit;
+    const size_t n2 = 2 * (size_t)n;
+    if (!tasks_d_.reserve(n2) || !out_d_.reserve(n2) || !bail_.reserve(n2) || !counters_.reserve(32)) return kErrInit;
+    if (!pin_tasks_.reserve(n2) || !pin_res_.reserve(n2) || !pin_handed_.reserve(2 + (size_t)n)) return kErrInit;
+    ensure_class_streams(2);
+    std::memcpy(pin_tasks_.p, t1.data(), sizeof(Task) * n);
+    std::memcpy(pin_tasks_.p + n, t2.data(), sizeof(Task) * n);
+    ck(hipMemcpyAsync(tasks_d_.p, pin_tasks_.p, sizeof(Task) * n2, hipMemcpyHostToDevice, st_));
+    SolveArgs a = {};
+    a.tasks = tasks_d_.p;
+    a.out = out_d_.p;
+    a.nitems = n;
+    a.bail = bail_.p;
+    a.nbail = counters_.p + 8;
+    a.pool = pool_.p;
+    a.refI = I_.p;
+    a.refGx = gx_.p;
+    a.refGy = gy_.p;
+    set_precision_args(a);
+    set_def_args(a);
+    a.w = w_;
+    a.h = h_;
+    a.dim = dim;
+    a.lm = 1;
+    a.tol = tol;
+    ck(launch_solve(a, use6, def_f32_ready_, st_, kSolveFast));
+    ck(hipMemcpyAsync(pin_handed_.p, a.nbail, sizeof(int32_t), hipMemcpyDeviceToHost, st_));
+    ck(hipStreamSynchronize(st_));
+    const int nb = pin_handed_.p[0];
+    ck(launch_solve(a, use6, def_f32_ready_, st_, kSolveFull));
+    if (nb > 0) {
+        // the second set over the handed-on items: item id k is task k of the
+        // second set (tasks_d_ + n) and result slot k of its half of out_d_;
+        // the first set's fast path is complete (the stream was synchronised)
+        hipStream_t cs = class_st_[0];
+        SolveArgs b = a;
+        b.tasks = tasks_d_.p + n;
+        b.out = out_d_.p + n;
+        b.items = bail_.p;
+        b.nitems = nb;
+        b.bail = bail_.p + n;
+        b.nbail = class_nbail_.p + 1;
+        ck(launch_solve(b, use6, def_f32_ready_, cs, kSolveBoth));
+        ck(hipEventRecord(class_ev_[1], cs));
+        ck(hipStreamWaitEvent(st_, class_ev_[1], 0));
+        ck(hipMemcpyAsync(pin_handed_.p + 1, class_nbail_.p + 1, sizeof(int32_t), hipMemcpyDeviceToHost, st_));
+        ck(hipMemcpyAsync(pin_handed_.p + 2, bail_.p, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, st_));
+    }
+    ck(hipMemcpyAsync(pin_res_.p, out_d_.p, sizeof(SolveOut) * n2, hipMemcpyDeviceToHost, st_));
+    ck(hipStreamSynchronize(st_));
+    std::memcpy(o1.data(), pin_res_.p, sizeof(SolveOut) * n);
+    launch_log.emplace_back(n, nb);
+    if (nb > 0) {
+        for (int k = 0; k < nb; ++k) {
+            const int id = pin_handed_.p[2 + k];
+            if (id < 0 || id >= n) return kErrInit;
+            handed[id] = 1;
+            o2[id] = pin_res_.p[n + id];
+        }
+        launch_log.emplace_back(nb, pin_handed_.p[1]);
+    }
+    return 0;
+}
+
 int Engine::solve_tasks_device(const Task* tasks, int n, SolveOut* out, bool use6, int dim, float tol) {
     if (n <= 0) return 0;
     if (!bail_.reserve(n) || !counters_.reserve(32)) return kErrInit;
@@ -1006,20 +1078,39 @@ int Engine::run(const uint8_t* def, bool on_device, const uint8_t* mask, const P
                     tasks[k].mode = cnn_rescue ? NO_SEARCH : NO_SIMPLEX;
                 }
             }
-            if (solve_tasks(tasks, res, p.use6, p.subset, tol) != 0) throw HipFail{hipErrorOutOfMemory};
+            // Under the CNN guesses on one subset size, the plain solves of the
+            // anchors whose CNN-seeded IC-GN the fast path handed on run while
+            // the full solver rescues them (its slowest anchor sets its
+            // launch's length); each anchor's result is the one the plain
+            // solve after the rescue would give.
+            std::vector<SolveOut> pre;
+            std::vector<uint8_t> handed;
+            if (!plain.empty() && grid_dims_.empty()) {
+                if (solve_tasks_spec(tasks, plain, res, pre, handed, p.use6, p.subset, tol) != 0)
+                    throw HipFail{hipErrorOutOfMemory};
+            } else if (solve_tasks(tasks, res, p.use6, p.subset, tol) != 0) {
+                throw HipFail{hipErrorOutOfMemory};
+            }
             if (!plain.empty()) {
                 // the plain anchor solve wherever the refined CNN guess failed
                 std::vector<Task> retry;
                 std::vector<int> ridx;
+                int failed = 0;
                 for (size_t k = 0; k < tasks.size(); ++k) {
                     const SolveOut& o = res[k];
                     if (o.init && o.status == 0 && o.corr <= kAnchorAcceptScore) continue;
+                    ++failed;
+                    if (!handed.empty() && handed[k]) {
+                        res[k] = pre[k];
+                        tasks[k] = plain[k];
+                        continue;
+                    }
                     retry.push_back(plain[k]);
                     ridx.push_back((int)k);
                 }
                 seed_stats.cnn += (int)tasks.size();
-                seed_stats.cnn_ok += (int)(tasks.size() - retry.size());
-                seed_stats.fallback += (int)retry.size();
+                seed_stats.cnn_ok += (int)tasks.size() - failed;
+                seed_stats.fallback += failed;
                 if (!retry.empty()) {
                     std::vector<SolveOut> rr;
                     if (solve_tasks(retry, rr, p.use6, p.subset, tol) != 0) throw HipFail{hipErrorOutOfMemory};

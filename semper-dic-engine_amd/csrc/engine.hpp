@@ -372,6 +372,14 @@ private:
     std::vector<DimClass> classes_;     // Path A launches: nodes of one size each
     DevBuf<int32_t> class_nodes_d_;
     int solve_tasks_dim(const std::vector<Task>& tasks, std::vector<SolveOut>& out, bool use6, int dim, float tol);
+    // Two task sets over the same n items, the second needed only where the
+    // first fails (one subset size): the first set's fast path, then its full
+    // solver on the engine's stream while the second set solves, on a stream
+    // of its own, the items that fast path handed on -- every item the full
+    // solver can still fail. o2[k] is set where handed[k]; each result is the
+    // one a launch of that task alone gives.
+    int solve_tasks_spec(const std::vector<Task>& t1, const std::vector<Task>& t2, std::vector<SolveOut>& o1,
+                         std::vector<SolveOut>& o2, std::vector<uint8_t>& handed, bool use6, int dim, float tol);
     struct TuneState {
         int next = 0;                      // runs so far
         int chosen = -1;                   // settled candidate
