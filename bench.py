@@ -412,7 +412,7 @@ def learned_seed_study(dev, sa, wl, steps, threads, name="c5_4096_bumps", in_fli
         assert dev.sdk_engine_set_reference(eng, dref, 1, w, h, None) == 0
         out = pinned_f32(dev, cap)
         met = np.zeros(24, np.float32)
-        if mode == "learned_seed_gpu" and forced:
+        if mode == "learned_seed_gpu" and forced and not in_flight:
             # the plain engine's settled launch shape, so that the tuners'
             # choices do not differ between the two
             assert dev.sdk_engine_set_flag(eng, 7, forced) == 0
@@ -424,12 +424,16 @@ def learned_seed_study(dev, sa, wl, steps, threads, name="c5_4096_bumps", in_fli
             if dev.sdk_engine_tuning(eng, P(shape), P(shape_ms)) == 0 and int(shape[0]) in (1, 2, 4):
                 forced = {1: 1, 2: 2, 4: 3}[int(shape[0])] + (3 if int(shape[1]) > 0 else 0)
         engines[mode] = (eng, out, met)
-    # the two methods' runs interleaved, each the median of its runs, so that
-    # drift of the clock or the host does not favour either
+    # Where both methods do the same work (the main workload, in_flight
+    # False), their runs are interleaved on one launch shape and each is the
+    # median of its runs, so that drift of the clock or the host does not
+    # favour either; otherwise each engine's runs are timed back to back, so
+    # that neither finds the caches as the other left them.
     runs = {mode: [] for mode in modes}
     last_n = {}
-    for _ in range(max(steps, 5)):
-        for mode in modes:
+    order = [modes] * max(steps, 5) if not in_flight else [(m,) * max(steps, 5) for m in modes]
+    for group in order:
+        for mode in group:
             eng, out, met = engines[mode]
             t0 = time.perf_counter()
             last_n[mode] = dev.sdk_engine_run(eng, ddef, 1, P(prm), P(out), cap, P(met), 24, tol)
